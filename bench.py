@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark: chi0 + eps^-1 q-points/s on one MI355X, split-FP32 vs FP64.
+
+Workload (one "step" = one q-point of the single-q realistic case):
+  Nk=64 k-points, Nocc=16, Nunocc=400 (n = 409,600 transitions),
+  mixed basis Nb=1200, Nw=12 imaginary frequencies (Gauss-Legendre on
+  [0,1] mapped to omega = x/(1-x)), dense random SPD Coulomb matrix with
+  eigenvalues log-uniform in [1e-3, 1e3] (v^1/2 exact).
+  Timed per step, inputs resident in HBM:
+    HILBERT NONE weights -> chi0 (rank-k MFMA, spin factor 2)
+    -> eps = 1 - v^1/2 chi0 v^1/2 (FP64 MFMA GEMMs) -> eps^-1 (FP64).
+  The product matrix M (n x Nb complex128) is synthetic (Gaussian,
+  Re/Im ~ N(0,1)/sqrt(Nb)) and generated on the device before timing.
+
+Prints ONE JSON line.  ``value`` is q-points/s of the split-FP32 form; the
+FP64 form, the error of split vs FP64, the chi0 roofline and the CPU
+baseline (the oracle's literal pair loop on a bounded sample) ride along.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = ("χ0+ε⁻¹ q-points/s & GFLOP/s, split-FP32 vs FP64 at "
+          "≤1e-8 rel; p50/p99 latency")
+PEAK_F64 = 78.6  # TFLOP/s dense FP64 matrix, MI355X
+PEAK_F32 = 157.3  # TFLOP/s dense FP32 matrix, MI355X
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--nk", type=int, default=64)
+    p.add_argument("--nocc", type=int, default=16)
+    p.add_argument("--nunocc", type=int, default=400)
+    p.add_argument("--nb", type=int, default=1200)
+    p.add_argument("--nw", type=int, default=12)
+    p.add_argument("--seed", type=int, default=2026)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--cpu-cols", type=int, default=128,
+                   help="CPU baseline sample: all pairs among this many columns")
+    return p.parse_args()
+
+
+def chi0_flops(nw, n, nb):
+    return 8.0 * nw * n * nb * (nb + 1) / 2
+
+
+def eps_flops(nw, nb):
+    return nw * (16.0 * nb**3 + 16.0 / 3 * nb**3)
+
+
+class Pipeline:
+    """Device-resident single-q pipeline: weights -> chi0 -> eps -> eps^-1."""
+
+    def __init__(self, ctx, a):
+        from project1_amd import synth
+
+        self.ctx, self.a = ctx, a
+        nk, no, nu, nb, nw = a.nk, a.nocc, a.nunocc, a.nb, a.nw
+        self.n = n = nk * no * nu
+        g = synth.rng(a.seed)
+        eo, eu = synth.energies(g, nk, no, nu)
+        self.omega, _ = synth.gl_frequencies(nw)
+        _, vh, lam = synth.coulomb(g, nb)
+        self.vh = vh
+        self.dM = ctx.alloc(16 * n * nb)
+        ctx.synth_normal(self.dM, 2 * n * nb, a.seed + 1, 1.0 / np.sqrt(nb))
+        self.deo, self.deu = ctx.to_dev(eo), ctx.to_dev(eu)
+        self.dnk = ctx.to_dev(np.ones(nk, np.int32))
+        self.dom = ctx.to_dev(self.omega)
+        self.dW = ctx.alloc(8 * n * nw)
+        self.dV = ctx.to_dev(np.asfortranarray(vh.astype(np.complex128)))
+        self.dX = ctx.alloc(16 * nb * nb * nw)
+        self.dE = ctx.alloc(16 * nb * nb * nw)
+        self.ev = [ctx.event() for _ in range(4)]
+        ctx.sync()
+
+    def step(self, prec):
+        ctx, a, ev = self.ctx, self.a, self.ev
+        ev[0].record()
+        ctx.weights_none_dev(self.deo, self.deu, a.nk, a.nocc, a.nunocc, a.nk, 1, self.dnk,
+                             self.dom, a.nw, self.dW)
+        ev[1].record()
+        ctx.chi0_dev(self.n, a.nb, self.dM, a.nb, False, a.nw, self.dW, prec, self.dX, a.nb,
+                     alpha=2.0)
+        ev[2].record()
+        ctx.dielectric_dev(self.dX, a.nb, a.nw, a.nb, 1, self.dV, self.dE, a.nb)
+        ctx.inverse_hpd_dev(self.dE, a.nb, a.nw, a.nb)
+        ev[3].record()
+        return ev[2].ms_since(ev[1]), ev[3].ms_since(ev[2]), ev[3].ms_since(ev[0])
+
+    def results(self):
+        a = self.a
+        chi = self.ctx.pack_upper(self.dX, a.nb, a.nw, a.nb)
+        einv = self.ctx.pack_upper(self.dE, a.nb, a.nw, a.nb)
+        return chi, einv
+
+
+def run_form(pipe, prec, steps, warmup):
+    for _ in range(warmup):
+        pipe.step(prec)
+    pipe.ctx.sync()
+    t0 = time.perf_counter()
+    chi_ms, eps_ms, tot_ms = [], [], []
+    for _ in range(steps):
+        c, e, t = pipe.step(prec)
+        chi_ms.append(c)
+        eps_ms.append(e)
+        tot_ms.append(t)
+    pipe.ctx.sync()
+    wall = time.perf_counter() - t0
+    return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, chi0_ms=float(np.mean(chi_ms)),
+                eps_ms=float(np.mean(eps_ms)), step_ms_p50=float(np.percentile(tot_ms, 50)),
+                step_ms_p99=float(np.percentile(tot_ms, 99)))
+
+
+def errs(a, b):
+    d = np.abs(a - b)
+    return dict(max_abs=float(d.max()), max_rel=float(d.max() / np.abs(b).max()),
+                fro_rel=float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+
+
+def cpu_baseline(pipe, a):
+    """The oracle's literal pair loop (susceptibility.f:1183-1240) on a sample."""
+    import oracle
+
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n, nb, nw = pipe.n, a.nb, a.nw
+    cols = np.linspace(0, nb - 1, min(a.cpu_cols, nb)).astype(int)
+    M = pipe.dM.download(np.complex128, (n, nb))
+    Ms = np.asfortranarray(M[:, cols])
+    del M
+    W = pipe.dW.download(np.float64, (nw, n)).T
+    nc = cols.size
+    npairs = nc * (nc + 1) // 2
+    t0 = time.perf_counter()
+    oracle.chi0_pairsum(Ms, W, nthreads=nthreads)
+    t_pairs = time.perf_counter() - t0
+    total_pairs = nb * (nb + 1) / 2
+    # one frequency of the dielectric step + inverse with the oracle
+    chi_full = pipe.ctx.pack_upper(pipe.dX, nb, 1, nb)[0]
+    t0 = time.perf_counter()
+    eps = oracle.dielectric_dense(chi_full, pipe.vh.astype(np.complex128))
+    oracle.inverse_hpd_packed(eps)
+    t_eps1 = time.perf_counter() - t0
+    t_q = t_pairs * total_pairs / npairs + t_eps1 * nw
+    return dict(value=1.0 / t_q, unit="q-points/s", cores=nthreads, kind="port",
+                sample=(f"literal pair loop over all {npairs} pairs of {nc} of {nb} columns "
+                        f"({t_pairs:.2f} s, extrapolated x{total_pairs / npairs:.1f}) + eps/eps^-1 "
+                        f"for 1 of {nw} frequencies ({t_eps1:.2f} s, x{nw}); chi0 scaled "
+                        f"linearly in pairs (equal cost per pair)"),
+                seconds_per_q_extrapolated=t_q)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+    from project1_amd import spexhip
+
+    ctx = spexhip.Context(0)
+    pipe = Pipeline(ctx, a)
+    res = {}
+    out = {}
+    for name, prec in (("fp64", spexhip.PREC_FP64), ("split_fp32", spexhip.PREC_SPLIT_FP32)):
+        r = run_form(pipe, prec, a.steps, a.warmup)
+        out[name] = pipe.results()
+        r["qpts_per_s"] = 1e3 / r["ms_per_step"]
+        r["gflops"] = (chi0_flops(a.nw, pipe.n, a.nb) + eps_flops(a.nw, a.nb)) / (r["ms_per_step"] * 1e6)
+        r["chi0_tflops"] = chi0_flops(a.nw, pipe.n, a.nb) / (r["chi0_ms"] * 1e9)
+        res[name] = r
+    e_chi = errs(out["split_fp32"][0], out["fp64"][0])
+    e_inv = errs(out["split_fp32"][1], out["fp64"][1])
+    head = res["split_fp32"]
+    line = {
+        "metric": METRIC,
+        "value": round(head["qpts_per_s"], 4),
+        "unit": "q-points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32x3-split/fp64-acc (eps in f64)",
+        "data": "synthetic (seeded Gaussian products on device, random SPD Coulomb matrix)",
+        "config": {"workload": "single q, Nk=64 Nocc=16 Nunocc=400 Nb=1200 Nw=12, dense v^1/2",
+                   "nk": a.nk, "nocc": a.nocc, "nunocc": a.nunocc, "nb": a.nb, "nw": a.nw,
+                   "parallelism": "1 GPU"},
+        "gflops": round(head["gflops"], 1),
+        "latency_ms": {"p50": round(head["step_ms_p50"], 3), "p99": round(head["step_ms_p99"], 3)},
+        "fp64": {k: round(v, 4) for k, v in res["fp64"].items()},
+        "split_fp32": {k: round(v, 4) for k, v in res["split_fp32"].items()},
+        "speedup_split_vs_fp64": round(res["fp64"]["ms_per_step"] / head["ms_per_step"], 3),
+        "err_split_vs_fp64": {"chi0": e_chi, "eps_inv": e_inv},
+        "roofline": {"bound": "mfma", "kernel": "chi0_split_kernel",
+                     "achieved": round(head["chi0_tflops"], 2), "peak": PEAK_F32,
+                     "unit": "TFLOP/s", "frac": round(head["chi0_tflops"] / PEAK_F32, 4),
+                     "traffic": None,
+                     "fp64_kernel": {"kernel": "chi0_f64_kernel",
+                                     "achieved": round(res["fp64"]["chi0_tflops"], 2),
+                                     "peak": PEAK_F64,
+                                     "frac": round(res["fp64"]["chi0_tflops"] / PEAK_F64, 4)}},
+    }
+    if not a.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(pipe, a)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,87 @@
+"""Build the spexhip shared library (HIP, gfx950) and the Fortran interface.
+
+Run as ``python -m project1_amd.build``; ``__graft_entry__.build()`` calls
+:func:`build_all`.  Objects go to ``project1_amd/_build/``; the library is
+``project1_amd/libspexhip.so`` (in-tree so it travels with the sources).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libspexhip.so"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SPEXHIP_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-fPIC",
+    "-std=c++17",
+    "-Wall",
+    "-Wno-unused-function",
+    "-Wno-unused-variable",
+    "-Wno-unused-result",
+    # packed FP32 VALU ops stay disabled in every device pass
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{ROOT / 'include'}",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(
+        l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
+    )
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out, file=sys.stderr)
+
+
+def _compile(src: Path) -> Path:
+    obj = BUILD / (src.stem + ".o")
+    _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+    return obj
+
+
+def build_lib(jobs: int = 8) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    srcs = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+    with cf.ThreadPoolExecutor(max_workers=min(jobs, len(srcs))) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+          *map(str, objs), "-lpthread"])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran() -> Path | None:
+    """Compile the ISO_C_BINDING interface module with amdflang, if present."""
+    fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+    src = PKG / "fortran" / "m_spexhip.f90"
+    if not (Path(fc).exists() and src.exists()):
+        return None
+    BUILD.mkdir(exist_ok=True)
+    obj = BUILD / "m_spexhip.o"
+    _run([fc, "-c", "-O2", "-fPIC", f"-J{BUILD}", str(src), "-o", str(obj)])
+    return obj
+
+
+def build_all() -> Path:
+    lib = build_lib()
+    build_fortran()
+    return lib
+
+
+if __name__ == "__main__":
+    print(build_all())

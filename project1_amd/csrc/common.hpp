@@ -1,0 +1,58 @@
+// Shared host/device helpers of the spexhip library (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/spexhip.h"
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+struct spexhip_ctx {
+  int device = 0;
+  size_t budget = 0;
+  size_t used = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::mutex mu;  // guards err, used, allocs
+  std::unordered_map<void *, size_t> allocs;
+  // scratch for host drop-in calls (grown on demand, owned by the context)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+};
+
+int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...);
+
+#define SPX_HIP(ctx, call)                                                       \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess)                                                        \
+      return spx_seterr((ctx), SPEXHIP_E_DEVICE, "%s failed: %s (%s:%d)", #call, \
+                        hipGetErrorString(e_), __FILE__, __LINE__);              \
+  } while (0)
+
+#define SPX_LAUNCHED(ctx) SPX_HIP(ctx, hipGetLastError())
+
+#define SPX_CHECK(ctx, cond, ...)                                   \
+  do {                                                              \
+    if (!(cond)) return spx_seterr((ctx), SPEXHIP_E_INVALID, __VA_ARGS__); \
+  } while (0)
+
+// Allocation tracked against the context budget.
+int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p);
+int spx_free(spexhip_ctx *ctx, void *p);
+// Context-owned scratch of at least `bytes` (contents undefined).
+int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p);
+// Is `p` (or p+bytes-1) inside an allocation of this context?
+bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes);
+
+// Internal launchers shared between translation units.
+int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
+                    const void *M, long ldm, int colmajor, int nw,
+                    const void *wght, int precision, double alpha, double beta,
+                    void *chi, int ldchi);

@@ -1,0 +1,187 @@
+// Context, error reporting and budget-tracked device memory.
+// The budget mirrors the reference's MEM accounting (checkmem.f:16-92):
+// an allocation that would exceed it fails with SPEXHIP_E_NOMEM instead of
+// aborting the process.
+#include "common.hpp"
+
+int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...) {
+  if (!ctx) return code;
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->err = buf;
+  return code;
+}
+
+int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p) {
+  if (bytes == 0) bytes = 256;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->budget && ctx->used + bytes > ctx->budget) {
+      ctx->err = "HBM budget exceeded: used " + std::to_string(ctx->used) +
+                 " + request " + std::to_string(bytes) + " > budget " +
+                 std::to_string(ctx->budget);
+      return SPEXHIP_E_NOMEM;
+    }
+    ctx->used += bytes;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->used -= bytes;
+    ctx->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? SPEXHIP_E_NOMEM : SPEXHIP_E_DEVICE;
+  }
+  std::lock_guard<std::mutex> g(ctx->mu);
+  ctx->allocs[*p] = bytes;
+  return SPEXHIP_OK;
+}
+
+int spx_free(spexhip_ctx *ctx, void *p) {
+  if (!p) return SPEXHIP_OK;
+  size_t bytes = 0;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->allocs.find(p);
+    if (it == ctx->allocs.end()) {
+      ctx->err = "spexhip_dev_free: pointer not owned by this context";
+      return SPEXHIP_E_INVALID;
+    }
+    bytes = it->second;
+    ctx->allocs.erase(it);
+    ctx->used -= bytes;
+  }
+  SPX_HIP(ctx, hipFree(p));
+  return SPEXHIP_OK;
+}
+
+bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes) {
+  std::lock_guard<std::mutex> g(ctx->mu);
+  const char *c = (const char *)p;
+  for (auto &kv : ctx->allocs) {
+    const char *b = (const char *)kv.first;
+    if (c >= b && c + bytes <= b + kv.second) return true;
+  }
+  return false;
+}
+
+int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p) {
+  if (ctx->scratch_bytes >= bytes) {
+    *p = ctx->scratch;
+    return SPEXHIP_OK;
+  }
+  if (ctx->scratch) {
+    SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = spx_free(ctx, ctx->scratch);
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    if (rc) return rc;
+  }
+  int rc = spx_alloc(ctx, bytes, &ctx->scratch);
+  if (rc) return rc;
+  ctx->scratch_bytes = bytes;
+  *p = ctx->scratch;
+  return SPEXHIP_OK;
+}
+
+extern "C" {
+
+int spexhip_create(int device, size_t budget, spexhip_ctx **out) {
+  if (!out) return SPEXHIP_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SPEXHIP_E_DEVICE;
+  if (device < 0 || device >= ndev) return SPEXHIP_E_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return SPEXHIP_E_DEVICE;
+  spexhip_ctx *c = new spexhip_ctx();
+  c->device = device;
+  c->budget = budget;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return SPEXHIP_E_DEVICE;
+  }
+  *out = c;
+  return SPEXHIP_OK;
+}
+
+int spexhip_destroy(spexhip_ctx *ctx) {
+  if (!ctx) return SPEXHIP_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
+  ctx->allocs.clear();
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return SPEXHIP_OK;
+}
+
+int spexhip_last_error(spexhip_ctx *ctx, char *buf, size_t len) {
+  if (!ctx || !buf || !len) return SPEXHIP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  snprintf(buf, len, "%s", ctx->err.c_str());
+  return SPEXHIP_OK;
+}
+
+int spexhip_sync(spexhip_ctx *ctx) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SPEXHIP_OK;
+}
+
+int spexhip_mem_used(spexhip_ctx *ctx, size_t *used, size_t *budget) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  if (used) *used = ctx->used;
+  if (budget) *budget = ctx->budget;
+  return SPEXHIP_OK;
+}
+
+int spexhip_dev_alloc(spexhip_ctx *ctx, size_t bytes, void **dev) {
+  if (!ctx || !dev) return SPEXHIP_E_INVALID;
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  return spx_alloc(ctx, bytes, dev);
+}
+
+int spexhip_dev_free(spexhip_ctx *ctx, void *dev) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return spx_free(ctx, dev);
+}
+
+int spexhip_h2d(spexhip_ctx *ctx, void *dev, const void *host, size_t bytes) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  if (!bytes) return SPEXHIP_OK;
+  SPX_CHECK(ctx, dev && host, "spexhip_h2d: null pointer");
+  SPX_CHECK(ctx, spx_owns(ctx, dev, bytes), "spexhip_h2d: destination outside a context allocation");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  SPX_HIP(ctx, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SPEXHIP_OK;
+}
+
+int spexhip_d2h(spexhip_ctx *ctx, void *host, const void *dev, size_t bytes) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  if (!bytes) return SPEXHIP_OK;
+  SPX_CHECK(ctx, dev && host, "spexhip_d2h: null pointer");
+  SPX_CHECK(ctx, spx_owns(ctx, dev, bytes), "spexhip_d2h: source outside a context allocation");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  SPX_HIP(ctx, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SPEXHIP_OK;
+}
+
+int spexhip_memset(spexhip_ctx *ctx, void *dev, int value, size_t bytes) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  if (!bytes) return SPEXHIP_OK;
+  SPX_CHECK(ctx, spx_owns(ctx, dev, bytes), "spexhip_memset: range outside a context allocation");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  SPX_HIP(ctx, hipMemsetAsync(dev, value, bytes, ctx->stream));
+  return SPEXHIP_OK;
+}
+
+}  // extern "C"

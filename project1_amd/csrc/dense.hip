@@ -1,0 +1,474 @@
+// Dielectric step and its inverse in FP64 on the GPU.
+//
+// Reference: correlation.f:612-621 (eps = 1 - v^1/2 chi v^1/2 per imaginary
+// frequency, packed, with p_diagmatdiag_dzpd1 wrapper.f:1264-1278 and
+// p_plusdiag1_zp :1317-1328) and correlation.f:701-747 (Minverse per
+// frequency -> inverse_zp1 ZHPTRF/ZHPTRI wrapper.f:2362-2374, or
+// PZPOTRF/PZPOTRI w_scalapack.inc:752-761 under MPI).
+//
+// Device matrices are full column-major nb x nb per frequency (ld >= nb,
+// frequency stride ld*nb).  v^1/2 is either diagonal (sqrt(coul), the
+// Coulomb eigenbasis of the reference) or a dense Hermitian matrix; the
+// dense products use an FP64 MFMA complex GEMM.  The inverse is an in-place
+// blocked Gauss-Jordan elimination without pivoting (eps is Hermitian
+// positive definite on the imaginary axis, eigenvalues >= 1, so every
+// leading block is HPD and no pivoting is needed), with 64-wide panels:
+// one workgroup inverts the diagonal block in LDS, an MFMA GEMM forms the
+// row panel, and a rank-64 MFMA update sweeps the whole matrix.
+#include "common.hpp"
+
+namespace {
+
+constexpr int TM = 64, KC = 16, NT = 256;
+
+// ------------------------------------------------- complex FP64 GEMM tile --
+// acc += A(i0:i0+64, 0:kdim) * B(0:kdim, j0:j0+64); A, B column-major.
+// Rows/cols outside [0,m) / [0,n) and k outside [0,kdim) read as zero.
+struct TileAcc {
+  d4_t r[2][2], i[2][2];
+};
+
+__device__ inline void tile_zero(TileAcc &a) {
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++) {
+      a.r[x][y] = d4_t{0, 0, 0, 0};
+      a.i[x][y] = d4_t{0, 0, 0, 0};
+    }
+}
+
+struct GemmLds {
+  double Ar[2][KC][TM], Ai[2][KC][TM], Br[2][KC][TM], Bi[2][KC][TM];
+};
+
+__device__ void gemm_tile(const double2 *__restrict__ A, long lda,
+                          const double2 *__restrict__ B, long ldb, int m, int n,
+                          int kdim, int i0, int j0, TileAcc &acc, GemmLds &s) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = wv >> 1, wc = wv & 1, lr = lane & 15, lk = lane >> 4;
+  double2 va[4], vb[4];
+  const int nst = (kdim + KC - 1) / KC;
+  auto fetch = [&](int st) {
+    const int k0 = st * KC;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int e = tid + NT * r;
+      {  // A: i fastest (coalesced down a column)
+        const int i = e % TM, k = e / TM;
+        const int gi = i0 + i, gk = k0 + k;
+        va[r] = (gi < m && gk < kdim) ? A[gi + lda * (long)gk] : make_double2(0, 0);
+      }
+      {  // B: k fastest
+        const int k = e % KC, j = e / KC;
+        const int gj = j0 + j, gk = k0 + k;
+        vb[r] = (gj < n && gk < kdim) ? B[gk + ldb * (long)gj] : make_double2(0, 0);
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int e = tid + NT * r;
+      {
+        const int i = e % TM, k = e / TM;
+        s.Ar[buf][k][i] = va[r].x;
+        s.Ai[buf][k][i] = va[r].y;
+      }
+      {
+        const int k = e % KC, j = e / KC;
+        s.Br[buf][k][j] = vb[r].x;
+        s.Bi[buf][k][j] = vb[r].y;
+      }
+    }
+  };
+  __syncthreads();  // LDS may still be in use by a previous call
+  fetch(0);
+  store(0);
+  __syncthreads();
+  for (int st = 0; st < nst; st++) {
+    const int buf = st & 1;
+    if (st + 1 < nst) fetch(st + 1);
+#pragma unroll
+    for (int kk = 0; kk < KC / 4; kk++) {
+      const int k = kk * 4 + lk;
+      double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) {
+        ar[x] = s.Ar[buf][k][wr * 32 + x * 16 + lr];
+        ai[x] = s.Ai[buf][k][wr * 32 + x * 16 + lr];
+        br[x] = s.Br[buf][k][wc * 32 + x * 16 + lr];
+        bi[x] = s.Bi[buf][k][wc * 32 + x * 16 + lr];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int y = 0; y < 2; y++) {
+          // (ar + i ai)(br + i bi) = ar br - ai bi + i(ar bi + ai br)
+          acc.r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc.r[x][y], 0, 0, 0);
+          acc.r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], acc.r[x][y], 0, 0, 0);
+          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acc.i[x][y], 0, 0, 0);
+          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acc.i[x][y], 0, 0, 0);
+        }
+    }
+    if (st + 1 < nst) store(buf ^ 1);
+    __syncthreads();
+  }
+}
+
+// Visit the tile's elements owned by this lane: f(i, j, re, im).  The
+// v_mfma_f64_16x16x4f64 result holds column j = lane%16, rows lane/16 + 4*r.
+template <class F>
+__device__ inline void tile_foreach(const TileAcc &acc, int i0, int j0, F f) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = wv >> 1, wc = wv & 1, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int x = 0; x < 2; x++)
+#pragma unroll
+    for (int y = 0; y < 2; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        f(i0 + wr * 32 + x * 16 + lk + 4 * r, j0 + wc * 32 + y * 16 + lr, acc.r[x][y][r],
+          acc.i[x][y][r]);
+}
+
+// C = alpha*A*B + beta*C + diag*I, batched over blockIdx.z with strides.
+__global__ __launch_bounds__(NT) void zgemm_kernel(int m, int n, int k, double alpha,
+                                                   const double2 *__restrict__ A, long lda, long sA,
+                                                   const double2 *__restrict__ B, long ldb, long sB,
+                                                   double beta, double2 *__restrict__ C, long ldc,
+                                                   long sC, double diag) {
+  __shared__ GemmLds s;
+  const int b = blockIdx.z;
+  const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TM;
+  TileAcc acc;
+  tile_zero(acc);
+  gemm_tile(A + sA * b, lda, B + sB * b, ldb, m, n, k, i0, j0, acc, s);
+  double2 *Cb = C + sC * b;
+  tile_foreach(acc, i0, j0, [&](int i, int j, double re, double im) {
+    if (i >= m || j >= n) return;
+    double2 *p = Cb + i + ldc * (long)j;
+    double2 o = make_double2(alpha * re, alpha * im);
+    if (beta != 0.0) {
+      const double2 old = *p;
+      o.x += beta * old.x;
+      o.y += beta * old.y;
+    }
+    if (i == j) o.x += diag;
+    *p = o;
+  });
+}
+
+// Lower triangle <- conj(upper) (I > J), optional real diagonal.
+__global__ void hermitize_kernel(int nb, double2 *__restrict__ X, long ldx, long sX,
+                                 int real_diag) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nn = (long)nb * nb;
+  if (g >= nn) return;
+  const int j = (int)(g / nb), i = (int)(g % nb);
+  double2 *Xb = X + sX * blockIdx.y;
+  if (i > j) {
+    double2 v = Xb[j + ldx * (long)i];
+    v.y = -v.y;
+    Xb[i + ldx * (long)j] = v;
+  } else if (i == j && real_diag) {
+    Xb[i + ldx * (long)i].y = 0.0;
+  }
+}
+
+// eps = 1 - diag(s) chi diag(s) on the full matrix (correlation.f:616-620).
+__global__ void eps_diag_kernel(int nb, const double *__restrict__ sq,
+                                const double2 *__restrict__ X, long ldx, long sX,
+                                double2 *__restrict__ E, long lde, long sE) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)nb * nb) return;
+  const int j = (int)(g / nb), i = (int)(g % nb);
+  const double2 v = X[sX * blockIdx.y + i + ldx * (long)j];
+  const double f = sq[i] * sq[j];
+  double2 o = make_double2(-f * v.x, -f * v.y);
+  if (i == j) o.x += 1.0;
+  E[sE * blockIdx.y + i + lde * (long)j] = o;
+}
+
+// ----------------------------------------------- Gauss-Jordan inversion --
+constexpr int GB = 64;  // panel width
+
+// P = inverse of the GB x GB diagonal block at (k0,k0) (rows/cols beyond nb
+// are treated as identity).  One workgroup per matrix (blockIdx.y).
+__global__ __launch_bounds__(NT) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
+                                                     long ldx, long sX, int k0,
+                                                     double2 *__restrict__ P, int *__restrict__ flag) {
+  __shared__ double ar[GB][GB + 1], ai[GB][GB + 1];
+  __shared__ double pr_s;
+  const int tid = threadIdx.x;
+  const double2 *Xb = X + sX * blockIdx.y;
+  for (int e = tid; e < GB * GB; e += NT) {
+    const int i = e % GB, j = e / GB;
+    double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
+    if (k0 + i < nb && k0 + j < nb) v = Xb[(k0 + i) + ldx * (long)(k0 + j)];
+    ar[i][j] = v.x;
+    ai[i][j] = v.y;
+  }
+  __syncthreads();
+  for (int p = 0; p < GB; p++) {
+    if (tid == 0) {
+      // Hermitian positive definite: the pivot is real and positive.
+      const double d = ar[p][p];
+      if (!(d > 0.0)) flag[0] = 1;
+      pr_s = 1.0 / d;
+    }
+    __syncthreads();
+    const double pr = pr_s;
+    // scale row p
+    for (int j = tid; j < GB; j += NT)
+      if (j != p) {
+        const double xr = ar[p][j], xi = ai[p][j];
+        ar[p][j] = xr * pr;
+        ai[p][j] = xi * pr;
+      }
+    __syncthreads();
+    // eliminate column p from the other rows
+    for (int e = tid; e < GB * GB; e += NT) {
+      const int i = e % GB, j = e / GB;
+      if (i == p || j == p) continue;
+      const double fr = ar[i][p], fi = ai[i][p];
+      const double xr = ar[p][j], xi = ai[p][j];
+      ar[i][j] -= fr * xr - fi * xi;
+      ai[i][j] -= fr * xi + fi * xr;
+    }
+    __syncthreads();
+    for (int i = tid; i < GB; i += NT)
+      if (i != p) {
+        const double fr = ar[i][p], fi = ai[i][p];
+        ar[i][p] = -fr * pr;
+        ai[i][p] = -fi * pr;
+      }
+    if (tid == 0) {
+      ar[p][p] = pr;
+      ai[p][p] = 0.0;
+    }
+    __syncthreads();
+  }
+  double2 *Pb = P + (long)GB * GB * blockIdx.y;
+  for (int e = tid; e < GB * GB; e += NT) {
+    const int i = e % GB, j = e / GB;
+    Pb[i + GB * j] = make_double2(ar[i][j], ai[i][j]);
+  }
+}
+
+// Copy the old column panel X(:, k0:k0+GB) into Cp (nb x GB, ld nbp).
+__global__ void gj_copy_col_kernel(int nb, const double2 *__restrict__ X, long ldx, long sX,
+                                   int k0, double2 *__restrict__ Cp, long ldcp) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)nb * GB) return;
+  const int c = (int)(g / nb), i = (int)(g % nb);
+  double2 v = make_double2(0, 0);
+  if (k0 + c < nb) v = X[sX * blockIdx.y + i + ldx * (long)(k0 + c)];
+  Cp[(long)ldcp * GB * blockIdx.y + i + ldcp * (long)c] = v;
+}
+
+// Rank-GB sweep of Gauss-Jordan step k0 (see file header):
+//   A(K,K) = P;  A(K,J) = R(:,J);  A(I,K) = -Cp(I,:) P;  A(I,J) -= Cp(I,:) R(:,J)
+__global__ __launch_bounds__(NT) void gj_update_kernel(int nb, double2 *__restrict__ X, long ldx,
+                                                       long sX, int k0,
+                                                       const double2 *__restrict__ P,
+                                                       const double2 *__restrict__ R, long ldr, long sR,
+                                                       const double2 *__restrict__ Cp, long ldcp) {
+  __shared__ GemmLds s;
+  const int b = blockIdx.z;
+  const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TM;
+  const bool rowK = (i0 == k0), colK = (j0 == k0);
+  double2 *Xb = X + sX * b;
+  const double2 *Pb = P + (long)GB * GB * b;
+  const double2 *Rb = R + sR * b;
+  const double2 *Cb = Cp + (long)ldcp * GB * b;
+  if (rowK) {
+    // A(K, J) = R(:, J)   (for J == K, R(:,K) == P by construction)
+    for (int e = threadIdx.x; e < TM * TM; e += NT) {
+      const int i = e % TM, j = e / TM;
+      if (i0 + i < nb && j0 + j < nb) Xb[(i0 + i) + ldx * (long)(j0 + j)] = colK ? Pb[i + GB * j] : Rb[i + ldr * (long)(j0 + j)];
+    }
+    return;
+  }
+  TileAcc acc;
+  tile_zero(acc);
+  if (colK)
+    gemm_tile(Cb, ldcp, Pb, GB, nb, GB, GB, i0, 0, acc, s);
+  else
+    gemm_tile(Cb, ldcp, Rb, ldr, nb, nb, GB, i0, j0, acc, s);
+  tile_foreach(acc, i0, j0, [&](int i, int j, double re, double im) {
+    if (i >= nb || j >= nb) return;
+    double2 *p = Xb + i + ldx * (long)j;
+    if (colK) {
+      *p = make_double2(-re, -im);
+    } else {
+      double2 o = *p;
+      o.x -= re;
+      o.y -= im;
+      *p = o;
+    }
+  });
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers --
+int spx_zgemm(spexhip_ctx *ctx, hipStream_t st, int m, int n, int k, double alpha,
+              const void *A, long lda, long sA, const void *B, long ldb, long sB, double beta,
+              void *C, long ldc, long sC, double diag, int batch) {
+  dim3 grid((m + TM - 1) / TM, (n + TM - 1) / TM, batch);
+  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(NT), 0, st, m, n, k, alpha, (const double2 *)A, lda,
+                     sA, (const double2 *)B, ldb, sB, beta, (double2 *)C, ldc, sC, diag);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
+int spx_hermitize(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx,
+                  int real_diag) {
+  const long nn = (long)nb * nb;
+  hipLaunchKernelGGL(hermitize_kernel, dim3((nn + 255) / 256, nw), dim3(256), 0, st, nb,
+                     (double2 *)X, ldx, ldx * (long)nb, real_diag);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
+// Workspace (bytes) of spx_inverse for nb, nw.
+size_t spx_inverse_ws(int nb, int nw) {
+  const long nbp = ((nb + GB - 1) / GB) * GB;
+  return 16 * (size_t)nw * ((size_t)GB * GB + (size_t)GB * nbp + (size_t)nbp * GB) + 256;
+}
+
+// In-place inverse of nw HPD matrices (full storage).  ws: spx_inverse_ws
+// bytes; flag: device int, set non-zero if a pivot is not positive.
+int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, void *ws,
+                int *flag) {
+  const long nbp = ((nb + GB - 1) / GB) * GB;
+  double2 *P = (double2 *)ws;
+  double2 *R = P + (long)nw * GB * GB;        // GB x nbp per matrix (ld GB)
+  double2 *Cp = R + (long)nw * GB * nbp;      // nbp x GB per matrix (ld nbp)
+  const long sX = ldx * (long)nb;
+  const int nt = (int)(nbp / GB);
+  for (int kb = 0; kb < nt; kb++) {
+    const int k0 = kb * GB;
+    hipLaunchKernelGGL(gj_diag_kernel, dim3(1, nw), dim3(NT), 0, st, nb, (const double2 *)X, ldx,
+                       sX, k0, P, flag);
+    SPX_LAUNCHED(ctx);
+    // R = P * X(K, :)   (GB x nb), batched
+    int rc = spx_zgemm(ctx, st, GB, nb, (nb - k0 < GB ? nb - k0 : GB), 1.0, P, GB, (long)GB * GB,
+                       (const double2 *)X + k0, ldx, sX, 0.0, R, GB, (long)GB * nbp, 0.0, nw);
+    if (rc) return rc;
+    const long tot = (long)nb * GB;
+    hipLaunchKernelGGL(gj_copy_col_kernel, dim3((tot + 255) / 256, nw), dim3(256), 0, st, nb,
+                       (const double2 *)X, ldx, sX, k0, Cp, nbp);
+    SPX_LAUNCHED(ctx);
+    hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nw), dim3(NT), 0, st, nb, (double2 *)X, ldx,
+                       sX, k0, (const double2 *)P, (const double2 *)R, (long)GB, (long)GB * nbp,
+                       (const double2 *)Cp, nbp);
+    SPX_LAUNCHED(ctx);
+  }
+  return SPEXHIP_OK;
+}
+
+// eps = 1 - Vh chi Vh.  vkind 0: vh = real vector sqrt(coul); 1: dense
+// complex nb x nb (column-major, ld nb).  chi is Hermitized first.  E may
+// alias X only for vkind 0.  tmp: nb*nb*nw complex (vkind 1).
+int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, int vkind,
+                   const void *vh, void *E, long lde, void *tmp) {
+  int rc = spx_hermitize(ctx, st, nb, nw, X, ldx, 0);
+  if (rc) return rc;
+  const long nn = (long)nb * nb;
+  if (vkind == 0) {
+    hipLaunchKernelGGL(eps_diag_kernel, dim3((nn + 255) / 256, nw), dim3(256), 0, st, nb,
+                       (const double *)vh, (const double2 *)X, ldx, ldx * (long)nb, (double2 *)E,
+                       lde, lde * (long)nb);
+    SPX_LAUNCHED(ctx);
+    return SPEXHIP_OK;
+  }
+  // tmp = chi * Vh ; E = I - Vh * tmp
+  rc = spx_zgemm(ctx, st, nb, nb, nb, 1.0, X, ldx, ldx * (long)nb, vh, nb, 0, 0.0, tmp, nb, nn, 0.0, nw);
+  if (rc) return rc;
+  return spx_zgemm(ctx, st, nb, nb, nb, -1.0, vh, nb, 0, tmp, nb, nn, 0.0, E, lde, lde * (long)nb, 1.0, nw);
+}
+
+extern "C" {
+
+int spexhip_dielectric_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx, int vkind,
+                           const void *vhalf, void *E, int lde) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && ldx >= nb && lde >= nb, "dielectric: bad sizes");
+  SPX_CHECK(ctx, vkind == 0 || vkind == 1, "dielectric: vkind must be 0 (diagonal) or 1 (dense)");
+  SPX_CHECK(ctx, spx_owns(ctx, X, 16 * (size_t)ldx * nb * nw) && spx_owns(ctx, E, 16 * (size_t)lde * nb * nw),
+            "dielectric: X/E not context allocations of the stated shape");
+  SPX_CHECK(ctx, spx_owns(ctx, vhalf, vkind ? 16 * (size_t)nb * nb : 8 * (size_t)nb),
+            "dielectric: vhalf not a context allocation of the stated shape");
+  SPX_CHECK(ctx, vkind == 0 || X != E, "dielectric: dense v^1/2 needs E != X");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *tmp = nullptr;
+  if (vkind) {
+    int rc = spx_scratch(ctx, 16 * (size_t)nb * nb * nw, &tmp);
+    if (rc) return rc;
+  }
+  return spx_dielectric(ctx, ctx->stream, nb, nw, X, ldx, vkind, vhalf, E, lde, tmp);
+}
+
+int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && ldx >= nb, "inverse: bad sizes");
+  SPX_CHECK(ctx, spx_owns(ctx, X, 16 * (size_t)ldx * nb * nw), "inverse: X not a context allocation of the stated shape");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *ws;
+  int rc = spx_scratch(ctx, spx_inverse_ws(nb, nw), &ws);
+  if (rc) return rc;
+  int *flag = (int *)((char *)ws + spx_inverse_ws(nb, nw) - 256);
+  SPX_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+  rc = spx_inverse(ctx, ctx->stream, nb, nw, X, ldx, ws, flag);
+  if (rc) return rc;
+  int h = 0;
+  SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h) return spx_seterr(ctx, SPEXHIP_E_NOTPD, "inverse: matrix not positive definite");
+  return SPEXHIP_OK;
+}
+
+int spexhip_dielectric_inverse(spexhip_ctx *ctx, int nb, int nw, const double *chi_packed,
+                               int vkind, const double *vhalf, double *einv_packed) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && chi_packed && vhalf && einv_packed, "dielectric_inverse: bad arguments");
+  SPX_CHECK(ctx, vkind == 0 || vkind == 1, "dielectric_inverse: vkind must be 0 or 1");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t xb = 16 * (size_t)nb * nb * nw;
+  const size_t vb = vkind ? 16 * (size_t)nb * nb : 8 * (size_t)nb;
+  void *X = nullptr, *E = nullptr, *V = nullptr;
+  int rc = spx_alloc(ctx, xb, &X);
+  if (!rc) rc = spx_alloc(ctx, xb, &E);
+  if (!rc) rc = spx_alloc(ctx, vb, &V);
+  if (!rc) rc = spexhip_unpack_h2d(ctx, nb, nw, chi_packed, X, nb);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(V, vhalf, vb, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "upload: %s", hipGetErrorString(e));
+  }
+  if (!rc) rc = spexhip_dielectric_dev(ctx, nb, nw, X, nb, vkind, V, E, nb);
+  if (!rc) rc = spexhip_inverse_hpd_dev(ctx, nb, nw, E, nb);
+  if (!rc) rc = spexhip_pack_upper_d2h(ctx, nb, nw, E, nb, einv_packed, 0);
+  (void)hipStreamSynchronize(ctx->stream);
+  spx_free(ctx, X);
+  spx_free(ctx, E);
+  spx_free(ctx, V);
+  return rc;
+}
+
+int spexhip_inverse_hpd_packed(spexhip_ctx *ctx, int nb, int count, double *packed) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && count > 0 && packed, "inverse_hpd_packed: bad arguments");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *X = nullptr;
+  int rc = spx_alloc(ctx, 16 * (size_t)nb * nb * count, &X);
+  if (!rc) rc = spexhip_unpack_h2d(ctx, nb, count, packed, X, nb);
+  if (!rc) rc = spexhip_inverse_hpd_dev(ctx, nb, count, X, nb);
+  if (!rc) rc = spexhip_pack_upper_d2h(ctx, nb, count, X, nb, packed, 0);
+  (void)hipStreamSynchronize(ctx->stream);
+  spx_free(ctx, X);
+  return rc;
+}
+
+}  // extern "C"

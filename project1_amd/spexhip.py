@@ -1,0 +1,309 @@
+"""ctypes binding of the spexhip C ABI (``include/spexhip.h``).
+
+This is the Python host mirror of the reference operator surface for the
+polarisation-function path:
+
+* :meth:`Context.susceptibility_none` -- the HILBERT NONE pair sum of one
+  memory packet (``susceptibility.f:1159-1251``), accumulating into the
+  packed ``susceph`` array exactly like the reference;
+* :meth:`Context.wavefproducts2_mt` / :meth:`Context.wavefproducts2_pw` --
+  ``wavefproducts.f:95`` / ``:465``;
+* :meth:`Context.dielectric_inverse` -- ``correlation.f:612-747``.
+
+A failing status raises :class:`SpexError` carrying the library message
+(the reference stops with ``Error(...)``, ``util.f:480-505``).  The library
+must be built (``python -m project1_amd.build``); there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().parent / "libspexhip.so"
+
+OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOTPD, E_AGAIN = range(6)
+PREC_FP64, PREC_SPLIT_FP32 = 0, 1
+
+_c_long = C.c_long
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+
+class SpexError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"spexhip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load ``libspexhip.so`` (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(
+                f"{_LIB_PATH} not built; run `python -m project1_amd.build`"
+            )
+        _lib = C.CDLL(str(_LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L: C.CDLL) -> None:
+    I = C.c_int
+    P = C.POINTER
+    sig = {
+        "spexhip_create": (I, [I, C.c_size_t, P(_vp)]),
+        "spexhip_destroy": (I, [_vp]),
+        "spexhip_last_error": (I, [_vp, C.c_char_p, C.c_size_t]),
+        "spexhip_sync": (I, [_vp]),
+        "spexhip_mem_used": (I, [_vp, P(C.c_size_t), P(C.c_size_t)]),
+        "spexhip_dev_alloc": (I, [_vp, C.c_size_t, P(_vp)]),
+        "spexhip_dev_free": (I, [_vp, _vp]),
+        "spexhip_h2d": (I, [_vp, _vp, _vp, C.c_size_t]),
+        "spexhip_d2h": (I, [_vp, _vp, _vp, C.c_size_t]),
+        "spexhip_memset": (I, [_vp, _vp, I, C.c_size_t]),
+        "spexhip_weights_none_dev": (I, [_vp, I, I, I, _vp, _vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_chi0_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
+                                 C.c_double, C.c_double, _vp, I]),
+        "spexhip_pack_upper_d2h": (I, [_vp, I, I, _vp, I, _vp, I]),
+        "spexhip_unpack_h2d": (I, [_vp, I, I, _vp, _vp, I]),
+        "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
+        "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
+        "spexhip_event_create": (I, [_vp, P(_vp)]),
+        "spexhip_event_record": (I, [_vp, _vp]),
+        "spexhip_event_elapsed": (I, [_vp, _vp, _vp, P(C.c_float)]),
+        "spexhip_event_destroy": (I, [_vp, _vp]),
+        "spexhip_dielectric_dev": (I, [_vp, I, I, _vp, I, I, _vp, _vp, I]),
+        "spexhip_inverse_hpd_dev": (I, [_vp, I, I, _vp, I]),
+        "spexhip_dielectric_inverse": (I, [_vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_inverse_hpd_packed": (I, [_vp, I, I, _vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _c128(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class DevBuf:
+    """A device allocation owned by a :class:`Context`."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        ctx._check(lib().spexhip_dev_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> "DevBuf":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes, (a.nbytes, self.nbytes)
+        self.ctx._check(lib().spexhip_h2d(self.ctx.h, self.ptr, _ptr(a), a.nbytes))
+        return self
+
+    def download(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        self.ctx._check(lib().spexhip_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def zero(self) -> "DevBuf":
+        self.ctx._check(lib().spexhip_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx._check(lib().spexhip_dev_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+
+class Event:
+    """A HIP event recorded on the context stream."""
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        p = _vp()
+        ctx._check(lib().spexhip_event_create(ctx.h, C.byref(p)))
+        self.h = p.value
+
+    def record(self) -> "Event":
+        self.ctx._check(lib().spexhip_event_record(self.ctx.h, self.h))
+        return self
+
+    def ms_since(self, start: "Event") -> float:
+        v = C.c_float()
+        self.ctx._check(lib().spexhip_event_elapsed(self.ctx.h, start.h, self.h, C.byref(v)))
+        return v.value
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().spexhip_event_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class Context:
+    """One device context (one HIP stream, one HBM budget)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = _vp()
+        rc = lib().spexhip_create(device, hbm_budget, C.byref(h))
+        if rc:
+            raise SpexError(rc, "spexhip_create failed (no usable GPU?)")
+        self.h = h.value
+
+    # -- plumbing ---------------------------------------------------------
+    def last_error(self) -> str:
+        buf = C.create_string_buffer(1024)
+        lib().spexhip_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def _check(self, rc: int) -> None:
+        if rc:
+            raise SpexError(rc, self.last_error())
+
+    def sync(self) -> None:
+        self._check(lib().spexhip_sync(self.h))
+
+    def mem_used(self) -> tuple[int, int]:
+        u, b = C.c_size_t(), C.c_size_t()
+        self._check(lib().spexhip_mem_used(self.h, C.byref(u), C.byref(b)))
+        return u.value, b.value
+
+    def alloc(self, nbytes: int) -> DevBuf:
+        return DevBuf(self, nbytes)
+
+    def to_dev(self, a: np.ndarray) -> DevBuf:
+        a = np.ascontiguousarray(a)
+        return DevBuf(self, a.nbytes).upload(a)
+
+    def close(self) -> None:
+        if self.h:
+            lib().spexhip_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- synthetic data and timing ------------------------------------------
+    def synth_normal(self, buf: DevBuf, count: int, seed: int, scale: float = 1.0) -> None:
+        self._check(lib().spexhip_synth_normal_dev(self.h, buf.ptr, count, seed, scale))
+
+    def event(self) -> "Event":
+        return Event(self)
+
+    # -- weights ----------------------------------------------------------
+    def weights_none_dev(self, e_occ: DevBuf, e_unocc: DevBuf, nk: int, nocc: int,
+                         nunocc: int, nkpt: int, nsym1: int, nkpts: DevBuf,
+                         omega: DevBuf, nw: int, out: DevBuf) -> None:
+        self._check(lib().spexhip_weights_none_dev(
+            self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, nkpt, nsym1,
+            nkpts.ptr, nw, omega.ptr, out.ptr))
+
+    # -- chi0 -------------------------------------------------------------
+    def chi0_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool,
+                 nw: int, wght: DevBuf, precision: int, chi: DevBuf, ldchi: int,
+                 alpha: float = 1.0, beta: float = 0.0) -> None:
+        self._check(lib().spexhip_chi0_dev(
+            self.h, n, nb, M.ptr, ldm, int(colmajor), nw, wght.ptr, precision,
+            alpha, beta, chi.ptr, ldchi))
+
+    def pack_upper(self, X: DevBuf, nb: int, nw: int, ldx: int,
+                   out: np.ndarray | None = None) -> np.ndarray:
+        npack = nb * (nb + 1) // 2
+        acc = out is not None
+        if out is None:
+            out = np.zeros((nw, npack), dtype=np.complex128)
+        assert out.dtype == np.complex128 and out.flags.c_contiguous and out.size == nw * npack
+        self._check(lib().spexhip_pack_upper_d2h(self.h, nb, nw, X.ptr, ldx, _ptr(out), int(acc)))
+        return out
+
+    def unpack(self, packed: np.ndarray, nb: int, nw: int, X: DevBuf, ldx: int) -> None:
+        packed = _c128(packed)
+        self._check(lib().spexhip_unpack_h2d(self.h, nb, nw, _ptr(packed), X.ptr, ldx))
+
+    def susceptibility_none(self, cprod: np.ndarray, wght: np.ndarray,
+                            susceph: np.ndarray, precision: int = PREC_FP64) -> np.ndarray:
+        """Accumulate one packet's pair sum into ``susceph`` (reference layout).
+
+        ``cprod`` is (n, nb) complex128 in Fortran order (or any array whose
+        column-major view is (n, nb)); ``wght`` is (n, nw) Fortran-ordered
+        real; ``susceph`` is (npack, nw) Fortran-ordered complex128, i.e. a
+        C-contiguous (nw, npack) array.  Mirrors ``susceptibility.f:1159-1251``.
+        """
+        cp = np.asfortranarray(cprod, dtype=np.complex128)
+        n, nb = cp.shape
+        w = np.asfortranarray(wght, dtype=np.float64)
+        assert w.shape[0] == n
+        nw = w.shape[1]
+        assert susceph.dtype == np.complex128 and susceph.size == nw * nb * (nb + 1) // 2
+        assert susceph.flags.f_contiguous or susceph.flags.c_contiguous
+        self._check(lib().spexhip_susceptibility_none(
+            self.h, n, nb, cp.ctypes.data, n, nw, w.ctypes.data, precision,
+            susceph.ctypes.data))
+        return susceph
+
+    # -- dielectric step ---------------------------------------------------
+    def dielectric_dev(self, X: DevBuf, nb: int, nw: int, ldx: int, vkind: int,
+                       vhalf: DevBuf, E: DevBuf, lde: int) -> None:
+        self._check(lib().spexhip_dielectric_dev(self.h, nb, nw, X.ptr, ldx, vkind,
+                                                 vhalf.ptr, E.ptr, lde))
+
+    def inverse_hpd_dev(self, X: DevBuf, nb: int, nw: int, ldx: int) -> None:
+        self._check(lib().spexhip_inverse_hpd_dev(self.h, nb, nw, X.ptr, ldx))
+
+    def dielectric_inverse(self, chi_packed: np.ndarray, vhalf: np.ndarray) -> np.ndarray:
+        """correlation.f:612-747 for packed chi (nw, npack): returns eps^-1 packed.
+
+        ``vhalf`` is either the vector sqrt(coul) (diagonal v^1/2) or a dense
+        (nb, nb) matrix.
+        """
+        chi = _c128(chi_packed)
+        nw = chi.shape[0] if chi.ndim == 2 else 1
+        vh = np.asarray(vhalf)
+        if vh.ndim == 1:
+            nb, vkind, vbuf = vh.size, 0, _f64(vh)
+        else:
+            nb, vkind, vbuf = vh.shape[0], 1, np.asfortranarray(vh, dtype=np.complex128)
+        assert chi.size == nw * nb * (nb + 1) // 2
+        out = np.zeros((nw, nb * (nb + 1) // 2), np.complex128)
+        self._check(lib().spexhip_dielectric_inverse(self.h, nb, nw, _ptr(chi), vkind,
+                                                     vbuf.ctypes.data, _ptr(out)))
+        return out
+
+    def inverse_hpd_packed(self, packed: np.ndarray) -> np.ndarray:
+        """Minverse on packed HPD matrices (count, npack), in place on a copy."""
+        p = np.array(packed, dtype=np.complex128, order="C", copy=True)
+        if p.ndim == 1:
+            p = p[None, :]
+        count, npack = p.shape
+        nb = int(round((np.sqrt(8 * npack + 1) - 1) / 2))
+        self._check(lib().spexhip_inverse_hpd_packed(self.h, nb, count, _ptr(p)))
+        return p
+
+
+def exported_symbols() -> list[str]:
+    """Function names declared in include/spexhip.h."""
+    import re
+
+    hdr = (_LIB_PATH.parent.parent / "include" / "spexhip.h").read_text()
+    return sorted(set(re.findall(r"\bint\s+(spexhip_\w+)\s*\(", hdr)))
