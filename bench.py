@@ -1,20 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: chi0 + eps^-1 q-points/s on one MI355X, split-FP32 vs FP64.
+"""Benchmark: chi0 + eps^-1 q-points/s on one MI355X, split forms vs FP64.
 
 Workload (one "step" = one q-point of the single-q realistic case):
-  Nk=64 k-points, Nocc=16, Nunocc=400 (n = 409,600 transitions),
-  mixed basis Nb=1200, Nw=12 imaginary frequencies (Gauss-Legendre on
-  [0,1] mapped to omega = x/(1-x)), dense random SPD Coulomb matrix with
-  eigenvalues log-uniform in [1e-3, 1e3] (v^1/2 exact).
-  Timed per step, inputs resident in HBM:
-    HILBERT NONE weights -> chi0 (rank-k MFMA, spin factor 2)
+  Nk=64 k-points (4x4x4), Nocc=16, Nunocc=400 (n = 409,600 transitions),
+  mixed basis Nb=1200 = 294 muffin-tin (2 atoms, lmax 6, 3 radial
+  functions) + 906 interstitial plane waves, Nw=12 imaginary frequencies
+  (Gauss-Legendre on [0,1] mapped to omega = x/(1-x)), 36^3 FFT grid,
+  dense random SPD Coulomb matrix (eigenvalues log-uniform in [1e-3, 1e3]).
+  Timed per step, inputs (wave functions, energies, v^1/2) resident in HBM:
+    products M (MT Gaunt contraction + IPW FFT convolution, FP64)
+    -> HILBERT NONE weights -> chi0 (rank-k, spin factor 2)
     -> eps = 1 - v^1/2 chi0 v^1/2 (FP64 MFMA GEMMs) -> eps^-1 (FP64).
-  The product matrix M (n x Nb complex128) is synthetic (Gaussian,
-  Re/Im ~ N(0,1)/sqrt(Nb)) and generated on the device before timing.
 
-Prints ONE JSON line.  ``value`` is q-points/s of the split-FP32 form; the
-FP64 form, the error of split vs FP64, the chi0 roofline and the CPU
-baseline (the oracle's literal pair loop on a bounded sample) ride along.
+Three chi0 forms run on the same inputs: plain FP64, split-FP32 (hi+lo,
+3 FP32 MFMA products) and the Ozaki int8 form.  Prints ONE JSON line;
+``value`` is q-points/s of the fastest split form whose chi0 and eps^-1
+stay within 1e-8 (max-abs and Frobenius relative) of the FP64 form.  The
+roofline of its chi0 kernel and the CPU baseline (the oracle's literal
+pair loop on a bounded sample) ride along.
 """
 from __future__ import annotations
 
@@ -48,6 +51,8 @@ def parse():
     p.add_argument("--nw", type=int, default=12)
     p.add_argument("--seed", type=int, default=2026)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-products", action="store_true",
+                   help="use Gaussian products generated once instead of computing M each step")
     p.add_argument("--cpu-cols", type=int, default=128,
                    help="CPU baseline sample: all pairs among this many columns")
     return p.parse_args()
@@ -76,7 +81,22 @@ class Pipeline:
         _, vh, lam = synth.coulomb(g, nb)
         self.vh = vh
         self.dM = ctx.alloc(16 * n * nb)
-        ctx.synth_normal(self.dM, 2 * n * nb, a.seed + 1, 1.0 / np.sqrt(nb))
+        self.products = not a.no_products
+        if self.products:
+            # synthetic wave functions (cubic nk mesh, |k+G| <= 7.1 -> Ng ~ 1500, 906 IPWs on
+            # a 36^3 grid; 2 atoms, lmax 6, 3 radial functions) -> M on the device each step
+            nmesh = round(nk ** (1 / 3))
+            assert nmesh ** 3 == nk, "--nk must be a cube for the product workload"
+            self.sys = sy = synth.product_system(
+                nmesh=nmesh, nband=no + nu, gcut=7.1, ngptm=nb - 294, nfft=(36, 36, 36), ntype=2,
+                lcut=6, nrad=3, lcutm=6, nradm=3, seed=a.seed + 2)
+            assert sy.nbasp + sy.ngptm == nb
+            self.mtp = ctx.mt_plan(sy.types, sy.cent, sy.cmt.shape[0])
+            self.pwp = ctx.pw_plan(sy.nfft, sy.gpt, sy.ngpt, sy.gptm, sy.vol)
+            self.dcmt = ctx.to_dev(np.asfortranarray(sy.cmt).ravel(order="F"))
+            self.dcpw = ctx.to_dev(np.asfortranarray(sy.cpw).ravel(order="F"))
+        else:
+            ctx.synth_normal(self.dM, 2 * n * nb, a.seed + 1, 1.0 / np.sqrt(nb))
         self.deo, self.deu = ctx.to_dev(eo), ctx.to_dev(eu)
         self.dnk = ctx.to_dev(np.ones(nk, np.int32))
         self.dom = ctx.to_dev(self.omega)
@@ -84,11 +104,21 @@ class Pipeline:
         self.dV = ctx.to_dev(np.asfortranarray(vh.astype(np.complex128)))
         self.dX = ctx.alloc(16 * nb * nb * nw)
         self.dE = ctx.alloc(16 * nb * nb * nw)
-        self.ev = [ctx.event() for _ in range(4)]
+        self.ev = [ctx.event() for _ in range(5)]
         ctx.sync()
+
+    def make_products(self):
+        ctx, a, sy = self.ctx, self.a, self.sys
+        ctx.wfprod_mt_dev(self.mtp, self.dcmt, sy.cmt.shape[1], sy.nband, sy.nkpt, sy.q, sy.k1,
+                          sy.k2, 0, a.nocc, a.nocc, a.nunocc, self.dM, a.nb, 0)
+        ctx.wfprod_pw_dev(self.pwp, self.dcpw, sy.nband, sy.k1, sy.k2, sy.g2, 0, a.nocc, a.nocc,
+                          a.nunocc, self.dM, a.nb, sy.nbasp)
 
     def step(self, prec):
         ctx, a, ev = self.ctx, self.a, self.ev
+        ev[4].record()
+        if self.products:
+            self.make_products()
         ev[0].record()
         ctx.weights_none_dev(self.deo, self.deu, a.nk, a.nocc, a.nunocc, a.nk, 1, self.dnk,
                              self.dom, a.nw, self.dW)
@@ -99,7 +129,8 @@ class Pipeline:
         ctx.dielectric_dev(self.dX, a.nb, a.nw, a.nb, 1, self.dV, self.dE, a.nb)
         ctx.inverse_hpd_dev(self.dE, a.nb, a.nw, a.nb)
         ev[3].record()
-        return ev[2].ms_since(ev[1]), ev[3].ms_since(ev[2]), ev[3].ms_since(ev[0])
+        return (ev[2].ms_since(ev[1]), ev[3].ms_since(ev[2]), ev[3].ms_since(ev[4]),
+                ev[0].ms_since(ev[4]))
 
     def results(self):
         a = self.a
@@ -113,16 +144,18 @@ def run_form(pipe, prec, steps, warmup):
         pipe.step(prec)
     pipe.ctx.sync()
     t0 = time.perf_counter()
-    chi_ms, eps_ms, tot_ms = [], [], []
+    chi_ms, eps_ms, tot_ms, prod_ms = [], [], [], []
     for _ in range(steps):
-        c, e, t = pipe.step(prec)
+        c, e, t, p = pipe.step(prec)
         chi_ms.append(c)
         eps_ms.append(e)
         tot_ms.append(t)
+        prod_ms.append(p)
     pipe.ctx.sync()
     wall = time.perf_counter() - t0
     return dict(wall_s=wall, ms_per_step=1e3 * wall / steps, chi0_ms=float(np.mean(chi_ms)),
-                eps_ms=float(np.mean(eps_ms)), step_ms_p50=float(np.percentile(tot_ms, 50)),
+                eps_ms=float(np.mean(eps_ms)), products_ms=float(np.mean(prod_ms)),
+                step_ms_p50=float(np.percentile(tot_ms, 50)),
                 step_ms_p99=float(np.percentile(tot_ms, 99)))
 
 
@@ -164,6 +197,10 @@ def cpu_baseline(pipe, a):
                 seconds_per_q_extrapolated=t_q)
 
 
+PEAK_I8 = 5000.0  # TOPS dense int8 matrix, MI355X
+FORMS = (("fp64", "PREC_FP64"), ("split_fp32", "PREC_SPLIT_FP32"), ("ozaki_int8", "PREC_OZAKI_INT8"))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -173,19 +210,44 @@ def main():
     from project1_amd import spexhip
 
     ctx = spexhip.Context(0)
+    ctx.set_workspace_limit(64 << 30)  # one frequency group for the Ozaki form at this size
     pipe = Pipeline(ctx, a)
-    res = {}
-    out = {}
-    for name, prec in (("fp64", spexhip.PREC_FP64), ("split_fp32", spexhip.PREC_SPLIT_FP32)):
+    res, out = {}, {}
+    flops = chi0_flops(a.nw, pipe.n, a.nb)
+    for name, pname in FORMS:
+        prec = getattr(spexhip, pname)
         r = run_form(pipe, prec, a.steps, a.warmup)
         out[name] = pipe.results()
         r["qpts_per_s"] = 1e3 / r["ms_per_step"]
-        r["gflops"] = (chi0_flops(a.nw, pipe.n, a.nb) + eps_flops(a.nw, a.nb)) / (r["ms_per_step"] * 1e6)
-        r["chi0_tflops"] = chi0_flops(a.nw, pipe.n, a.nb) / (r["chi0_ms"] * 1e9)
+        r["gflops"] = (flops + eps_flops(a.nw, a.nb)) / (r["ms_per_step"] * 1e6)
+        r["chi0_tflops"] = flops / (r["chi0_ms"] * 1e9)
+        if name == "ozaki_int8":
+            prep, rank, groups = ctx.chi0_phase_ms()
+            r.update(prep_ms=prep, rank_ms=rank, groups=groups)
         res[name] = r
-    e_chi = errs(out["split_fp32"][0], out["fp64"][0])
-    e_inv = errs(out["split_fp32"][1], out["fp64"][1])
-    head = res["split_fp32"]
+    errs_ = {k: {"chi0": errs(out[k][0], out["fp64"][0]), "eps_inv": errs(out[k][1], out["fp64"][1])}
+             for k in ("split_fp32", "ozaki_int8")}
+
+    def ok(k):
+        e = errs_[k]
+        return all(e[q][m] <= 1e-8 for q in ("chi0", "eps_inv") for m in ("max_rel", "fro_rel"))
+
+    head_name = "ozaki_int8" if ok("ozaki_int8") else "split_fp32"
+    head = res[head_name]
+    if head_name == "ozaki_int8":
+        i8 = 10.0 * flops / (head["rank_ms"] * 1e9)  # 40 int8 MACs per complex MAC
+        roof = {"bound": "mfma", "kernel": "ozk_herk_kernel", "achieved": round(i8, 1),
+                "peak": PEAK_I8, "unit": "TFLOP/s (int8 ops)", "frac": round(i8 / PEAK_I8, 4),
+                "traffic": None,
+                "note": "achieved = 80 int8 ops per complex multiply-add of chi0 "
+                        "(10 slice pairs x 4 real products) / rank-k kernel time"}
+    else:
+        roof = {"bound": "mfma", "kernel": "chi0_split_kernel",
+                "achieved": round(head["chi0_tflops"], 2), "peak": PEAK_F32, "unit": "TFLOP/s",
+                "frac": round(head["chi0_tflops"] / PEAK_F32, 4), "traffic": None}
+    roof["fp64_kernel"] = {"kernel": "chi0_f64_kernel", "achieved": round(res["fp64"]["chi0_tflops"], 2),
+                           "peak": PEAK_F64, "unit": "TFLOP/s",
+                           "frac": round(res["fp64"]["chi0_tflops"] / PEAK_F64, 4)}
     line = {
         "metric": METRIC,
         "value": round(head["qpts_per_s"], 4),
@@ -197,25 +259,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32x3-split/fp64-acc (eps in f64)",
-        "data": "synthetic (seeded Gaussian products on device, random SPD Coulomb matrix)",
-        "config": {"workload": "single q, Nk=64 Nocc=16 Nunocc=400 Nb=1200 Nw=12, dense v^1/2",
+        "dtype": ("int8x4-slice/int32-exact/fp64-recombine (eps in f64)" if head_name == "ozaki_int8"
+                  else "fp32x3-split/fp64-acc (eps in f64)"),
+        "data": ("synthetic (seeded wave functions -> products on device each step, random SPD "
+                 "Coulomb matrix)" if pipe.products else
+                 "synthetic (seeded Gaussian products on device, random SPD Coulomb matrix)"),
+        "config": {"workload": ("single q, Nk=64 Nocc=16 Nunocc=400 Nb=1200 (294 MT + 906 IPW) "
+                                "Nw=12, 36^3 FFT grid, dense v^1/2; products->chi0->eps^-1"),
                    "nk": a.nk, "nocc": a.nocc, "nunocc": a.nunocc, "nb": a.nb, "nw": a.nw,
-                   "parallelism": "1 GPU"},
+                   "parallelism": "1 GPU", "headline_form": head_name},
         "gflops": round(head["gflops"], 1),
         "latency_ms": {"p50": round(head["step_ms_p50"], 3), "p99": round(head["step_ms_p99"], 3)},
-        "fp64": {k: round(v, 4) for k, v in res["fp64"].items()},
-        "split_fp32": {k: round(v, 4) for k, v in res["split_fp32"].items()},
-        "speedup_split_vs_fp64": round(res["fp64"]["ms_per_step"] / head["ms_per_step"], 3),
-        "err_split_vs_fp64": {"chi0": e_chi, "eps_inv": e_inv},
-        "roofline": {"bound": "mfma", "kernel": "chi0_split_kernel",
-                     "achieved": round(head["chi0_tflops"], 2), "peak": PEAK_F32,
-                     "unit": "TFLOP/s", "frac": round(head["chi0_tflops"] / PEAK_F32, 4),
-                     "traffic": None,
-                     "fp64_kernel": {"kernel": "chi0_f64_kernel",
-                                     "achieved": round(res["fp64"]["chi0_tflops"], 2),
-                                     "peak": PEAK_F64,
-                                     "frac": round(res["fp64"]["chi0_tflops"] / PEAK_F64, 4)}},
+        "forms": {k: {kk: round(v, 4) for kk, v in r.items()} for k, r in res.items()},
+        "speedup_vs_fp64": {k: round(res["fp64"]["ms_per_step"] / res[k]["ms_per_step"], 3)
+                            for k in ("split_fp32", "ozaki_int8")},
+        "err_vs_fp64": errs_,
+        "roofline": roof,
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pipe, a)

@@ -41,9 +41,16 @@ enum {
 
 enum {
   SPEXHIP_PREC_FP64 = 0,       /* plain FP64 MFMA                          */
-  SPEXHIP_PREC_SPLIT_FP32 = 1  /* hi+lo FP32 operands, 3 FP32 MFMA products,
+  SPEXHIP_PREC_SPLIT_FP32 = 1, /* hi+lo FP32 operands, 3 FP32 MFMA products,
                                   FP32 accumulation in short K-chunks flushed
                                   into FP64 accumulators                     */
+  SPEXHIP_PREC_OZAKI_INT8 = 2  /* weights balanced as sqrt|D| on both sides,
+                                  4 per-column-scaled int8 slices per
+                                  component, slice products with p+q <= 5 on
+                                  int8 MFMA with exact int32 accumulation,
+                                  FP64 recombination; needs weights of one
+                                  sign per frequency (HILBERT NONE on the
+                                  imaginary axis), else SPEXHIP_E_INVALID    */
 };
 
 typedef struct spexhip_ctx spexhip_ctx;
@@ -57,6 +64,14 @@ int spexhip_last_error(spexhip_ctx *ctx, char *buf, size_t len);
 int spexhip_sync(spexhip_ctx *ctx);
 /* bytes currently allocated through this context */
 int spexhip_mem_used(spexhip_ctx *ctx, size_t *used, size_t *budget);
+
+/* Largest slice workspace the Ozaki chi0 form may use (default 24 GiB); the
+ * frequencies are processed in groups that fit.                            */
+int spexhip_set_workspace_limit(spexhip_ctx *ctx, size_t bytes);
+/* Event-timed phases of the last Ozaki chi0 call: prep = weights, scales
+ * and slices of the first frequency group; rank = rank-k kernels (and the
+ * slicing of any later group).  groups = number of frequency groups.      */
+int spexhip_chi0_phase_ms(spexhip_ctx *ctx, float *prep_ms, float *rank_ms, int *groups);
 
 /* ---- device memory (library-owned; counted against the budget) -------- */
 int spexhip_dev_alloc(spexhip_ctx *ctx, size_t bytes, void **dev);
@@ -122,6 +137,75 @@ int spexhip_susceptibility_none(spexhip_ctx *ctx, long n, int nb,
                                 const double *cprod, long ldc, int nw,
                                 const double *wght, int precision,
                                 double *susceph);
+
+/* ---- wave-function products -------------------------------------------- */
+/* The reference reads the basis and wave functions from module global
+ * (cmt, cpw, gpt, gptm, nfft, imati, integral, gaunt -- wavefproducts.f:
+ * 101-222, 476-560; fft.f:17-28).  Here that setup is a plan built once. */
+typedef struct spexhip_plan spexhip_plan;
+
+/* One muffin-tin atom type (wavefproducts.f:177-222).  integral(d1,d2,d3)
+ * is the reference's radial-integral array integral(ln,ln1,lln) for the
+ * type, column-major.  Centres of a type are consecutive.                  */
+typedef struct {
+  int lcut;           /* lcutp(itype)                 */
+  const int *nindx;   /* nindx(0:lcut, itype)         */
+  int lcutm;          /* lcutm(itype)                 */
+  const int *nindxm;  /* nindxm(0:lcutm, itype)       */
+  int neq;            /* neq(itype)                   */
+  const double *integral;
+  int d1, d2, d3;
+} spexhip_mt_type;
+
+/* Interstitial plane-wave setup (fft.f:36-96, wavefproducts.f:576-690).   */
+typedef struct {
+  int nfft[3];        /* FFT grid (16..64 per axis)                        */
+  int imati[9];       /* imati(i,j) stored row-major [3*i+j]               */
+  double vol;         /* unit-cell volume (sqrtvoli = 1/sqrt(vol))         */
+  int nkpt, maxgpt;
+  const int *gpt;     /* [nkpt][maxgpt][3] G vectors of each k's basis     */
+  const int *ngpt;    /* [nkpt]                                            */
+  int ngptm;          /* mixed-basis IPWs of this q                        */
+  const int *gptm;    /* [ngptm][3]                                        */
+  const double *step_r; /* optional theta(r) on the grid (x fastest), applied
+                           to phi1 like fullpw (:631-633); NULL for none    */
+} spexhip_pw_desc;
+
+int spexhip_mt_plan_create(spexhip_ctx *ctx, int ntype, const spexhip_mt_type *types,
+                           const double *cent /* [ncent][3] fractional */, int maxlmindx,
+                           spexhip_plan **plan);
+int spexhip_pw_plan_create(spexhip_ctx *ctx, const spexhip_pw_desc *desc, spexhip_plan **plan);
+int spexhip_plan_destroy(spexhip_ctx *ctx, spexhip_plan *plan);
+/* number of product columns the plan writes (nbasp or ngptm) */
+int spexhip_plan_ncols(spexhip_ctx *ctx, const spexhip_plan *plan, int *ncols);
+
+/* wavefproducts2_mt (wavefproducts.f:95-431) on the device.  cmt_dev is the
+ * reference array cmt(maxlmindx,ncent,nband,nkpt) (complex, column-major);
+ * q = kpt(:,ik0); k1/k2 (host, nk) are the packet's k and k+q indices,
+ * 0-based; bands b1lo..b1lo+nocc-1 and b2lo..b2lo+nunocc-1 (0-based).
+ * Writes columns col0..col0+nbasp-1 of the transition-major M_dev.        */
+int spexhip_wfprod2_mt_dev(spexhip_ctx *ctx, const spexhip_plan *plan, const void *cmt_dev,
+                           int ncent, int nband, int nkpt, const double *q, int nk,
+                           const int *k1, const int *k2, int b1lo, int nocc, int b2lo,
+                           int nunocc, void *M_dev, long ldm, int col0);
+/* wavefproducts2_pw FFT path (wavefproducts.f:576-758) on the device.
+ * cpw_dev is cpw(maxgpt,nband,nkpt); g2 (host, nk x 3) is the umklapp
+ * gpt2(:,ik) = nint(kpt(k2) - q - kpt(k1)) (:547-552).  Writes columns
+ * col0..col0+ngptm-1 of M_dev.                                              */
+int spexhip_wfprod2_pw_dev(spexhip_ctx *ctx, const spexhip_plan *plan, const void *cpw_dev,
+                           int nband, int nk, const int *k1, const int *k2, const int *g2,
+                           int b1lo, int nocc, int b2lo, int nunocc, void *M_dev, long ldm,
+                           int col0);
+/* Host drop-ins with the reference output layout cprod(t, I) (column-major,
+ * leading dimension n = nk*nocc*nunocc), overwritten like the reference
+ * (cprod = 0 then +=, :180-186 / :561-568).                                */
+int spexhip_wavefproducts2_mt(spexhip_ctx *ctx, const spexhip_plan *plan, const double *cmt,
+                              int ncent, int nband, int nkpt, const double *q, int nk,
+                              const int *k1, const int *k2, int b1lo, int nocc, int b2lo,
+                              int nunocc, double *cprod);
+int spexhip_wavefproducts2_pw(spexhip_ctx *ctx, const spexhip_plan *plan, const double *cpw,
+                              int nband, int nk, const int *k1, const int *k2, const int *g2,
+                              int b1lo, int nocc, int b2lo, int nunocc, double *cprod);
 
 /* ---- dielectric step and inverse (FP64) -------------------------------- */
 /* eps = 1 - v^1/2 chi v^1/2 on nw full device matrices, correlation.f:

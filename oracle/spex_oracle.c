@@ -387,11 +387,13 @@ void or_wfprod_mt(int ntype, const or_mt_type *types, const double *cent,
 
 /* ------------------------------------------------------------------ */
 /* wavefproducts.f:766-997 (explicit convolution, SUBTYPE 2, noinv).     */
-/*  non-fullpw: cprod(b2,b1,k,ig) += dotprod(cpw3,cpw2)*sqrtvoli with      */
-/*    cpw3 = c1(G2-Gm+g2) for G2-Gm+g2 in the G list of k1 (:953-972);     */
-/*  fullpw: cpw0(G0) = sum_G1 cstep(G0-G1) c1(G1) (:870-888) and           */
-/*    cprod += sqrtvoli*conjg(macmat(cpw2,cpw3)),                          */
-/*    cpw3(ig2,ig) = cpw0(G2-Gm+g2) (:917-945).                            */
+/* point(g) over the box of g = G2 - Gm + g2 (:795-841):                  */
+/*  non-fullpw: point(g) = index of g in the G list of k1 (pgptinv), and  */
+/*    cprod(b2,b1,k,ig) += dotprod(cpw3,cpw2)*sqrtvoli, cpw3 = c1(point)  */
+/*    (:953-972);                                                          */
+/*  fullpw: cpw0(G0) = sum_G1 cstep(G0-G1) c1(G1) (:870-888) and          */
+/*    cprod += sqrtvoli*conjg(macmat(cpw2,cpw3)), cpw3 = cpw0(point)       */
+/*    (:917-945).                                                          */
 void or_wfprod_pw(int nk, const int *k1, const int *k2, const int *g2,
                   const int *gpt, const int *ngpt, int maxgpt,
                   const double *cpw, int nband, int ngptm, const int *gptm,
@@ -408,7 +410,46 @@ void or_wfprod_pw(int nk, const int *k1, const int *k2, const int *g2,
     int kk1 = k1[ik], kk2 = k2[ik];
     const int *G1 = gpt + 3L * maxgpt * kk1, *G2 = gpt + 3L * maxgpt * kk2;
     int n1 = ngpt[kk1], n2 = ngpt[kk2];
+    /* box extents of g = G2 - Gm + g2 (def == 0 pass, :803-808) */
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) { lo[d] = 1 << 30; hi[d] = -(1 << 30); }
+    for (int ig2 = 0; ig2 < n2; ig2++)
+      for (int ig = 0; ig < ngptm; ig++)
+        for (int d = 0; d < 3; d++) {
+          int g = G2[3 * ig2 + d] - gptm[3 * ig + d] + g2[3 * ik + d];
+          if (g < lo[d]) lo[d] = g;
+          if (g > hi[d]) hi[d] = g;
+        }
+    long ex[3];
+    for (int d = 0; d < 3; d++) ex[d] = hi[d] - lo[d] + 1;
+    if (n2 == 0 || ngptm == 0) continue;
+    int *point = calloc(ex[0] * ex[1] * ex[2], sizeof(int));
+#define PIDX(g0, g1, gg2) (((long)((gg2) - lo[2]) * ex[1] + ((g1) - lo[1])) * ex[0] + ((g0) - lo[0]))
+    if (!fullpw) {
+      for (int ig1 = 0; ig1 < n1; ig1++) {
+        int a = G1[3 * ig1], b = G1[3 * ig1 + 1], c = G1[3 * ig1 + 2];
+        if (a < lo[0] || a > hi[0] || b < lo[1] || b > hi[1] || c < lo[2] || c > hi[2]) continue;
+        point[PIDX(a, b, c)] = ig1 + 1;
+      }
+    }
+    /* fullpw: every g in the box is a G0 of cpw0 */
+    long nbox = ex[0] * ex[1] * ex[2];
+    zc *cpw0 = fullpw ? malloc(sizeof(zc) * nbox) : NULL;
     for (int ib = 0; ib < nocc; ib++) {
+      if (fullpw) {
+        for (long p = 0; p < nbox; p++) {
+          int g0 = (int)(p % ex[0]) + lo[0], gq1 = (int)((p / ex[0]) % ex[1]) + lo[1],
+              gq2 = (int)(p / (ex[0] * ex[1])) + lo[2];
+          zc s = 0;
+          for (int ig1 = 0; ig1 < n1; ig1++) {
+            int d0 = g0 - G1[3 * ig1], d1 = gq1 - G1[3 * ig1 + 1], d2 = gq2 - G1[3 * ig1 + 2];
+            if (abs(d0) > sb || abs(d1) > sb || abs(d2) > sb) continue;
+            s += ZAT(cstep, ((long)(d2 + sb) * sw + (d1 + sb)) * sw + (d0 + sb)) *
+                 CPW(ig1, b1lo + ib, kk1);
+          }
+          cpw0[p] = s;
+        }
+      }
       for (int ig = 0; ig < ngptm; ig++) {
         for (int b2 = 0; b2 < nunocc; b2++) {
           long t = b2 + (long)nunocc * (ib + (long)nocc * ik);
@@ -417,24 +458,13 @@ void or_wfprod_pw(int nk, const int *k1, const int *k2, const int *g2,
             int g[3];
             for (int d = 0; d < 3; d++)
               g[d] = G2[3 * ig2 + d] - gptm[3 * ig + d] + g2[3 * ik + d];
-            zc c1 = 0;
+            long p = PIDX(g[0], g[1], g[2]);
+            zc c1;
             if (!fullpw) {
-              int found = -1;
-              for (int ig1 = 0; ig1 < n1; ig1++)
-                if (G1[3 * ig1] == g[0] && G1[3 * ig1 + 1] == g[1] && G1[3 * ig1 + 2] == g[2]) {
-                  found = ig1;
-                  break;
-                }
-              if (found < 0) continue;
-              c1 = CPW(found, b1lo + ib, kk1);
+              if (!point[p]) continue;
+              c1 = CPW(point[p] - 1, b1lo + ib, kk1);
             } else {
-              for (int ig1 = 0; ig1 < n1; ig1++) {
-                int d0 = g[0] - G1[3 * ig1], d1 = g[1] - G1[3 * ig1 + 1],
-                    d2 = g[2] - G1[3 * ig1 + 2];
-                if (abs(d0) > sb || abs(d1) > sb || abs(d2) > sb) continue;
-                zc st = ZAT(cstep, ((long)(d2 + sb) * sw + (d1 + sb)) * sw + (d0 + sb));
-                c1 += st * CPW(ig1, b1lo + ib, kk1);
-              }
+              c1 = cpw0[p];
             }
             acc += conj(c1) * CPW(ig2, b2lo + b2, kk2);
           }
@@ -442,6 +472,9 @@ void or_wfprod_pw(int nk, const int *k1, const int *k2, const int *g2,
         }
       }
     }
+    free(point);
+    free(cpw0);
+#undef PIDX
   }
 #undef CPW
 }

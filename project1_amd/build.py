@@ -65,16 +65,23 @@ def build_lib(jobs: int = 8) -> Path:
     return LIB
 
 
+FDRIVER = PKG / "fortran" / "spexhip_fdriver"
+
+
 def build_fortran() -> Path | None:
-    """Compile the ISO_C_BINDING interface module with amdflang, if present."""
+    """Compile the ISO_C_BINDING interface module and the Fortran driver
+    (amdflang), linked against libspexhip.so, if the compiler is present."""
     fc = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
     src = PKG / "fortran" / "m_spexhip.f90"
+    drv = PKG / "fortran" / "fdriver.f90"
     if not (Path(fc).exists() and src.exists()):
         return None
     BUILD.mkdir(exist_ok=True)
     obj = BUILD / "m_spexhip.o"
     _run([fc, "-c", "-O2", "-fPIC", f"-J{BUILD}", str(src), "-o", str(obj)])
-    return obj
+    _run([fc, "-O2", f"-I{BUILD}", str(drv), str(obj), f"-L{PKG}", "-lspexhip",
+          f"-Wl,-rpath,$ORIGIN/..", "-o", str(FDRIVER)])
+    return FDRIVER
 
 
 def build_all() -> Path:

@@ -342,6 +342,8 @@ __global__ void weights_none_kernel(int nk, int nocc, int nunocc,
   const double d = eu[(long)k * nunocc + b2] - eo[(long)k * nocc + b1];
   const double kw = (double)nkpts[k];
   for (int f = 0; f < nw; f++) {
+    // same operation order and roundings as the reference expression (no FMA)
+#pragma clang fp contract(off)
     const double w = om[f];
     wght[t + n * f] = -2 * d / (w * w + d * d) / nkpt / nsym1 * kw;
   }
@@ -389,8 +391,12 @@ int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
   SPX_CHECK(ctx, n >= 0 && nb > 0 && nw > 0, "chi0: bad sizes n=%ld nb=%d nw=%d", n, nb, nw);
   SPX_CHECK(ctx, ldchi >= nb, "chi0: ldchi=%d < nb=%d", ldchi, nb);
   SPX_CHECK(ctx, colmajor ? ldm >= n : ldm >= nb, "chi0: ldm=%ld too small", ldm);
-  SPX_CHECK(ctx, precision == SPEXHIP_PREC_FP64 || precision == SPEXHIP_PREC_SPLIT_FP32,
+  SPX_CHECK(ctx, precision == SPEXHIP_PREC_FP64 || precision == SPEXHIP_PREC_SPLIT_FP32 ||
+                     precision == SPEXHIP_PREC_OZAKI_INT8,
             "chi0: unknown precision %d", precision);
+  if (precision == SPEXHIP_PREC_OZAKI_INT8)
+    return spx_chi0_ozaki(ctx, st, n, nb, M, ldm, colmajor, nw, wght, alpha, beta, chi, ldchi,
+                          ctx->ozaki_ws_limit);
   const int nbt = (nb + TM - 1) / TM;
   dim3 grid(nbt * (nbt + 1) / 2, nw), block(NT);
   if (n == 0) {

@@ -24,6 +24,12 @@ struct spexhip_ctx {
   // scratch for host drop-in calls (grown on demand, owned by the context)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // largest slice workspace of the Ozaki form (frequencies are grouped to fit)
+  size_t ozaki_ws_limit = (size_t)24 << 30;
+  // events bracketing the phases of the last Ozaki chi0 call (slice of the
+  // first frequency group | rank-k updates and later groups)
+  hipEvent_t phase_ev[3] = {nullptr, nullptr, nullptr};
+  int phase_groups = 0;
 };
 
 int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...);
@@ -52,6 +58,9 @@ int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p);
 bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes);
 
 // Internal launchers shared between translation units.
+int spx_chi0_ozaki(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,
+                   int colmajor, int nw, const void *wght, double alpha, double beta, void *chi,
+                   int ldchi, size_t ws_limit);
 int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
                     const void *M, long ldm, int colmajor, int nw,
                     const void *wght, int precision, double alpha, double beta,

@@ -30,6 +30,7 @@ int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p) {
   }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) {
+    (void)hipGetLastError();  // do not leave the failure for the next launch check
     std::lock_guard<std::mutex> g(ctx->mu);
     ctx->used -= bytes;
     ctx->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
@@ -103,6 +104,8 @@ int spexhip_create(int device, size_t budget, spexhip_ctx **out) {
     delete c;
     return SPEXHIP_E_DEVICE;
   }
+  for (auto &e : c->phase_ev)
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
   *out = c;
   return SPEXHIP_OK;
 }
@@ -113,6 +116,8 @@ int spexhip_destroy(spexhip_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
   ctx->allocs.clear();
+  for (auto &e : ctx->phase_ev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SPEXHIP_OK;
@@ -137,6 +142,22 @@ int spexhip_mem_used(spexhip_ctx *ctx, size_t *used, size_t *budget) {
   std::lock_guard<std::mutex> g(ctx->mu);
   if (used) *used = ctx->used;
   if (budget) *budget = ctx->budget;
+  return SPEXHIP_OK;
+}
+
+int spexhip_set_workspace_limit(spexhip_ctx *ctx, size_t bytes) {
+  if (!ctx || bytes == 0) return SPEXHIP_E_INVALID;
+  ctx->ozaki_ws_limit = bytes;
+  return SPEXHIP_OK;
+}
+
+int spexhip_chi0_phase_ms(spexhip_ctx *ctx, float *prep_ms, float *rank_ms, int *groups) {
+  if (!ctx || !prep_ms || !rank_ms) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, ctx->phase_groups > 0, "no Ozaki chi0 call recorded yet");
+  SPX_HIP(ctx, hipEventSynchronize(ctx->phase_ev[2]));
+  SPX_HIP(ctx, hipEventElapsedTime(prep_ms, ctx->phase_ev[0], ctx->phase_ev[1]));
+  SPX_HIP(ctx, hipEventElapsedTime(rank_ms, ctx->phase_ev[1], ctx->phase_ev[2]));
+  if (groups) *groups = ctx->phase_groups;
   return SPEXHIP_OK;
 }
 

@@ -15,146 +15,32 @@
 // leading block is HPD and no pivoting is needed), with 64-wide panels:
 // one workgroup inverts the diagonal block in LDS, an MFMA GEMM forms the
 // row panel, and a rank-64 MFMA update sweeps the whole matrix.
-#include "common.hpp"
+#include "gemm_f64.hpp"
+
+using namespace spxg;
 
 namespace {
 
-constexpr int TM = 64, KC = 16, NT = 256;
-
-// ------------------------------------------------- complex FP64 GEMM tile --
-// acc += A(i0:i0+64, 0:kdim) * B(0:kdim, j0:j0+64); A, B column-major.
-// Rows/cols outside [0,m) / [0,n) and k outside [0,kdim) read as zero.
-struct TileAcc {
-  d4_t r[2][2], i[2][2];
-};
-
-__device__ inline void tile_zero(TileAcc &a) {
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++) {
-      a.r[x][y] = d4_t{0, 0, 0, 0};
-      a.i[x][y] = d4_t{0, 0, 0, 0};
-    }
-}
-
-struct GemmLds {
-  double Ar[2][KC][TM], Ai[2][KC][TM], Br[2][KC][TM], Bi[2][KC][TM];
-};
-
-__device__ void gemm_tile(const double2 *__restrict__ A, long lda,
-                          const double2 *__restrict__ B, long ldb, int m, int n,
-                          int kdim, int i0, int j0, TileAcc &acc, GemmLds &s) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1, lr = lane & 15, lk = lane >> 4;
-  double2 va[4], vb[4];
-  const int nst = (kdim + KC - 1) / KC;
-  auto fetch = [&](int st) {
-    const int k0 = st * KC;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int e = tid + NT * r;
-      {  // A: i fastest (coalesced down a column)
-        const int i = e % TM, k = e / TM;
-        const int gi = i0 + i, gk = k0 + k;
-        va[r] = (gi < m && gk < kdim) ? A[gi + lda * (long)gk] : make_double2(0, 0);
-      }
-      {  // B: k fastest
-        const int k = e % KC, j = e / KC;
-        const int gj = j0 + j, gk = k0 + k;
-        vb[r] = (gj < n && gk < kdim) ? B[gk + ldb * (long)gj] : make_double2(0, 0);
-      }
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int e = tid + NT * r;
-      {
-        const int i = e % TM, k = e / TM;
-        s.Ar[buf][k][i] = va[r].x;
-        s.Ai[buf][k][i] = va[r].y;
-      }
-      {
-        const int k = e % KC, j = e / KC;
-        s.Br[buf][k][j] = vb[r].x;
-        s.Bi[buf][k][j] = vb[r].y;
-      }
-    }
-  };
-  __syncthreads();  // LDS may still be in use by a previous call
-  fetch(0);
-  store(0);
-  __syncthreads();
-  for (int st = 0; st < nst; st++) {
-    const int buf = st & 1;
-    if (st + 1 < nst) fetch(st + 1);
-#pragma unroll
-    for (int kk = 0; kk < KC / 4; kk++) {
-      const int k = kk * 4 + lk;
-      double ar[2], ai[2], br[2], bi[2];
-#pragma unroll
-      for (int x = 0; x < 2; x++) {
-        ar[x] = s.Ar[buf][k][wr * 32 + x * 16 + lr];
-        ai[x] = s.Ai[buf][k][wr * 32 + x * 16 + lr];
-        br[x] = s.Br[buf][k][wc * 32 + x * 16 + lr];
-        bi[x] = s.Bi[buf][k][wc * 32 + x * 16 + lr];
-      }
-#pragma unroll
-      for (int x = 0; x < 2; x++)
-#pragma unroll
-        for (int y = 0; y < 2; y++) {
-          // (ar + i ai)(br + i bi) = ar br - ai bi + i(ar bi + ai br)
-          acc.r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc.r[x][y], 0, 0, 0);
-          acc.r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], acc.r[x][y], 0, 0, 0);
-          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acc.i[x][y], 0, 0, 0);
-          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acc.i[x][y], 0, 0, 0);
-        }
-    }
-    if (st + 1 < nst) store(buf ^ 1);
-    __syncthreads();
-  }
-}
-
-// Visit the tile's elements owned by this lane: f(i, j, re, im).  The
-// v_mfma_f64_16x16x4f64 result holds column j = lane%16, rows lane/16 + 4*r.
-template <class F>
-__device__ inline void tile_foreach(const TileAcc &acc, int i0, int j0, F f) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int wr = wv >> 1, wc = wv & 1, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-  for (int x = 0; x < 2; x++)
-#pragma unroll
-    for (int y = 0; y < 2; y++)
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-        f(i0 + wr * 32 + x * 16 + lk + 4 * r, j0 + wc * 32 + y * 16 + lr, acc.r[x][y][r],
-          acc.i[x][y][r]);
-}
-
-// C = alpha*A*B + beta*C + diag*I, batched over blockIdx.z with strides.
-__global__ __launch_bounds__(NT) void zgemm_kernel(int m, int n, int k, double alpha,
-                                                   const double2 *__restrict__ A, long lda, long sA,
-                                                   const double2 *__restrict__ B, long ldb, long sB,
-                                                   double beta, double2 *__restrict__ C, long ldc,
-                                                   long sC, double diag) {
+// C = alpha*A*B + beta*C + diag*I, batched over blockIdx.z (see ZgemmArgs).
+__global__ __launch_bounds__(NT) void zgemm_kernel(ZgemmArgs a) {
   __shared__ GemmLds s;
   const int b = blockIdx.z;
   const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TM;
   TileAcc acc;
   tile_zero(acc);
-  gemm_tile(A + sA * b, lda, B + sB * b, ldb, m, n, k, i0, j0, acc, s);
-  double2 *Cb = C + sC * b;
+  gemm_tile(a.A + (b / a.dA) * a.sA, a.lda, a.B + (b / a.dB) * a.sB, a.ldb, a.m, a.n, a.k, i0,
+            j0, acc, s);
+  double2 *Cb = a.C + (b / a.dC) * a.sC;
   tile_foreach(acc, i0, j0, [&](int i, int j, double re, double im) {
-    if (i >= m || j >= n) return;
-    double2 *p = Cb + i + ldc * (long)j;
-    double2 o = make_double2(alpha * re, alpha * im);
-    if (beta != 0.0) {
+    if (i >= a.m || j >= a.n) return;
+    double2 *p = Cb + i + a.ldc * (long)j;
+    double2 o = make_double2(a.alpha.x * re - a.alpha.y * im, a.alpha.x * im + a.alpha.y * re);
+    if (a.beta != 0.0) {
       const double2 old = *p;
-      o.x += beta * old.x;
-      o.y += beta * old.y;
+      o.x += a.beta * old.x;
+      o.y += a.beta * old.y;
     }
-    if (i == j) o.x += diag;
+    if (i == j) o.x += a.diag;
     *p = o;
   });
 }
@@ -313,14 +199,20 @@ __global__ __launch_bounds__(NT) void gj_update_kernel(int nb, double2 *__restri
 }  // namespace
 
 // ------------------------------------------------------------ launchers --
+int spx_zgemm_ex(spexhip_ctx *ctx, hipStream_t st, const ZgemmArgs &a, int batch) {
+  if (a.m <= 0 || a.n <= 0 || batch <= 0) return SPEXHIP_OK;
+  dim3 grid((a.m + TM - 1) / TM, (a.n + TM - 1) / TM, batch);
+  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(NT), 0, st, a);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
 int spx_zgemm(spexhip_ctx *ctx, hipStream_t st, int m, int n, int k, double alpha,
               const void *A, long lda, long sA, const void *B, long ldb, long sB, double beta,
               void *C, long ldc, long sC, double diag, int batch) {
-  dim3 grid((m + TM - 1) / TM, (n + TM - 1) / TM, batch);
-  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(NT), 0, st, m, n, k, alpha, (const double2 *)A, lda,
-                     sA, (const double2 *)B, ldb, sB, beta, (double2 *)C, ldc, sC, diag);
-  SPX_LAUNCHED(ctx);
-  return SPEXHIP_OK;
+  ZgemmArgs a{m, n, k, make_double2(alpha, 0.0), (const double2 *)A, lda, sA, 1,
+              (const double2 *)B, ldb, sB, 1, beta, (double2 *)C, ldc, sC, 1, diag};
+  return spx_zgemm_ex(ctx, st, a, batch);
 }
 
 int spx_hermitize(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx,

@@ -1,0 +1,42 @@
+// Layout conversion for the host drop-in entry points: device products are
+// transition-major (t*ldm + I); the reference's cprod is column-major
+// (t fastest, wavefproducts.f:120,490).
+#include "common.hpp"
+
+namespace {
+
+__global__ void transpose_kernel(const double2 *__restrict__ M, long rows, int cols, long ldm,
+                                 double2 *__restrict__ T) {
+  __shared__ double2 tile[32][33];
+  const long r0 = (long)blockIdx.x * 32;
+  const int c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8 threads
+  for (int j = ty; j < 32; j += 8) {
+    const long r = r0 + j;
+    const int c = c0 + tx;
+    if (r < rows && c < cols) tile[j][tx] = M[r * ldm + c];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j;
+    const long r = r0 + tx;
+    if (r < rows && c < cols) T[r + rows * (long)c] = tile[tx][j];
+  }
+}
+
+}  // namespace
+
+int spx_transpose_d2h(spexhip_ctx *ctx, hipStream_t st, const void *M, long rows, int cols,
+                      long ldm, double *host) {
+  void *T;
+  const size_t bytes = 16 * (size_t)rows * cols;
+  int rc = spx_scratch(ctx, bytes, &T);
+  if (rc) return rc;
+  dim3 grid((rows + 31) / 32, (cols + 31) / 32);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, st, (const double2 *)M, rows, cols, ldm,
+                     (double2 *)T);
+  SPX_LAUNCHED(ctx);
+  SPX_HIP(ctx, hipMemcpyAsync(host, T, bytes, hipMemcpyDeviceToHost, st));
+  SPX_HIP(ctx, hipStreamSynchronize(st));
+  return SPEXHIP_OK;
+}

@@ -1,0 +1,367 @@
+// Interstitial (IPW) part of the wave-function products by FFT convolution,
+// replacing the FFTW path of wavefproducts.f:576-758 (plans fft.f:90-93).
+//
+// Reference (SUBTYPE 2, noinv, fullpw optional):
+//   phi1(r) = conjg(FFT_bwd(c1)(r) / sqrt(vol)) [* theta(r)]    (:624-633)
+//   psi2(r) = FFT_bwd(c2)(r) / nnfft                             (:658-665)
+//   cprod(b2,b1,k,ig) = FFT_fwd(phi1*psi2)(imati.(Gm_ig - g_k) mod nfft)
+//                                                                (:670-684)
+// with FFT_bwd = sum_G c(G) e^{+2 pi i G.r/N}, FFT_fwd = sum_r p(r)
+// e^{-2 pi i G.r/N}, both unnormalised (FFTW BACKWARD / FORWARD).
+//
+// Every transform here is a pruned separable DFT evaluated as FP64 complex
+// MFMA products on v_mfma_f64_16x16x4f64.  Wave-function coefficients live
+// in a 16^3 box of Fourier components (-8..7 per axis), so the backward
+// transform starts from that box and the forward transform only produces
+// the box (then gathers the mixed-basis IPWs):
+//   pw_inverse  (one workgroup per state, waves take z planes):
+//     U(gx,gy) = sum_gz e^{+i..gz z} C(gz,gy,gx)           (VALU, from LDS)
+//     V(gx,y)  = U x E2         (MFMA, K = 16 gy)
+//     psi(y,x) = V^T x E1       (MFMA, K = 16 gx) -> grid, scaled
+//   pw_pair_xy (one wave per pair (b1,b2), workgroup = 4 b1 sharing b2):
+//     R^T(y,gx) = (phi1*psi2)(y,x) x F1(x,gx)   (MFMA, K = N1)
+//     Q(gy,gx)  = F2(gy,y) x R^T(y,gx)          (MFMA, K = N2)  -> slab
+//   pw_pair_z  (one workgroup per pair):
+//     cprod(t, ig) = sum_z e^{-i..gz z} Q_z(gy,gx) at the IPW's box point.
+// The accumulator layout of one MFMA (column j = lane%16, rows lane/16+4r)
+// is exactly the operand layout the next MFMA needs, so no transposes go
+// through LDS between passes.
+#include <cmath>
+#include <vector>
+
+#include "plan.hpp"
+
+namespace {
+
+constexpr int NT = 256;
+
+__device__ inline double2 twid(long m, int N, double sign) {
+  long r = m % N;
+  if (r < 0) r += N;
+  double s, c;
+  sincospi(2.0 * (double)r / (double)N, &s, &c);
+  return make_double2(c, sign * s);
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// acc += a * b (complex, one 16x16x4 MFMA step)
+__device__ inline void cmfma(d4_t &ar, d4_t &ai, double2 a, double2 b) {
+  ar = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, ar, 0, 0, 0);
+  ar = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, ar, 0, 0, 0);
+  ai = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, ai, 0, 0, 0);
+  ai = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, ai, 0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Backward transform of states: s < nA -> (kA, bandA+s) as phi1, else
+// (kB, bandB+s-nA) as psi2.  grids[s] is N3 x N2 x N1 (x fastest).
+template <int NB>
+__global__ __launch_bounds__(NT) void pw_inverse_kernel(
+    const double2 *__restrict__ cpw, int maxgpt, int nband, const int *__restrict__ gidx,
+    const int *__restrict__ ngpt, int kA, int bandA, int nA, int kB, int bandB, int N1, int N2,
+    int N3, double scaleA, double scaleB, const double *__restrict__ step,
+    double2 *__restrict__ grids) {
+  __shared__ double Cr[PW_BOX * PW_BOX * PW_BOX], Ci[PW_BOX * PW_BOX * PW_BOX];
+  __shared__ double2 tz[64][PW_BOX];  // N3 <= 64
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  const int s = blockIdx.x;
+  const bool occ = s < nA;
+  const int kq = occ ? kA : kB;
+  const int band = occ ? bandA + s : bandB + (s - nA);
+  for (int e = tid; e < PW_BOX * PW_BOX * PW_BOX; e += NT) {
+    Cr[e] = 0.0;
+    Ci[e] = 0.0;
+  }
+  for (int e = tid; e < N3 * PW_BOX; e += NT) {
+    const int z = e / PW_BOX, g = e % PW_BOX;
+    tz[z][g] = twid((long)(g - PW_OFF) * z, N3, +1.0);
+  }
+  __syncthreads();
+  const int ng = ngpt[kq];
+  const double2 *c = cpw + (long)maxgpt * (band + (long)nband * kq);
+  const int *gi = gidx + (long)maxgpt * kq;
+  for (int ig = tid; ig < ng; ig += NT) {
+    const int b = gi[ig];
+    Cr[b] = c[ig].x;
+    Ci[b] = c[ig].y;
+  }
+  __syncthreads();
+
+  // constant MFMA operands: E2(gy, y) and E1(gx, x), lane layout B[k][j]
+  double2 E2[NB][4], E1[NB][4];
+#pragma unroll
+  for (int b = 0; b < NB; b++)
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int g = 4 * kk + lk - PW_OFF, j = 16 * b + lr;
+      E2[b][kk] = j < N2 ? twid((long)g * j, N2, +1.0) : make_double2(0.0, 0.0);
+      E1[b][kk] = j < N1 ? twid((long)g * j, N1, +1.0) : make_double2(0.0, 0.0);
+    }
+
+  const long NN = (long)N1 * N2 * N3;
+  double2 *out = grids + (long)s * NN;
+  const double scale = occ ? scaleA : scaleB;
+  for (int z = wv; z < N3; z += NT / 64) {
+    double2 U[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; kk++) {
+      const int gy = 4 * kk + lk;
+      double ur = 0.0, ui = 0.0;
+#pragma unroll 4
+      for (int gz = 0; gz < PW_BOX; gz++) {
+        const double2 t = tz[z][gz];
+        const int b = gz * PW_BOX * PW_BOX + gy * PW_BOX + lr;
+        const double cr = Cr[b], ci = Ci[b];
+        ur += t.x * cr - t.y * ci;
+        ui += t.x * ci + t.y * cr;
+      }
+      U[kk] = make_double2(ur, ui);
+    }
+#pragma unroll
+    for (int mb = 0; mb < NB; mb++) {
+      if (16 * mb >= N2) continue;
+      d4_t vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) cmfma(vr, vi, U[kk], E2[mb][kk]);
+#pragma unroll
+      for (int nb = 0; nb < NB; nb++) {
+        if (16 * nb >= N1) continue;
+        d4_t orr = {0, 0, 0, 0}, oi = {0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) cmfma(orr, oi, make_double2(vr[kk], vi[kk]), E1[nb][kk]);
+        const int x = 16 * nb + lr;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int y = 16 * mb + lk + 4 * r;
+          if (x < N1 && y < N2) {
+            const long g = x + (long)N1 * (y + (long)N2 * z);
+            double2 v = make_double2(orr[r], oi[r]);
+            if (occ) {
+              v = make_double2(v.x * scale, -v.y * scale);
+              if (step) {
+                const double th = step[g];
+                v.x *= th;
+                v.y *= th;
+              }
+            } else {
+              v.x *= scale;
+              v.y *= scale;
+            }
+            out[g] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward passes x and y of the products of pairs (b1, b2).
+// grids: [0, nocc) phi1 of the occupied states, [nocc, nocc+nunocc) psi2.
+// slab[(b1*nunocc + b2)][z][gy][gx], 16 x 16 per plane.
+template <int NB>
+__global__ __launch_bounds__(NT) void pw_pair_xy_kernel(const double2 *__restrict__ grids,
+                                                        int nocc, int nunocc, int N1, int N2,
+                                                        int N3, double2 *__restrict__ slab) {
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  const int b2 = blockIdx.x;
+  const int b1 = blockIdx.y * (NT / 64) + wv;
+  if (b1 >= nocc) return;  // whole wave; this kernel has no workgroup barrier
+  const long NN = (long)N1 * N2 * N3;
+  const double2 *p1 = grids + (long)b1 * NN;
+  const double2 *p2 = grids + (long)(nocc + b2) * NN;
+  double2 *S = slab + ((long)b1 * nunocc + b2) * N3 * PW_BOX * PW_BOX;
+  constexpr int KX = 4 * NB;  // x steps (N1 <= 16*NB)
+  double2 F1[KX], F2[KX];
+#pragma unroll
+  for (int kk = 0; kk < KX; kk++) {
+    const int x = 4 * kk + lk;  // B[k = x][j = gx]
+    F1[kk] = x < N1 ? twid((long)(lr - PW_OFF) * x, N1, -1.0) : make_double2(0.0, 0.0);
+    const int y = 4 * kk + lk;  // A[i = gy][k = y]
+    F2[kk] = y < N2 ? twid((long)(lr - PW_OFF) * y, N2, -1.0) : make_double2(0.0, 0.0);
+  }
+  for (int z = 0; z < N3; z++) {
+    d4_t xr[NB], xi[NB];
+#pragma unroll
+    for (int mb = 0; mb < NB; mb++) {
+      xr[mb] = d4_t{0, 0, 0, 0};
+      xi[mb] = d4_t{0, 0, 0, 0};
+      if (16 * mb >= N2) continue;
+      const int y = 16 * mb + lr;
+#pragma unroll
+      for (int kk = 0; kk < KX; kk++) {
+        if (4 * kk < N1) {  // wave-uniform guard (keeps the loop unrollable)
+          const int x = 4 * kk + lk;
+          double2 a = make_double2(0.0, 0.0);
+          if (x < N1 && y < N2) {
+            const long g = x + (long)N1 * (y + (long)N2 * z);
+            a = cmul(p1[g], p2[g]);
+          }
+          cmfma(xr[mb], xi[mb], a, F1[kk]);
+        }
+      }
+    }
+    d4_t qr = {0, 0, 0, 0}, qi = {0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < KX; kk++)
+      if (4 * kk < N2) cmfma(qr, qi, F2[kk], make_double2(xr[kk / 4][kk % 4], xi[kk / 4][kk % 4]));
+    double2 *Sz = S + (long)z * PW_BOX * PW_BOX;
+#pragma unroll
+    for (int r = 0; r < 4; r++) Sz[(lk + 4 * r) * PW_BOX + lr] = make_double2(qr[r], qi[r]);
+  }
+}
+
+// Forward pass z and gather of the mixed-basis IPWs (oidx: box index
+// gz*256 + gy*16 + gx of imati.(Gm - g_k) per IPW).
+__global__ __launch_bounds__(NT) void pw_pair_z_kernel(const double2 *__restrict__ slab,
+                                                       int nunocc, int N3,
+                                                       const int *__restrict__ oidx, int ngptm,
+                                                       double2 *__restrict__ M, long ldm, long t0,
+                                                       int col0) {
+  __shared__ double2 tz[64][PW_BOX];  // N3 <= 64
+  const int tid = threadIdx.x;
+  for (int e = tid; e < N3 * PW_BOX; e += NT) {
+    const int z = e / PW_BOX, g = e % PW_BOX;
+    tz[z][g] = twid((long)(g - PW_OFF) * z, N3, -1.0);
+  }
+  __syncthreads();
+  const long pair = blockIdx.x;  // b1*nunocc + b2
+  const double2 *S = slab + pair * N3 * PW_BOX * PW_BOX;
+  double2 *out = M + (t0 + pair) * ldm + col0;
+  for (int ig = tid; ig < ngptm; ig += NT) {
+    const int b = oidx[ig];
+    const int gz = b >> 8, gyx = b & 255;
+    double ar = 0.0, ai = 0.0;
+    for (int z = 0; z < N3; z++) {
+      const double2 t = tz[z][gz];
+      const double2 v = S[(long)z * PW_BOX * PW_BOX + gyx];
+      ar += t.x * v.x - t.y * v.y;
+      ai += t.x * v.y + t.y * v.x;
+    }
+    out[ig] = make_double2(ar, ai);
+  }
+}
+
+int box_index(const int *imati, const int *v, int &idx) {
+  int g[3];
+  for (int i = 0; i < 3; i++) {
+    g[i] = imati[3 * i] * v[0] + imati[3 * i + 1] * v[1] + imati[3 * i + 2] * v[2];
+    if (g[i] < -PW_OFF || g[i] >= PW_BOX - PW_OFF) return 1;
+  }
+  idx = ((g[2] + PW_OFF) * PW_BOX + (g[1] + PW_OFF)) * PW_BOX + (g[0] + PW_OFF);
+  return 0;
+}
+
+template <int NB>
+int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const void *cpw, int nband,
+              int nk, const int *k1, const int *k2, const int *oidx_dev, int b1lo, int nocc,
+              int b2lo, int nunocc, void *M, long ldm, int col0, void *ws) {
+  const int N1 = p->nfft[0], N2 = p->nfft[1], N3 = p->nfft[2];
+  const long NN = (long)N1 * N2 * N3;
+  double2 *grids = (double2 *)ws;
+  double2 *slab = grids + (long)(nocc + nunocc) * NN;
+  const int *ngpt_dev = p->gidx + (long)p->nkpt * p->maxgpt;
+  for (int ik = 0; ik < nk; ik++) {
+    hipLaunchKernelGGL(pw_inverse_kernel<NB>, dim3(nocc + nunocc), dim3(NT), 0, st,
+                       (const double2 *)cpw, p->maxgpt, nband, p->gidx, ngpt_dev, k1[ik], b1lo,
+                       nocc, k2[ik], b2lo, N1, N2, N3, 1.0 / std::sqrt(p->vol), 1.0 / (double)NN,
+                       p->step, grids);
+    SPX_LAUNCHED(ctx);
+    hipLaunchKernelGGL(pw_pair_xy_kernel<NB>, dim3(nunocc, (nocc + 3) / 4), dim3(NT), 0, st, grids,
+                       nocc, nunocc, N1, N2, N3, slab);
+    SPX_LAUNCHED(ctx);
+    hipLaunchKernelGGL(pw_pair_z_kernel, dim3(nocc * nunocc), dim3(NT), 0, st, slab, nunocc, N3,
+                       oidx_dev + (long)ik * p->ngptm, p->ngptm, (double2 *)M, ldm,
+                       (long)ik * nocc * nunocc, col0);
+    SPX_LAUNCHED(ctx);
+  }
+  return SPEXHIP_OK;
+}
+
+}  // namespace
+
+int spx_pw_plan_build(spexhip_ctx *ctx, const spexhip_pw_desc *d, spexhip_plan *p) {
+  p->kind = 2;
+  for (int i = 0; i < 3; i++) {
+    SPX_CHECK(ctx, d->nfft[i] >= PW_BOX && d->nfft[i] <= 64,
+              "pw plan: nfft[%d]=%d outside [16, 64]", i, d->nfft[i]);
+    p->nfft[i] = d->nfft[i];
+  }
+  for (int i = 0; i < 9; i++) p->imati[i] = d->imati[i];
+  SPX_CHECK(ctx, d->vol > 0 && d->nkpt > 0 && d->maxgpt > 0 && d->ngptm >= 0 && d->gpt && d->ngpt,
+            "pw plan: bad descriptor");
+  p->vol = d->vol;
+  p->nkpt = d->nkpt;
+  p->maxgpt = d->maxgpt;
+  p->ngptm = d->ngptm;
+  p->ngpt.assign(d->ngpt, d->ngpt + d->nkpt);
+  p->gptm.assign(d->gptm, d->gptm + 3 * d->ngptm);
+  std::vector<int> tab((size_t)d->nkpt * d->maxgpt + d->nkpt, 0);
+  for (int k = 0; k < d->nkpt; k++) {
+    SPX_CHECK(ctx, d->ngpt[k] >= 0 && d->ngpt[k] <= d->maxgpt, "pw plan: ngpt[%d] out of range", k);
+    for (int ig = 0; ig < d->ngpt[k]; ig++) {
+      int idx;
+      if (box_index(p->imati, d->gpt + 3 * ((long)k * d->maxgpt + ig), idx))
+        return spx_seterr(ctx, SPEXHIP_E_INVALID,
+                          "pw plan: G vector %d of k %d outside the -8..7 Fourier box", ig, k);
+      tab[(size_t)k * d->maxgpt + ig] = idx;
+    }
+    tab[(size_t)d->nkpt * d->maxgpt + k] = d->ngpt[k];
+  }
+  int rc = spx_alloc(ctx, sizeof(int) * tab.size(), (void **)&p->gidx);
+  if (rc) return rc;
+  p->owned.push_back(p->gidx);
+  SPX_HIP(ctx, hipMemcpy(p->gidx, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+  if (d->step_r) {
+    const size_t nn = (size_t)p->nfft[0] * p->nfft[1] * p->nfft[2];
+    rc = spx_alloc(ctx, 8 * nn, (void **)&p->step);
+    if (rc) return rc;
+    p->owned.push_back(p->step);
+    SPX_HIP(ctx, hipMemcpy(p->step, d->step_r, 8 * nn, hipMemcpyHostToDevice));
+  }
+  return SPEXHIP_OK;
+}
+
+size_t spx_wfprod_pw_ws(const spexhip_plan *p, int nk, int nocc, int nunocc) {
+  const long NN = (long)p->nfft[0] * p->nfft[1] * p->nfft[2];
+  return 16 * ((size_t)(nocc + nunocc) * NN + (size_t)nocc * nunocc * p->nfft[2] * PW_BOX * PW_BOX) +
+         4 * (size_t)nk * p->ngptm + 256;
+}
+
+// k1, k2 host arrays (nk); g2 host (nk x 3).  Writes columns col0 .. col0 +
+// ngptm - 1 of M (transition-major, ld ldm).
+int spx_wfprod_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const void *cpw,
+                  int nband, int nk, const int *k1, const int *k2, const int *g2, int b1lo,
+                  int nocc, int b2lo, int nunocc, void *M, long ldm, int col0, void *ws) {
+  std::vector<int> oidx((size_t)nk * p->ngptm);
+  for (int ik = 0; ik < nk; ik++) {
+    SPX_CHECK(ctx, k1[ik] >= 0 && k1[ik] < p->nkpt && k2[ik] >= 0 && k2[ik] < p->nkpt,
+              "wfprod_pw: k index out of range");
+    for (int ig = 0; ig < p->ngptm; ig++) {
+      int v[3];
+      for (int i = 0; i < 3; i++) v[i] = p->gptm[3 * ig + i] - g2[3 * ik + i];
+      int idx;
+      if (box_index(p->imati, v, idx))
+        return spx_seterr(ctx, SPEXHIP_E_INVALID,
+                          "wfprod_pw: IPW %d - umklapp of k %d outside the -8..7 Fourier box", ig, ik);
+      oidx[(size_t)ik * p->ngptm + ig] = idx;
+    }
+  }
+  const size_t wsb = spx_wfprod_pw_ws(p, nk, nocc, nunocc);
+  int *oidx_dev = (int *)((char *)ws + wsb - 256 - 4 * (size_t)nk * p->ngptm);
+  SPX_HIP(ctx, hipMemcpyAsync(oidx_dev, oidx.data(), 4 * oidx.size(), hipMemcpyHostToDevice, st));
+  SPX_HIP(ctx, hipStreamSynchronize(st));  // oidx is a host temporary
+  const int nbmax = (std::max(p->nfft[0], p->nfft[1]) + 15) / 16;
+  switch (nbmax) {
+    case 1:
+    case 2:
+      return launch_pw<2>(ctx, st, p, cpw, nband, nk, k1, k2, oidx_dev, b1lo, nocc, b2lo, nunocc, M, ldm, col0, ws);
+    case 3:
+      return launch_pw<3>(ctx, st, p, cpw, nband, nk, k1, k2, oidx_dev, b1lo, nocc, b2lo, nunocc, M, ldm, col0, ws);
+    default:
+      return launch_pw<4>(ctx, st, p, cpw, nband, nk, k1, k2, oidx_dev, b1lo, nocc, b2lo, nunocc, M, ldm, col0, ws);
+  }
+}

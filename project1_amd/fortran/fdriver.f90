@@ -1,0 +1,84 @@
+! Fortran host exercising the ISO_C_BINDING boundary: one packet of the
+! HILBERT NONE pair sum and the dielectric inverse, compared with a plain
+! Fortran pair loop written the way susceptibility.f:1183-1240 does it.
+! Exit status 0 on agreement (normwise relative error < 1e-12).
+program fdriver
+  use, intrinsic :: iso_c_binding
+  use m_spexhip
+  implicit none
+  integer, parameter :: n = 96, nb = 40, nw = 3, npack = nb*(nb+1)/2
+  complex(c_double_complex), target :: cprod(n, nb), sus(npack, nw), ref(npack, nw), einv(npack, nw)
+  real(c_double), target :: wght(n, nw), sq(nb)
+  real(c_double) :: re(n), im(n), err, nrm, x
+  type(c_ptr) :: ctx
+  integer :: i, j, k, ij, f, rc
+
+  do j = 1, nb
+    do k = 1, n
+      x = 0.37d0*k + 1.13d0*j
+      cprod(k, j) = cmplx(sin(x), cos(1.7d0*x), kind=c_double_complex) / sqrt(dble(nb))
+    end do
+  end do
+  do f = 1, nw
+    do k = 1, n
+      wght(k, f) = -2d0*(0.5d0 + mod(k, 7)) / ((0.3d0*f)**2 + (0.5d0 + mod(k, 7))**2) / 8
+    end do
+  end do
+  do i = 1, nb
+    sq(i) = sqrt(0.1d0 + i)
+  end do
+
+  ! plain Fortran pair loop (reference order: j outer, i <= j, one dot per frequency)
+  ref = 0
+  do j = 1, nb
+    do i = 1, j
+      do k = 1, n
+        re(k) = real(cprod(k, i) * conjg(cprod(k, j)))
+        im(k) = aimag(cprod(k, i) * conjg(cprod(k, j)))
+      end do
+      ij = j*(j-1)/2 + i
+      do f = 1, nw
+        ref(ij, f) = ref(ij, f) + cmplx(dot_product(re, wght(:, f)), dot_product(im, wght(:, f)), &
+                                        kind=c_double_complex)
+      end do
+    end do
+  end do
+
+  rc = spexhip_create(0_c_int, 0_c_size_t, ctx)
+  if (rc /= 0) then
+    write(0, *) 'spexhip_create failed', rc
+    stop 2
+  end if
+  sus = 0
+  call spexhip_pairsum(ctx, n, nb, cprod, nw, wght, sus)
+  err = maxval(abs(sus - ref))
+  nrm = maxval(abs(ref))
+  write(*, '(A,ES10.3)') 'fortran pairsum max-rel error: ', err / nrm
+  if (err / nrm > 1d-12) stop 3
+
+  sus = 2 * sus  ! spin factor (susceptibility.f:1336)
+  call spexhip_dielec_inverse(ctx, nb, nw, sus, sq, einv)
+  ! eps^-1 of the first frequency times eps must be the identity
+  block
+    complex(c_double_complex) :: e(nb, nb), ei(nb, nb), prod(nb, nb)
+    ij = 0
+    do j = 1, nb
+      do i = 1, j
+        ij = ij + 1
+        e(i, j) = -sq(i) * sq(j) * sus(ij, 1)
+        e(j, i) = conjg(e(i, j))
+        ei(i, j) = einv(ij, 1)
+        ei(j, i) = conjg(ei(i, j))
+      end do
+      e(j, j) = e(j, j) + 1
+    end do
+    prod = matmul(ei, e)
+    do i = 1, nb
+      prod(i, i) = prod(i, i) - 1
+    end do
+    write(*, '(A,ES10.3)') 'fortran eps^-1 * eps - 1 max: ', maxval(abs(prod))
+    if (maxval(abs(prod)) > 1d-10) stop 4
+  end block
+  rc = spexhip_destroy(ctx)
+  write(*, '(A)') 'fortran driver OK'
+end program fdriver

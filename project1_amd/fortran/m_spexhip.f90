@@ -1,0 +1,200 @@
+! ISO_C_BINDING interface to libspexhip (include/spexhip.h) for the SPEX
+! Fortran host.  The shims keep the reference argument lists of the three
+! call sites that change:
+!   susceptibility.f:1159-1251  pair loop (HILBERT NONE)   -> spexhip_pairsum
+!   susceptibility.f:778,790    wavefproducts2_mt/_pw      -> spexhip_wfprod2_mt/_pw
+!   correlation.f:612-747       dielectric matrix, Minverse -> spexhip_dielec_inverse
+! A non-zero status stops the run with the library message, like the
+! reference's Error(...) -> spex_error (util.f:480-505).
+module m_spexhip
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer(c_int), parameter, public :: SPEXHIP_PREC_FP64 = 0, SPEXHIP_PREC_SPLIT_FP32 = 1
+
+  type, bind(C), public :: spexhip_mt_type
+    integer(c_int) :: lcut
+    type(c_ptr)    :: nindx
+    integer(c_int) :: lcutm
+    type(c_ptr)    :: nindxm
+    integer(c_int) :: neq
+    type(c_ptr)    :: integral
+    integer(c_int) :: d1, d2, d3
+  end type
+
+  type, bind(C), public :: spexhip_pw_desc
+    integer(c_int) :: nfft(3)
+    integer(c_int) :: imati(9)
+    real(c_double) :: vol
+    integer(c_int) :: nkpt, maxgpt
+    type(c_ptr)    :: gpt, ngpt
+    integer(c_int) :: ngptm
+    type(c_ptr)    :: gptm
+    type(c_ptr)    :: step_r
+  end type
+
+  public :: spexhip_create, spexhip_destroy, spexhip_mt_plan_create, spexhip_pw_plan_create
+  public :: spexhip_plan_destroy
+  public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
+  public :: spexhip_minverse
+
+  interface
+    integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
+      import :: c_int, c_size_t, c_ptr
+      integer(c_int), value :: device
+      integer(c_size_t), value :: budget
+      type(c_ptr) :: ctx
+    end function
+    integer(c_int) function spexhip_destroy(ctx) bind(C, name='spexhip_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function c_last_error(ctx, buf, len) bind(C, name='spexhip_last_error')
+      import :: c_int, c_ptr, c_char, c_size_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char) :: buf(*)
+      integer(c_size_t), value :: len
+    end function
+    integer(c_int) function c_susceptibility_none(ctx, n, nb, cprod, ldc, nw, wght, prec, sus) &
+        bind(C, name='spexhip_susceptibility_none')
+      import :: c_int, c_long, c_ptr
+      type(c_ptr), value :: ctx, cprod, wght, sus
+      integer(c_long), value :: n, ldc
+      integer(c_int), value :: nb, nw, prec
+    end function
+    integer(c_int) function spexhip_mt_plan_create(ctx, ntype, types, cent, maxlmindx, plan) &
+        bind(C, name='spexhip_mt_plan_create')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, types, cent
+      integer(c_int), value :: ntype, maxlmindx
+      type(c_ptr) :: plan
+    end function
+    integer(c_int) function spexhip_pw_plan_create(ctx, desc, plan) bind(C, name='spexhip_pw_plan_create')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, desc
+      type(c_ptr) :: plan
+    end function
+    integer(c_int) function spexhip_plan_destroy(ctx, plan) bind(C, name='spexhip_plan_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, plan
+    end function
+    integer(c_int) function c_wfprod_mt(ctx, plan, cmt, ncent, nband, nkpt, q, nk, k1, k2, b1lo, &
+        nocc, b2lo, nunocc, cprod) bind(C, name='spexhip_wavefproducts2_mt')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, plan, cmt, q, k1, k2, cprod
+      integer(c_int), value :: ncent, nband, nkpt, nk, b1lo, nocc, b2lo, nunocc
+    end function
+    integer(c_int) function c_wfprod_pw(ctx, plan, cpw, nband, nk, k1, k2, g2, b1lo, nocc, b2lo, &
+        nunocc, cprod) bind(C, name='spexhip_wavefproducts2_pw')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, plan, cpw, k1, k2, g2, cprod
+      integer(c_int), value :: nband, nk, b1lo, nocc, b2lo, nunocc
+    end function
+    integer(c_int) function c_dielectric_inverse(ctx, nb, nw, chi, vkind, vhalf, einv) &
+        bind(C, name='spexhip_dielectric_inverse')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, chi, vhalf, einv
+      integer(c_int), value :: nb, nw, vkind
+    end function
+    integer(c_int) function c_inverse_hpd_packed(ctx, nb, count, packed) &
+        bind(C, name='spexhip_inverse_hpd_packed')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, packed
+      integer(c_int), value :: nb, count
+    end function
+  end interface
+
+contains
+
+  ! Error(...) behaviour: print the library message and stop (util.f:480-505).
+  subroutine check(ctx, rc, where)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    character(kind=c_char) :: buf(512)
+    character(512) :: msg
+    integer :: i, irc
+    if (rc == 0) return
+    buf = c_null_char
+    irc = c_last_error(ctx, buf, 512_c_size_t)
+    msg = ' '
+    do i = 1, 512
+      if (buf(i) == c_null_char) exit
+      msg(i:i) = buf(i)
+    end do
+    write(0, '(A,I0,A)') 'SPEX-ERROR (spexhip, '//where//', status ', rc, '): '//trim(msg)
+    error stop 1
+  end subroutine
+
+  ! Pair loop of one memory packet, HILBERT NONE (susceptibility.f:1159-1251):
+  ! susceph(ij,ifrq) += sum_k wghtSr(k,ifrq) * cprod(k,i) * conjg(cprod(k,j)).
+  subroutine spexhip_pairsum(ctx, n, dim, cprod, nfrqh, wghtSr, susceph, precision)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: n, dim, nfrqh
+    complex(c_double_complex), intent(in), target :: cprod(n, dim)
+    real(c_double), intent(in), target :: wghtSr(n, nfrqh)
+    complex(c_double_complex), intent(inout), target :: susceph(dim*(dim+1)/2, nfrqh)
+    integer(c_int), intent(in), optional :: precision
+    integer(c_int) :: prec
+    prec = SPEXHIP_PREC_FP64
+    if (present(precision)) prec = precision
+    call check(ctx, c_susceptibility_none(ctx, int(n, c_long), int(dim, c_int), c_loc(cprod), &
+               int(n, c_long), int(nfrqh, c_int), c_loc(wghtSr), prec, c_loc(susceph)), 'pairsum')
+  end subroutine
+
+  ! wavefproducts2_mt(cprod,k1,nk,ik0,ispin1,ispin2,b1low,b1up,b2low,b2up)
+  ! with 1-based k and band indices as in the reference; q = kpt(:,ik0).
+  subroutine spexhip_wfprod2_mt(ctx, plan, cprod, cmt, q, k1, k2, nk, b1low, b1up, b2low, b2up)
+    type(c_ptr), intent(in) :: ctx, plan
+    integer, intent(in) :: nk, k1(nk), k2(nk), b1low, b1up, b2low, b2up
+    complex(c_double_complex), intent(in), target :: cmt(:,:,:,:)
+    real(c_double), intent(in), target :: q(3)
+    complex(c_double_complex), intent(out), target :: cprod(b2low:b2up, b1low:b1up, nk, *)
+    integer(c_int), target :: k1c(nk), k2c(nk)
+    k1c = k1 - 1
+    k2c = k2 - 1
+    call check(ctx, c_wfprod_mt(ctx, plan, c_loc(cmt), int(size(cmt,2), c_int), int(size(cmt,3), c_int), &
+               int(size(cmt,4), c_int), c_loc(q), int(nk, c_int), c_loc(k1c), c_loc(k2c), &
+               int(b1low-1, c_int), int(b1up-b1low+1, c_int), int(b2low-1, c_int), &
+               int(b2up-b2low+1, c_int), c_loc(cprod)), 'wavefproducts2_mt')
+  end subroutine
+
+  ! wavefproducts2_pw FFT path; gpt2(3,nk) is the umklapp of :547-552.
+  subroutine spexhip_wfprod2_pw(ctx, plan, cprod, cpw, k1, k2, gpt2, nk, b1low, b1up, b2low, b2up)
+    type(c_ptr), intent(in) :: ctx, plan
+    integer, intent(in) :: nk, k1(nk), k2(nk), b1low, b1up, b2low, b2up
+    integer, intent(in) :: gpt2(3, nk)
+    complex(c_double_complex), intent(in), target :: cpw(:,:,:)
+    complex(c_double_complex), intent(out), target :: cprod(b2low:b2up, b1low:b1up, nk, *)
+    integer(c_int), target :: k1c(nk), k2c(nk), g2c(3, nk)
+    k1c = k1 - 1
+    k2c = k2 - 1
+    g2c = gpt2
+    call check(ctx, c_wfprod_pw(ctx, plan, c_loc(cpw), int(size(cpw,2), c_int), int(nk, c_int), &
+               c_loc(k1c), c_loc(k2c), c_loc(g2c), int(b1low-1, c_int), int(b1up-b1low+1, c_int), &
+               int(b2low-1, c_int), int(b2up-b2low+1, c_int), c_loc(cprod)), 'wavefproducts2_pw')
+  end subroutine
+
+  ! correlation.f:612-747: dielec = 1 - v^1/2 suscep v^1/2, then Minverse, per
+  ! frequency.  sqcoul = sqrt(coul) (Coulomb eigenbasis).
+  subroutine spexhip_dielec_inverse(ctx, nbas, nfreq, suscep, sqcoul, dielec)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nbas, nfreq
+    complex(c_double_complex), intent(in), target :: suscep(nbas*(nbas+1)/2, nfreq)
+    real(c_double), intent(in), target :: sqcoul(nbas)
+    complex(c_double_complex), intent(out), target :: dielec(nbas*(nbas+1)/2, nfreq)
+    call check(ctx, c_dielectric_inverse(ctx, int(nbas, c_int), int(nfreq, c_int), c_loc(suscep), &
+               0_c_int, c_loc(sqcoul), c_loc(dielec)), 'dielectric inverse')
+  end subroutine
+
+  ! Minverse(dielec(:,i)) on packed Hermitian positive definite matrices.
+  subroutine spexhip_minverse(ctx, mat)
+    type(c_ptr), intent(in) :: ctx
+    complex(c_double_complex), intent(inout), target :: mat(:)
+    integer :: n
+    n = nint(sqrt(0.25d0 + 2*size(mat)) - 0.5d0)
+    call check(ctx, c_inverse_hpd_packed(ctx, int(n, c_int), 1_c_int, c_loc(mat)), 'Minverse')
+  end subroutine
+
+end module m_spexhip

@@ -25,7 +25,7 @@ import numpy as np
 _LIB_PATH = Path(__file__).resolve().parent / "libspexhip.so"
 
 OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOTPD, E_AGAIN = range(6)
-PREC_FP64, PREC_SPLIT_FP32 = 0, 1
+PREC_FP64, PREC_SPLIT_FP32, PREC_OZAKI_INT8 = 0, 1, 2
 
 _c_long = C.c_long
 _vp = C.c_void_p
@@ -54,6 +54,38 @@ def lib() -> C.CDLL:
     return _lib
 
 
+class MtType(C.Structure):
+    """spexhip_mt_type"""
+    _fields_ = [("lcut", C.c_int), ("nindx", _vp), ("lcutm", C.c_int), ("nindxm", _vp),
+                ("neq", C.c_int), ("integral", _vp), ("d1", C.c_int), ("d2", C.c_int),
+                ("d3", C.c_int)]
+
+
+class PwDesc(C.Structure):
+    """spexhip_pw_desc"""
+    _fields_ = [("nfft", C.c_int * 3), ("imati", C.c_int * 9), ("vol", C.c_double),
+                ("nkpt", C.c_int), ("maxgpt", C.c_int), ("gpt", _vp), ("ngpt", _vp),
+                ("ngptm", C.c_int), ("gptm", _vp), ("step_r", _vp)]
+
+
+class Plan:
+    """A muffin-tin or plane-wave product plan (owns host copies it points to)."""
+
+    def __init__(self, ctx: "Context", handle: int, keep: list):
+        self.ctx, self.h, self._keep = ctx, handle, keep
+
+    @property
+    def ncols(self) -> int:
+        v = C.c_int()
+        self.ctx._check(lib().spexhip_plan_ncols(self.ctx.h, self.h, C.byref(v)))
+        return v.value
+
+    def destroy(self) -> None:
+        if self.h:
+            self.ctx._check(lib().spexhip_plan_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
 def _declare(L: C.CDLL) -> None:
     I = C.c_int
     P = C.POINTER
@@ -79,6 +111,19 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_event_record": (I, [_vp, _vp]),
         "spexhip_event_elapsed": (I, [_vp, _vp, _vp, P(C.c_float)]),
         "spexhip_event_destroy": (I, [_vp, _vp]),
+        "spexhip_set_workspace_limit": (I, [_vp, C.c_size_t]),
+        "spexhip_chi0_phase_ms": (I, [_vp, P(C.c_float), P(C.c_float), P(I)]),
+        "spexhip_mt_plan_create": (I, [_vp, I, _vp, _vp, I, P(_vp)]),
+        "spexhip_pw_plan_create": (I, [_vp, P(PwDesc), P(_vp)]),
+        "spexhip_plan_destroy": (I, [_vp, _vp]),
+        "spexhip_plan_ncols": (I, [_vp, _vp, P(I)]),
+        "spexhip_wfprod2_mt_dev": (I, [_vp, _vp, _vp, I, I, I, _vp, I, _vp, _vp, I, I, I, I,
+                                       _vp, _c_long, I]),
+        "spexhip_wfprod2_pw_dev": (I, [_vp, _vp, _vp, I, I, _vp, _vp, _vp, I, I, I, I, _vp,
+                                       _c_long, I]),
+        "spexhip_wavefproducts2_mt": (I, [_vp, _vp, _vp, I, I, I, _vp, I, _vp, _vp, I, I, I, I,
+                                          _vp]),
+        "spexhip_wavefproducts2_pw": (I, [_vp, _vp, _vp, I, I, _vp, _vp, _vp, I, I, I, I, _vp]),
         "spexhip_dielectric_dev": (I, [_vp, I, I, _vp, I, I, _vp, _vp, I]),
         "spexhip_inverse_hpd_dev": (I, [_vp, I, I, _vp, I]),
         "spexhip_dielectric_inverse": (I, [_vp, I, I, _vp, I, _vp, _vp]),
@@ -204,6 +249,14 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def set_workspace_limit(self, nbytes: int) -> None:
+        self._check(lib().spexhip_set_workspace_limit(self.h, int(nbytes)))
+
+    def chi0_phase_ms(self) -> tuple[float, float, int]:
+        a, b, g = C.c_float(), C.c_float(), C.c_int()
+        self._check(lib().spexhip_chi0_phase_ms(self.h, C.byref(a), C.byref(b), C.byref(g)))
+        return a.value, b.value, g.value
+
     # -- synthetic data and timing ------------------------------------------
     def synth_normal(self, buf: DevBuf, count: int, seed: int, scale: float = 1.0) -> None:
         self._check(lib().spexhip_synth_normal_dev(self.h, buf.ptr, count, seed, scale))
@@ -261,6 +314,93 @@ class Context:
             self.h, n, nb, cp.ctypes.data, n, nw, w.ctypes.data, precision,
             susceph.ctypes.data))
         return susceph
+
+    # -- wave-function products -------------------------------------------
+    def mt_plan(self, types: list, cent: np.ndarray, maxlmindx: int) -> Plan:
+        """types: dicts lcut, nindx, lcutm, nindxm, neq, integral (3-d, real)."""
+        keep = []
+        arr = (MtType * len(types))()
+        for t, T in zip(arr, types):
+            nindx = np.ascontiguousarray(T["nindx"], np.int32)
+            nindxm = np.ascontiguousarray(T["nindxm"], np.int32)
+            integ = np.asfortranarray(T["integral"], np.float64)
+            keep += [nindx, nindxm, integ]
+            t.lcut, t.lcutm, t.neq = int(T["lcut"]), int(T["lcutm"]), int(T["neq"])
+            t.nindx, t.nindxm, t.integral = _ptr(nindx), _ptr(nindxm), _ptr(integ)
+            t.d1, t.d2, t.d3 = integ.shape
+        cent = _f64(cent)
+        h = _vp()
+        self._check(lib().spexhip_mt_plan_create(self.h, len(types), C.cast(arr, _vp),
+                                                 _ptr(cent), maxlmindx, C.byref(h)))
+        return Plan(self, h.value, keep)
+
+    def pw_plan(self, nfft, gpt, ngpt, gptm, vol, imati=None, step_r=None) -> Plan:
+        gpt = np.ascontiguousarray(gpt, np.int32)
+        ngpt = np.ascontiguousarray(ngpt, np.int32)
+        gptm = np.ascontiguousarray(gptm, np.int32)
+        d = PwDesc()
+        d.nfft[:] = [int(v) for v in nfft]
+        im = np.eye(3, dtype=np.int32) if imati is None else np.asarray(imati, np.int32)
+        d.imati[:] = [int(v) for v in im.reshape(-1)]
+        d.vol = float(vol)
+        d.nkpt, d.maxgpt = gpt.shape[0], gpt.shape[1]
+        d.gpt, d.ngpt = _ptr(gpt), _ptr(ngpt)
+        d.ngptm, d.gptm = gptm.shape[0], _ptr(gptm)
+        keep = [gpt, ngpt, gptm]
+        if step_r is not None:
+            st = _f64(step_r)
+            keep.append(st)
+            d.step_r = _ptr(st)
+        h = _vp()
+        self._check(lib().spexhip_pw_plan_create(self.h, C.byref(d), C.byref(h)))
+        return Plan(self, h.value, keep)
+
+    def wfprod_mt_dev(self, plan: Plan, cmt: DevBuf, ncent, nband, nkpt, q, k1, k2, b1lo,
+                      nocc, b2lo, nunocc, M: DevBuf, ldm: int, col0: int = 0) -> None:
+        q = _f64(q)
+        k1 = np.ascontiguousarray(k1, np.int32)
+        k2 = np.ascontiguousarray(k2, np.int32)
+        self._check(lib().spexhip_wfprod2_mt_dev(self.h, plan.h, cmt.ptr, ncent, nband, nkpt,
+                                                 _ptr(q), k1.size, _ptr(k1), _ptr(k2), b1lo,
+                                                 nocc, b2lo, nunocc, M.ptr, ldm, col0))
+
+    def wfprod_pw_dev(self, plan: Plan, cpw: DevBuf, nband, k1, k2, g2, b1lo, nocc, b2lo,
+                      nunocc, M: DevBuf, ldm: int, col0: int = 0) -> None:
+        k1 = np.ascontiguousarray(k1, np.int32)
+        k2 = np.ascontiguousarray(k2, np.int32)
+        g2 = np.ascontiguousarray(g2, np.int32)
+        self._check(lib().spexhip_wfprod2_pw_dev(self.h, plan.h, cpw.ptr, nband, k1.size,
+                                                 _ptr(k1), _ptr(k2), _ptr(g2), b1lo, nocc, b2lo,
+                                                 nunocc, M.ptr, ldm, col0))
+
+    def wavefproducts2_mt(self, plan: Plan, cmt, q, k1, k2, b1lo, nocc, b2lo, nunocc):
+        """Host drop-in of wavefproducts2_mt: returns cprod (n, nbasp), Fortran order."""
+        cmt = np.asfortranarray(cmt, np.complex128)
+        maxlm, ncent, nband, nkpt = cmt.shape
+        q = _f64(q)
+        k1 = np.ascontiguousarray(k1, np.int32)
+        k2 = np.ascontiguousarray(k2, np.int32)
+        n = k1.size * nocc * nunocc
+        out = np.zeros((plan.ncols, n), np.complex128)  # C (ncols, n) == F (n, ncols)
+        self._check(lib().spexhip_wavefproducts2_mt(self.h, plan.h, cmt.ctypes.data, ncent,
+                                                    nband, nkpt, _ptr(q), k1.size, _ptr(k1),
+                                                    _ptr(k2), b1lo, nocc, b2lo, nunocc,
+                                                    _ptr(out)))
+        return out.T
+
+    def wavefproducts2_pw(self, plan: Plan, cpw, k1, k2, g2, b1lo, nocc, b2lo, nunocc):
+        """Host drop-in of wavefproducts2_pw: returns cprod (n, ngptm), Fortran order."""
+        cpw = np.asfortranarray(cpw, np.complex128)
+        nband = cpw.shape[1]
+        k1 = np.ascontiguousarray(k1, np.int32)
+        k2 = np.ascontiguousarray(k2, np.int32)
+        g2 = np.ascontiguousarray(g2, np.int32)
+        n = k1.size * nocc * nunocc
+        out = np.zeros((plan.ncols, n), np.complex128)
+        self._check(lib().spexhip_wavefproducts2_pw(self.h, plan.h, cpw.ctypes.data, nband,
+                                                    k1.size, _ptr(k1), _ptr(k2), _ptr(g2),
+                                                    b1lo, nocc, b2lo, nunocc, _ptr(out)))
+        return out.T
 
     # -- dielectric step ---------------------------------------------------
     def dielectric_dev(self, X: DevBuf, nb: int, nw: int, ldx: int, vkind: int,

@@ -100,6 +100,126 @@ def chi0_case(nk=8, nocc=8, nunocc=56, nb=256, nw=6, seed=1) -> Chi0Case:
     return Chi0Case(nk, nocc, nunocc, nb, nw, M, eo, eu, om, wq)
 
 
+@dataclass
+class ProductSystem:
+    """Synthetic inputs of the wave-function products for one q-point."""
+
+    nkpt: int
+    kpt: np.ndarray      # (nkpt, 3) fractional
+    q: np.ndarray        # (3,) = kpt[iq]
+    k1: np.ndarray       # (nk,) packet k indices (0-based)
+    k2: np.ndarray       # (nk,) index of k1+q folded into the mesh
+    g2: np.ndarray       # (nk, 3) umklapp nint(kpt(k2) - q - kpt(k1))
+    nband: int
+    # plane waves
+    gpt: np.ndarray      # (nkpt, maxgpt, 3)
+    ngpt: np.ndarray     # (nkpt,)
+    cpw: np.ndarray      # (maxgpt, nband, nkpt) complex, Fortran order
+    gptm: np.ndarray     # (ngptm, 3)
+    nfft: tuple
+    vol: float
+    # muffin tin
+    types: list          # dicts: lcut, nindx, lcutm, nindxm, neq, integral
+    cent: np.ndarray     # (ncent, 3)
+    cmt: np.ndarray      # (maxlm, ncent, nband, nkpt) complex, Fortran order
+
+    @property
+    def nbasp(self):
+        return sum(T["neq"] * sum((2 * l + 1) * T["nindxm"][l] for l in range(T["lcutm"] + 1))
+                   for T in self.types)
+
+    @property
+    def ngptm(self):
+        return self.gptm.shape[0]
+
+    @property
+    def maxgpt(self):
+        return self.gpt.shape[1]
+
+
+def kmesh(n: int):
+    g = np.arange(n) / n
+    kx, ky, kz = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([kx.ravel(), ky.ravel(), kz.ravel()], axis=1)
+
+
+def kptsum_index(kpt, n, a, b):
+    """Index of kpt[a] + kpt[b] folded into [0,1)^3 (vector.f:107-124)."""
+    v = np.mod(kpt[a] + kpt[b], 1.0)
+    g = np.rint(v * n).astype(int) % n
+    return int(g[0] * n * n + g[1] * n + g[2])
+
+
+def integral_dims(lcut, nindx, lcutm, nindxm):
+    """Extents of integral(ln, ln1, lln), wavefproducts.f:200-222."""
+    lm = sum(nindx[: lcut + 1])
+    lln = sum(nindxm[: lcutm + 1])
+    lm1 = 0
+    for ll in range(lcutm + 1):
+        for l in range(lcut + 1):
+            s = sum(nindx[l1] for l1 in range(abs(ll - l), min(ll + l, lcut) + 1, 2))
+            lm1 = max(lm1, s)
+    return lm, lm1, lln
+
+
+def product_system(nmesh=2, nband=20, gcut=3.1, ngptm=None, gcutm=2.1, nfft=(16, 16, 16),
+                   ntype=2, lcut=2, nrad=2, lcutm=2, nradm=2, iq=1, seed=5,
+                   damping=8.0) -> ProductSystem:
+    """Seeded product inputs on an nmesh^3 k mesh (simple cubic lattice).
+
+    Plane waves: G with |k+G| <= gcut, coefficients CN(0,1) exp(-|G|^2/damping).
+    Mixed-basis IPWs: the ngptm shortest G (or |G| <= gcutm).  MT: ntype
+    types with one atom each, lcut/nrad radial functions, random radial
+    integrals ~N(0,1)/10 and coefficients CN(0,1)/sqrt(maxlm).
+    """
+    g = rng(seed)
+    kpt = kmesh(nmesh)
+    nkpt = kpt.shape[0]
+    q = kpt[iq].copy()
+    k1 = np.arange(nkpt, dtype=np.int32)
+    k2 = np.array([kptsum_index(kpt, nmesh, iq, k) for k in k1], np.int32)
+    g2 = np.rint(kpt[k2] - q - kpt[k1]).astype(np.int32)
+    r = int(np.ceil(gcut)) + 1
+    ax = np.arange(-r, r + 1)
+    G = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    lists = []
+    for k in range(nkpt):
+        kk = kpt[k] + G
+        sel = G[np.einsum("ij,ij->i", kk, kk) <= gcut**2]
+        lists.append(sel)
+    maxgpt = max(len(s) for s in lists)
+    gpt = np.zeros((nkpt, maxgpt, 3), np.int32)
+    ngpt = np.zeros(nkpt, np.int32)
+    cpw = np.zeros((maxgpt, nband, nkpt), np.complex128, order="F")
+    for k, s in enumerate(lists):
+        ngpt[k] = len(s)
+        gpt[k, : len(s)] = s
+        damp = np.exp(-np.einsum("ij,ij->i", s, s) / damping)
+        c = (g.standard_normal((len(s), nband)) + 1j * g.standard_normal((len(s), nband)))
+        cpw[: len(s), :, k] = c * damp[:, None]
+    n2 = np.einsum("ij,ij->i", G, G)
+    order = np.lexsort((G[:, 2], G[:, 1], G[:, 0], n2))
+    if ngptm is None:
+        gptm = G[order][n2[order] <= gcutm**2]
+    else:
+        gptm = G[order][:ngptm]
+    types = []
+    maxlm = 0
+    for it in range(ntype):
+        nindx = np.full(lcut + 1, nrad, np.int32)
+        nindxm = np.full(lcutm + 1, nradm, np.int32)
+        d = integral_dims(lcut, nindx, lcutm, nindxm)
+        integ = np.asfortranarray(g.standard_normal(d) / 10)
+        types.append(dict(lcut=lcut, nindx=nindx, lcutm=lcutm, nindxm=nindxm, neq=1, integral=integ))
+        maxlm = max(maxlm, int(sum((2 * l + 1) * nindx[l] for l in range(lcut + 1))))
+    ncent = sum(T["neq"] for T in types)
+    cent = g.uniform(0, 1, (ncent, 3))
+    cmt = np.asfortranarray((g.standard_normal((maxlm, ncent, nband, nkpt)) +
+                             1j * g.standard_normal((maxlm, ncent, nband, nkpt))) / np.sqrt(maxlm))
+    return ProductSystem(nkpt, kpt, q, k1, k2, g2, nband, gpt, ngpt, cpw,
+                         np.ascontiguousarray(gptm, np.int32), tuple(nfft), 97.3, types, cent, cmt)
+
+
 def weights_none(e_occ, e_unocc, omega, nkpt=None):
     """Closed-form HILBERT NONE weights (n, nw), used by host-side callers.
 

@@ -35,7 +35,8 @@ def test_golden_tiny_fp64(ctx):
     np.testing.assert_allclose(got.T, g["chi_packed"], rtol=0, atol=1e-14)
 
 
-@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, TOL_FP64), (spexhip.PREC_SPLIT_FP32, TOL_SPLIT)])
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, TOL_FP64), (spexhip.PREC_SPLIT_FP32, TOL_SPLIT),
+                                           (spexhip.PREC_OZAKI_INT8, TOL_SPLIT)])
 @pytest.mark.parametrize("colmajor", [False, True])
 def test_cpu_case_vs_oracle(ctx, precision, tol, colmajor):
     """The CPU reference case: Nk=8, 8/56 bands, Nb=256, 6 frequencies."""
@@ -54,7 +55,9 @@ def test_ragged_shapes(ctx, shape):
     c = synth.chi0_case(nk=nk, nocc=nocc, nunocc=nunocc, nb=nb, nw=3, seed=sum(shape))
     w = oracle.weights_none(c.e_occ, c.e_unocc, c.omega)
     ref = oracle.chi0_pairsum(c.M, w)
-    for prec, tol in ((spexhip.PREC_FP64, TOL_FP64), (spexhip.PREC_SPLIT_FP32, 1e-7)):
+    # split-FP32 at tiny K: the FP32 accumulation error is not averaged down
+    for prec, tol in ((spexhip.PREC_FP64, TOL_FP64), (spexhip.PREC_SPLIT_FP32, 3e-7),
+                      (spexhip.PREC_OZAKI_INT8, TOL_SPLIT)):
         got = run_chi0(ctx, c.M, w, prec)
         mx, fro = normwise(got, ref)
         assert fro < tol, (prec, mx, fro)
@@ -135,3 +138,23 @@ def test_budget_admission():
         c.alloc(1 << 20).free()
     finally:
         c.close()
+
+
+def test_ozaki_rejects_mixed_sign_weights(ctx):
+    c = synth.chi0_case(nk=1, nocc=2, nunocc=8, nb=20, nw=2, seed=3)
+    w = np.array(oracle.weights_none(c.e_occ, c.e_unocc, c.omega))
+    w[3, 1] = 0.1  # a positive weight among negative ones
+    with pytest.raises(spexhip.SpexError) as e:
+        run_chi0(ctx, c.M, w, spexhip.PREC_OZAKI_INT8)
+    assert e.value.code == spexhip.E_INVALID
+
+
+def test_ozaki_positive_weights_and_large_k(ctx):
+    """All-positive weights (sign +1) and K spanning several int32 flushes."""
+    c = synth.chi0_case(nk=4, nocc=8, nunocc=300, nb=72, nw=2, seed=4)  # n = 9600 > 4096
+    w = -np.array(oracle.weights_none(c.e_occ, c.e_unocc, c.omega))
+    ref = oracle.chi0_herk(c.M, np.asfortranarray(w))
+    got = run_chi0(ctx, c.M, np.asfortranarray(w), spexhip.PREC_OZAKI_INT8)
+    mx, fro = normwise(got, ref)
+    print(f"ozaki +w large K: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < TOL_SPLIT
