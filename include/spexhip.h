@@ -116,6 +116,42 @@ int spexhip_chi0_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev,
                      int precision, double alpha, double beta, void *chi_dev,
                      int ldchi);
 
+/* ---- grouped chi0 over many q-points (persistent launch) ---------------- */
+/* One q-point's pair sum (susceptibility.f:1159-1251) as a problem of a
+ * grouped launch: M_dev transition-major (t*ldm + I), wght_dev (n x nw, t
+ * fastest), chi_dev full column-major per frequency;
+ * chi = beta*chi + alpha*sum_t w M conj(M).                               */
+typedef struct {
+  long n;
+  int nb, nw;
+  const void *M_dev;
+  long ldm;
+  const void *wght_dev;
+  void *chi_dev;
+  int ldchi;
+  double alpha, beta;
+} spexhip_chi0_problem;
+typedef struct spexhip_group spexhip_group;
+
+/* Tile configurations of the grouped kernel: TM x TM output tiles, KC-deep
+ * K slices, persistent workgroups per CU.  *count in: capacity, out: total. */
+int spexhip_chi0_configs(int *count, int *tm, int *kc, int *per_cu);
+/* Build the device work list once (problems ordered longest first); launch
+ * it any number of times (graph-capturable); destroy.  precision FP64 or
+ * SPLIT_FP32; config < 0 selects configuration 0.                           */
+int spexhip_group_create(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *probs,
+                         int precision, int config, spexhip_group **group);
+int spexhip_group_launch(spexhip_ctx *ctx, const spexhip_group *group);
+int spexhip_group_destroy(spexhip_ctx *ctx, spexhip_group *group);
+/* create + launch + destroy */
+int spexhip_chi0_grouped_dev(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *probs,
+                             int precision, int config);
+/* Autotuner: times every configuration on this batch (reps timed launches
+ * after one warm-up; outputs overwritten) and returns the fastest in *best;
+ * ms (optional, capacity = config count) receives the per-launch times.     */
+int spexhip_chi0_autotune(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *probs,
+                          int precision, int reps, int *best, float *ms);
+
 /* Pack the upper triangle of a device chi/eps stack into the reference
  * packed layout (susceptibility.f:1210) on the host; accumulate!=0 adds
  * into packed_host like susceph(ij,ifrq) = susceph(ij,ifrq) + ...        */
@@ -206,6 +242,56 @@ int spexhip_wavefproducts2_mt(spexhip_ctx *ctx, const spexhip_plan *plan, const 
 int spexhip_wavefproducts2_pw(spexhip_ctx *ctx, const spexhip_plan *plan, const double *cpw,
                               int nband, int nk, const int *k1, const int *k2, const int *g2,
                               int b1lo, int nocc, int b2lo, int nunocc, double *cprod);
+
+/* ---- per-q pipeline ---------------------------------------------------- */
+/* One q-point of correlation.f:404-747: products (susceptibility.f:756-832)
+ * -> HILBERT NONE weights (:949-975) -> chi0 pair sum (:1159-1251) ->
+ * eps = 1 - v^1/2 chi0 v^1/2 -> eps^-1 (correlation.f:612-747), on its own
+ * stream, every buffer allocated at creation.  A launch only enqueues
+ * kernels, so it can be captured into a HIP graph once and replayed.      */
+typedef struct spexhip_qpipe spexhip_qpipe;
+typedef struct {
+  int nk, nocc, nunocc;      /* packet: n = nk*nocc*nunocc transitions       */
+  int nb, nw;                /* mixed-basis size, imaginary frequencies (<=16) */
+  int precision;             /* SPEXHIP_PREC_*                                */
+  int vkind;                 /* 0: vhalf = sqrt(coul) vector; 1: dense        */
+  double alpha;              /* spin factor on chi0 (2 for nspin=1)           */
+  int ozaki_group;           /* frequencies sliced together (0: all)          */
+  /* products: either given ...                                              */
+  const void *M_dev;         /* transition-major n x ldm, or NULL             */
+  long ldm;
+  /* ... or computed from plans (MT columns first, then IPWs)                */
+  const spexhip_plan *mt_plan, *pw_plan;
+  const void *cmt_dev;       /* cmt(maxlmindx,ncent,nband,nkpt)               */
+  int ncent, nband, nkpt;
+  double q[3];
+  const void *cpw_dev;       /* cpw(maxgpt,nband,nkpt)                        */
+  const int *k1, *k2, *g2;   /* host: nk, nk, nk*3 (copied at creation)       */
+  int b1lo, b2lo;
+  /* weights and dielectric inputs (device)                                  */
+  const void *e_occ_dev, *e_unocc_dev, *nkpts_dev;
+  int nkpt_w, nsym1;
+  const void *omega_dev;
+  const void *vhalf_dev;
+} spexhip_qdesc;
+
+/* HBM bytes a pipeline of this shape allocates (admission by footprint). */
+int spexhip_qpipe_footprint(const spexhip_qdesc *desc, size_t *bytes);
+int spexhip_qpipe_create(spexhip_ctx *ctx, const spexhip_qdesc *desc, spexhip_qpipe **qp);
+/* Enqueue one run (asynchronous).  use_graph: capture on the first call,
+ * replay the graph afterwards.                                             */
+int spexhip_qpipe_launch(spexhip_ctx *ctx, spexhip_qpipe *qp, int use_graph);
+/* SPEXHIP_OK when the last launch finished, SPEXHIP_E_AGAIN while running. */
+int spexhip_qpipe_query(spexhip_ctx *ctx, spexhip_qpipe *qp);
+/* Wait for the last launch; ms = its device time.  Reports E_NOTPD or
+ * E_INVALID (mixed-sign weights in the Ozaki form) raised during the run. */
+int spexhip_qpipe_wait(spexhip_ctx *ctx, spexhip_qpipe *qp, float *ms);
+/* Packed-upper chi0 and/or eps^-1 (nb(nb+1)/2 x nw) to the host; NULL skips. */
+int spexhip_qpipe_result(spexhip_ctx *ctx, spexhip_qpipe *qp, double *chi_packed,
+                         double *einv_packed);
+int spexhip_qpipe_buffers(spexhip_ctx *ctx, spexhip_qpipe *qp, void **M_dev, void **chi_dev,
+                          void **einv_dev);
+int spexhip_qpipe_destroy(spexhip_ctx *ctx, spexhip_qpipe *qp);
 
 /* ---- dielectric step and inverse (FP64) -------------------------------- */
 /* eps = 1 - v^1/2 chi v^1/2 on nw full device matrices, correlation.f:

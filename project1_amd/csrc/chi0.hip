@@ -13,316 +13,19 @@
 //     FP64 accumulators after every K-chunk of 16 transitions.
 #include "common.hpp"
 
+#include "chi0_tile.hpp"
+
 namespace {
 
-constexpr int TM = 64;    // output tile edge (complex elements)
-constexpr int KC = 16;    // transitions per pipeline stage
-constexpr int NT = 256;   // threads per workgroup (4 waves, 2x2 layout)
+using chi0t::Lds;
+using chi0t::Problem;
 
-__device__ inline void tri_index(int x, int &bi, int &bj) {
-  int j = (int)((sqrtf(8.f * (float)x + 1.f) - 1.f) * 0.5f);
-  while ((j + 1) * (j + 2) / 2 <= x) j++;
-  while (j * (j + 1) / 2 > x) j--;
-  bj = j;
-  bi = x - j * (j + 1) / 2;
-}
-
-// Element e = tid + NT*r of a KC x TM stage; mapping keeps global loads
-// coalesced for either operand layout.
-template <bool COLMAJOR>
-__device__ inline void stage_coords(int e, int &k, int &i) {
-  if (COLMAJOR) {
-    k = e % KC;
-    i = e / KC;
-  } else {
-    k = e / TM;
-    i = e % TM;
-  }
-}
-
-template <bool COLMAJOR>
-__device__ inline void load_stage(const double2 *__restrict__ M, long ldm, long n,
-                                  int nb, long t0, int c0, int tid, double2 v[4]) {
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    int k, i;
-    stage_coords<COLMAJOR>(tid + NT * r, k, i);
-    long t = t0 + k;
-    int c = c0 + i;
-    double2 z = make_double2(0.0, 0.0);
-    if (t < n && c < nb) z = COLMAJOR ? M[t + ldm * (long)c] : M[t * ldm + c];
-    v[r] = z;
-  }
-}
-
-// ---------------------------------------------------------------- FP64 --
-template <bool COLMAJOR>
-__global__ __launch_bounds__(NT) void chi0_f64_kernel(
-    long n, int nb, const double2 *__restrict__ M, long ldm,
-    const double *__restrict__ wght, int nw, double2 *__restrict__ chi,
-    long ldc, double alpha, double beta) {
-  __shared__ double Lr[2][KC][TM], Li[2][KC][TM], Rr[2][KC][TM], Ri[2][KC][TM];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
+template <int TM, int KC, bool COLMAJOR, bool SPLIT>
+__global__ __launch_bounds__(chi0t::NT) void chi0_kernel(Problem P) {
+  __shared__ Lds<TM, KC, SPLIT> s;
   int bi, bj;
-  tri_index(blockIdx.x, bi, bj);
-  const int w = blockIdx.y;
-  const int I0 = bi * TM, J0 = bj * TM;
-  const double *D = wght + (long)w * n;
-
-  d4_t accr[2][2], acci[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      accr[a][b] = d4_t{0, 0, 0, 0};
-      acci[a][b] = d4_t{0, 0, 0, 0};
-    }
-
-  double2 va[4], vb[4];
-  double dv[4];
-  const long nst = (n + KC - 1) / KC;
-
-  auto fetch = [&](long s) {
-    long t0 = s * KC;
-    load_stage<COLMAJOR>(M, ldm, n, nb, t0, I0, tid, va);
-    load_stage<COLMAJOR>(M, ldm, n, nb, t0, J0, tid, vb);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int k, i;
-      stage_coords<COLMAJOR>(tid + NT * r, k, i);
-      long t = t0 + k;
-      dv[r] = t < n ? D[t] : 0.0;
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int k, i;
-      stage_coords<COLMAJOR>(tid + NT * r, k, i);
-      Lr[buf][k][i] = dv[r] * va[r].x;
-      Li[buf][k][i] = dv[r] * va[r].y;
-      Rr[buf][k][i] = vb[r].x;
-      Ri[buf][k][i] = vb[r].y;
-    }
-  };
-
-  fetch(0);
-  store(0);
-  __syncthreads();
-  const int lr = lane & 15, lk = lane >> 4;
-  for (long s = 0; s < nst; s++) {
-    const int buf = s & 1;
-    if (s + 1 < nst) fetch(s + 1);
-#pragma unroll
-    for (int kk = 0; kk < KC / 4; kk++) {
-      const int k = kk * 4 + lk;
-      double ar[2], ai[2], br[2], bim[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        ar[a] = Lr[buf][k][wr * 32 + a * 16 + lr];
-        ai[a] = Li[buf][k][wr * 32 + a * 16 + lr];
-        br[a] = Rr[buf][k][wc * 32 + a * 16 + lr];
-        bim[a] = Ri[buf][k][wc * 32 + a * 16 + lr];
-      }
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          // (ar + i ai)(br - i bim) = ar br + ai bim + i(ai br - ar bim)
-          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], accr[a][b], 0, 0, 0);
-          accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], bim[b], accr[a][b], 0, 0, 0);
-          acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
-          acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar[a], bim[b], acci[a][b], 0, 0, 0);
-        }
-    }
-    if (s + 1 < nst) store(buf ^ 1);
-    __syncthreads();
-  }
-
-  // v_mfma_f64_16x16x4f64 result layout: lane holds column j = lane%16 and
-  // rows i = lane/16 + 4*r (r = 0..3).
-  double2 *C = chi + (long)w * ldc * nb;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int J = J0 + wc * 32 + b * 16 + lr;
-      if (J >= nb) continue;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int I = I0 + wr * 32 + a * 16 + lk + 4 * r;
-        if (I >= nb) continue;
-        double2 *p = C + I + ldc * (long)J;
-        double2 o = make_double2(alpha * accr[a][b][r], alpha * acci[a][b][r]);
-        if (beta != 0.0) {
-          double2 old = *p;
-          o.x += beta * old.x;
-          o.y += beta * old.y;
-        }
-        *p = o;
-      }
-    }
-}
-
-// ------------------------------------------------------------ split FP32 --
-__device__ inline void split2(double x, float &hi, float &lo) {
-  hi = (float)x;
-  lo = (float)(x - (double)hi);
-}
-
-template <bool COLMAJOR>
-__global__ __launch_bounds__(NT) void chi0_split_kernel(
-    long n, int nb, const double2 *__restrict__ M, long ldm,
-    const double *__restrict__ wght, int nw, double2 *__restrict__ chi,
-    long ldc, double alpha, double beta) {
-  // hi/lo planes of L (weighted left operand) and R (right operand)
-  __shared__ float Lrh[2][KC][TM], Lrl[2][KC][TM], Lih[2][KC][TM], Lil[2][KC][TM];
-  __shared__ float Rrh[2][KC][TM], Rrl[2][KC][TM], Rih[2][KC][TM], Ril[2][KC][TM];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
-  int bi, bj;
-  tri_index(blockIdx.x, bi, bj);
-  const int w = blockIdx.y;
-  const int I0 = bi * TM, J0 = bj * TM;
-  const double *D = wght + (long)w * n;
-
-  d4_t accr[2][2], acci[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      accr[a][b] = d4_t{0, 0, 0, 0};
-      acci[a][b] = d4_t{0, 0, 0, 0};
-    }
-
-  double2 va[4], vb[4];
-  double dv[4];
-  const long nst = (n + KC - 1) / KC;
-
-  auto fetch = [&](long s) {
-    long t0 = s * KC;
-    load_stage<COLMAJOR>(M, ldm, n, nb, t0, I0, tid, va);
-    load_stage<COLMAJOR>(M, ldm, n, nb, t0, J0, tid, vb);
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int k, i;
-      stage_coords<COLMAJOR>(tid + NT * r, k, i);
-      long t = t0 + k;
-      dv[r] = t < n ? D[t] : 0.0;
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      int k, i;
-      stage_coords<COLMAJOR>(tid + NT * r, k, i);
-      float h, l;
-      split2(dv[r] * va[r].x, h, l);
-      Lrh[buf][k][i] = h;
-      Lrl[buf][k][i] = l;
-      split2(dv[r] * va[r].y, h, l);
-      Lih[buf][k][i] = h;
-      Lil[buf][k][i] = l;
-      split2(vb[r].x, h, l);
-      Rrh[buf][k][i] = h;
-      Rrl[buf][k][i] = l;
-      split2(vb[r].y, h, l);
-      Rih[buf][k][i] = h;
-      Ril[buf][k][i] = l;
-    }
-  };
-
-  fetch(0);
-  store(0);
-  __syncthreads();
-  const int lr = lane & 15, lk = lane >> 4;
-  for (long s = 0; s < nst; s++) {
-    const int buf = s & 1;
-    if (s + 1 < nst) fetch(s + 1);
-    f4_t hr[2][2], hi_[2][2], xr[2][2], xi[2][2];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++) {
-        hr[a][b] = f4_t{0, 0, 0, 0};
-        hi_[a][b] = f4_t{0, 0, 0, 0};
-        xr[a][b] = f4_t{0, 0, 0, 0};
-        xi[a][b] = f4_t{0, 0, 0, 0};
-      }
-#pragma unroll
-    for (int kk = 0; kk < KC / 4; kk++) {
-      const int k = kk * 4 + lk;
-      float arh[2], arl[2], aih[2], ail[2], brh[2], brl[2], bih[2], bil[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int ia = wr * 32 + a * 16 + lr, jb = wc * 32 + a * 16 + lr;
-        arh[a] = Lrh[buf][k][ia];
-        arl[a] = Lrl[buf][k][ia];
-        aih[a] = Lih[buf][k][ia];
-        ail[a] = Lil[buf][k][ia];
-        brh[a] = Rrh[buf][k][jb];
-        brl[a] = Rrl[buf][k][jb];
-        bih[a] = Rih[buf][k][jb];
-        bil[a] = Ril[buf][k][jb];
-      }
-#pragma unroll
-      for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          // hi*hi
-          hr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(arh[a], brh[b], hr[a][b], 0, 0, 0);
-          hr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], bih[b], hr[a][b], 0, 0, 0);
-          hi_[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], brh[b], hi_[a][b], 0, 0, 0);
-          hi_[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arh[a], bih[b], hi_[a][b], 0, 0, 0);
-          // hi*lo + lo*hi
-          xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(arh[a], brl[b], xr[a][b], 0, 0, 0);
-          xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(arl[a], brh[b], xr[a][b], 0, 0, 0);
-          xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], bil[b], xr[a][b], 0, 0, 0);
-          xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ail[a], bih[b], xr[a][b], 0, 0, 0);
-          xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], brl[b], xi[a][b], 0, 0, 0);
-          xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ail[a], brh[b], xi[a][b], 0, 0, 0);
-          xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arh[a], bil[b], xi[a][b], 0, 0, 0);
-          xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arl[a], bih[b], xi[a][b], 0, 0, 0);
-        }
-    }
-    // FP64 correction pass for this K-chunk
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          accr[a][b][r] += (double)hr[a][b][r] + (double)xr[a][b][r];
-          acci[a][b][r] += (double)hi_[a][b][r] + (double)xi[a][b][r];
-        }
-    if (s + 1 < nst) store(buf ^ 1);
-    __syncthreads();
-  }
-
-  // v_mfma_f32_16x16x4f32 result layout: lane holds column j = lane%16 and
-  // rows i = 4*(lane/16) + r.
-  double2 *C = chi + (long)w * ldc * nb;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int J = J0 + wc * 32 + b * 16 + lr;
-      if (J >= nb) continue;
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int I = I0 + wr * 32 + a * 16 + 4 * lk + r;
-        if (I >= nb) continue;
-        double2 *p = C + I + ldc * (long)J;
-        double2 o = make_double2(alpha * accr[a][b][r], alpha * acci[a][b][r]);
-        if (beta != 0.0) {
-          double2 old = *p;
-          o.x += beta * old.x;
-          o.y += beta * old.y;
-        }
-        *p = o;
-      }
-    }
+  chi0t::tri_index(blockIdx.x, bi, bj);
+  chi0t::tile<TM, KC, COLMAJOR, SPLIT>(P, bi, bj, blockIdx.y, s);
 }
 
 // ------------------------------------------------------------ weights --
@@ -384,6 +87,19 @@ __global__ void unpack_kernel(int nb, int nw, const double2 *__restrict__ P,
 
 }  // namespace
 
+template <int TM, int KC>
+static void launch_cfg(hipStream_t st, const Problem &P, int colmajor, int split) {
+  const int nbt = (P.nb + TM - 1) / TM;
+  dim3 grid(nbt * (nbt + 1) / 2, P.nw), block(chi0t::NT);
+  if (split) {
+    if (colmajor) hipLaunchKernelGGL((chi0_kernel<TM, KC, true, true>), grid, block, 0, st, P);
+    else hipLaunchKernelGGL((chi0_kernel<TM, KC, false, true>), grid, block, 0, st, P);
+  } else {
+    if (colmajor) hipLaunchKernelGGL((chi0_kernel<TM, KC, true, false>), grid, block, 0, st, P);
+    else hipLaunchKernelGGL((chi0_kernel<TM, KC, false, false>), grid, block, 0, st, P);
+  }
+}
+
 int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
                     const void *M, long ldm, int colmajor, int nw,
                     const void *wght, int precision, double alpha, double beta,
@@ -397,26 +113,33 @@ int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
   if (precision == SPEXHIP_PREC_OZAKI_INT8)
     return spx_chi0_ozaki(ctx, st, n, nb, M, ldm, colmajor, nw, wght, alpha, beta, chi, ldchi,
                           ctx->ozaki_ws_limit);
-  const int nbt = (nb + TM - 1) / TM;
-  dim3 grid(nbt * (nbt + 1) / 2, nw), block(NT);
-  if (n == 0) {
-    // empty transition set: chi = beta*chi
-    alpha = 0.0;
-  }
-  const double2 *Mp = (const double2 *)M;
-  const double *W = (const double *)wght;
-  double2 *C = (double2 *)chi;
-  if (precision == SPEXHIP_PREC_FP64) {
-    if (colmajor)
-      hipLaunchKernelGGL(chi0_f64_kernel<true>, grid, block, 0, st, n, nb, Mp, ldm, W, nw, C, (long)ldchi, alpha, beta);
-    else
-      hipLaunchKernelGGL(chi0_f64_kernel<false>, grid, block, 0, st, n, nb, Mp, ldm, W, nw, C, (long)ldchi, alpha, beta);
-  } else {
-    if (colmajor)
-      hipLaunchKernelGGL(chi0_split_kernel<true>, grid, block, 0, st, n, nb, Mp, ldm, W, nw, C, (long)ldchi, alpha, beta);
-    else
-      hipLaunchKernelGGL(chi0_split_kernel<false>, grid, block, 0, st, n, nb, Mp, ldm, W, nw, C, (long)ldchi, alpha, beta);
-  }
+  Problem P{n, nb, nw, (const double2 *)M, ldm, (const double *)wght, (double2 *)chi,
+            (long)ldchi, n == 0 ? 0.0 : alpha, beta};
+  const int split = precision == SPEXHIP_PREC_SPLIT_FP32;
+  launch_cfg<64, 16>(st, P, colmajor, split);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
+int spx_weights_launch(spexhip_ctx *ctx, hipStream_t st, int nk, int nocc, int nunocc,
+                       const void *e_occ, const void *e_unocc, int nkpt, int nsym1,
+                       const void *nkpts, int nw, const void *omega, void *wght) {
+  const long n = (long)nk * nocc * nunocc;
+  if (n == 0) return SPEXHIP_OK;
+  const int bs = 256;
+  hipLaunchKernelGGL(weights_none_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, nk, nocc,
+                     nunocc, (const double *)e_occ, (const double *)e_unocc, nkpt, nsym1,
+                     (const int *)nkpts, nw, (const double *)omega, (double *)wght);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
+int spx_pack_upper(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, const void *X, long ldx,
+                   void *P) {
+  const long npack = (long)nb * (nb + 1) / 2;
+  const int bs = 256;
+  hipLaunchKernelGGL(pack_upper_kernel, dim3((npack * nw + bs - 1) / bs), dim3(bs), 0, st, nb, nw,
+                     (const double2 *)X, ldx, (double2 *)P);
   SPX_LAUNCHED(ctx);
   return SPEXHIP_OK;
 }

@@ -6,24 +6,30 @@
 // When D(.,w) has one sign (always on the imaginary axis for HILBERT NONE,
 // D = -2d/(w^2+d^2)/nkpt < 0), write D = sgn |D| and B(t,I) = sqrt|D(t,w)|
 // M(t,I), so chi = sgn * B^T conj(B) -- a Hermitian rank-k update of one
-// matrix per frequency, with the weight dynamic range halved on each side.
-// Each real component of B is split per column I with scale
+// matrix per frequency, with the weight dynamic range split between the
+// two sides.  Each real component of B is split per column I with scale
 // s(I,w) = max_t max(|Re B|, |Im B|):
 //     B/s = sum_{p=1..4} a_p w_p + r,  w_p = 1/(127*254^(p-1)),
-//     a_1 in [-127,127], a_2..a_4 in [-127,127], |r| <= 0.5*w_4,
+//     a_p in [-127,127], |r| <= 0.5*w_4,
 // (rounding to nearest at every level), so the representation error is
 // below 2.4e-10 * s.  Products of slices with p+q <= 5 are accumulated
 // exactly in int32 on v_mfma_i32_16x16x64_i8 -- one int32 accumulator per
 // level l = p+q, since w_p w_q depends on p+q only -- and flushed into FP64
 // accumulators (weight w_l) every 4096 transitions, far below the int32
-// overflow bound (4 pairs * 2 products * 127^2 * 4096 < 2^31).
-// Re chi = sum Br Br + Bi Bi,  Im chi = sum Bi Br - Br Bi (negated Br on
-// the row side), then chi = sgn * s_I s_J * (...).
+// overflow bound (4 pairs * 2 products * 127^2 * 4096 < 2^31):
+//     Re chi = sum Br_I Br_J + Bi_I Bi_J,  Im chi = sum Bi_I Br_J - Br_I Bi_J,
+// then chi = sgn * s_I s_J * (...).
 //
-// Layout of the slice array (one frequency):
-//   S[chunk c][part][slice p][I (nbp)][64 bytes of t = 64c .. 64c+63],
-//   part 0 = Re, 1 = Im;  so a 64-transition chunk of a 64-column panel is
-//   8 contiguous 4 KB blocks.
+// Slice layout (one frequency), "MFMA-ready": for 64-transition chunk c,
+// plane pl = part*4 + slice (part 0 = Re, 1 = Im) and 16-column block ib,
+// a 1 KB block holding lane l's operand bytes at offset 16*l, lane l =
+// row (l%16) of the block and transitions 64c + 16*(l/16) .. +15 -- the
+// operand layout of v_mfma_i32_16x16x64_i8.  The rank-k kernel copies
+// these blocks straight into LDS with global_load_lds_dwordx4 (one block
+// per wave-instruction) and reads each operand with one conflict-free
+// 16-byte LDS load per lane.
+#include <algorithm>
+
 #include "common.hpp"
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
@@ -34,26 +40,26 @@ constexpr int NS = 4;      // slices
 constexpr int TK = 64;     // transitions per chunk (MFMA K)
 constexpr int TT = 64;     // output tile edge
 constexpr int NT = 256;
-constexpr int ROWB = 80;   // LDS row stride in bytes (64 + 16 pad: no bank conflicts)
 constexpr int FLUSH = 64;  // chunks between int32 -> FP64 flushes
+constexpr int NPL = 2 * NS;  // planes per chunk
 
-__device__ inline unsigned neg_bytes(unsigned w) {
-  // per-byte two's complement negation without cross-byte carries
-  const unsigned nw = ~w;
-  return ((nw & 0x7F7F7F7Fu) + 0x01010101u) ^ (nw & 0x80808080u);
+__device__ inline long blk_off(long c, int pl, int ib, int nib) {
+  return (((long)c * NPL + pl) * nib + ib) * 1024L;
 }
 
-// sqd[w][t] = sqrt|D(t,w)|, and sign flags per frequency (bit0: D>0 seen,
-// bit1: D<0 seen).
-__global__ void ozk_sqd_kernel(long n, int nw, const double *__restrict__ D,
-                               double *__restrict__ sqd, int *__restrict__ flags) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n * nw) return;
-  const double d = D[g];
-  sqd[g] = sqrt(fabs(d));
-  const int w = (int)(g / n);
-  if (d > 0) atomicOr(flags + w, 1);
-  if (d < 0) atomicOr(flags + w, 2);
+// sqd[w][t] = sqrt|D(t,w)| and sign flags per frequency (bit0: D>0 seen,
+// bit1: D<0 seen); one atomic per wave.
+__global__ void ozk_sqd_kernel(long n, const double *__restrict__ D, double *__restrict__ sqd,
+                               int *__restrict__ flags) {
+  const int w = blockIdx.y;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0;
+  if (t < n) {
+    d = D[(long)w * n + t];
+    sqd[(long)w * n + t] = sqrt(fabs(d));
+  }
+  const unsigned long long pos = __ballot(d > 0.0), neg = __ballot(d < 0.0);
+  if ((threadIdx.x & 63) == 0 && (pos || neg)) atomicOr(flags + w, (pos ? 1 : 0) | (neg ? 2 : 0));
 }
 
 // scale[w][I] = max_t sqd[w][t] * max(|Re M|, |Im M|), via atomicMax on the
@@ -81,29 +87,36 @@ __global__ void ozk_scale_kernel(long n, int nb, const double2 *__restrict__ M, 
     if (w < nw) atomicMax(scale + (long)w * nb + I, (unsigned long long)__double_as_longlong(m[w]));
 }
 
-__device__ inline void slice4(double u, signed char a[NS]) {
+__device__ inline void slice4(double u, int a[NS]) {
   // u in [-127, 127]
   double r = u;
 #pragma unroll
   for (int p = 0; p < NS; p++) {
     const double q = rint(r);
-    a[p] = (signed char)(int)q;
+    a[p] = (int)q;
     r = (r - q) * 254.0;
   }
 }
 
-// Slices for frequencies w0 .. w0+nwg-1 into S (nwg consecutive arrays of
-// sbytes each).  One thread: one column I, 16 consecutive transitions.
+// Slices of frequencies w0 .. w0+nwg-1 (nwg arrays of sbytes each).  One
+// wave per (chunk, 16-column block); lane l = row l%16, transitions
+// 64c + 16(l/16) + 0..15.
 template <bool COLMAJOR>
-__global__ void ozk_slice_kernel(long n, int nb, int nbp, const double2 *__restrict__ M, long ldm,
-                                 int w0, int nwg, const double *__restrict__ sqd,
-                                 const double *__restrict__ scale, unsigned char *__restrict__ S,
-                                 long sbytes) {
-  const int I = blockIdx.y * blockDim.x + threadIdx.x;
-  if (I >= nbp) return;
-  const long c = blockIdx.x >> 2;
-  const int g16 = (blockIdx.x & 3) * 16;
-  const long tb = c * TK + g16;
+__global__ __launch_bounds__(NT) void ozk_slice_kernel(long n, int nb, int nbp,
+                                                       const double2 *__restrict__ M, long ldm,
+                                                       int w0, int nwg,
+                                                       const double *__restrict__ sqd,
+                                                       const double *__restrict__ scale,
+                                                       unsigned char *__restrict__ S, long sbytes,
+                                                       long nwaves) {
+  const long wid = (long)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  if (wid >= nwaves) return;
+  const int nib = nbp / 16;
+  const long c = wid / nib;
+  const int ib = (int)(wid % nib);
+  const int l = threadIdx.x & 63, row = l & 15, g = l >> 4;
+  const int I = ib * 16 + row;
+  const long tb = c * TK + 16 * g;
   double2 v[16];
 #pragma unroll
   for (int j = 0; j < 16; j++) {
@@ -115,31 +128,33 @@ __global__ void ozk_slice_kernel(long n, int nb, int nbp, const double2 *__restr
     const int w = w0 + wg;
     const double s = I < nb ? scale[(long)w * nb + I] : 0.0;
     const double inv = s > 0.0 ? 127.0 / s : 0.0;
-    unsigned char out[2][NS][16];
+    unsigned int q[NPL][4];
+#pragma unroll
+    for (int pl = 0; pl < NPL; pl++)
+#pragma unroll
+      for (int d = 0; d < 4; d++) q[pl][d] = 0u;
 #pragma unroll
     for (int j = 0; j < 16; j++) {
       const long t = tb + j;
       const double f = t < n ? sqd[(long)w * n + t] * inv : 0.0;
-      signed char a[NS];
+      int a[NS];
       slice4(v[j].x * f, a);
 #pragma unroll
-      for (int p = 0; p < NS; p++) out[0][p][j] = (unsigned char)a[p];
+      for (int p = 0; p < NS; p++) q[p][j >> 2] |= ((unsigned)a[p] & 0xFFu) << (8 * (j & 3));
       slice4(v[j].y * f, a);
 #pragma unroll
-      for (int p = 0; p < NS; p++) out[1][p][j] = (unsigned char)a[p];
+      for (int p = 0; p < NS; p++) q[NS + p][j >> 2] |= ((unsigned)a[p] & 0xFFu) << (8 * (j & 3));
     }
-    unsigned char *base = S + (long)wg * sbytes + c * (2L * NS * nbp * TK);
+    unsigned char *base = S + (long)wg * sbytes;
 #pragma unroll
-    for (int part = 0; part < 2; part++)
-#pragma unroll
-      for (int p = 0; p < NS; p++) {
-        uint4 q;
-        q.x = out[part][p][0] | (out[part][p][1] << 8) | (out[part][p][2] << 16) | ((unsigned)out[part][p][3] << 24);
-        q.y = out[part][p][4] | (out[part][p][5] << 8) | (out[part][p][6] << 16) | ((unsigned)out[part][p][7] << 24);
-        q.z = out[part][p][8] | (out[part][p][9] << 8) | (out[part][p][10] << 16) | ((unsigned)out[part][p][11] << 24);
-        q.w = out[part][p][12] | (out[part][p][13] << 8) | (out[part][p][14] << 16) | ((unsigned)out[part][p][15] << 24);
-        *(uint4 *)(base + ((long)(part * NS + p) * nbp + I) * TK + g16) = q;
-      }
+    for (int pl = 0; pl < NPL; pl++) {
+      uint4 o;
+      o.x = q[pl][0];
+      o.y = q[pl][1];
+      o.z = q[pl][2];
+      o.w = q[pl][3];
+      *(uint4 *)(base + blk_off(c, pl, ib, nib) + 16 * l) = o;
+    }
   }
 }
 
@@ -151,62 +166,44 @@ __device__ inline void tri_index(int x, int &bi, int &bj) {
   bi = x - j * (j + 1) / 2;
 }
 
-// Rank-k update on int8 slices.  LDS per chunk: row panel (Re, Im, -Re) and
-// column panel (Re, Im), NS slices each, 64 rows of 64 bytes (stride ROWB).
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Rank-k update on int8 slices: 64x64 tile per workgroup, 4 waves (2x2) of
+// 32x32 (2x2 blocks of 16x16).  Per chunk the workgroup stages 8 planes x
+// 4 row blocks + 8 planes x 4 column blocks (64 KB) into one of two LDS
+// buffers; each wave issues 16 global_load_lds_dwordx4 (1 KB each).
 __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     long nchunk, int nb, int nbp, const unsigned char *__restrict__ S, long sbytes, int w0,
     const double *__restrict__ scale, const double *__restrict__ sgn, double2 *__restrict__ chi,
     long ldc, double alpha, double beta) {
-  __shared__ __attribute__((aligned(16))) unsigned char Ls[5 * NS * TT * ROWB];  // 100 KB
+  // [buf][panel][plane][block of 4][1 KB]
+  __shared__ __attribute__((aligned(1024))) unsigned char Ls[2 * 2 * NPL * 4 * 1024];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1, lr = lane & 15, lk = lane >> 4;
+  const int wr = wv >> 1, wc = wv & 1;
   int bi, bj;
   tri_index(blockIdx.x, bi, bj);
   const int wg = blockIdx.y, w = w0 + wg;
-  const int I0 = bi * TT, J0 = bj * TT;
+  const int nib = nbp / 16;
   const unsigned char *Sw = S + (long)wg * sbytes;
-  const long cstride = 2L * NS * nbp * TK;
 
-  // chunk loader: 8 planes x 2 panels x 64 rows x 64 B = 64 KB = 256 threads x 16 uint4
-  uint4 pre[16];
-  auto fetch = [&](long c) {
-    const unsigned char *base = Sw + c * cstride;
+  // wave wv loads the 16 blocks e = wv*16 .. wv*16+15 of the 64 per chunk:
+  // e -> panel = e/32, plane = (e/4)%8, block = e%4
+  auto issue = [&](long c, int buf) {
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-      const int e = tid + NT * r;      // 0 .. 4095 uint4 slots
-      const int panel = e >> 11;       // 0: rows (I), 1: cols (J)
-      const int q = e & 2047;
-      const int plane = q >> 8;        // part*NS + p
-      const int row = (q >> 2) & 63;
-      const int seg = q & 3;
-      const int col = (panel ? J0 : I0) + row;
-      pre[r] = *(const uint4 *)(base + ((long)plane * nbp + col) * TK + seg * 16);
+      const int e = wv * 16 + r;
+      const int panel = e >> 5, pl = (e >> 2) & 7, blk = e & 3;
+      const int ibk = (panel ? bj : bi) * 4 + blk;
+      const unsigned char *src = Sw + blk_off(c, pl, ibk, nib) + 16 * lane;
+      unsigned char *dst = Ls + ((((buf * 2 + panel) * NPL + pl) * 4 + blk) * 1024);
+      __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)dst, 16, 0, 0);
     }
   };
-  auto stash = [&]() {
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int e = tid + NT * r;
-      const int panel = e >> 11;
-      const int q = e & 2047;
-      const int plane = q >> 8;
-      const int row = (q >> 2) & 63;
-      const int seg = q & 3;
-      // planes 0..NS-1 Re, NS..2NS-1 Im; row panel also keeps -Re at 2NS..3NS-1
-      const int lp = panel ? 3 * NS + plane : plane;
-      *(uint4 *)(Ls + (lp * TT + row) * ROWB + seg * 16) = pre[r];
-      if (!panel && plane < NS) {
-        uint4 ng;
-        ng.x = neg_bytes(pre[r].x);
-        ng.y = neg_bytes(pre[r].y);
-        ng.z = neg_bytes(pre[r].z);
-        ng.w = neg_bytes(pre[r].w);
-        *(uint4 *)(Ls + ((2 * NS + plane) * TT + row) * ROWB + seg * 16) = ng;
-      }
-    }
+  auto opnd = [&](int buf, int panel, int pl, int blk) -> i4_t {
+    return *(const i4_t *)(Ls + ((((buf * 2 + panel) * NPL + pl) * 4 + blk) * 1024) + 16 * lane);
   };
 
-  i4_t ar[NS][2][2], ai[NS][2][2];  // level p+q = 2..5 stored at index p+q-2
+  i4_t re[NS][2][2], ia[NS][2][2], ib_[NS][2][2];  // level p+q-2 -> index
   d4_t fr[2][2], fi[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -216,8 +213,9 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
       fi[a][b] = d4_t{0, 0, 0, 0};
 #pragma unroll
       for (int l = 0; l < NS; l++) {
-        ar[l][a][b] = i4_t{0, 0, 0, 0};
-        ai[l][a][b] = i4_t{0, 0, 0, 0};
+        re[l][a][b] = i4_t{0, 0, 0, 0};
+        ia[l][a][b] = i4_t{0, 0, 0, 0};
+        ib_[l][a][b] = i4_t{0, 0, 0, 0};
       }
     }
   auto flush = [&]() {
@@ -234,8 +232,8 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
             double lw = 1.0 / (127.0 * 127.0);
 #pragma unroll
             for (int e = 0; e < l; e++) lw *= 1.0 / 254.0;
-            sr += lw * (double)ar[l][a][b][r];
-            si += lw * (double)ai[l][a][b][r];
+            sr += lw * (double)re[l][a][b][r];
+            si += lw * ((double)ia[l][a][b][r] - (double)ib_[l][a][b][r]);
           }
           fr[a][b][r] += sr;
           fi[a][b][r] += si;
@@ -246,60 +244,53 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
       for (int b = 0; b < 2; b++)
 #pragma unroll
         for (int l = 0; l < NS; l++) {
-          ar[l][a][b] = i4_t{0, 0, 0, 0};
-          ai[l][a][b] = i4_t{0, 0, 0, 0};
+          re[l][a][b] = i4_t{0, 0, 0, 0};
+          ia[l][a][b] = i4_t{0, 0, 0, 0};
+          ib_[l][a][b] = i4_t{0, 0, 0, 0};
         }
   };
 
-  fetch(0);
-  stash();
-  __syncthreads();
+  issue(0, 0);
   for (long c = 0; c < nchunk; c++) {
-    if (c + 1 < nchunk) fetch(c + 1);
-    // column operands for all slices: Re (planes 3NS+q) and Im (planes 4NS+q)
-    i4_t Br[NS][2], Bi[NS][2];
+    const int buf = c & 1;
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c have landed
+    __syncthreads();                // ... and every other wave's
+    if (c + 1 < nchunk) issue(c + 1, buf ^ 1);  // buffer buf^1 was last read before the barrier
 #pragma unroll
-    for (int q = 0; q < NS; q++)
+    for (int q = 0; q < NS; q++) {
+      i4_t Br[2], Bi[2];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        const int row = wc * 32 + b * 16 + lr;
-        Br[q][b] = *(const i4_t *)(Ls + ((3 * NS + q) * TT + row) * ROWB + lk * 16);
-        Bi[q][b] = *(const i4_t *)(Ls + ((4 * NS + q) * TT + row) * ROWB + lk * 16);
+        Br[b] = opnd(buf, 1, q, wc * 2 + b);
+        Bi[b] = opnd(buf, 1, NS + q, wc * 2 + b);
       }
 #pragma unroll
-    for (int p = 0; p < NS; p++) {
-      i4_t Ar[2], Ai[2], An[2];
-#pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int row = wr * 32 + a * 16 + lr;
-        Ar[a] = *(const i4_t *)(Ls + ((0 * NS + p) * TT + row) * ROWB + lk * 16);
-        Ai[a] = *(const i4_t *)(Ls + ((1 * NS + p) * TT + row) * ROWB + lk * 16);
-        An[a] = *(const i4_t *)(Ls + ((2 * NS + p) * TT + row) * ROWB + lk * 16);
-      }
-#pragma unroll
-      for (int q = 0; q < NS - p; q++) {
+      for (int p = 0; p < NS - q; p++) {
         const int l = p + q;  // slices p+1, q+1: level p+q+2 -> index p+q
+        i4_t Ar[2], Ai[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+          Ar[a] = opnd(buf, 0, p, wr * 2 + a);
+          Ai[a] = opnd(buf, 0, NS + p, wr * 2 + a);
+        }
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
           for (int b = 0; b < 2; b++) {
-            ar[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[q][b], ar[l][a][b], 0, 0, 0);
-            ar[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[q][b], ar[l][a][b], 0, 0, 0);
-            ai[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[q][b], ai[l][a][b], 0, 0, 0);
-            ai[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(An[a], Bi[q][b], ai[l][a][b], 0, 0, 0);
+            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[b], re[l][a][b], 0, 0, 0);
+            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[b], re[l][a][b], 0, 0, 0);
+            ia[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[b], ia[l][a][b], 0, 0, 0);
+            ib_[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Bi[b], ib_[l][a][b], 0, 0, 0);
           }
       }
     }
     if ((c + 1) % FLUSH == 0) flush();
-    __syncthreads();
-    if (c + 1 < nchunk) {
-      stash();
-      __syncthreads();
-    }
   }
   flush();
 
   // v_mfma_i32_16x16x64_i8 result layout: column j = lane%16, rows 4*(lane/16)+r.
+  const int lr = lane & 15, lk = lane >> 4;
+  const int I0 = bi * TT, J0 = bj * TT;
   double2 *C = chi + (long)w * ldc * nb;
   const double sg = sgn[w] * alpha;
 #pragma unroll
@@ -326,58 +317,79 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     }
 }
 
+// sgn[w] = +1 / -1 from the sign flags; a frequency with weights of both
+// signs gets sgn = 0 and raises *err (checked by the caller after the run).
+__global__ void ozk_sign_kernel(int nw, const int *__restrict__ flags, double *__restrict__ sgn,
+                                int *__restrict__ err) {
+  const int w = threadIdx.x;
+  if (w >= nw) return;
+  const int f = flags[w];
+  sgn[w] = f == 3 ? 0.0 : (f == 1 ? 1.0 : -1.0);
+  if (f == 3) err[0] = 1;
+}
+
+__global__ void ozk_zero_kernel(unsigned long long *__restrict__ p, long count) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < count) p[g] = 0ull;
+}
+
 }  // namespace
 
-// Workspace of the Ozaki form for a frequency group of nwg.
+// Slice bytes of one frequency.
 size_t spx_ozaki_slice_bytes(long n, int nb) {
   const long nchunk = (n + TK - 1) / TK;
   const int nbp = ((nb + TT - 1) / TT) * TT;
-  return (size_t)nchunk * 2 * NS * nbp * TK;
+  return (size_t)nchunk * NPL * nbp * TK;
 }
 
-// Returns SPEXHIP_E_INVALID (with a message) when some frequency has
-// weights of both signs; the caller then uses another precision.
-int spx_chi0_ozaki(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,
-                   int colmajor, int nw, const void *wght, double alpha, double beta, void *chi,
-                   int ldchi, size_t ws_limit) {
-  SPX_CHECK(ctx, nw <= 16, "chi0 ozaki: at most 16 frequencies per call (got %d)", nw);
+static size_t ozaki_fixed_bytes(long n, int nb, int nw) {
+  size_t fixed = 8 * (size_t)n * nw + 8 * (size_t)nb * nw + 8 * (size_t)nw + 4 * 16;
+  return (fixed + 1023) & ~(size_t)1023;
+}
+
+// Workspace of the Ozaki form when nwg frequencies are sliced at a time.
+size_t spx_ozaki_ws_bytes(long n, int nb, int nw, int nwg) {
+  return ozaki_fixed_bytes(n, nb, nw) + spx_ozaki_slice_bytes(n, nb) * (size_t)nwg;
+}
+
+// Asynchronous Ozaki chi0 on stream st with caller workspace ws (ws_bytes >=
+// spx_ozaki_ws_bytes(n, nb, nw, 1), ws 1 KB aligned).  No host
+// synchronisation and no memset nodes, so it may be captured in a graph.
+// *err_dev (device int, caller zeroed) becomes non-zero if some frequency
+// has weights of both signs.
+int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M,
+                         long ldm, int colmajor, int nw, const void *wght, double alpha,
+                         double beta, void *chi, int ldchi, void *ws, size_t ws_bytes,
+                         int *err_dev) {
+  SPX_CHECK(ctx, nw >= 1 && nw <= 16, "chi0 ozaki: 1..16 frequencies per call (got %d)", nw);
   if (n == 0) return spx_chi0_launch(ctx, st, 0, nb, M, ldm, colmajor, nw, wght,
                                      SPEXHIP_PREC_FP64, alpha, beta, chi, ldchi);
+  SPX_CHECK(ctx, ((uintptr_t)ws & 1023) == 0, "chi0 ozaki: workspace not 1 KB aligned");
   const long nchunk = (n + TK - 1) / TK;
   const int nbp = ((nb + TT - 1) / TT) * TT;
   const size_t sb = spx_ozaki_slice_bytes(n, nb);
-  // frequency group size bounded by the workspace limit
-  int nwg = nw;
-  while (nwg > 1 && sb * nwg > ws_limit) nwg--;
-  size_t fixed = 8 * (size_t)n * nw + 8 * (size_t)nb * nw + 8 * (size_t)nw + 4 * 16;
-  fixed = (fixed + 255) & ~(size_t)255;
-  void *ws;
-  int rc = spx_scratch(ctx, fixed + sb * nwg, &ws);
-  if (rc) return rc;
+  const size_t fixed = ozaki_fixed_bytes(n, nb, nw);
+  SPX_CHECK(ctx, ws_bytes >= fixed + sb, "chi0 ozaki: workspace too small");
+  const int nwg = (int)std::min<size_t>((size_t)nw, (ws_bytes - fixed) / sb);
   double *sqd = (double *)ws;
   double *scale = sqd + (size_t)n * nw;
   double *sgn = scale + (size_t)nb * nw;
   int *flags = (int *)(sgn + nw);
   unsigned char *S = (unsigned char *)ws + fixed;
-  SPX_HIP(ctx, hipMemsetAsync(flags, 0, 4 * 16, st));
-  SPX_HIP(ctx, hipMemsetAsync(scale, 0, 8 * (size_t)nb * nw, st));
-  {
-    const long tot = n * nw;
-    hipLaunchKernelGGL(ozk_sqd_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, n, nw,
-                       (const double *)wght, sqd, flags);
+  const bool timed = st == ctx->stream;
+  {  // zero the scale maxima and the sign flags (fill kernels: capture-safe)
+    const long cnt = (long)nb * nw;
+    hipLaunchKernelGGL(ozk_zero_kernel, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                       (unsigned long long *)scale, cnt);
+    SPX_LAUNCHED(ctx);
+    hipLaunchKernelGGL(ozk_zero_kernel, dim3(1), dim3(64), 0, st, (unsigned long long *)flags, 8L);
     SPX_LAUNCHED(ctx);
   }
-  int hflags[16];
-  SPX_HIP(ctx, hipMemcpyAsync(hflags, flags, 4 * nw, hipMemcpyDeviceToHost, st));
-  SPX_HIP(ctx, hipStreamSynchronize(st));
-  double hsgn[16];
-  for (int w = 0; w < nw; w++) {
-    if (hflags[w] == 3)
-      return spx_seterr(ctx, SPEXHIP_E_INVALID,
-                        "chi0 ozaki: weights of frequency %d have both signs", w);
-    hsgn[w] = hflags[w] == 1 ? 1.0 : -1.0;
-  }
-  SPX_HIP(ctx, hipMemcpyAsync(sgn, hsgn, 8 * nw, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(ozk_sqd_kernel, dim3((n + 255) / 256, nw), dim3(256), 0, st, n,
+                     (const double *)wght, sqd, flags);
+  SPX_LAUNCHED(ctx);
+  hipLaunchKernelGGL(ozk_sign_kernel, dim3(1), dim3(64), 0, st, nw, flags, sgn, err_dev);
+  SPX_LAUNCHED(ctx);
   {
     const long tchunk = 4096;
     dim3 grid((nb + 255) / 256, (n + tchunk - 1) / tchunk);
@@ -390,26 +402,53 @@ int spx_chi0_ozaki(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void 
     SPX_LAUNCHED(ctx);
   }
   const int nbt = nbp / TT;
-  if (ctx->phase_ev[0]) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[0], st));
+  const long nwaves = nchunk * (nbp / 16);
+  if (timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[0], st));
   for (int w0 = 0; w0 < nw; w0 += nwg) {
     const int g = nw - w0 < nwg ? nw - w0 : nwg;
-    dim3 sgrid(nchunk * 4, (nbp + 255) / 256);
+    dim3 sgrid((nwaves + NT / 64 - 1) / (NT / 64));
     if (colmajor)
-      hipLaunchKernelGGL(ozk_slice_kernel<true>, sgrid, dim3(256), 0, st, n, nb, nbp,
-                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb);
+      hipLaunchKernelGGL(ozk_slice_kernel<true>, sgrid, dim3(NT), 0, st, n, nb, nbp,
+                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, nwaves);
     else
-      hipLaunchKernelGGL(ozk_slice_kernel<false>, sgrid, dim3(256), 0, st, n, nb, nbp,
-                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb);
+      hipLaunchKernelGGL(ozk_slice_kernel<false>, sgrid, dim3(NT), 0, st, n, nb, nbp,
+                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, nwaves);
     SPX_LAUNCHED(ctx);
-    if (w0 == 0 && ctx->phase_ev[1]) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
+    if (w0 == 0 && timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
     dim3 hgrid(nbt * (nbt + 1) / 2, g);
     hipLaunchKernelGGL(ozk_herk_kernel, hgrid, dim3(NT), 0, st, nchunk, nb, nbp, S, (long)sb, w0,
                        scale, sgn, (double2 *)chi, (long)ldchi, alpha, beta);
     SPX_LAUNCHED(ctx);
   }
-  if (ctx->phase_ev[2]) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[2], st));
-  ctx->phase_groups = (nw + nwg - 1) / nwg;
-  // the host copy of the signs must outlive the async copy
+  if (timed) {
+    SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[2], st));
+    ctx->phase_groups = (nw + nwg - 1) / nwg;
+  }
+  return SPEXHIP_OK;
+}
+
+// Synchronous form used by spexhip_chi0_dev: context scratch, then a check
+// of the sign flag.
+int spx_chi0_ozaki(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,
+                   int colmajor, int nw, const void *wght, double alpha, double beta, void *chi,
+                   int ldchi, size_t ws_limit) {
+  SPX_CHECK(ctx, nw >= 1 && nw <= 16, "chi0 ozaki: 1..16 frequencies per call (got %d)", nw);
+  int nwg = nw;
+  while (nwg > 1 && spx_ozaki_ws_bytes(n, nb, nw, nwg) + 1024 > ws_limit) nwg--;
+  const size_t wsb = spx_ozaki_ws_bytes(n, nb, nw, nwg) + 1024;
+  void *ws;
+  int rc = spx_scratch(ctx, wsb, &ws);
+  if (rc) return rc;
+  int *err = (int *)((char *)ws + wsb - 1024);
+  SPX_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), st));
+  rc = spx_chi0_ozaki_async(ctx, st, n, nb, M, ldm, colmajor, nw, wght, alpha, beta, chi, ldchi,
+                            ws, wsb - 1024, err);
+  if (rc) return rc;
+  int h = 0;
+  SPX_HIP(ctx, hipMemcpyAsync(&h, err, sizeof(int), hipMemcpyDeviceToHost, st));
   SPX_HIP(ctx, hipStreamSynchronize(st));
+  if (h)
+    return spx_seterr(ctx, SPEXHIP_E_INVALID,
+                      "chi0 ozaki: the weights of some frequency have both signs");
   return SPEXHIP_OK;
 }

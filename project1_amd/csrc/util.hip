@@ -24,7 +24,23 @@ __global__ void transpose_kernel(const double2 *__restrict__ M, long rows, int c
   }
 }
 
+__global__ void fill_zero_kernel(unsigned int *__restrict__ p, long count) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < count) p[g] = 0u;
+}
+
 }  // namespace
+
+// Zero `bytes` (a multiple of 4) with a kernel rather than a memset node, so
+// that the call may be captured into a graph.
+int spx_fill_zero(spexhip_ctx *ctx, hipStream_t st, void *p, size_t bytes) {
+  const long cnt = (long)(bytes / 4);
+  if (cnt == 0) return SPEXHIP_OK;
+  hipLaunchKernelGGL(fill_zero_kernel, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                     (unsigned int *)p, cnt);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
 
 int spx_transpose_d2h(spexhip_ctx *ctx, hipStream_t st, const void *M, long rows, int cols,
                       long ldm, double *host) {

@@ -214,6 +214,98 @@ __global__ __launch_bounds__(NT) void pw_pair_xy_kernel(const double2 *__restric
   }
 }
 
+// Fused forward transform of the products of pairs (b1, b2): passes x and y
+// on MFMA as in pw_pair_xy_kernel, then per plane the 16x16 result goes
+// through a per-wave LDS tile into the z pass, accumulated directly at the
+// mixed-basis IPWs each lane owns (ig = lane + 64 j), with twiddles
+// e^{-2 pi i gz z/N3} from an LDS table.  No slab round trip through HBM.
+// Requires ngptm <= 64*PW_FOUT.
+constexpr int PW_FOUT = 16;
+template <int NB>
+__global__ __launch_bounds__(NT) void pw_pair_fused_kernel(
+    const double2 *__restrict__ grids, int nocc, int nunocc, int N1, int N2, int N3,
+    const int *__restrict__ oidx, int ngptm, double2 *__restrict__ M, long ldm, long t0, int col0) {
+  __shared__ double2 tz[64][PW_BOX];               // e^{-2 pi i (g-8) z/N3}
+  __shared__ double2 Qs[NT / 64][PW_BOX * PW_BOX];  // per-wave plane result
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  for (int e = tid; e < N3 * PW_BOX; e += NT) {
+    const int z = e / PW_BOX, g = e % PW_BOX;
+    tz[z][g] = twid((long)(g - PW_OFF) * z, N3, -1.0);
+  }
+  __syncthreads();
+  const int b2 = blockIdx.x;
+  const int b1 = blockIdx.y * (NT / 64) + wv;
+  if (b1 >= nocc) return;  // whole wave; no workgroup barrier after this point
+  const long NN = (long)N1 * N2 * N3;
+  const double2 *p1 = grids + (long)b1 * NN;
+  const double2 *p2 = grids + (long)(nocc + b2) * NN;
+  constexpr int KX = 4 * NB;
+  double2 F1[KX], F2[KX];
+#pragma unroll
+  for (int kk = 0; kk < KX; kk++) {
+    const int x = 4 * kk + lk;
+    F1[kk] = x < N1 ? twid((long)(lr - PW_OFF) * x, N1, -1.0) : make_double2(0.0, 0.0);
+    const int y = 4 * kk + lk;
+    F2[kk] = y < N2 ? twid((long)(lr - PW_OFF) * y, N2, -1.0) : make_double2(0.0, 0.0);
+  }
+  int ob[PW_FOUT];
+  double2 acc[PW_FOUT];
+#pragma unroll
+  for (int j = 0; j < PW_FOUT; j++) {
+    const int ig = l + 64 * j;
+    ob[j] = ig < ngptm ? oidx[ig] : -1;
+    acc[j] = make_double2(0.0, 0.0);
+  }
+  double2 *Q = Qs[wv];
+  for (int z = 0; z < N3; z++) {
+    d4_t xr[NB], xi[NB];
+#pragma unroll
+    for (int mb = 0; mb < NB; mb++) {
+      xr[mb] = d4_t{0, 0, 0, 0};
+      xi[mb] = d4_t{0, 0, 0, 0};
+      if (16 * mb >= N2) continue;
+      const int y = 16 * mb + lr;
+#pragma unroll
+      for (int kk = 0; kk < KX; kk++) {
+        if (4 * kk < N1) {
+          const int x = 4 * kk + lk;
+          double2 a = make_double2(0.0, 0.0);
+          if (x < N1 && y < N2) {
+            const long g = x + (long)N1 * (y + (long)N2 * z);
+            a = cmul(p1[g], p2[g]);
+          }
+          cmfma(xr[mb], xi[mb], a, F1[kk]);
+        }
+      }
+    }
+    d4_t qr = {0, 0, 0, 0}, qi = {0, 0, 0, 0};
+#pragma unroll
+    for (int kk = 0; kk < KX; kk++)
+      if (4 * kk < N2) cmfma(qr, qi, F2[kk], make_double2(xr[kk / 4][kk % 4], xi[kk / 4][kk % 4]));
+    // Q(gy, gx): lane holds gx = lr, gy = lk + 4r
+#pragma unroll
+    for (int r = 0; r < 4; r++) Q[(lk + 4 * r) * PW_BOX + lr] = make_double2(qr[r], qi[r]);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+    for (int j = 0; j < PW_FOUT; j++) {
+      if (ob[j] >= 0) {
+        const double2 t = tz[z][ob[j] >> 8];
+        const double2 v = Q[ob[j] & 255];
+        acc[j].x += t.x * v.x - t.y * v.y;
+        acc[j].y += t.x * v.y + t.y * v.x;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();  // Q is rewritten by the next plane
+  }
+  double2 *out = M + (t0 + (long)b1 * nunocc + b2) * ldm + col0;
+#pragma unroll
+  for (int j = 0; j < PW_FOUT; j++) {
+    const int ig = l + 64 * j;
+    if (ig < ngptm) out[ig] = acc[j];
+  }
+}
+
 // Forward pass z and gather of the mixed-basis IPWs (oidx: box index
 // gz*256 + gy*16 + gx of imati.(Gm - g_k) per IPW).
 __global__ __launch_bounds__(NT) void pw_pair_z_kernel(const double2 *__restrict__ slab,
@@ -270,6 +362,13 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
                        nocc, k2[ik], b2lo, N1, N2, N3, 1.0 / std::sqrt(p->vol), 1.0 / (double)NN,
                        p->step, grids);
     SPX_LAUNCHED(ctx);
+    if (p->ngptm <= 64 * PW_FOUT) {
+      hipLaunchKernelGGL(pw_pair_fused_kernel<NB>, dim3(nunocc, (nocc + 3) / 4), dim3(NT), 0, st,
+                         grids, nocc, nunocc, N1, N2, N3, oidx_dev + (long)ik * p->ngptm, p->ngptm,
+                         (double2 *)M, ldm, (long)ik * nocc * nunocc, col0);
+      SPX_LAUNCHED(ctx);
+      continue;
+    }
     hipLaunchKernelGGL(pw_pair_xy_kernel<NB>, dim3(nunocc, (nocc + 3) / 4), dim3(NT), 0, st, grids,
                        nocc, nunocc, N1, N2, N3, slab);
     SPX_LAUNCHED(ctx);
@@ -325,21 +424,14 @@ int spx_pw_plan_build(spexhip_ctx *ctx, const spexhip_pw_desc *d, spexhip_plan *
   return SPEXHIP_OK;
 }
 
+size_t spx_wfprod_pw_ws_core(const spexhip_plan *p, int nocc, int nunocc);
 size_t spx_wfprod_pw_ws(const spexhip_plan *p, int nk, int nocc, int nunocc) {
-  const long NN = (long)p->nfft[0] * p->nfft[1] * p->nfft[2];
-  return 16 * ((size_t)(nocc + nunocc) * NN + (size_t)nocc * nunocc * p->nfft[2] * PW_BOX * PW_BOX) +
-         4 * (size_t)nk * p->ngptm + 256;
+  return spx_wfprod_pw_ws_core(p, nocc, nunocc) + 4 * (size_t)nk * p->ngptm + 256;
 }
 
-// k1, k2 host arrays (nk); g2 host (nk x 3).  Writes columns col0 .. col0 +
-// ngptm - 1 of M (transition-major, ld ldm).
-int spx_wfprod_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const void *cpw,
-                  int nband, int nk, const int *k1, const int *k2, const int *g2, int b1lo,
-                  int nocc, int b2lo, int nunocc, void *M, long ldm, int col0, void *ws) {
-  std::vector<int> oidx((size_t)nk * p->ngptm);
-  for (int ik = 0; ik < nk; ik++) {
-    SPX_CHECK(ctx, k1[ik] >= 0 && k1[ik] < p->nkpt && k2[ik] >= 0 && k2[ik] < p->nkpt,
-              "wfprod_pw: k index out of range");
+// Box indices of imati.(Gm - g2(k)) for every packet k and IPW (host).
+int spx_pw_oidx(spexhip_ctx *ctx, const spexhip_plan *p, int nk, const int *g2, int *oidx) {
+  for (int ik = 0; ik < nk; ik++)
     for (int ig = 0; ig < p->ngptm; ig++) {
       int v[3];
       for (int i = 0; i < 3; i++) v[i] = p->gptm[3 * ig + i] - g2[3 * ik + i];
@@ -349,11 +441,24 @@ int spx_wfprod_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const
                           "wfprod_pw: IPW %d - umklapp of k %d outside the -8..7 Fourier box", ig, ik);
       oidx[(size_t)ik * p->ngptm + ig] = idx;
     }
-  }
-  const size_t wsb = spx_wfprod_pw_ws(p, nk, nocc, nunocc);
-  int *oidx_dev = (int *)((char *)ws + wsb - 256 - 4 * (size_t)nk * p->ngptm);
-  SPX_HIP(ctx, hipMemcpyAsync(oidx_dev, oidx.data(), 4 * oidx.size(), hipMemcpyHostToDevice, st));
-  SPX_HIP(ctx, hipStreamSynchronize(st));  // oidx is a host temporary
+  return SPEXHIP_OK;
+}
+
+// Workspace without the oidx table.
+size_t spx_wfprod_pw_ws_core(const spexhip_plan *p, int nocc, int nunocc) {
+  const long NN = (long)p->nfft[0] * p->nfft[1] * p->nfft[2];
+  return 16 * ((size_t)(nocc + nunocc) * NN + (size_t)nocc * nunocc * p->nfft[2] * PW_BOX * PW_BOX);
+}
+
+// Asynchronous launches (no host synchronisation): k1, k2 are host arrays
+// used as launch arguments, oidx_dev a device table from spx_pw_oidx.
+int spx_wfprod_pw_async(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const void *cpw,
+                        int nband, int nk, const int *k1, const int *k2, const int *oidx_dev,
+                        int b1lo, int nocc, int b2lo, int nunocc, void *M, long ldm, int col0,
+                        void *ws) {
+  for (int ik = 0; ik < nk; ik++)
+    SPX_CHECK(ctx, k1[ik] >= 0 && k1[ik] < p->nkpt && k2[ik] >= 0 && k2[ik] < p->nkpt,
+              "wfprod_pw: k index out of range");
   const int nbmax = (std::max(p->nfft[0], p->nfft[1]) + 15) / 16;
   switch (nbmax) {
     case 1:
@@ -364,4 +469,19 @@ int spx_wfprod_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const
     default:
       return launch_pw<4>(ctx, st, p, cpw, nband, nk, k1, k2, oidx_dev, b1lo, nocc, b2lo, nunocc, M, ldm, col0, ws);
   }
+}
+
+// Synchronous form: builds and uploads the oidx table into the tail of ws.
+int spx_wfprod_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const void *cpw,
+                  int nband, int nk, const int *k1, const int *k2, const int *g2, int b1lo,
+                  int nocc, int b2lo, int nunocc, void *M, long ldm, int col0, void *ws) {
+  std::vector<int> oidx((size_t)nk * p->ngptm);
+  int rc = spx_pw_oidx(ctx, p, nk, g2, oidx.data());
+  if (rc) return rc;
+  int *oidx_dev = (int *)((char *)ws + spx_wfprod_pw_ws_core(p, nocc, nunocc));
+  SPX_HIP(ctx, hipMemcpyAsync(oidx_dev, oidx.data(), 4 * oidx.size(), hipMemcpyHostToDevice, st));
+  rc = spx_wfprod_pw_async(ctx, st, p, cpw, nband, nk, k1, k2, oidx_dev, b1lo, nocc, b2lo, nunocc,
+                           M, ldm, col0, ws);
+  SPX_HIP(ctx, hipStreamSynchronize(st));  // oidx is a host temporary
+  return rc;
 }

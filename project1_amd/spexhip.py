@@ -68,6 +68,90 @@ class PwDesc(C.Structure):
                 ("ngptm", C.c_int), ("gptm", _vp), ("step_r", _vp)]
 
 
+class Chi0Problem(C.Structure):
+    """spexhip_chi0_problem"""
+    _fields_ = [("n", C.c_long), ("nb", C.c_int), ("nw", C.c_int), ("M_dev", _vp),
+                ("ldm", C.c_long), ("wght_dev", _vp), ("chi_dev", _vp), ("ldchi", C.c_int),
+                ("alpha", C.c_double), ("beta", C.c_double)]
+
+
+def chi0_configs():
+    """[(tm, kc, per_cu)] of the grouped kernel."""
+    n = C.c_int(64)
+    tm, kc, pc = (C.c_int * 64)(), (C.c_int * 64)(), (C.c_int * 64)()
+    rc = lib().spexhip_chi0_configs(C.byref(n), tm, kc, pc)
+    if rc:
+        raise SpexError(rc, "spexhip_chi0_configs")
+    return [(tm[i], kc[i], pc[i]) for i in range(n.value)]
+
+
+class QDesc(C.Structure):
+    """spexhip_qdesc"""
+    _fields_ = [("nk", C.c_int), ("nocc", C.c_int), ("nunocc", C.c_int), ("nb", C.c_int),
+                ("nw", C.c_int), ("precision", C.c_int), ("vkind", C.c_int),
+                ("alpha", C.c_double), ("ozaki_group", C.c_int),
+                ("M_dev", _vp), ("ldm", C.c_long),
+                ("mt_plan", _vp), ("pw_plan", _vp), ("cmt_dev", _vp), ("ncent", C.c_int),
+                ("nband", C.c_int), ("nkpt", C.c_int), ("q", C.c_double * 3), ("cpw_dev", _vp),
+                ("k1", _vp), ("k2", _vp), ("g2", _vp), ("b1lo", C.c_int), ("b2lo", C.c_int),
+                ("e_occ_dev", _vp), ("e_unocc_dev", _vp), ("nkpts_dev", _vp),
+                ("nkpt_w", C.c_int), ("nsym1", C.c_int), ("omega_dev", _vp), ("vhalf_dev", _vp)]
+
+
+class QPipe:
+    """Per-q pipeline (products -> chi0 -> eps^-1) on its own stream."""
+
+    def __init__(self, ctx: "Context", desc: QDesc, keep: list):
+        self.ctx, self._keep, self.desc = ctx, keep, desc
+        h = _vp()
+        ctx._check(lib().spexhip_qpipe_create(ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h.value
+
+    def launch(self, graph: bool = False) -> None:
+        self.ctx._check(lib().spexhip_qpipe_launch(self.ctx.h, self.h, int(graph)))
+
+    def done(self) -> bool:
+        rc = lib().spexhip_qpipe_query(self.ctx.h, self.h)
+        if rc == E_AGAIN:
+            return False
+        self.ctx._check(rc)
+        return True
+
+    def wait(self) -> float:
+        v = C.c_float()
+        self.ctx._check(lib().spexhip_qpipe_wait(self.ctx.h, self.h, C.byref(v)))
+        return v.value
+
+    def result(self, chi: bool = True, einv: bool = True):
+        nb, nw = self.desc.nb, self.desc.nw
+        npack = nb * (nb + 1) // 2
+        a = np.zeros((nw, npack), np.complex128) if chi else None
+        b = np.zeros((nw, npack), np.complex128) if einv else None
+        self.ctx._check(lib().spexhip_qpipe_result(self.ctx.h, self.h,
+                                                   _ptr(a) if chi else None,
+                                                   _ptr(b) if einv else None))
+        return a, b
+
+    def buffers(self):
+        m, x, e = _vp(), _vp(), _vp()
+        self.ctx._check(lib().spexhip_qpipe_buffers(self.ctx.h, self.h, C.byref(m), C.byref(x),
+                                                    C.byref(e)))
+        return m.value, x.value, e.value
+
+    def destroy(self) -> None:
+        if self.h:
+            self.ctx._check(lib().spexhip_qpipe_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
+def qpipe_footprint(desc: QDesc) -> int:
+    v = C.c_size_t()
+    rc = lib().spexhip_qpipe_footprint(C.byref(desc), C.byref(v))
+    if rc:
+        raise SpexError(rc, "qpipe_footprint")
+    return v.value
+
+
 class Plan:
     """A muffin-tin or plane-wave product plan (owns host copies it points to)."""
 
@@ -111,6 +195,20 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_event_record": (I, [_vp, _vp]),
         "spexhip_event_elapsed": (I, [_vp, _vp, _vp, P(C.c_float)]),
         "spexhip_event_destroy": (I, [_vp, _vp]),
+        "spexhip_chi0_configs": (I, [P(I), P(I), P(I), P(I)]),
+        "spexhip_group_create": (I, [_vp, I, P(Chi0Problem), I, I, P(_vp)]),
+        "spexhip_group_launch": (I, [_vp, _vp]),
+        "spexhip_group_destroy": (I, [_vp, _vp]),
+        "spexhip_chi0_grouped_dev": (I, [_vp, I, P(Chi0Problem), I, I]),
+        "spexhip_chi0_autotune": (I, [_vp, I, P(Chi0Problem), I, I, P(I), P(C.c_float)]),
+        "spexhip_qpipe_footprint": (I, [P(QDesc), P(C.c_size_t)]),
+        "spexhip_qpipe_create": (I, [_vp, P(QDesc), P(_vp)]),
+        "spexhip_qpipe_launch": (I, [_vp, _vp, I]),
+        "spexhip_qpipe_query": (I, [_vp, _vp]),
+        "spexhip_qpipe_wait": (I, [_vp, _vp, P(C.c_float)]),
+        "spexhip_qpipe_result": (I, [_vp, _vp, _vp, _vp]),
+        "spexhip_qpipe_buffers": (I, [_vp, _vp, P(_vp), P(_vp), P(_vp)]),
+        "spexhip_qpipe_destroy": (I, [_vp, _vp]),
         "spexhip_set_workspace_limit": (I, [_vp, C.c_size_t]),
         "spexhip_chi0_phase_ms": (I, [_vp, P(C.c_float), P(C.c_float), P(I)]),
         "spexhip_mt_plan_create": (I, [_vp, I, _vp, _vp, I, P(_vp)]),
@@ -279,6 +377,44 @@ class Context:
         self._check(lib().spexhip_chi0_dev(
             self.h, n, nb, M.ptr, ldm, int(colmajor), nw, wght.ptr, precision,
             alpha, beta, chi.ptr, ldchi))
+
+    @staticmethod
+    def _problems(probs):
+        arr = (Chi0Problem * len(probs))()
+        for a, p in zip(arr, probs):
+            a.n, a.nb, a.nw, a.ldm, a.ldchi = p["n"], p["nb"], p["nw"], p["ldm"], p["ldchi"]
+            a.M_dev, a.wght_dev, a.chi_dev = p["M"].ptr, p["wght"].ptr, p["chi"].ptr
+            a.alpha, a.beta = p.get("alpha", 1.0), p.get("beta", 0.0)
+        return arr
+
+    def chi0_grouped(self, probs: list, precision: int, config: int = -1) -> None:
+        """One persistent launch over problems (dicts n, nb, nw, M, ldm, wght, chi, ldchi,
+        alpha, beta with DevBuf values)."""
+        arr = self._problems(probs)
+        self._check(lib().spexhip_chi0_grouped_dev(self.h, len(probs), arr, precision, config))
+
+    def group(self, probs: list, precision: int, config: int = -1) -> int:
+        arr = self._problems(probs)
+        h = _vp()
+        self._check(lib().spexhip_group_create(self.h, len(probs), arr, precision, config,
+                                               C.byref(h)))
+        return h.value
+
+    def group_launch(self, g: int) -> None:
+        self._check(lib().spexhip_group_launch(self.h, g))
+
+    def group_destroy(self, g: int) -> None:
+        self._check(lib().spexhip_group_destroy(self.h, g))
+
+    def chi0_autotune(self, probs: list, precision: int, reps: int = 1):
+        """(best config index, [ms per config])"""
+        arr = self._problems(probs)
+        n = len(chi0_configs())
+        ms = (C.c_float * n)()
+        best = C.c_int()
+        self._check(lib().spexhip_chi0_autotune(self.h, len(probs), arr, precision, reps,
+                                                C.byref(best), ms))
+        return best.value, list(ms)
 
     def pack_upper(self, X: DevBuf, nb: int, nw: int, ldx: int,
                    out: np.ndarray | None = None) -> np.ndarray:

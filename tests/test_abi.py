@@ -41,3 +41,31 @@ def test_null_context_is_rejected_without_gpu_work():
     assert L.spexhip_destroy(None) == spexhip.OK
     buf = ctypes.create_string_buffer(16)
     assert L.spexhip_last_error(None, buf, 16) == spexhip.E_INVALID
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the ABI structs have the C sizes and field offsets."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        pytest.skip("no C compiler")
+    structs = {"spexhip_mt_type": spexhip.MtType, "spexhip_pw_desc": spexhip.PwDesc,
+               "spexhip_qdesc": spexhip.QDesc}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "spexhip.h"', "int main(void){"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, f"-I{ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                 check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(out[cname]) == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)

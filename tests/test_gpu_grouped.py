@@ -1,0 +1,68 @@
+"""Persistent grouped chi0 over uneven q-points vs the oracle; autotuner."""
+import numpy as np
+import pytest
+
+import oracle
+from project1_amd import spexhip, synth
+from tests.conftest import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(ctx, shapes, nw=3, seed=0):
+    probs, refs, keep = [], [], []
+    for i, (nk, nocc, nunocc, nb) in enumerate(shapes):
+        c = synth.chi0_case(nk=nk, nocc=nocc, nunocc=nunocc, nb=nb, nw=nw, seed=seed + i)
+        w = oracle.weights_none(c.e_occ, c.e_unocc, c.omega)
+        refs.append(2 * oracle.chi0_herk(c.M, w))
+        dM, dW = ctx.to_dev(c.M), ctx.to_dev(np.ascontiguousarray(w.T))
+        dC = ctx.alloc(16 * nb * nb * nw).zero()
+        keep += [dM, dW, dC]
+        probs.append(dict(n=c.n, nb=nb, nw=nw, M=dM, ldm=nb, wght=dW, chi=dC, ldchi=nb, alpha=2.0))
+    return probs, refs, keep
+
+
+SHAPES = [(2, 3, 17, 70), (3, 2, 40, 150), (1, 4, 9, 33), (2, 5, 31, 129), (4, 2, 23, 64), (1, 1, 1, 5)]
+
+
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-12), (spexhip.PREC_SPLIT_FP32, 3e-7)])
+def test_grouped_every_config_matches_oracle(ctx, precision, tol):
+    probs, refs, keep = make_batch(ctx, SHAPES)
+    for cfg in range(len(spexhip.chi0_configs())):
+        for p in probs:
+            p["chi"].zero()
+        ctx.chi0_grouped(probs, precision, cfg)
+        for p, ref in zip(probs, refs):
+            got = ctx.pack_upper(p["chi"], p["nb"], p["nw"], p["nb"]).T
+            mx, fro = normwise(got, ref)
+            assert fro < tol, (cfg, p["nb"], mx, fro)
+    for b in keep:
+        b.free()
+
+
+def test_group_relaunch_and_autotune(ctx):
+    probs, refs, keep = make_batch(ctx, SHAPES[:4], seed=10)
+    best, ms = ctx.chi0_autotune(probs, spexhip.PREC_FP64, reps=2)
+    print("autotune ms per config:", ["%.3f" % m for m in ms], "best", best)
+    assert 0 <= best < len(ms) and all(m > 0 for m in ms)
+    g = ctx.group(probs, spexhip.PREC_FP64, best)
+    for rep in range(2):  # the work-queue counter is reset by every launch
+        for p in probs:
+            p["chi"].zero()
+        ctx.group_launch(g)
+        ctx.sync()
+        for p, ref in zip(probs, refs):
+            got = ctx.pack_upper(p["chi"], p["nb"], p["nw"], p["nb"]).T
+            assert normwise(got, ref)[1] < 1e-12
+    ctx.group_destroy(g)
+    for b in keep:
+        b.free()
+
+
+def test_grouped_rejects_ozaki(ctx):
+    probs, refs, keep = make_batch(ctx, SHAPES[:1])
+    with pytest.raises(spexhip.SpexError) as e:
+        ctx.chi0_grouped(probs, spexhip.PREC_OZAKI_INT8)
+    assert e.value.code == spexhip.E_INVALID
+    for b in keep:
+        b.free()
