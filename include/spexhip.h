@@ -62,7 +62,7 @@ int spexhip_create(int device, size_t hbm_budget_bytes, spexhip_ctx **out);
 int spexhip_destroy(spexhip_ctx *ctx);
 int spexhip_last_error(spexhip_ctx *ctx, char *buf, size_t len);
 int spexhip_sync(spexhip_ctx *ctx);
-/* bytes currently allocated through this context */
+/* bytes currently allocated through this context (size-class rounded) */
 int spexhip_mem_used(spexhip_ctx *ctx, size_t *used, size_t *budget);
 
 /* Largest slice workspace the Ozaki chi0 form may use (default 24 GiB); the
@@ -74,6 +74,11 @@ int spexhip_set_workspace_limit(spexhip_ctx *ctx, size_t bytes);
 int spexhip_chi0_phase_ms(spexhip_ctx *ctx, float *prep_ms, float *rank_ms, int *groups);
 
 /* ---- device memory (library-owned; counted against the budget) -------- */
+/* Sizes are rounded up to one of four classes per power of two.  A freed
+ * block goes to the context's cache and serves the next allocation of its
+ * class (no hipFree, which would wait for every stream); the cache is
+ * returned when an allocation fails and at spexhip_destroy.  Free a block
+ * only after all work that uses it has finished.                         */
 int spexhip_dev_alloc(spexhip_ctx *ctx, size_t bytes, void **dev);
 int spexhip_dev_free(spexhip_ctx *ctx, void *dev);
 int spexhip_h2d(spexhip_ctx *ctx, void *dev, const void *host, size_t bytes);
@@ -86,6 +91,11 @@ int spexhip_memset(spexhip_ctx *ctx, void *dev, int value, size_t bytes);
  * (seed, i), so any launch shape gives the same numbers.                   */
 int spexhip_synth_normal_dev(spexhip_ctx *ctx, void *out_dev, long count,
                              unsigned long long seed, double scale);
+/* Complex (rows, cols, nmat) column-major array: element i gets the same
+ * draw (pair 2i, 2i+1 of the generator above) times scale[row + rows*mat],
+ * e.g. plane-wave coefficients damped per (G, k) and zero past ngpt(k).   */
+int spexhip_synth_normal_rows_dev(spexhip_ctx *ctx, void *out_dev, long rows, long cols,
+                                  long nmat, unsigned long long seed, const void *scale_dev);
 /* HIP events recorded on the context's stream (the stream every kernel of
  * the context runs on); elapsed waits for the second event.                */
 int spexhip_event_create(spexhip_ctx *ctx, void **ev);
@@ -292,6 +302,31 @@ int spexhip_qpipe_result(spexhip_ctx *ctx, spexhip_qpipe *qp, double *chi_packed
 int spexhip_qpipe_buffers(spexhip_ctx *ctx, spexhip_qpipe *qp, void **M_dev, void **chi_dev,
                           void **einv_dev);
 int spexhip_qpipe_destroy(spexhip_ctx *ctx, spexhip_qpipe *qp);
+
+/* ---- serving: continuous batching of q-point requests ------------------ */
+/* No reference counterpart (SPEX hands q-points to idle MPI ranks through
+ * the MPIKPT task queue, correlation.f:316-402).  A scheduler thread admits
+ * queued requests in arrival order whenever fewer than max_inflight run and
+ * their footprint (spexhip_qpipe_footprint) fits hbm_budget; backfill != 0
+ * lets smaller requests pass one that does not fit yet.  Admitted requests
+ * run concurrently, each on its own stream with its own buffers.           */
+typedef struct spexhip_server spexhip_server;
+int spexhip_server_create(spexhip_ctx *ctx, int max_inflight, size_t hbm_budget, int backfill,
+                          spexhip_server **srv);
+/* Copies the request (its host k lists too); E_NOMEM if it can never fit. */
+int spexhip_server_submit(spexhip_server *srv, const spexhip_qdesc *desc, long *id);
+/* state: 0 queued, 1 running, 2 done, 3 failed */
+int spexhip_server_poll(spexhip_server *srv, long id, int *state);
+/* Blocks until the request finished; latency = submit -> completion, queue
+ * = submit -> launch, device = pipeline time on the GPU (ms).              */
+int spexhip_server_wait(spexhip_server *srv, long id, double *latency_ms, double *queue_ms,
+                        double *device_ms);
+int spexhip_server_result(spexhip_server *srv, long id, double *chi_packed, double *einv_packed);
+/* Frees the request's device memory, returning it to the admission budget. */
+int spexhip_server_release(spexhip_server *srv, long id);
+int spexhip_server_stats(spexhip_server *srv, long *admitted, long *completed,
+                         size_t *peak_used, int *peak_inflight);
+int spexhip_server_destroy(spexhip_server *srv);
 
 /* ---- dielectric step and inverse (FP64) -------------------------------- */
 /* eps = 1 - v^1/2 chi v^1/2 on nw full device matrices, correlation.f:

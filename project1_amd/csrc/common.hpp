@@ -4,6 +4,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -19,8 +20,10 @@ struct spexhip_ctx {
   size_t used = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  std::mutex mu;  // guards err, used, allocs
+  std::mutex mu;  // guards err, used, allocs, cache
   std::unordered_map<void *, size_t> allocs;
+  std::multimap<size_t, void *> cache;  // freed blocks by size class
+  size_t cached = 0;
   // scratch for host drop-in calls (grown on demand, owned by the context)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

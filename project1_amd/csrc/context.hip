@@ -16,46 +16,80 @@ int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...) {
   return code;
 }
 
-int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p) {
-  if (bytes == 0) bytes = 256;
+// Allocations are rounded up to a size class (four per power of two) and a
+// freed block is kept in the context's cache for the next request of its
+// class, so a steady stream of pipelines of varying size does not call
+// hipFree, which waits for every stream of the device.  The cache is
+// returned to the device when an allocation fails and when the context ends.
+static size_t size_class(size_t b) {
+  if (b <= 4096) return 4096;
+  const int e = 63 - __builtin_clzll(b);
+  const size_t step = (size_t)1 << (e - 2);
+  return (b + step - 1) / step * step;
+}
+
+static void trim_cache(spexhip_ctx *ctx) {
+  std::multimap<size_t, void *> blocks;
   {
     std::lock_guard<std::mutex> g(ctx->mu);
-    if (ctx->budget && ctx->used + bytes > ctx->budget) {
+    blocks.swap(ctx->cache);
+    ctx->cached = 0;
+  }
+  for (auto &kv : blocks) (void)hipFree(kv.second);
+}
+
+int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p) {
+  const size_t cls = size_class(bytes);
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->budget && ctx->used + cls > ctx->budget) {
       ctx->err = "HBM budget exceeded: used " + std::to_string(ctx->used) +
-                 " + request " + std::to_string(bytes) + " > budget " +
+                 " + request " + std::to_string(cls) + " > budget " +
                  std::to_string(ctx->budget);
       return SPEXHIP_E_NOMEM;
     }
-    ctx->used += bytes;
+    ctx->used += cls;
+    auto it = ctx->cache.find(cls);
+    if (it != ctx->cache.end()) {
+      *p = it->second;
+      ctx->cache.erase(it);
+      ctx->cached -= cls;
+      ctx->allocs[*p] = cls;
+      return SPEXHIP_OK;
+    }
   }
-  hipError_t e = hipMalloc(p, bytes);
+  hipError_t e = hipMalloc(p, cls);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    trim_cache(ctx);
+    e = hipMalloc(p, cls);
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();  // do not leave the failure for the next launch check
     std::lock_guard<std::mutex> g(ctx->mu);
-    ctx->used -= bytes;
+    ctx->used -= cls;
     ctx->err = std::string("hipMalloc failed: ") + hipGetErrorString(e);
     return e == hipErrorOutOfMemory ? SPEXHIP_E_NOMEM : SPEXHIP_E_DEVICE;
   }
   std::lock_guard<std::mutex> g(ctx->mu);
-  ctx->allocs[*p] = bytes;
+  ctx->allocs[*p] = cls;
   return SPEXHIP_OK;
 }
 
+// The caller guarantees that no queued work still uses the block.
 int spx_free(spexhip_ctx *ctx, void *p) {
   if (!p) return SPEXHIP_OK;
-  size_t bytes = 0;
-  {
-    std::lock_guard<std::mutex> g(ctx->mu);
-    auto it = ctx->allocs.find(p);
-    if (it == ctx->allocs.end()) {
-      ctx->err = "spexhip_dev_free: pointer not owned by this context";
-      return SPEXHIP_E_INVALID;
-    }
-    bytes = it->second;
-    ctx->allocs.erase(it);
-    ctx->used -= bytes;
+  std::lock_guard<std::mutex> g(ctx->mu);
+  auto it = ctx->allocs.find(p);
+  if (it == ctx->allocs.end()) {
+    ctx->err = "spexhip_dev_free: pointer not owned by this context";
+    return SPEXHIP_E_INVALID;
   }
-  SPX_HIP(ctx, hipFree(p));
+  const size_t bytes = it->second;
+  ctx->allocs.erase(it);
+  ctx->used -= bytes;
+  ctx->cache.emplace(bytes, p);
+  ctx->cached += bytes;
   return SPEXHIP_OK;
 }
 
@@ -116,6 +150,7 @@ int spexhip_destroy(spexhip_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
   ctx->allocs.clear();
+  trim_cache(ctx);
   for (auto &e : ctx->phase_ev)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);

@@ -126,7 +126,7 @@ int spexhip_qpipe_create(spexhip_ctx *ctx, const spexhip_qdesc *desc, spexhip_qp
     const int ncols = (d.mt_plan ? d.mt_plan->nbasp : 0) + (d.pw_plan ? d.pw_plan->ngptm : 0);
     SPX_CHECK(ctx, ncols == d.nb, "qpipe: plans give %d product columns but nb=%d", ncols, d.nb);
     SPX_CHECK(ctx, d.k1 && d.k2 && (!d.pw_plan || d.g2), "qpipe: missing k lists");
-    SPX_CHECK(ctx, !d.mt_plan || (d.cmt_dev && d.ncent == d.mt_plan->ncent), "qpipe: bad MT inputs");
+    SPX_CHECK(ctx, !d.mt_plan || (d.cmt_dev && d.ncent >= d.mt_plan->ncent), "qpipe: bad MT inputs");
     SPX_CHECK(ctx, !d.pw_plan || d.cpw_dev, "qpipe: missing cpw");
   }
   SPX_HIP(ctx, hipSetDevice(ctx->device));

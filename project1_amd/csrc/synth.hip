@@ -35,9 +35,42 @@ __global__ void normal_kernel(double *__restrict__ out, long count, unsigned lon
   if (2 * i + 1 < count) out[2 * i + 1] = r * s;
 }
 
+// Complex (rows, cols, nmat) column-major array; element i is the draw of
+// counter i (as normal_kernel) times scale[row + rows * mat].
+__global__ void normal_rows_kernel(double2 *__restrict__ out, long rows, long cols, long nmat,
+                                   unsigned long long seed, const double *__restrict__ scale) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cols * nmat) return;
+  const unsigned long long base = splitmix64(seed ^ 0xD1B54A32D192ED03ull);
+  const double u1 = u01(splitmix64(base + 2 * (unsigned long long)i));
+  const double u2 = u01(splitmix64(base + 2 * (unsigned long long)i + 1));
+  const long r = i % rows, m = i / (rows * cols);
+  const double rr = sqrt(-2.0 * log(u1)) * scale[r + rows * m];
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  out[i] = make_double2(rr * c, rr * s);
+}
+
 }  // namespace
 
 extern "C" {
+
+int spexhip_synth_normal_rows_dev(spexhip_ctx *ctx, void *out, long rows, long cols, long nmat,
+                                  unsigned long long seed, const void *scale_dev) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, rows >= 0 && cols >= 0 && nmat >= 0, "synth_normal_rows: negative extent");
+  const long count = rows * cols * nmat;
+  if (count == 0) return SPEXHIP_OK;
+  SPX_CHECK(ctx, spx_owns(ctx, out, 16 * (size_t)count),
+            "synth_normal_rows: output not a context allocation of rows*cols*nmat complex");
+  SPX_CHECK(ctx, spx_owns(ctx, scale_dev, 8 * (size_t)rows * nmat),
+            "synth_normal_rows: scale not a context allocation of rows*nmat doubles");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(normal_rows_kernel, dim3((count + 255) / 256), dim3(256), 0, ctx->stream,
+                     (double2 *)out, rows, cols, nmat, seed, (const double *)scale_dev);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
 
 int spexhip_synth_normal_dev(spexhip_ctx *ctx, void *out, long count, unsigned long long seed,
                              double scale) {

@@ -91,7 +91,7 @@ int spexhip_wfprod2_mt_dev(spexhip_ctx *ctx, const spexhip_plan *plan, const voi
                            long ldm, int col0) {
   if (!ctx) return SPEXHIP_E_INVALID;
   SPX_CHECK(ctx, plan && plan->kind == 1, "wfprod_mt: not a muffin-tin plan");
-  SPX_CHECK(ctx, ncent == plan->ncent, "wfprod_mt: ncent=%d but the plan has %d centres", ncent, plan->ncent);
+  SPX_CHECK(ctx, ncent >= plan->ncent, "wfprod_mt: ncent=%d but the plan has %d centres", ncent, plan->ncent);
   SPX_CHECK(ctx, q && k1 && k2, "wfprod_mt: null argument");
   int rc = check_packet(ctx, nk, k1, k2, nkpt, b1lo, nocc, b2lo, nunocc, nband);
   if (rc) return rc;

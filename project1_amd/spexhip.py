@@ -152,6 +152,67 @@ def qpipe_footprint(desc: QDesc) -> int:
     return v.value
 
 
+class Server:
+    """Continuous-batching request loop over per-q pipelines (spexhip_server_*).
+
+    Requests are admitted in arrival order while a slot is free and their HBM
+    footprint fits ``hbm_budget``; with ``backfill`` a smaller later request may
+    pass a large one that does not fit yet."""
+
+    QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, ctx: "Context", max_inflight: int, hbm_budget: int, backfill: bool = True):
+        self.ctx = ctx
+        h = _vp()
+        ctx._check(lib().spexhip_server_create(ctx.h, max_inflight, hbm_budget, int(backfill),
+                                               C.byref(h)))
+        self.h = h.value
+        self._desc = {}
+
+    def submit(self, desc: QDesc) -> int:
+        v = C.c_long()
+        self.ctx._check(lib().spexhip_server_submit(self.h, C.byref(desc), C.byref(v)))
+        self._desc[v.value] = (desc.nb, desc.nw)
+        return v.value
+
+    def state(self, rid: int) -> int:
+        v = C.c_int()
+        self.ctx._check(lib().spexhip_server_poll(self.h, rid, C.byref(v)))
+        return v.value
+
+    def wait(self, rid: int) -> tuple[float, float, float]:
+        """Block until done; returns (latency_ms, queue_ms, device_ms)."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        self.ctx._check(lib().spexhip_server_wait(self.h, rid, C.byref(a), C.byref(b),
+                                                  C.byref(c)))
+        return a.value, b.value, c.value
+
+    def result(self, rid: int, chi: bool = True, einv: bool = True):
+        nb, nw = self._desc[rid]
+        npack = nb * (nb + 1) // 2
+        a = np.zeros((nw, npack), np.complex128) if chi else None
+        b = np.zeros((nw, npack), np.complex128) if einv else None
+        self.ctx._check(lib().spexhip_server_result(self.h, rid, _ptr(a) if chi else None,
+                                                    _ptr(b) if einv else None))
+        return a, b
+
+    def release(self, rid: int) -> None:
+        self.ctx._check(lib().spexhip_server_release(self.h, rid))
+        self._desc.pop(rid, None)
+
+    def stats(self) -> dict:
+        a, c, u, i = C.c_long(), C.c_long(), C.c_size_t(), C.c_int()
+        self.ctx._check(lib().spexhip_server_stats(self.h, C.byref(a), C.byref(c), C.byref(u),
+                                                   C.byref(i)))
+        return {"admitted": a.value, "completed": c.value, "peak_hbm_bytes": u.value,
+                "peak_inflight": i.value}
+
+    def destroy(self) -> None:
+        if self.h:
+            lib().spexhip_server_destroy(self.h)
+            self.h = None
+
+
 class Plan:
     """A muffin-tin or plane-wave product plan (owns host copies it points to)."""
 
@@ -191,6 +252,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_unpack_h2d": (I, [_vp, I, I, _vp, _vp, I]),
         "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
         "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
+        "spexhip_synth_normal_rows_dev": (I, [_vp, _vp, _c_long, _c_long, _c_long,
+                                              C.c_ulonglong, _vp]),
         "spexhip_event_create": (I, [_vp, P(_vp)]),
         "spexhip_event_record": (I, [_vp, _vp]),
         "spexhip_event_elapsed": (I, [_vp, _vp, _vp, P(C.c_float)]),
@@ -209,6 +272,15 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_qpipe_result": (I, [_vp, _vp, _vp, _vp]),
         "spexhip_qpipe_buffers": (I, [_vp, _vp, P(_vp), P(_vp), P(_vp)]),
         "spexhip_qpipe_destroy": (I, [_vp, _vp]),
+        "spexhip_server_create": (I, [_vp, I, C.c_size_t, I, P(_vp)]),
+        "spexhip_server_submit": (I, [_vp, P(QDesc), P(_c_long)]),
+        "spexhip_server_poll": (I, [_vp, _c_long, P(I)]),
+        "spexhip_server_wait": (I, [_vp, _c_long, P(C.c_double), P(C.c_double),
+                                    P(C.c_double)]),
+        "spexhip_server_result": (I, [_vp, _c_long, _vp, _vp]),
+        "spexhip_server_release": (I, [_vp, _c_long]),
+        "spexhip_server_stats": (I, [_vp, P(_c_long), P(_c_long), P(C.c_size_t), P(I)]),
+        "spexhip_server_destroy": (I, [_vp]),
         "spexhip_set_workspace_limit": (I, [_vp, C.c_size_t]),
         "spexhip_chi0_phase_ms": (I, [_vp, P(C.c_float), P(C.c_float), P(I)]),
         "spexhip_mt_plan_create": (I, [_vp, I, _vp, _vp, I, P(_vp)]),
@@ -358,6 +430,11 @@ class Context:
     # -- synthetic data and timing ------------------------------------------
     def synth_normal(self, buf: DevBuf, count: int, seed: int, scale: float = 1.0) -> None:
         self._check(lib().spexhip_synth_normal_dev(self.h, buf.ptr, count, seed, scale))
+
+    def synth_normal_rows(self, buf: DevBuf, rows: int, cols: int, nmat: int, seed: int,
+                          scale: DevBuf) -> None:
+        self._check(lib().spexhip_synth_normal_rows_dev(self.h, buf.ptr, rows, cols, nmat, seed,
+                                                        scale.ptr))
 
     def event(self) -> "Event":
         return Event(self)

@@ -25,6 +25,29 @@ def rng(seed: int) -> np.random.Generator:
     return np.random.Generator(np.random.PCG64(seed))
 
 
+def _splitmix64(x: np.ndarray) -> np.ndarray:
+    x = x + np.uint64(0x9E3779B97F4A7C15)
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def device_normal(count: int, seed: int) -> np.ndarray:
+    """Host mirror of the device generator (spexhip_synth_normal_dev):
+    complex draw i = (value 2i, value 2i+1) is one Box-Muller pair from the
+    splitmix64 hashes of counters 2i, 2i+1.  Agrees with the device to a few
+    ulp (log, sqrt and sincospi round differently)."""
+    with np.errstate(over="ignore"):
+        base = _splitmix64(np.array([np.uint64(seed) ^ np.uint64(0xD1B54A32D192ED03)]))[0]
+        i = np.arange(count, dtype=np.uint64)
+        h1 = _splitmix64(base + np.uint64(2) * i)
+        h2 = _splitmix64(base + np.uint64(2) * i + np.uint64(1))
+    u1 = ((h1 >> np.uint64(11)).astype(np.float64) + 1.0) / 9007199254740992.0
+    u2 = ((h2 >> np.uint64(11)).astype(np.float64) + 1.0) / 9007199254740992.0
+    r = np.sqrt(-2.0 * np.log(u1))
+    return r * np.cos(2 * np.pi * u2) + 1j * r * np.sin(2 * np.pi * u2)
+
+
 def gl_frequencies(nw: int):
     """Gauss-Legendre nodes on [0,1] mapped to omega = x/(1-x)."""
     u, wu = np.polynomial.legendre.leggauss(nw)
