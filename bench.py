@@ -55,6 +55,14 @@ def parse():
                    help="use Gaussian products generated once instead of computing M each step")
     p.add_argument("--cpu-cols", type=int, default=128,
                    help="CPU baseline sample: all pairs among this many columns")
+    p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                   help="only run 5 plain device copies of MIB MiB (byte-counter calibration)")
+    p.add_argument("--no-batch", action="store_true", help="skip the 32-q batched leg")
+    p.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    p.add_argument("--serve-seconds", type=float, default=20.0,
+                   help="open-loop window of the serving leg per precision")
+    p.add_argument("--serve-cap", type=int, default=8,
+                   help="requests of the closed-loop capacity run per precision")
     return p.parse_args()
 
 
@@ -132,6 +140,16 @@ class Pipeline:
         return (ev[2].ms_since(ev[1]), ev[3].ms_since(ev[2]), ev[3].ms_since(ev[4]),
                 ev[0].ms_since(ev[4]))
 
+    def free(self):
+        bufs = [self.dM, self.deo, self.deu, self.dnk, self.dom, self.dW, self.dV, self.dX,
+                self.dE]
+        if self.products:
+            bufs += [self.dcmt, self.dcpw]
+            self.mtp.destroy()
+            self.pwp.destroy()
+        for b in bufs:
+            b.free()
+
     def results(self):
         a = self.a
         chi = self.ctx.pack_upper(self.dX, a.nb, a.nw, a.nb)
@@ -197,6 +215,63 @@ def cpu_baseline(pipe, a):
                 seconds_per_q_extrapolated=t_q)
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def batch_leg(ctx, a, budget=160 << 30):
+    """32 uneven q-points (Nk=64, Nocc=16, Nunocc ~ U[200,600], Nb ~ U[600,1600],
+    Nw=12) in split-FP32: per k packet one grouped chi0 launch over all q
+    with the autotuned tile configuration; the fork/join pipeline is
+    captured into one HIP graph.  Two q are checked against their FP64
+    per-q pipelines."""
+    from project1_amd import serving, spexhip, synth
+
+    lat = serving.Lattice(ctx, nocc=16, nunocc=600, seed=a.seed + 5)
+    g = synth.rng(a.seed + 6)
+    specs = [serving.Spec(nmesh=4, nb=int(g.integers(600, 1601)), nw=12, seed=a.seed * 10 + i,
+                          nunocc=int(g.integers(200, 601)), iq=1 + i) for i in range(32)]
+    reqs = [serving.Request(lat, sp, spexhip.PREC_SPLIT_FP32) for sp in specs]
+    descs = [r.desc for r in reqs]
+    kp = next(k for k in (64, 32, 16, 8, 4) if spexhip.Batch.footprint(descs, k) <= budget)
+    t0 = time.perf_counter()
+    b = spexhip.Batch(ctx, descs, kpacket=kp, config=-1, tune_k=16384)
+    t_create = time.perf_counter() - t0
+    cfg, npacket, tune = b.config()
+    log(f"batch: {len(reqs)} q, kpacket {kp}, config {cfg}, created in {t_create:.1f} s")
+    b.launch(graph=False)
+    ms_eager = b.wait()
+    log(f"batch: eager run {ms_eager:.0f} ms")
+    b.launch(graph=True)  # captured once, then replayed
+    ms_graph = b.wait()
+    log(f"batch: graph run {ms_graph:.0f} ms")
+    flops = sum(chi0_flops(r.spec.nw, r.n, r.spec.nb) + eps_flops(r.spec.nw, r.spec.nb)
+                for r in reqs)
+    check = {}
+    for i in sorted({0, int(np.argmax([sp.nb for sp in specs]))}):
+        ref = serving.Request(lat, specs[i], spexhip.PREC_FP64)
+        p = spexhip.QPipe(ctx, ref.desc, [])
+        p.launch()
+        p.wait()
+        rchi, reinv = p.result()
+        chi, einv = b.result(i)
+        check[f"q{i}_nb{specs[i].nb}"] = {"chi0": errs(chi, rchi), "eps_inv": errs(einv, reinv)}
+        p.destroy()
+        ref.free()
+    b.destroy()
+    for r in reqs:
+        r.free()
+    lat.free()
+    tm, kc, pc = spexhip.chi0_configs()[cfg]
+    return {"workload": "32 q, Nk=64, Nocc=16, Nunocc~U[200,600], Nb~U[600,1600], Nw=12, split-FP32",
+            "kpacket": kp, "packets": npacket,
+            "config": {"index": cfg, "tile": tm, "kc": kc, "wg_per_cu": pc},
+            "autotune_ms": [round(t, 3) for t in tune], "create_s": round(t_create, 2),
+            "ms_eager": round(ms_eager, 1), "ms_graph": round(ms_graph, 1),
+            "qpts_per_s": round(32e3 / ms_graph, 3), "gflops": round(flops / ms_graph / 1e6, 1),
+            "err_vs_fp64_per_q": check}
+
+
 PEAK_I8 = 5000.0  # TOPS dense int8 matrix, MI355X
 FORMS = (("fp64", "PREC_FP64"), ("split_fp32", "PREC_SPLIT_FP32"), ("ozaki_int8", "PREC_OZAKI_INT8"))
 
@@ -210,6 +285,16 @@ def main():
     from project1_amd import spexhip
 
     ctx = spexhip.Context(0)
+    if a.copy_calib:
+        nbytes = a.copy_calib << 20
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        ctx.synth_normal(src, nbytes // 8, 1)
+        for _ in range(5):
+            ctx.copy(dst, src, nbytes)
+        ctx.sync()
+        print(json.dumps({"copy_kernel_bytes_each_way": nbytes, "launches": 5}))
+        ctx.close()
+        return
     ctx.set_workspace_limit(64 << 30)  # one frequency group for the Ozaki form at this size
     pipe = Pipeline(ctx, a)
     res, out = {}, {}
@@ -217,6 +302,7 @@ def main():
     for name, pname in FORMS:
         prec = getattr(spexhip, pname)
         r = run_form(pipe, prec, a.steps, a.warmup)
+        log(f"{name}: {r['ms_per_step']:.1f} ms/step (chi0 {r['chi0_ms']:.1f} ms)")
         out[name] = pipe.results()
         r["qpts_per_s"] = 1e3 / r["ms_per_step"]
         r["gflops"] = (flops + eps_flops(a.nw, a.nb)) / (r["ms_per_step"] * 1e6)
@@ -278,6 +364,16 @@ def main():
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pipe, a)
+    pipe.free()
+    if not a.no_batch:
+        line["batch"] = batch_leg(ctx, a)
+        log(f"batch {json.dumps(line['batch'])}")
+    if not a.no_serve:
+        from project1_amd import serving
+
+        line["serving"] = serving.serve_benchmark(
+            ctx, {"fp64": spexhip.PREC_FP64, "split_fp32": spexhip.PREC_SPLIT_FP32},
+            a.serve_seconds, a.serve_cap, log=lambda m: log(m) if isinstance(m, str) else None)
     print(json.dumps(line), flush=True)
     ctx.close()
 

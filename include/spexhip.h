@@ -84,6 +84,10 @@ int spexhip_dev_free(spexhip_ctx *ctx, void *dev);
 int spexhip_h2d(spexhip_ctx *ctx, void *dev, const void *host, size_t bytes);
 int spexhip_d2h(spexhip_ctx *ctx, void *host, const void *dev, size_t bytes);
 int spexhip_memset(spexhip_ctx *ctx, void *dev, int value, size_t bytes);
+/* Device-to-device copy on the context's stream (asynchronous; sizes and
+ * pointers multiples of 16 B).  Also the reference stream for calibrating
+ * hardware byte counters: it reads and writes exactly `bytes` each.      */
+int spexhip_copy_dev(spexhip_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 /* ---- synthetic inputs and timing --------------------------------------- */
 /* No reference counterpart (SPEX reads DFT files).  Fills count doubles with
@@ -203,7 +207,12 @@ typedef struct {
   int d1, d2, d3;
 } spexhip_mt_type;
 
-/* Interstitial plane-wave setup (fft.f:36-96, wavefproducts.f:576-690).   */
+/* Interstitial plane-wave setup (fft.f:36-96, wavefproducts.f:576-690).
+ * Without step_r the products are pure convolutions: on any grid at least
+ * W+9 points wide per axis (W = width of the states' Fourier support) they
+ * are exact, so the plan computes them on the smallest such grid (>= 16)
+ * when the caller's grid is one of them -- same numbers, fewer operations.
+ * With step_r, or a caller grid that aliases, the caller's grid is used.  */
 typedef struct {
   int nfft[3];        /* FFT grid (16..64 per axis)                        */
   int imati[9];       /* imati(i,j) stored row-major [3*i+j]               */
@@ -302,6 +311,33 @@ int spexhip_qpipe_result(spexhip_ctx *ctx, spexhip_qpipe *qp, double *chi_packed
 int spexhip_qpipe_buffers(spexhip_ctx *ctx, spexhip_qpipe *qp, void **M_dev, void **chi_dev,
                           void **einv_dev);
 int spexhip_qpipe_destroy(spexhip_ctx *ctx, spexhip_qpipe *qp);
+
+/* ---- batched uneven q-points (one grouped chi0 launch per k packet) ---- */
+/* Many q of one correlation run (correlation.f:404-747 per q) at once.  The
+ * k range of every q is cut into packets of at most kpacket k-points (the
+ * (nk, no, nu) packets of susceptibility.f:614-713); per packet, each q's
+ * products and weights run on one of up to 8 streams, then ONE persistent
+ * grouped chi0 launch covers all q (chi0 += packet); after the last packet
+ * each q's eps and eps^-1 run on its stream.  Descriptors as for
+ * spexhip_qpipe (products from plans; all FP64 or all SPLIT_FP32).
+ * config < 0: the tile configuration is chosen by timing every
+ * configuration on packet 0 with all K scaled by one factor so that the
+ * longest K is tune_k (tune_k <= 0: unscaled).                           */
+typedef struct spexhip_batch spexhip_batch;
+int spexhip_batch_footprint(int nq, const spexhip_qdesc *descs, int kpacket, size_t *bytes);
+int spexhip_batch_create(spexhip_ctx *ctx, int nq, const spexhip_qdesc *descs, int kpacket,
+                         int config, int tune_k, spexhip_batch **batch);
+/* chosen configuration, number of packets, autotuner timings (ms, one per
+ * configuration; zeros when the configuration was given)                  */
+int spexhip_batch_config(spexhip_ctx *ctx, const spexhip_batch *batch, int *config,
+                         int *npacket, float *tune_ms);
+/* Enqueue one run; use_graph: capture the fork/join graph on the first call
+ * and replay it afterwards.                                               */
+int spexhip_batch_launch(spexhip_ctx *ctx, spexhip_batch *batch, int use_graph);
+int spexhip_batch_wait(spexhip_ctx *ctx, spexhip_batch *batch, float *ms);
+int spexhip_batch_result(spexhip_ctx *ctx, spexhip_batch *batch, int iq, double *chi_packed,
+                         double *einv_packed);
+int spexhip_batch_destroy(spexhip_ctx *ctx, spexhip_batch *batch);
 
 /* ---- serving: continuous batching of q-point requests ------------------ */
 /* No reference counterpart (SPEX hands q-points to idle MPI ranks through

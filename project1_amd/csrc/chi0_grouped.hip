@@ -153,6 +153,10 @@ static int group_launch(spexhip_ctx *ctx, hipStream_t st, const spexhip_group *g
   return SPEXHIP_OK;
 }
 
+int spx_group_launch(spexhip_ctx *ctx, hipStream_t st, const spexhip_group *g) {
+  return group_launch(ctx, st, g);
+}
+
 extern "C" {
 
 int spexhip_chi0_configs(int *count, int *tm, int *kc, int *per_cu) {

@@ -12,12 +12,18 @@
 //     B/s = sum_{p=1..4} a_p w_p + r,  w_p = 1/(127*254^(p-1)),
 //     a_p in [-127,127], |r| <= 0.5*w_4,
 // (rounding to nearest at every level), so the representation error is
-// below 2.4e-10 * s.  Products of slices with p+q <= 5 are accumulated
-// exactly in int32 on v_mfma_i32_16x16x64_i8 -- one int32 accumulator per
-// level l = p+q, since w_p w_q depends on p+q only -- and flushed into FP64
-// accumulators (weight w_l) every 4096 transitions, far below the int32
-// overflow bound (4 pairs * 2 products * 127^2 * 4096 < 2^31):
-//     Re chi = sum Br_I Br_J + Bi_I Bi_J,  Im chi = sum Bi_I Br_J - Br_I Bi_J,
+// below 2.4e-10 * s.  Products of slices with p+q <= 5, plus the real
+// part of the pair (3,3), are accumulated exactly in int32 on
+// v_mfma_i32_16x16x64_i8 -- one int32 accumulator per level l = p+q, since
+// w_p w_q depends on p+q only -- and flushed into FP64 accumulators (weight
+// w_l) every 4096 transitions, far below the int32 overflow bound
+// (4 pairs * 2 products * 127^2 * 4096 < 2^31).  Of the dropped pairs
+// (p+q >= 6), those with p != q are products of independent digits and
+// average out over the transitions; (3,3) is kept because on the diagonal
+// (and for correlated columns) it is a sum of squares, a bias of about
+// 254^-4/3 (s/rms)^2 relative to chi.
+//
+//     Re chi = sum Br_I Br_J + Bi_I Bi_J,  Im chi = sum Bi_I Br_J + (-Br_I) Bi_J,
 // then chi = sgn * s_I s_J * (...).
 //
 // Slice layout (one frequency), "MFMA-ready": for 64-transition chunk c,
@@ -158,6 +164,19 @@ __global__ __launch_bounds__(NT) void ozk_slice_kernel(long n, int nb, int nbp,
   }
 }
 
+// Bytewise negation of 16 int8 in [-127, 127] (two's complement per byte,
+// no carries between bytes): -x = (~x & 0x7f..) + 1 with the sign bit of ~x
+// xor-ed back in.
+__device__ inline i4_t neg_i8x16(i4_t x) {
+  i4_t r;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const unsigned n = ~(unsigned)x[k];
+    r[k] = (int)(((n & 0x7f7f7f7fu) + 0x01010101u) ^ (n & 0x80808080u));
+  }
+  return r;
+}
+
 __device__ inline void tri_index(int x, int &bi, int &bj) {
   int j = (int)((sqrtf(8.f * (float)x + 1.f) - 1.f) * 0.5f);
   while ((j + 1) * (j + 2) / 2 <= x) j++;
@@ -203,7 +222,11 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     return *(const i4_t *)(Ls + ((((buf * 2 + panel) * NPL + pl) * 4 + blk) * 1024) + 16 * lane);
   };
 
-  i4_t re[NS][2][2], ia[NS][2][2], ib_[NS][2][2];  // level p+q-2 -> index
+  // level l = p+q (0-based slices) -> one int32 accumulator for Re and one
+  // for Im per 16x16 block; Im = Ai.Br + (-Ar).Bi.  r4: Re of the pair of
+  // third slices (0-based p = q = 2), the one level-4 term that does not average out (on the
+  // diagonal it is a sum of squares).
+  i4_t re[NS][2][2], im[NS][2][2], r4[2][2];
   d4_t fr[2][2], fi[2][2];
 #pragma unroll
   for (int a = 0; a < 2; a++)
@@ -211,11 +234,11 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     for (int b = 0; b < 2; b++) {
       fr[a][b] = d4_t{0, 0, 0, 0};
       fi[a][b] = d4_t{0, 0, 0, 0};
+      r4[a][b] = i4_t{0, 0, 0, 0};
 #pragma unroll
       for (int l = 0; l < NS; l++) {
         re[l][a][b] = i4_t{0, 0, 0, 0};
-        ia[l][a][b] = i4_t{0, 0, 0, 0};
-        ib_[l][a][b] = i4_t{0, 0, 0, 0};
+        im[l][a][b] = i4_t{0, 0, 0, 0};
       }
     }
   auto flush = [&]() {
@@ -225,15 +248,16 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
       for (int b = 0; b < 2; b++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          // level weights w_p w_q = 1/(127^2 254^(p+q-2)); small terms first
-          double sr = 0.0, si = 0.0;
+          // level weights w_p w_q = 1/(127^2 254^(p+q)); small terms first
+          double sr = (double)r4[a][b][r] * (1.0 / (127.0 * 127.0 * 254.0 * 254.0 * 254.0 * 254.0));
+          double si = 0.0;
 #pragma unroll
           for (int l = NS - 1; l >= 0; l--) {
             double lw = 1.0 / (127.0 * 127.0);
 #pragma unroll
             for (int e = 0; e < l; e++) lw *= 1.0 / 254.0;
             sr += lw * (double)re[l][a][b][r];
-            si += lw * ((double)ia[l][a][b][r] - (double)ib_[l][a][b][r]);
+            si += lw * (double)im[l][a][b][r];
           }
           fr[a][b][r] += sr;
           fi[a][b][r] += si;
@@ -241,13 +265,14 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++)
+      for (int b = 0; b < 2; b++) {
+        r4[a][b] = i4_t{0, 0, 0, 0};
 #pragma unroll
         for (int l = 0; l < NS; l++) {
           re[l][a][b] = i4_t{0, 0, 0, 0};
-          ia[l][a][b] = i4_t{0, 0, 0, 0};
-          ib_[l][a][b] = i4_t{0, 0, 0, 0};
+          im[l][a][b] = i4_t{0, 0, 0, 0};
         }
+      }
   };
 
   issue(0, 0);
@@ -256,31 +281,51 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c have landed
     __syncthreads();                // ... and every other wave's
     if (c + 1 < nchunk) issue(c + 1, buf ^ 1);  // buffer buf^1 was last read before the barrier
+    // every column-side operand of the chunk stays in registers; the row
+    // side is read once per slice
+    i4_t Br[NS][2], Bi[NS][2];
 #pragma unroll
-    for (int q = 0; q < NS; q++) {
-      i4_t Br[2], Bi[2];
+    for (int q = 0; q < NS; q++)
 #pragma unroll
       for (int b = 0; b < 2; b++) {
-        Br[b] = opnd(buf, 1, q, wc * 2 + b);
-        Bi[b] = opnd(buf, 1, NS + q, wc * 2 + b);
+        Br[q][b] = opnd(buf, 1, q, wc * 2 + b);
+        Bi[q][b] = opnd(buf, 1, NS + q, wc * 2 + b);
       }
 #pragma unroll
-      for (int p = 0; p < NS - q; p++) {
-        const int l = p + q;  // slices p+1, q+1: level p+q+2 -> index p+q
-        i4_t Ar[2], Ai[2];
+    for (int p = 0; p < NS; p++) {
+      i4_t Ar[2], Ai[2], An[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-          Ar[a] = opnd(buf, 0, p, wr * 2 + a);
-          Ai[a] = opnd(buf, 0, NS + p, wr * 2 + a);
-        }
+      for (int a = 0; a < 2; a++) {
+        Ar[a] = opnd(buf, 0, p, wr * 2 + a);
+        Ai[a] = opnd(buf, 0, NS + p, wr * 2 + a);
+        An[a] = neg_i8x16(Ar[a]);
+      }
+#pragma unroll
+      for (int q = 0; q < NS - p; q++) {
+        const int l = p + q;
+        // independent accumulators first, the dependent second halves after
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
           for (int b = 0; b < 2; b++) {
-            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[b], re[l][a][b], 0, 0, 0);
-            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[b], re[l][a][b], 0, 0, 0);
-            ia[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[b], ia[l][a][b], 0, 0, 0);
-            ib_[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Bi[b], ib_[l][a][b], 0, 0, 0);
+            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[q][b], re[l][a][b], 0, 0, 0);
+            im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[q][b], im[l][a][b], 0, 0, 0);
+          }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[q][b], re[l][a][b], 0, 0, 0);
+            im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(An[a], Bi[q][b], im[l][a][b], 0, 0, 0);
+          }
+      }
+      if (p == 2) {
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+          for (int b = 0; b < 2; b++) {
+            r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[2][b], r4[a][b], 0, 0, 0);
+            r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[2][b], r4[a][b], 0, 0, 0);
           }
       }
     }

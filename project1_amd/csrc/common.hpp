@@ -21,7 +21,10 @@ struct spexhip_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   std::mutex mu;  // guards err, used, allocs, cache
-  std::unordered_map<void *, size_t> allocs;
+  struct Block {
+    size_t bytes, cls;  // requested size (ownership checks), size class (cache)
+  };
+  std::unordered_map<void *, Block> allocs;
   std::multimap<size_t, void *> cache;  // freed blocks by size class
   size_t cached = 0;
   // scratch for host drop-in calls (grown on demand, owned by the context)

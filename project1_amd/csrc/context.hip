@@ -54,7 +54,7 @@ int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p) {
       *p = it->second;
       ctx->cache.erase(it);
       ctx->cached -= cls;
-      ctx->allocs[*p] = cls;
+      ctx->allocs[*p] = {bytes, cls};
       return SPEXHIP_OK;
     }
   }
@@ -72,7 +72,7 @@ int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p) {
     return e == hipErrorOutOfMemory ? SPEXHIP_E_NOMEM : SPEXHIP_E_DEVICE;
   }
   std::lock_guard<std::mutex> g(ctx->mu);
-  ctx->allocs[*p] = cls;
+  ctx->allocs[*p] = {bytes, cls};
   return SPEXHIP_OK;
 }
 
@@ -85,11 +85,11 @@ int spx_free(spexhip_ctx *ctx, void *p) {
     ctx->err = "spexhip_dev_free: pointer not owned by this context";
     return SPEXHIP_E_INVALID;
   }
-  const size_t bytes = it->second;
+  const size_t cls = it->second.cls;
   ctx->allocs.erase(it);
-  ctx->used -= bytes;
-  ctx->cache.emplace(bytes, p);
-  ctx->cached += bytes;
+  ctx->used -= cls;
+  ctx->cache.emplace(cls, p);
+  ctx->cached += cls;
   return SPEXHIP_OK;
 }
 
@@ -98,7 +98,7 @@ bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes) {
   const char *c = (const char *)p;
   for (auto &kv : ctx->allocs) {
     const char *b = (const char *)kv.first;
-    if (c >= b && c + bytes <= b + kv.second) return true;
+    if (c >= b && c + bytes <= b + kv.second.bytes) return true;
   }
   return false;
 }

@@ -8,6 +8,8 @@
 int spx_weights_launch(spexhip_ctx *ctx, hipStream_t st, int nk, int nocc, int nunocc,
                        const void *e_occ, const void *e_unocc, int nkpt, int nsym1,
                        const void *nkpts, int nw, const void *omega, void *wght);
+// grouped chi0 (persistent, many problems)
+int spx_group_launch(spexhip_ctx *ctx, hipStream_t st, const spexhip_group *g);
 // chi0 Ozaki form
 size_t spx_ozaki_ws_bytes(long n, int nb, int nw, int nwg);
 int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M,

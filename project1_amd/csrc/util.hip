@@ -1,6 +1,8 @@
 // Layout conversion for the host drop-in entry points: device products are
 // transition-major (t*ldm + I); the reference's cprod is column-major
 // (t fastest, wavefproducts.f:120,490).
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace {
@@ -22,6 +24,13 @@ __global__ void transpose_kernel(const double2 *__restrict__ M, long rows, int c
     const long r = r0 + tx;
     if (r < rows && c < cols) T[r + rows * (long)c] = tile[tx][j];
   }
+}
+
+// Plain streaming copy, 16 B per thread per step (grid-stride).
+__global__ void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, long n16) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 __global__ void fill_zero_kernel(unsigned int *__restrict__ p, long count) {
@@ -54,5 +63,21 @@ int spx_transpose_d2h(spexhip_ctx *ctx, hipStream_t st, const void *M, long rows
   SPX_LAUNCHED(ctx);
   SPX_HIP(ctx, hipMemcpyAsync(host, T, bytes, hipMemcpyDeviceToHost, st));
   SPX_HIP(ctx, hipStreamSynchronize(st));
+  return SPEXHIP_OK;
+}
+
+extern "C" int spexhip_copy_dev(spexhip_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  if (!bytes) return SPEXHIP_OK;
+  SPX_CHECK(ctx, bytes % 16 == 0 && ((size_t)dst | (size_t)src) % 16 == 0,
+            "copy: pointers and size must be multiples of 16 bytes");
+  SPX_CHECK(ctx, spx_owns(ctx, dst, bytes) && spx_owns(ctx, src, bytes),
+            "copy: ranges outside context allocations");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const long n16 = (long)(bytes / 16);
+  const long grid = std::min<long>((n16 + 255) / 256, 8192);
+  hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const uint4 *)src,
+                     (uint4 *)dst, n16);
+  SPX_LAUNCHED(ctx);
   return SPEXHIP_OK;
 }

@@ -26,6 +26,7 @@
 // The accumulator layout of one MFMA (column j = lane%16, rows lane/16+4r)
 // is exactly the operand layout the next MFMA needs, so no transposes go
 // through LDS between passes.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -225,12 +226,21 @@ template <int NB>
 __global__ __launch_bounds__(NT) void pw_pair_fused_kernel(
     const double2 *__restrict__ grids, int nocc, int nunocc, int N1, int N2, int N3,
     const int *__restrict__ oidx, int ngptm, double2 *__restrict__ M, long ldm, long t0, int col0) {
+  constexpr int KX = 4 * NB;
   __shared__ double2 tz[64][PW_BOX];               // e^{-2 pi i (g-8) z/N3}
   __shared__ double2 Qs[NT / 64][PW_BOX * PW_BOX];  // per-wave plane result
+  // MFMA operands of the x and y passes, per lane: e^{-2 pi i (lr-8) x/N}
+  // with x = 4kk + lk (kept in LDS rather than registers, for occupancy)
+  __shared__ double2 F1s[KX][64], F2s[KX][64];
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
   for (int e = tid; e < N3 * PW_BOX; e += NT) {
     const int z = e / PW_BOX, g = e % PW_BOX;
     tz[z][g] = twid((long)(g - PW_OFF) * z, N3, -1.0);
+  }
+  for (int e = tid; e < KX * 64; e += NT) {
+    const int kk = e >> 6, ln = e & 63, x = 4 * kk + (ln >> 4), g = (ln & 15) - PW_OFF;
+    F1s[kk][ln] = x < N1 ? twid((long)g * x, N1, -1.0) : make_double2(0.0, 0.0);
+    F2s[kk][ln] = x < N2 ? twid((long)g * x, N2, -1.0) : make_double2(0.0, 0.0);
   }
   __syncthreads();
   const int b2 = blockIdx.x;
@@ -239,15 +249,6 @@ __global__ __launch_bounds__(NT) void pw_pair_fused_kernel(
   const long NN = (long)N1 * N2 * N3;
   const double2 *p1 = grids + (long)b1 * NN;
   const double2 *p2 = grids + (long)(nocc + b2) * NN;
-  constexpr int KX = 4 * NB;
-  double2 F1[KX], F2[KX];
-#pragma unroll
-  for (int kk = 0; kk < KX; kk++) {
-    const int x = 4 * kk + lk;
-    F1[kk] = x < N1 ? twid((long)(lr - PW_OFF) * x, N1, -1.0) : make_double2(0.0, 0.0);
-    const int y = 4 * kk + lk;
-    F2[kk] = y < N2 ? twid((long)(lr - PW_OFF) * y, N2, -1.0) : make_double2(0.0, 0.0);
-  }
   int ob[PW_FOUT];
   double2 acc[PW_FOUT];
 #pragma unroll
@@ -274,14 +275,14 @@ __global__ __launch_bounds__(NT) void pw_pair_fused_kernel(
             const long g = x + (long)N1 * (y + (long)N2 * z);
             a = cmul(p1[g], p2[g]);
           }
-          cmfma(xr[mb], xi[mb], a, F1[kk]);
+          cmfma(xr[mb], xi[mb], a, F1s[kk][l]);
         }
       }
     }
     d4_t qr = {0, 0, 0, 0}, qi = {0, 0, 0, 0};
 #pragma unroll
     for (int kk = 0; kk < KX; kk++)
-      if (4 * kk < N2) cmfma(qr, qi, F2[kk], make_double2(xr[kk / 4][kk % 4], xi[kk / 4][kk % 4]));
+      if (4 * kk < N2) cmfma(qr, qi, F2s[kk][l], make_double2(xr[kk / 4][kk % 4], xi[kk / 4][kk % 4]));
     // Q(gy, gx): lane holds gx = lr, gy = lk + 4r
 #pragma unroll
     for (int r = 0; r < 4; r++) Q[(lk + 4 * r) * PW_BOX + lr] = make_double2(qr[r], qi[r]);
@@ -337,8 +338,9 @@ __global__ __launch_bounds__(NT) void pw_pair_z_kernel(const double2 *__restrict
   }
 }
 
-int box_index(const int *imati, const int *v, int &idx) {
-  int g[3];
+int box_index(const int *imati, const int *v, int &idx, int *gout = nullptr) {
+  int g3[3];
+  int *g = gout ? gout : g3;
   for (int i = 0; i < 3; i++) {
     g[i] = imati[3 * i] * v[0] + imati[3 * i + 1] * v[1] + imati[3 * i + 2] * v[2];
     if (g[i] < -PW_OFF || g[i] >= PW_BOX - PW_OFF) return 1;
@@ -399,17 +401,34 @@ int spx_pw_plan_build(spexhip_ctx *ctx, const spexhip_pw_desc *d, spexhip_plan *
   p->ngpt.assign(d->ngpt, d->ngpt + d->nkpt);
   p->gptm.assign(d->gptm, d->gptm + 3 * d->ngptm);
   std::vector<int> tab((size_t)d->nkpt * d->maxgpt + d->nkpt, 0);
+  int lo[3] = {PW_BOX, PW_BOX, PW_BOX}, hi[3] = {-PW_BOX, -PW_BOX, -PW_BOX};
   for (int k = 0; k < d->nkpt; k++) {
     SPX_CHECK(ctx, d->ngpt[k] >= 0 && d->ngpt[k] <= d->maxgpt, "pw plan: ngpt[%d] out of range", k);
     for (int ig = 0; ig < d->ngpt[k]; ig++) {
-      int idx;
-      if (box_index(p->imati, d->gpt + 3 * ((long)k * d->maxgpt + ig), idx))
+      int idx, g[3];
+      if (box_index(p->imati, d->gpt + 3 * ((long)k * d->maxgpt + ig), idx, g))
         return spx_seterr(ctx, SPEXHIP_E_INVALID,
                           "pw plan: G vector %d of k %d outside the -8..7 Fourier box", ig, k);
       tab[(size_t)k * d->maxgpt + ig] = idx;
+      for (int i = 0; i < 3; i++) {
+        lo[i] = std::min(lo[i], g[i]);
+        hi[i] = std::max(hi[i], g[i]);
+      }
     }
     tab[(size_t)d->nkpt * d->maxgpt + k] = d->ngpt[k];
   }
+  // Without theta the pair product is a pure convolution of two states with
+  // Fourier support of width W = hi - lo per axis, i.e. product frequencies
+  // in [-W, W]; the outputs lie in the -8..7 box.  On a grid with
+  // N >= W + 9 no product frequency aliases onto an output, so the DFT gives
+  // the exact convolution -- the same numbers on every such grid, the
+  // caller's included.  The plan then uses the smallest such grid (at least
+  // the 16-point box); a caller grid that aliases is kept as given.
+  if (!d->step_r)
+    for (int i = 0; i < 3; i++) {
+      const int nmin = std::max(PW_BOX, hi[i] >= lo[i] ? hi[i] - lo[i] + PW_OFF + 1 : PW_BOX);
+      if (p->nfft[i] > nmin) p->nfft[i] = nmin;
+    }
   int rc = spx_alloc(ctx, sizeof(int) * tab.size(), (void **)&p->gidx);
   if (rc) return rc;
   p->owned.push_back(p->gidx);

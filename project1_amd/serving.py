@@ -56,6 +56,8 @@ class Spec:
     nb: int
     nw: int
     seed: int
+    nunocc: int = 0  # 0: the lattice's band window
+    iq: int = 1      # q = k-point iq of the mesh
 
 
 def request_mix(count: int, seed: int, meshes=MESHES, nb_range=NB_RANGE,
@@ -132,10 +134,8 @@ class Mesh:
         self.lat, self.nmesh = lat, nmesh
         self.kpt = kpt = synth.kmesh(nmesh)
         self.nkpt = nk = kpt.shape[0]
-        self.q = kpt[1 % nk].copy()
         self.k1 = np.arange(nk, dtype=np.int32)
-        self.k2 = np.array([synth.kptsum_index(kpt, nmesh, 1 % nk, k) for k in self.k1], np.int32)
-        self.g2 = np.ascontiguousarray(np.rint(kpt[self.k2] - self.q - kpt[self.k1]), np.int32)
+        self._q: dict[int, tuple] = {}
         r = int(np.ceil(lat.gcut)) + 1
         ax = np.arange(-r, r + 1)
         G = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
@@ -162,6 +162,18 @@ class Mesh:
         ddamp.free()
         self.pw_plans: dict[int, spexhip.Plan] = {}
         self.lock = threading.Lock()
+
+    def qpoint(self, iq: int):
+        """(q, k2, g2) for q = kpt[iq]: k2 = index of k+q folded into the mesh,
+        g2 = its umklapp vector (vector.f:107-124, wavefproducts.f:543-552)."""
+        iq %= self.nkpt
+        if iq not in self._q:
+            q = self.kpt[iq].copy()
+            k2 = np.array([synth.kptsum_index(self.kpt, self.nmesh, iq, k) for k in self.k1],
+                          np.int32)
+            g2 = np.ascontiguousarray(np.rint(self.kpt[k2] - q - self.kpt[self.k1]), np.int32)
+            self._q[iq] = (q, k2, g2)
+        return self._q[iq]
 
     def pw_plan(self, ngptm: int) -> spexhip.Plan:
         with self.lock:
@@ -190,28 +202,32 @@ class Request:
         ngptm = nb - natoms * lat.per_atom
         assert ngptm > 0, (nb, natoms)
         mt, pw = lat.mt_plan(natoms), mesh.pw_plan(ngptm)
+        nunocc = spec.nunocc or lat.nunocc
+        assert lat.nocc + nunocc <= lat.nband, (nunocc, lat.nband)
+        q, k2, g2 = mesh.qpoint(spec.iq)
         g = synth.rng(spec.seed)
-        eo, eu = synth.energies(g, nk, lat.nocc, lat.nunocc)
+        eo, eu = synth.energies(g, nk, lat.nocc, nunocc)
         om, _ = synth.gl_frequencies(nw)
         vh = np.asfortranarray(lat.vhalf[:nb, :nb].astype(np.complex128))
         self.bufs = [ctx.to_dev(eo), ctx.to_dev(eu), ctx.to_dev(np.ones(nk, np.int32)),
                      ctx.to_dev(om), ctx.to_dev(vh)]
         d = spexhip.QDesc()
-        d.nk, d.nocc, d.nunocc, d.nb, d.nw = nk, lat.nocc, lat.nunocc, nb, nw
+        d.nk, d.nocc, d.nunocc, d.nb, d.nw = nk, lat.nocc, nunocc, nb, nw
         d.precision, d.vkind, d.alpha, d.ozaki_group = precision, 1, 2.0, ozaki_group
         d.mt_plan, d.pw_plan = mt.h, pw.h
         d.cmt_dev, d.ncent, d.nband, d.nkpt = mesh.dcmt.ptr, lat.max_atoms, lat.nband, nk
-        d.q[:] = [float(v) for v in mesh.q]
+        d.q[:] = [float(v) for v in q]
         d.cpw_dev = mesh.dcpw.ptr
-        d.k1, d.k2, d.g2 = mesh.k1.ctypes.data, mesh.k2.ctypes.data, mesh.g2.ctypes.data
+        d.k1, d.k2, d.g2 = mesh.k1.ctypes.data, k2.ctypes.data, g2.ctypes.data
         d.b1lo, d.b2lo = 0, lat.nocc
         d.e_occ_dev, d.e_unocc_dev, d.nkpts_dev = (self.bufs[0].ptr, self.bufs[1].ptr,
                                                    self.bufs[2].ptr)
         d.nkpt_w, d.nsym1 = nk, 1
         d.omega_dev, d.vhalf_dev = self.bufs[3].ptr, self.bufs[4].ptr
         self.desc = d
-        self._keep = (mesh,)  # k lists referenced by the descriptor
-        self.n = nk * lat.nocc * lat.nunocc
+        self._keep = (mesh, k2, g2)  # k lists referenced by the descriptor
+        self.e_occ, self.e_unocc, self.omega = eo, eu, om
+        self.n = nk * lat.nocc * nunocc
         self.flops = chi0_flops(nw, self.n, nb) + eps_flops(nw, nb)
         self.footprint = spexhip.qpipe_footprint(d)
 
@@ -368,6 +384,8 @@ def serve_benchmark(ctx: spexhip.Context, precisions: dict, seconds: float, cap_
     specs = request_mix(4096, seed)
     for m in sorted({s.nmesh for s in specs}):
         lat.mesh(m)
+    for na in sorted({lat.natoms_for(s.nb) for s in specs}):
+        lat.mt_plan(na)
     out = {"mix": {"Nk": [m**3 for m in MESHES], "Nb": list(NB_RANGE), "Nw": list(NW_CHOICES),
                    "Nocc": lat.nocc, "Nunocc": lat.nunocc, "max_inflight": max_inflight,
                    "hbm_budget_gib": hbm_budget / 2**30, "load": load}}

@@ -152,6 +152,59 @@ def qpipe_footprint(desc: QDesc) -> int:
     return v.value
 
 
+class Batch:
+    """Many q-points: per k packet one grouped chi0 launch over all q, each
+    q's products and eps / eps^-1 on its own stream; graph-replayable."""
+
+    def __init__(self, ctx: "Context", descs: list, kpacket: int, config: int = -1,
+                 tune_k: int = 16384):
+        self.ctx = ctx
+        self.descs = descs
+        self._arr = (QDesc * len(descs))(*descs)
+        h = _vp()
+        ctx._check(lib().spexhip_batch_create(ctx.h, len(descs), C.cast(self._arr, _vp), kpacket,
+                                              config, tune_k, C.byref(h)))
+        self.h = h.value
+
+    @staticmethod
+    def footprint(descs: list, kpacket: int) -> int:
+        arr = (QDesc * len(descs))(*descs)
+        v = C.c_size_t()
+        rc = lib().spexhip_batch_footprint(len(descs), C.cast(arr, _vp), kpacket, C.byref(v))
+        if rc:
+            raise SpexError(rc, "batch_footprint")
+        return v.value
+
+    def config(self):
+        c, n = C.c_int(), C.c_int()
+        t = (C.c_float * 16)()
+        self.ctx._check(lib().spexhip_batch_config(self.ctx.h, self.h, C.byref(c), C.byref(n), t))
+        return c.value, n.value, [t[i] for i in range(len(chi0_configs()))]
+
+    def launch(self, graph: bool = False) -> None:
+        self.ctx._check(lib().spexhip_batch_launch(self.ctx.h, self.h, int(graph)))
+
+    def wait(self) -> float:
+        v = C.c_float()
+        self.ctx._check(lib().spexhip_batch_wait(self.ctx.h, self.h, C.byref(v)))
+        return v.value
+
+    def result(self, iq: int, chi: bool = True, einv: bool = True):
+        nb, nw = self.descs[iq].nb, self.descs[iq].nw
+        npack = nb * (nb + 1) // 2
+        a = np.zeros((nw, npack), np.complex128) if chi else None
+        b = np.zeros((nw, npack), np.complex128) if einv else None
+        self.ctx._check(lib().spexhip_batch_result(self.ctx.h, self.h, iq,
+                                                   _ptr(a) if chi else None,
+                                                   _ptr(b) if einv else None))
+        return a, b
+
+    def destroy(self) -> None:
+        if self.h:
+            self.ctx._check(lib().spexhip_batch_destroy(self.ctx.h, self.h))
+            self.h = None
+
+
 class Server:
     """Continuous-batching request loop over per-q pipelines (spexhip_server_*).
 
@@ -245,6 +298,7 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_h2d": (I, [_vp, _vp, _vp, C.c_size_t]),
         "spexhip_d2h": (I, [_vp, _vp, _vp, C.c_size_t]),
         "spexhip_memset": (I, [_vp, _vp, I, C.c_size_t]),
+        "spexhip_copy_dev": (I, [_vp, _vp, _vp, C.c_size_t]),
         "spexhip_weights_none_dev": (I, [_vp, I, I, I, _vp, _vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_chi0_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
                                  C.c_double, C.c_double, _vp, I]),
@@ -272,6 +326,13 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_qpipe_result": (I, [_vp, _vp, _vp, _vp]),
         "spexhip_qpipe_buffers": (I, [_vp, _vp, P(_vp), P(_vp), P(_vp)]),
         "spexhip_qpipe_destroy": (I, [_vp, _vp]),
+        "spexhip_batch_footprint": (I, [I, _vp, I, P(C.c_size_t)]),
+        "spexhip_batch_create": (I, [_vp, I, _vp, I, I, I, P(_vp)]),
+        "spexhip_batch_config": (I, [_vp, _vp, P(I), P(I), P(C.c_float)]),
+        "spexhip_batch_launch": (I, [_vp, _vp, I]),
+        "spexhip_batch_wait": (I, [_vp, _vp, P(C.c_float)]),
+        "spexhip_batch_result": (I, [_vp, _vp, I, _vp, _vp]),
+        "spexhip_batch_destroy": (I, [_vp, _vp]),
         "spexhip_server_create": (I, [_vp, I, C.c_size_t, I, P(_vp)]),
         "spexhip_server_submit": (I, [_vp, P(QDesc), P(_c_long)]),
         "spexhip_server_poll": (I, [_vp, _c_long, P(I)]),
@@ -435,6 +496,9 @@ class Context:
                           scale: DevBuf) -> None:
         self._check(lib().spexhip_synth_normal_rows_dev(self.h, buf.ptr, rows, cols, nmat, seed,
                                                         scale.ptr))
+
+    def copy(self, dst: DevBuf, src: DevBuf, nbytes: int) -> None:
+        self._check(lib().spexhip_copy_dev(self.h, dst.ptr, src.ptr, nbytes))
 
     def event(self) -> "Event":
         return Event(self)
