@@ -47,9 +47,17 @@ def _run(cmd: list[str]) -> None:
         print(out, file=sys.stderr)
 
 
+# Files whose MFMA results feed further MFMAs or VALU work (chained DFT
+# passes, per-plane accumulators): MFMA destinations and C operands in
+# VGPRs, which removes the AGPR<->VGPR copies the default form inserts.  The
+# int8 rank-k kernel keeps the default (its 36 accumulators live in AGPRs).
+VGPR_FORM = {"wfprod_pw"}
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    _run([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
+    extra = ["-mllvm", "-amdgpu-mfma-vgpr-form"] if src.stem in VGPR_FORM else []
+    _run([HIPCC, *HIP_FLAGS, *extra, "-c", str(src), "-o", str(obj)])
     return obj
 
 

@@ -35,6 +35,7 @@
 // per wave-instruction) and reads each operand with one conflict-free
 // 16-byte LDS load per lane.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -224,44 +225,10 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
 
   // level l = p+q (0-based slices) -> one int32 accumulator for Re and one
   // for Im per 16x16 block; Im = Ai.Br + (-Ar).Bi.  r4: Re of the pair of
-  // third slices (0-based p = q = 2), the one level-4 term that does not average out (on the
-  // diagonal it is a sum of squares).
+  // third slices (0-based p = q = 2), the one level-4 term that does not
+  // average out (on the diagonal it is a sum of squares).
   i4_t re[NS][2][2], im[NS][2][2], r4[2][2];
-  d4_t fr[2][2], fi[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      fr[a][b] = d4_t{0, 0, 0, 0};
-      fi[a][b] = d4_t{0, 0, 0, 0};
-      r4[a][b] = i4_t{0, 0, 0, 0};
-#pragma unroll
-      for (int l = 0; l < NS; l++) {
-        re[l][a][b] = i4_t{0, 0, 0, 0};
-        im[l][a][b] = i4_t{0, 0, 0, 0};
-      }
-    }
-  auto flush = [&]() {
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-      for (int b = 0; b < 2; b++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          // level weights w_p w_q = 1/(127^2 254^(p+q)); small terms first
-          double sr = (double)r4[a][b][r] * (1.0 / (127.0 * 127.0 * 254.0 * 254.0 * 254.0 * 254.0));
-          double si = 0.0;
-#pragma unroll
-          for (int l = NS - 1; l >= 0; l--) {
-            double lw = 1.0 / (127.0 * 127.0);
-#pragma unroll
-            for (int e = 0; e < l; e++) lw *= 1.0 / 254.0;
-            sr += lw * (double)re[l][a][b][r];
-            si += lw * (double)im[l][a][b][r];
-          }
-          fr[a][b][r] += sr;
-          fi[a][b][r] += si;
-        }
+  auto zero_acc = [&]() {
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -274,9 +241,58 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
         }
       }
   };
+  // The FP64 accumulation lives in the output itself: each flush adds the
+  // scaled level sums to chi (the first one also applies beta), so no FP64
+  // accumulators occupy registers during the int8 loop.  The workgroup owns
+  // its tile, so the read-modify-write needs no synchronisation.
+  // v_mfma_i32_16x16x64_i8 result layout: column j = lane%16, rows 4*(lane/16)+r.
+  const int lr = lane & 15, lk = lane >> 4;
+  const int I0 = bi * TT + wr * 32 + 4 * lk, J0 = bj * TT + wc * 32 + lr;
+  double2 *C = chi + (long)w * ldc * nb;
+  const double sg = sgn[w] * alpha;
+  auto flush = [&](bool first) {
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int J = J0 + b * 16;
+      if (J >= nb) continue;
+      const double sJ = sg * scale[(long)w * nb + J];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          const int I = I0 + a * 16 + r;
+          if (I >= nb) continue;
+          // level weights w_p w_q = 1/(127^2 254^(p+q)); small terms first
+          double sr = (double)r4[a][b][r] * (1.0 / (127.0 * 127.0 * 254.0 * 254.0 * 254.0 * 254.0));
+          double si = 0.0;
+#pragma unroll
+          for (int l = NS - 1; l >= 0; l--) {
+            double lw = 1.0 / (127.0 * 127.0);
+#pragma unroll
+            for (int e = 0; e < l; e++) lw *= 1.0 / 254.0;
+            sr += lw * (double)re[l][a][b][r];
+            si += lw * (double)im[l][a][b][r];
+          }
+          const double f = sJ * scale[(long)w * nb + I];
+          double2 *p = C + I + ldc * (long)J;
+          double2 o = make_double2(f * sr, f * si);
+          if (!first || beta != 0.0) {
+            const double2 old = *p;
+            const double bb = first ? beta : 1.0;
+            o.x += bb * old.x;
+            o.y += bb * old.y;
+          }
+          *p = o;
+        }
+    }
+  };
 
-  issue(0, 0);
-  for (long c = 0; c < nchunk; c++) {
+  // One chunk of 64 transitions.  FIRST: the first chunk after a flush --
+  // every accumulator's first MFMA takes C = 0 (an inline constant), so the
+  // accumulators are never zeroed through registers and stay in AGPRs.
+  auto chunk = [&](long c, auto first_tag) {
+    constexpr bool FIRST = decltype(first_tag)::value;
+    const i4_t z4 = {0, 0, 0, 0};
     const int buf = c & 1;
     __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c have landed
     __syncthreads();                // ... and every other wave's
@@ -303,13 +319,14 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
 #pragma unroll
       for (int q = 0; q < NS - p; q++) {
         const int l = p + q;
+        const bool z = FIRST && p == 0;  // first pair of level l in this order
         // independent accumulators first, the dependent second halves after
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
           for (int b = 0; b < 2; b++) {
-            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[q][b], re[l][a][b], 0, 0, 0);
-            im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[q][b], im[l][a][b], 0, 0, 0);
+            re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[q][b], z ? z4 : re[l][a][b], 0, 0, 0);
+            im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[q][b], z ? z4 : im[l][a][b], 0, 0, 0);
           }
 #pragma unroll
         for (int a = 0; a < 2; a++)
@@ -324,42 +341,24 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
         for (int a = 0; a < 2; a++)
 #pragma unroll
           for (int b = 0; b < 2; b++) {
-            r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[2][b], r4[a][b], 0, 0, 0);
+            r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[2][b], FIRST ? z4 : r4[a][b], 0, 0, 0);
             r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[2][b], r4[a][b], 0, 0, 0);
           }
       }
     }
-    if ((c + 1) % FLUSH == 0) flush();
-  }
-  flush();
+  };
 
-  // v_mfma_i32_16x16x64_i8 result layout: column j = lane%16, rows 4*(lane/16)+r.
-  const int lr = lane & 15, lk = lane >> 4;
-  const int I0 = bi * TT, J0 = bj * TT;
-  double2 *C = chi + (long)w * ldc * nb;
-  const double sg = sgn[w] * alpha;
-#pragma unroll
-  for (int a = 0; a < 2; a++)
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int J = J0 + wc * 32 + b * 16 + lr;
-      if (J >= nb) continue;
-      const double sJ = scale[(long)w * nb + J];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int I = I0 + wr * 32 + a * 16 + 4 * lk + r;
-        if (I >= nb) continue;
-        const double f = sg * scale[(long)w * nb + I] * sJ;
-        double2 *p = C + I + ldc * (long)J;
-        double2 o = make_double2(f * fr[a][b][r], f * fi[a][b][r]);
-        if (beta != 0.0) {
-          const double2 old = *p;
-          o.x += beta * old.x;
-          o.y += beta * old.y;
-        }
-        *p = o;
-      }
-    }
+  issue(0, 0);
+  for (long c0 = 0; c0 < nchunk; c0 += FLUSH) {
+    const long ce = c0 + FLUSH < nchunk ? c0 + FLUSH : nchunk;
+    chunk(c0, std::true_type{});
+    for (long c = c0 + 1; c < ce; c++) chunk(c, std::false_type{});
+    flush(c0 == 0);
+  }
+  if (nchunk == 0) {  // no transitions: chi = beta * chi
+    zero_acc();
+    flush(true);
+  }
 }
 
 // sgn[w] = +1 / -1 from the sign flags; a frequency with weights of both

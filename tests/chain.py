@@ -6,6 +6,16 @@ import numpy as np
 import oracle
 
 
+U64 = 2.0**-53
+
+
+def einv_tol(cond, nb, chi_err=0.0):
+    """Bound for the normwise difference of two eps^-1: backward-stable FP64
+    inversions differ by up to ~c n u cond(eps), and a relative chi0
+    difference chi_err propagates as ~cond(eps) chi_err."""
+    return 1e-13 + 10 * cond * (nb * U64 + chi_err)
+
+
 def eps_cond(chi, vh):
     """Largest 2-norm condition number of eps = 1 - v^1/2 chi0 v^1/2 over the
     frequencies (chi: nw x packed).  Two backward-stable FP64 inversions of
