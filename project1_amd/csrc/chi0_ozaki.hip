@@ -82,6 +82,7 @@ __global__ void ozk_scale_kernel(long n, int nb, const double2 *__restrict__ M, 
   double m[16];
 #pragma unroll
   for (int w = 0; w < 16; w++) m[w] = 0.0;
+#pragma unroll 4
   for (long t = t0; t < t1; t++) {
     const double2 v = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
     const double a = fmax(fabs(v.x), fabs(v.y));
@@ -435,7 +436,7 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
   hipLaunchKernelGGL(ozk_sign_kernel, dim3(1), dim3(64), 0, st, nw, flags, sgn, err_dev);
   SPX_LAUNCHED(ctx);
   {
-    const long tchunk = 4096;
+    const long tchunk = 1024;  // enough workgroups to keep HBM busy; few atomics per column
     dim3 grid((nb + 255) / 256, (n + tchunk - 1) / tchunk);
     if (colmajor)
       hipLaunchKernelGGL(ozk_scale_kernel<true>, grid, dim3(256), 0, st, n, nb, (const double2 *)M,
