@@ -48,6 +48,7 @@ constexpr int TK = 64;     // transitions per chunk (MFMA K)
 constexpr int TT = 64;     // output tile edge
 constexpr int NT = 256;
 constexpr int FLUSH = 64;  // chunks between int32 -> FP64 flushes
+constexpr int PFD = 3;     // L2 prefetch distance in chunks
 constexpr int NPL = 2 * NS;  // planes per chunk
 
 __device__ inline long blk_off(long c, int pl, int ib, int nib) {
@@ -187,6 +188,42 @@ __device__ inline void tri_index(int x, int &bi, int &bj) {
   bi = x - j * (j + 1) / 2;
 }
 
+// Linear workgroup id -> (frequency in the group, 64x64 tile bi <= bj).
+// Workgroups go to the 8 XCDs round-robin (XCD = id % 8) and each XCD has
+// its own L2, so every XCD is given one contiguous range of the logical
+// order -- frequency-major, tiles in 4x4 super-blocks (column of
+// super-blocks by column) -- and the ~32 tiles resident on an XCD at a time
+// share a few row blocks of slices instead of all of them.
+__device__ inline void xcd_tile(long L, long P, int T, int nbt, int &wg, int &bi, int &bj) {
+  const long per = P / 8, rem = P % 8;
+  const long x = L % 8, j = L / 8;
+  const long lg = x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
+  wg = (int)(lg / T);
+  int t = (int)(lg % T);
+  bi = bj = 0;
+  const int ns = (nbt + 3) / 4;
+  for (int SJ = 0; SJ < ns; SJ++) {
+    const int cols = min(4, nbt - 4 * SJ);
+    for (int SI = 0; SI <= SJ; SI++) {
+      const int rows = min(4, nbt - 4 * SI);
+      const int cnt = SI < SJ ? rows * cols : rows * (rows + 1) / 2;
+      if (t < cnt) {
+        if (SI < SJ) {
+          bi = 4 * SI + t / cols;
+          bj = 4 * SJ + t % cols;
+        } else {
+          int i, jj;
+          tri_index(t, i, jj);
+          bi = 4 * SI + i;
+          bj = 4 * SI + jj;
+        }
+        return;
+      }
+      t -= cnt;
+    }
+  }
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Rank-k update on int8 slices: 64x64 tile per workgroup, 4 waves (2x2) of
@@ -194,16 +231,17 @@ typedef __attribute__((address_space(3))) void lds_void;
 // 4 row blocks + 8 planes x 4 column blocks (64 KB) into one of two LDS
 // buffers; each wave issues 16 global_load_lds_dwordx4 (1 KB each).
 __global__ __launch_bounds__(NT) void ozk_herk_kernel(
-    long nchunk, int nb, int nbp, const unsigned char *__restrict__ S, long sbytes, int w0,
-    const double *__restrict__ scale, const double *__restrict__ sgn, double2 *__restrict__ chi,
-    long ldc, double alpha, double beta) {
-  // [buf][panel][plane][block of 4][1 KB]
-  __shared__ __attribute__((aligned(1024))) unsigned char Ls[2 * 2 * NPL * 4 * 1024];
+    long nchunk, int nb, int nbp, int ntile, const unsigned char *__restrict__ S, long sbytes,
+    int w0, const double *__restrict__ scale, const double *__restrict__ sgn,
+    double2 *__restrict__ chi, long ldc, double alpha, double beta) {
+  // [buf][panel][plane][block of 4][1 KB], then 256 B of prefetch scratch
+  constexpr int LPF = 2 * 2 * NPL * 4 * 1024;
+  __shared__ __attribute__((aligned(1024))) unsigned char Ls[LPF + 256];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = wv >> 1, wc = wv & 1;
-  int bi, bj;
-  tri_index(blockIdx.x, bi, bj);
-  const int wg = blockIdx.y, w = w0 + wg;
+  int bi, bj, wg;
+  xcd_tile(blockIdx.x, (long)gridDim.x, ntile, nbp / TT, wg, bi, bj);
+  const int w = w0 + wg;
   const int nib = nbp / 16;
   const unsigned char *Sw = S + (long)wg * sbytes;
 
@@ -219,6 +257,28 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
       unsigned char *dst = Ls + ((((buf * 2 + panel) * NPL + pl) * 4 + blk) * 1024);
       __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)dst, 16, 0, 0);
     }
+  };
+  // L2 prefetch of chunk c: one dword of each 128-byte line of this wave's
+  // 16 blocks (128 lines, two instructions), dropped into a scratch LDS word
+  // per lane.  The double buffer alone gives a copy one chunk of MFMA work
+  // to land, less than an HBM round trip under load; with the prefetch
+  // PFD-1 chunks ahead the copies hit L2.
+  auto prefetch = [&](long c) {
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int e = wv * 16 + h * 8 + (lane >> 3);
+      const int panel = e >> 5, pl = (e >> 2) & 7, blk = e & 3;
+      const int ibk = (panel ? bj : bi) * 4 + blk;
+      const unsigned char *src = Sw + blk_off(c, pl, ibk, nib) + 128 * (lane & 7);
+      __builtin_amdgcn_global_load_lds((const void *)src, (lds_void *)(Ls + LPF), 4, 0, 0);
+    }
+  };
+  // s_waitcnt with vmcnt <= n (loads retire in issue order, so the copies
+  // issued before the last n prefetch instructions have landed), lgkmcnt 0.
+  auto wait_copies = [&](int n) {
+    if (n >= 4) __builtin_amdgcn_s_waitcnt(0x74);
+    else if (n >= 2) __builtin_amdgcn_s_waitcnt(0x72);
+    else __builtin_amdgcn_s_waitcnt(0x70);
   };
   auto opnd = [&](int buf, int panel, int pl, int blk) -> i4_t {
     return *(const i4_t *)(Ls + ((((buf * 2 + panel) * NPL + pl) * 4 + blk) * 1024) + 16 * lane);
@@ -295,9 +355,16 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     constexpr bool FIRST = decltype(first_tag)::value;
     const i4_t z4 = {0, 0, 0, 0};
     const int buf = c & 1;
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c have landed
-    __syncthreads();                // ... and every other wave's
+    // this wave's copies of chunk c have landed (prefetches issued after
+    // them may still be in flight) ...
+    wait_copies(c == 0 ? 2 * (int)(nchunk - 1 < PFD - 1 ? nchunk - 1 : PFD - 1)
+                       : (c - 1 + PFD < nchunk ? 2 : 0));
+    // ... and every other wave's (a plain barrier: the wait above already
+    // made this wave's copies visible, and a fence would drain the
+    // prefetches too)
+    __builtin_amdgcn_s_barrier();
     if (c + 1 < nchunk) issue(c + 1, buf ^ 1);  // buffer buf^1 was last read before the barrier
+    if (c + PFD < nchunk) prefetch(c + PFD);
     // every column-side operand of the chunk stays in registers; the row
     // side is read once per slice
     i4_t Br[NS][2], Bi[NS][2];
@@ -350,6 +417,7 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
   };
 
   issue(0, 0);
+  for (long c = 1; c < PFD && c < nchunk; c++) prefetch(c);
   for (long c0 = 0; c0 < nchunk; c0 += FLUSH) {
     const long ce = c0 + FLUSH < nchunk ? c0 + FLUSH : nchunk;
     chunk(c0, std::true_type{});
@@ -460,9 +528,9 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
                          (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, nwaves);
     SPX_LAUNCHED(ctx);
     if (w0 == 0 && timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
-    dim3 hgrid(nbt * (nbt + 1) / 2, g);
-    hipLaunchKernelGGL(ozk_herk_kernel, hgrid, dim3(NT), 0, st, nchunk, nb, nbp, S, (long)sb, w0,
-                       scale, sgn, (double2 *)chi, (long)ldchi, alpha, beta);
+    const int ntile = nbt * (nbt + 1) / 2;
+    hipLaunchKernelGGL(ozk_herk_kernel, dim3(ntile * g), dim3(NT), 0, st, nchunk, nb, nbp, ntile,
+                       S, (long)sb, w0, scale, sgn, (double2 *)chi, (long)ldchi, alpha, beta);
     SPX_LAUNCHED(ctx);
   }
   if (timed) {

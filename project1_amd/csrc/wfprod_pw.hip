@@ -307,6 +307,112 @@ __global__ __launch_bounds__(NT) void pw_pair_fused_kernel(
   }
 }
 
+constexpr int gcd16(int n) { return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : 1; }
+
+// pw_pair_fused_kernel for compile-time in-plane sizes N1 x N2: the x pass
+// covers G = 16/gcd(N2,16) consecutive z planes at once, with the G*N2
+// rows (y, z) packed into RB = G*N2/16 MFMA row blocks and no padding rows
+// (a 24-point grid: 2 planes = 3 blocks instead of 2 x 2 half-empty ones).
+// Row R of the packed x-pass result is (y, zz) = (R % N2, R / N2); the y
+// pass of plane zz takes K-step kq (y = 4kq..4kq+3) from row block
+// (N2*zz + 4kq)/16, register ((N2*zz + 4kq) % 16)/4 -- the accumulator
+// layout (rows lane/16 + 4r) is again the operand layout.
+template <int N1, int N2>
+__global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
+    const double2 *__restrict__ grids, int nocc, int nunocc, int N3, const int *__restrict__ oidx,
+    int ngptm, double2 *__restrict__ M, long ldm, long t0, int col0) {
+  static_assert(N1 % 4 == 0 && N2 % 4 == 0 && N1 >= PW_BOX && N2 >= PW_BOX, "grid");
+  constexpr int G = 16 / gcd16(N2), RB = G * N2 / 16, KX = N1 / 4, KY = N2 / 4;
+  __shared__ double2 tz[64][PW_BOX];
+  __shared__ double2 Qs[NT / 64][PW_BOX * PW_BOX];
+  __shared__ double2 F1s[KX][64], F2s[KY][64];
+  const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  for (int e = tid; e < N3 * PW_BOX; e += NT) {
+    const int z = e / PW_BOX, g = e % PW_BOX;
+    tz[z][g] = twid((long)(g - PW_OFF) * z, N3, -1.0);
+  }
+  for (int e = tid; e < KX * 64; e += NT) {
+    const int kk = e >> 6, ln = e & 63;
+    F1s[kk][ln] = twid((long)((ln & 15) - PW_OFF) * (4 * kk + (ln >> 4)), N1, -1.0);
+  }
+  for (int e = tid; e < KY * 64; e += NT) {
+    const int kk = e >> 6, ln = e & 63;
+    F2s[kk][ln] = twid((long)((ln & 15) - PW_OFF) * (4 * kk + (ln >> 4)), N2, -1.0);
+  }
+  __syncthreads();
+  const int b2 = blockIdx.x;
+  const int b1 = blockIdx.y * (NT / 64) + wv;
+  if (b1 >= nocc) return;  // whole wave; no workgroup barrier after this point
+  const long NN = (long)N1 * N2 * N3;
+  const double2 *p1 = grids + (long)b1 * NN;
+  const double2 *p2 = grids + (long)(nocc + b2) * NN;
+  int ob[PW_FOUT];
+  double2 acc[PW_FOUT];
+#pragma unroll
+  for (int j = 0; j < PW_FOUT; j++) {
+    const int ig = l + 64 * j;
+    ob[j] = ig < ngptm ? oidx[ig] : -1;
+    acc[j] = make_double2(0.0, 0.0);
+  }
+  double2 *Q = Qs[wv];
+  for (int z0 = 0; z0 < N3; z0 += G) {
+    d4_t xr[RB], xi[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+      xr[rb] = d4_t{0, 0, 0, 0};
+      xi[rb] = d4_t{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int kk = 0; kk < KX; kk++) {
+      const double2 f = F1s[kk][l];
+      const int x = 4 * kk + lk;
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++) {
+        const int R = 16 * rb + lr, y = R % N2, z = z0 + R / N2;
+        double2 a = make_double2(0.0, 0.0);
+        if (z < N3) {
+          const long g = x + (long)N1 * (y + (long)N2 * z);
+          a = cmul(p1[g], p2[g]);
+        }
+        cmfma(xr[rb], xi[rb], a, f);
+      }
+    }
+#pragma unroll
+    for (int zz = 0; zz < G; zz++) {
+      const int z = z0 + zz;
+      if (z < N3) {  // wave-uniform
+        d4_t qr = {0, 0, 0, 0}, qi = {0, 0, 0, 0};
+#pragma unroll
+        for (int kq = 0; kq < KY; kq++) {
+          const int R0 = N2 * zz + 4 * kq, rb = R0 / 16, r = (R0 % 16) / 4;
+          cmfma(qr, qi, F2s[kq][l], make_double2(xr[rb][r], xi[rb][r]));
+        }
+        // Q(gy, gx): lane holds gx = lr, gy = lk + 4r
+#pragma unroll
+        for (int r = 0; r < 4; r++) Q[(lk + 4 * r) * PW_BOX + lr] = make_double2(qr[r], qi[r]);
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+        for (int j = 0; j < PW_FOUT; j++) {
+          if (ob[j] >= 0) {
+            const double2 t = tz[z][ob[j] >> 8];
+            const double2 v = Q[ob[j] & 255];
+            acc[j].x += t.x * v.x - t.y * v.y;
+            acc[j].y += t.x * v.y + t.y * v.x;
+          }
+        }
+        __builtin_amdgcn_wave_barrier();  // Q is rewritten by the next plane
+      }
+    }
+  }
+  double2 *out = M + (t0 + (long)b1 * nunocc + b2) * ldm + col0;
+#pragma unroll
+  for (int j = 0; j < PW_FOUT; j++) {
+    const int ig = l + 64 * j;
+    if (ig < ngptm) out[ig] = acc[j];
+  }
+}
+
 // Forward pass z and gather of the mixed-basis IPWs (oidx: box index
 // gz*256 + gy*16 + gx of imati.(Gm - g_k) per IPW).
 __global__ __launch_bounds__(NT) void pw_pair_z_kernel(const double2 *__restrict__ slab,
@@ -364,6 +470,19 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
                        nocc, k2[ik], b2lo, N1, N2, N3, 1.0 / std::sqrt(p->vol), 1.0 / (double)NN,
                        p->step, grids);
     SPX_LAUNCHED(ctx);
+    if (p->ngptm <= 64 * PW_FOUT && N1 == N2 && (N1 == 16 || N1 == 20 || N1 == 24 || N1 == 32)) {
+      const dim3 grid(nunocc, (nocc + 3) / 4);
+      const int *ox = oidx_dev + (long)ik * p->ngptm;
+      const long tt = (long)ik * nocc * nunocc;
+      switch (N1) {
+        case 16: hipLaunchKernelGGL((pw_pair_joint_kernel<16, 16>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 20: hipLaunchKernelGGL((pw_pair_joint_kernel<20, 20>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 24: hipLaunchKernelGGL((pw_pair_joint_kernel<24, 24>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        default: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+      }
+      SPX_LAUNCHED(ctx);
+      continue;
+    }
     if (p->ngptm <= 64 * PW_FOUT) {
       hipLaunchKernelGGL(pw_pair_fused_kernel<NB>, dim3(nunocc, (nocc + 3) / 4), dim3(NT), 0, st,
                          grids, nocc, nunocc, N1, N2, N3, oidx_dev + (long)ik * p->ngptm, p->ngptm,
@@ -424,9 +543,16 @@ int spx_pw_plan_build(spexhip_ctx *ctx, const spexhip_pw_desc *d, spexhip_plan *
   // the exact convolution -- the same numbers on every such grid, the
   // caller's included.  The plan then uses the smallest such grid (at least
   // the 16-point box); a caller grid that aliases is kept as given.
+  // Sizes are rounded up to 16, 20, 24 or 32 (the in-plane sizes with a
+  // specialised pair kernel) when that is still below the caller's grid.
   if (!d->step_r)
     for (int i = 0; i < 3; i++) {
-      const int nmin = std::max(PW_BOX, hi[i] >= lo[i] ? hi[i] - lo[i] + PW_OFF + 1 : PW_BOX);
+      int nmin = std::max(PW_BOX, hi[i] >= lo[i] ? hi[i] - lo[i] + PW_OFF + 1 : PW_BOX);
+      for (int nice : {16, 20, 24, 32})
+        if (nmin <= nice && nice <= p->nfft[i]) {
+          nmin = nice;
+          break;
+        }
       if (p->nfft[i] > nmin) p->nfft[i] = nmin;
     }
   int rc = spx_alloc(ctx, sizeof(int) * tab.size(), (void **)&p->gidx);

@@ -66,9 +66,13 @@ def _pw_dev(ctx, s, b1lo, nocc, b2lo, nunocc, step=None, nfft=None):
     return M[:, 2:2 + s.ngptm]
 
 
-@pytest.mark.parametrize("nfft", [(16, 16, 16), (18, 20, 24), (36, 36, 36)])
-def test_pw_vs_oracle(ctx, nfft):
-    s = synth.product_system(nmesh=2, nband=9, gcut=3.1, gcutm=2.1, nfft=nfft, seed=41)
+@pytest.mark.parametrize("nfft,gcut", [((16, 16, 16), 3.1), ((18, 20, 24), 3.1),
+                                       ((36, 36, 36), 3.1), ((36, 36, 36), 5.1)])
+def test_pw_vs_oracle(ctx, nfft, gcut):
+    """Without theta the plan evaluates on the smallest alias-free grid of
+    16/20/24/32 points (16 here for gcut 3.1, 20 for gcut 5.1); the oracle's
+    explicit convolution is grid-free."""
+    s = synth.product_system(nmesh=2, nband=9, gcut=gcut, gcutm=2.1, nfft=nfft, seed=41)
     nocc, nunocc = 3, 5
     ref = oracle.wfprod_pw(s.k1, s.k2, s.g2, s.gpt, s.ngpt, s.cpw, s.gptm, s.vol, 0, nocc, 4, nunocc)
     got = _pw_dev(ctx, s, 0, nocc, 4, nunocc)
@@ -77,16 +81,23 @@ def test_pw_vs_oracle(ctx, nfft):
     assert mx < TOL
 
 
-def test_pw_with_step_function(ctx):
-    s = synth.product_system(nmesh=2, nband=6, gcut=3.1, gcutm=2.1, nfft=(20, 20, 20), seed=42)
+@pytest.mark.parametrize("nfft", [(20, 20, 20), (20, 20, 18), (24, 24, 21), (32, 32, 32),
+                                  (18, 20, 24)])
+def test_pw_with_step_function(ctx, nfft):
+    """With theta the caller's grid is used as given: in-plane sizes with a
+    packed-plane kernel (20, 24, 32; z extents that leave a partial plane
+    group) and the generic kernel (18 x 20)."""
+    s = synth.product_system(nmesh=2, nband=6, gcut=3.1, gcutm=2.1, nfft=nfft, seed=42)
     N = s.nfft
-    x = np.arange(N[0]) / N[0]
-    th = 0.8 + 0.2 * np.cos(2 * np.pi * x)[None, None, :] * np.cos(2 * np.pi * x)[None, :, None]
-    th = np.broadcast_to(th, N[::-1]).reshape(-1).copy()
+    x, y, z = (np.arange(n) / n for n in N)
+    th = (0.8 + 0.2 * np.cos(2 * np.pi * x)[None, None, :] * np.cos(2 * np.pi * y)[None, :, None]
+          + 0.05 * np.sin(2 * np.pi * z)[:, None, None])
+    th = np.ascontiguousarray(th).reshape(-1)
     nocc, nunocc = 2, 3
     ref = pw_products_fft(s, 0, nocc, 2, nunocc, step_r=th)
     got = _pw_dev(ctx, s, 0, nocc, 2, nunocc, step=th)
     mx, fro = normwise(got, ref)
+    print(f"PW theta nfft={nfft}: max-rel {mx:.2e} fro-rel {fro:.2e}")
     assert mx < TOL, (mx, fro)
 
 
