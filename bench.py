@@ -321,12 +321,13 @@ def main():
     head_name = "ozaki_int8" if ok("ozaki_int8") else "split_fp32"
     head = res[head_name]
     if head_name == "ozaki_int8":
-        i8 = 10.0 * flops / (head["rank_ms"] * 1e9)  # 40 int8 MACs per complex MAC
+        i8 = 10.5 * flops / (head["rank_ms"] * 1e9)  # 84 int8 ops per complex MAC
         roof = {"bound": "mfma", "kernel": "ozk_herk_kernel", "achieved": round(i8, 1),
                 "peak": PEAK_I8, "unit": "TFLOP/s (int8 ops)", "frac": round(i8 / PEAK_I8, 4),
                 "traffic": None,
-                "note": "achieved = 80 int8 ops per complex multiply-add of chi0 "
-                        "(10 slice pairs x 4 real products) / rank-k kernel time"}
+                "note": "achieved = 84 int8 ops per complex multiply-add of chi0 (10 slice "
+                        "pairs x 4 real products + 2 for the (3,3) pair, 2 ops each) / "
+                        "rank-k kernel time"}
     else:
         roof = {"bound": "mfma", "kernel": "chi0_split_kernel",
                 "achieved": round(head["chi0_tflops"], 2), "peak": PEAK_F32, "unit": "TFLOP/s",
@@ -372,7 +373,8 @@ def main():
         from project1_amd import serving
 
         line["serving"] = serving.serve_benchmark(
-            ctx, {"fp64": spexhip.PREC_FP64, "split_fp32": spexhip.PREC_SPLIT_FP32},
+            ctx, {"fp64": spexhip.PREC_FP64, "split_fp32": spexhip.PREC_SPLIT_FP32,
+                  "ozaki_int8": spexhip.PREC_OZAKI_INT8},
             a.serve_seconds, a.serve_cap, log=lambda m: log(m) if isinstance(m, str) else None)
     print(json.dumps(line), flush=True)
     ctx.close()

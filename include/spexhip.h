@@ -292,6 +292,10 @@ typedef struct {
   int nkpt_w, nsym1;
   const void *omega_dev;
   const void *vhalf_dev;
+  /* k-points per packet (0: all).  The k range runs in packets as in
+   * susceptibility.f:614-713: products, weights and chi0 += packet, with
+   * buffers sized for one packet (M, weights, Ozaki slices).               */
+  int kpacket;
 } spexhip_qdesc;
 
 /* HBM bytes a pipeline of this shape allocates (admission by footprint). */

@@ -48,7 +48,7 @@ constexpr int TK = 64;     // transitions per chunk (MFMA K)
 constexpr int TT = 64;     // output tile edge
 constexpr int NT = 256;
 constexpr int FLUSH = 64;  // chunks between int32 -> FP64 flushes
-constexpr int PFD = 3;     // L2 prefetch distance in chunks
+constexpr int PFD = 1;     // L2 prefetch distance in chunks (1: none beyond the double buffer)
 constexpr int NPL = 2 * NS;  // planes per chunk
 
 __device__ inline long blk_off(long c, int pl, int ib, int nib) {
@@ -188,18 +188,13 @@ __device__ inline void tri_index(int x, int &bi, int &bj) {
   bi = x - j * (j + 1) / 2;
 }
 
-// Linear workgroup id -> (frequency in the group, 64x64 tile bi <= bj).
-// Workgroups go to the 8 XCDs round-robin (XCD = id % 8) and each XCD has
-// its own L2, so every XCD is given one contiguous range of the logical
-// order -- frequency-major, tiles in 4x4 super-blocks (column of
-// super-blocks by column) -- and the ~32 tiles resident on an XCD at a time
-// share a few row blocks of slices instead of all of them.
-__device__ inline void xcd_tile(long L, long P, int T, int nbt, int &wg, int &bi, int &bj) {
-  const long per = P / 8, rem = P % 8;
-  const long x = L % 8, j = L / 8;
-  const long lg = x < rem ? x * (per + 1) + j : rem * (per + 1) + (x - rem) * per + j;
-  wg = (int)(lg / T);
-  int t = (int)(lg % T);
+// Logical tile index -> (frequency in the group, 64x64 tile bi <= bj):
+// frequency-major, and within a frequency 4x4 super-blocks of tiles,
+// column of super-blocks by column, so that consecutive tiles share row
+// blocks of slices.
+__device__ inline void tile_of(long t0, int T, int nbt, int &wg, int &bi, int &bj) {
+  wg = (int)(t0 / T);
+  int t = (int)(t0 % T);
   bi = bj = 0;
   const int ns = (nbt + 3) / 4;
   for (int SJ = 0; SJ < ns; SJ++) {
@@ -230,20 +225,27 @@ typedef __attribute__((address_space(3))) void lds_void;
 // 32x32 (2x2 blocks of 16x16).  Per chunk the workgroup stages 8 planes x
 // 4 row blocks + 8 planes x 4 column blocks (64 KB) into one of two LDS
 // buffers; each wave issues 16 global_load_lds_dwordx4 (1 KB each).
+//
+// Persistent: one workgroup per CU (G of them, a multiple of 8) walks the
+// tiles in rounds of G.  Every tile streams the same K, so the tiles of a
+// round run side by side, and within a round XCD x (workgroups x, x+8, ...)
+// holds G/8 consecutive tiles of the logical order: tiles that share row
+// blocks read them through one L2 at nearly the same time.
 __global__ __launch_bounds__(NT) void ozk_herk_kernel(
-    long nchunk, int nb, int nbp, int ntile, const unsigned char *__restrict__ S, long sbytes,
-    int w0, const double *__restrict__ scale, const double *__restrict__ sgn,
+    long nchunk, int nb, int nbp, int ntile, int ngroup, const unsigned char *__restrict__ S,
+    long sbytes, int w0, const double *__restrict__ scale, const double *__restrict__ sgn,
     double2 *__restrict__ chi, long ldc, double alpha, double beta) {
   // [buf][panel][plane][block of 4][1 KB], then 256 B of prefetch scratch
   constexpr int LPF = 2 * 2 * NPL * 4 * 1024;
   __shared__ __attribute__((aligned(1024))) unsigned char Ls[LPF + 256];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = wv >> 1, wc = wv & 1;
-  int bi, bj, wg;
-  xcd_tile(blockIdx.x, (long)gridDim.x, ntile, nbp / TT, wg, bi, bj);
-  const int w = w0 + wg;
-  const int nib = nbp / 16;
-  const unsigned char *Sw = S + (long)wg * sbytes;
+  const int nib = nbp / 16, nbt = nbp / TT;
+  const long ntot = (long)ntile * ngroup, G = gridDim.x;
+  const long slot = (long)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
+  // current tile (set at the top of each round)
+  int bi = 0, bj = 0, wg = 0, w = 0;
+  const unsigned char *Sw = S;
 
   // wave wv loads the 16 blocks e = wv*16 .. wv*16+15 of the 64 per chunk:
   // e -> panel = e/32, plane = (e/4)%8, block = e%4
@@ -308,9 +310,9 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
   // its tile, so the read-modify-write needs no synchronisation.
   // v_mfma_i32_16x16x64_i8 result layout: column j = lane%16, rows 4*(lane/16)+r.
   const int lr = lane & 15, lk = lane >> 4;
-  const int I0 = bi * TT + wr * 32 + 4 * lk, J0 = bj * TT + wc * 32 + lr;
-  double2 *C = chi + (long)w * ldc * nb;
-  const double sg = sgn[w] * alpha;
+  int I0 = 0, J0 = 0;
+  double2 *C = chi;
+  double sg = 0.0;
   auto flush = [&](bool first) {
 #pragma unroll
     for (int b = 0; b < 2; b++) {
@@ -357,14 +359,15 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     const int buf = c & 1;
     // this wave's copies of chunk c have landed (prefetches issued after
     // them may still be in flight) ...
-    wait_copies(c == 0 ? 2 * (int)(nchunk - 1 < PFD - 1 ? nchunk - 1 : PFD - 1)
-                       : (c - 1 + PFD < nchunk ? 2 : 0));
+    wait_copies(PFD <= 1 ? 0
+                : c == 0 ? 2 * (int)(nchunk - 1 < PFD - 1 ? nchunk - 1 : PFD - 1)
+                         : (c - 1 + PFD < nchunk ? 2 : 0));
     // ... and every other wave's (a plain barrier: the wait above already
     // made this wave's copies visible, and a fence would drain the
     // prefetches too)
     __builtin_amdgcn_s_barrier();
     if (c + 1 < nchunk) issue(c + 1, buf ^ 1);  // buffer buf^1 was last read before the barrier
-    if (c + PFD < nchunk) prefetch(c + PFD);
+    if (PFD > 1 && c + PFD < nchunk) prefetch(c + PFD);
     // every column-side operand of the chunk stays in registers; the row
     // side is read once per slice
     i4_t Br[NS][2], Bi[NS][2];
@@ -416,17 +419,27 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     }
   };
 
-  issue(0, 0);
-  for (long c = 1; c < PFD && c < nchunk; c++) prefetch(c);
-  for (long c0 = 0; c0 < nchunk; c0 += FLUSH) {
-    const long ce = c0 + FLUSH < nchunk ? c0 + FLUSH : nchunk;
-    chunk(c0, std::true_type{});
-    for (long c = c0 + 1; c < ce; c++) chunk(c, std::false_type{});
-    flush(c0 == 0);
-  }
-  if (nchunk == 0) {  // no transitions: chi = beta * chi
-    zero_acc();
-    flush(true);
+  for (long tix = slot; tix < ntot; tix += G) {
+    tile_of(tix, ntile, nbt, wg, bi, bj);
+    w = w0 + wg;
+    Sw = S + (long)wg * sbytes;
+    I0 = bi * TT + wr * 32 + 4 * lk;
+    J0 = bj * TT + wc * 32 + lr;
+    C = chi + (long)w * ldc * nb;
+    sg = sgn[w] * alpha;
+    __syncthreads();  // the previous tile's LDS reads are done before new copies land
+    issue(0, 0);
+    for (long c = 1; c < PFD && c < nchunk; c++) prefetch(c);
+    for (long c0 = 0; c0 < nchunk; c0 += FLUSH) {
+      const long ce = c0 + FLUSH < nchunk ? c0 + FLUSH : nchunk;
+      chunk(c0, std::true_type{});
+      for (long c = c0 + 1; c < ce; c++) chunk(c, std::false_type{});
+      flush(c0 == 0);
+    }
+    if (nchunk == 0) {  // no transitions: chi = beta * chi
+      zero_acc();
+      flush(true);
+    }
   }
 }
 
@@ -470,6 +483,14 @@ size_t spx_ozaki_ws_bytes(long n, int nb, int nw, int nwg) {
 // synchronisation and no memset nodes, so it may be captured in a graph.
 // *err_dev (device int, caller zeroed) becomes non-zero if some frequency
 // has weights of both signs.
+static int ozk_ncu(spexhip_ctx *ctx) {
+  int n = 0;
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess ||
+      n <= 0)
+    n = 256;
+  return n;
+}
+
 int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M,
                          long ldm, int colmajor, int nw, const void *wght, double alpha,
                          double beta, void *chi, int ldchi, void *ws, size_t ws_bytes,
@@ -529,7 +550,10 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
     SPX_LAUNCHED(ctx);
     if (w0 == 0 && timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
     const int ntile = nbt * (nbt + 1) / 2;
-    hipLaunchKernelGGL(ozk_herk_kernel, dim3(ntile * g), dim3(NT), 0, st, nchunk, nb, nbp, ntile,
+    // persistent grid: one workgroup per CU (128 KB of LDS each), a multiple of 8
+    const long want = ((long)ntile * g + 7) / 8 * 8;
+    const int grid = (int)std::min<long>(want, std::max(8, ozk_ncu(ctx) / 8 * 8));
+    hipLaunchKernelGGL(ozk_herk_kernel, dim3(grid), dim3(NT), 0, st, nchunk, nb, nbp, ntile, g,
                        S, (long)sb, w0, scale, sgn, (double2 *)chi, (long)ldchi, alpha, beta);
     SPX_LAUNCHED(ctx);
   }

@@ -16,6 +16,7 @@ struct spexhip_qpipe {
   spexhip_qdesc d{};
   std::vector<int> k1, k2, g2;
   long n = 0;
+  int kp = 0, npacket = 1;  // k-points per packet, packets
   hipStream_t st = nullptr;
   void *M = nullptr;
   bool ownM = false;
@@ -38,9 +39,15 @@ struct Sizes {
   size_t M, D, X, ws, tab;
 };
 
+int packet_k(const spexhip_qdesc &d) {
+  return d.kpacket > 0 && d.kpacket < d.nk ? d.kpacket : (d.nk > 0 ? d.nk : 1);
+}
+
+// Buffers are sized for one packet of packet_k(d) k-points.
 Sizes sizes_of(const spexhip_qdesc &d) {
   Sizes s{};
-  const long n = (long)d.nk * d.nocc * d.nunocc;
+  const int kp = packet_k(d);
+  const long n = (long)kp * d.nocc * d.nunocc;
   s.M = d.M_dev ? 0 : align256(16 * (size_t)n * d.nb);
   s.D = align256(8 * (size_t)n * d.nw);
   s.X = align256(16 * (size_t)d.nb * d.nb * d.nw);
@@ -49,40 +56,55 @@ Sizes sizes_of(const spexhip_qdesc &d) {
   ws = std::max(ws, 16 * (size_t)d.nb * (d.nb + 1) / 2 * d.nw);  // packing
   if (d.precision == SPEXHIP_PREC_OZAKI_INT8)
     ws = std::max(ws, spx_ozaki_ws_bytes(n, d.nb, d.nw, std::min(d.nw, d.ozaki_group > 0 ? d.ozaki_group : d.nw)));
-  if (d.mt_plan) ws = std::max(ws, spx_wfprod_mt_ws(d.mt_plan, d.nk, d.nocc, d.nunocc));
+  if (d.mt_plan) ws = std::max(ws, spx_wfprod_mt_ws(d.mt_plan, kp, d.nocc, d.nunocc));
   if (d.pw_plan) ws = std::max(ws, spx_wfprod_pw_ws_core(d.pw_plan, d.nocc, d.nunocc));
   s.ws = align256(ws);
   s.tab = align256(8 * (size_t)d.nk + 4 * (size_t)d.nk * (d.pw_plan ? d.pw_plan->ngptm : 0) + 64);
   return s;
 }
 
+// The k range runs in packets (susceptibility.f:614-713): per packet the
+// products, weights and chi0 += packet; eps and eps^-1 after the last one.
 int enqueue(spexhip_ctx *ctx, spexhip_qpipe *q) {
   const spexhip_qdesc &d = q->d;
   hipStream_t st = q->st;
   int rc = spx_fill_zero(ctx, st, q->flags, 64);
   if (rc) return rc;
-  if (d.mt_plan) {
-    rc = spx_wfprod_mt(ctx, st, d.mt_plan, d.cmt_dev, d.ncent, d.nband, d.nkpt, d.q, d.nk, q->kdev,
-                       q->kdev + d.nk, d.b1lo, d.nocc, d.b2lo, d.nunocc, q->M, q->ldm, 0, q->ws);
+  for (int p = 0; p < q->npacket; p++) {
+    const int k0 = p * q->kp;
+    const int cnt = std::max(0, std::min(q->kp, d.nk - k0));
+    const long np = (long)cnt * d.nocc * d.nunocc;
+    void *Mp = d.M_dev ? (void *)((char *)q->M + 16 * (size_t)k0 * d.nocc * d.nunocc * q->ldm)
+                       : q->M;
+    if (cnt > 0) {
+      if (d.mt_plan) {
+        rc = spx_wfprod_mt(ctx, st, d.mt_plan, d.cmt_dev, d.ncent, d.nband, d.nkpt, d.q, cnt,
+                           q->kdev + k0, q->kdev + d.nk + k0, d.b1lo, d.nocc, d.b2lo, d.nunocc,
+                           Mp, q->ldm, 0, q->ws);
+        if (rc) return rc;
+      }
+      if (d.pw_plan) {
+        const int col0 = d.mt_plan ? d.mt_plan->nbasp : 0;
+        rc = spx_wfprod_pw_async(ctx, st, d.pw_plan, d.cpw_dev, d.nband, cnt, q->k1.data() + k0,
+                                 q->k2.data() + k0, q->oidx + (size_t)k0 * d.pw_plan->ngptm,
+                                 d.b1lo, d.nocc, d.b2lo, d.nunocc, Mp, q->ldm, col0, q->ws);
+        if (rc) return rc;
+      }
+      rc = spx_weights_launch(ctx, st, cnt, d.nocc, d.nunocc,
+                              (const double *)d.e_occ_dev + (size_t)k0 * d.nocc,
+                              (const double *)d.e_unocc_dev + (size_t)k0 * d.nunocc, d.nkpt_w,
+                              d.nsym1, (const int *)d.nkpts_dev + k0, d.nw, d.omega_dev, q->D);
+      if (rc) return rc;
+    }
+    const double beta = p > 0 ? 1.0 : 0.0;
+    if (d.precision == SPEXHIP_PREC_OZAKI_INT8)
+      rc = spx_chi0_ozaki_async(ctx, st, np, d.nb, Mp, q->ldm, 0, d.nw, q->D, d.alpha, beta, q->X,
+                                d.nb, q->ws, q->wsb, q->flags + 1);
+    else
+      rc = spx_chi0_launch(ctx, st, np, d.nb, Mp, q->ldm, 0, d.nw, q->D, d.precision, d.alpha,
+                           beta, q->X, d.nb);
     if (rc) return rc;
   }
-  if (d.pw_plan) {
-    const int col0 = d.mt_plan ? d.mt_plan->nbasp : 0;
-    rc = spx_wfprod_pw_async(ctx, st, d.pw_plan, d.cpw_dev, d.nband, d.nk, q->k1.data(),
-                             q->k2.data(), q->oidx, d.b1lo, d.nocc, d.b2lo, d.nunocc, q->M, q->ldm,
-                             col0, q->ws);
-    if (rc) return rc;
-  }
-  rc = spx_weights_launch(ctx, st, d.nk, d.nocc, d.nunocc, d.e_occ_dev, d.e_unocc_dev, d.nkpt_w,
-                          d.nsym1, d.nkpts_dev, d.nw, d.omega_dev, q->D);
-  if (rc) return rc;
-  if (d.precision == SPEXHIP_PREC_OZAKI_INT8)
-    rc = spx_chi0_ozaki_async(ctx, st, q->n, d.nb, q->M, q->ldm, 0, d.nw, q->D, d.alpha, 0.0, q->X,
-                              d.nb, q->ws, q->wsb, q->flags + 1);
-  else
-    rc = spx_chi0_launch(ctx, st, q->n, d.nb, q->M, q->ldm, 0, d.nw, q->D, d.precision, d.alpha,
-                         0.0, q->X, d.nb);
-  if (rc) return rc;
   rc = spx_dielectric(ctx, st, d.nb, d.nw, q->X, d.nb, d.vkind, d.vhalf_dev, q->E, d.nb, q->ws);
   if (rc) return rc;
   return spx_inverse(ctx, st, d.nb, d.nw, q->E, d.nb, q->ws, q->flags);
@@ -133,6 +155,8 @@ int spexhip_qpipe_create(spexhip_ctx *ctx, const spexhip_qdesc *desc, spexhip_qp
   spexhip_qpipe *q = new spexhip_qpipe();
   q->d = d;
   q->n = (long)d.nk * d.nocc * d.nunocc;
+  q->kp = packet_k(d);
+  q->npacket = d.nk > 0 ? (d.nk + q->kp - 1) / q->kp : 1;
   if (!d.M_dev) {
     q->k1.assign(d.k1, d.k1 + d.nk);
     q->k2.assign(d.k2, d.k2 + d.nk);

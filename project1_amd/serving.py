@@ -193,7 +193,8 @@ class Mesh:
 class Request:
     """Device inputs and the C descriptor of one q-point request."""
 
-    def __init__(self, lat: Lattice, spec: Spec, precision: int, ozaki_group: int = 0):
+    def __init__(self, lat: Lattice, spec: Spec, precision: int, ozaki_group: int = 0,
+                 packet_cap: int = 24 << 30):
         ctx = lat.ctx
         self.spec, self.precision = spec, precision
         mesh = lat.mesh(spec.nmesh)
@@ -224,6 +225,13 @@ class Request:
                                                    self.bufs[2].ptr)
         d.nkpt_w, d.nsym1 = nk, 1
         d.omega_dev, d.vhalf_dev = self.bufs[3].ptr, self.bufs[4].ptr
+        # k packets: the largest packet whose pipeline fits packet_cap bytes
+        # (the reference sizes its packets against MEM the same way)
+        kp = nk
+        d.kpacket = kp
+        while kp > 1 and spexhip.qpipe_footprint(d) > packet_cap:
+            kp = (kp + 1) // 2
+            d.kpacket = kp
         self.desc = d
         self._keep = (mesh, k2, g2)  # k lists referenced by the descriptor
         self.e_occ, self.e_unocc, self.omega = eo, eu, om

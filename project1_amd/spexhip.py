@@ -95,7 +95,8 @@ class QDesc(C.Structure):
                 ("nband", C.c_int), ("nkpt", C.c_int), ("q", C.c_double * 3), ("cpw_dev", _vp),
                 ("k1", _vp), ("k2", _vp), ("g2", _vp), ("b1lo", C.c_int), ("b2lo", C.c_int),
                 ("e_occ_dev", _vp), ("e_unocc_dev", _vp), ("nkpts_dev", _vp),
-                ("nkpt_w", C.c_int), ("nsym1", C.c_int), ("omega_dev", _vp), ("vhalf_dev", _vp)]
+                ("nkpt_w", C.c_int), ("nsym1", C.c_int), ("omega_dev", _vp), ("vhalf_dev", _vp),
+                ("kpacket", C.c_int)]
 
 
 class QPipe:

@@ -155,6 +155,18 @@ def test_ozaki_rejects_mixed_sign_weights(ctx):
     assert e.value.code == spexhip.E_INVALID
 
 
+def test_ozaki_many_tiles_several_rounds(ctx):
+    """nb = 640 (55 tiles per frequency) x 12 frequencies = 660 tiles: more
+    than one round of the persistent grid, every frequency group position."""
+    c = synth.chi0_case(nk=2, nocc=4, nunocc=100, nb=640, nw=12, seed=6)
+    w = np.asfortranarray(oracle.weights_none(c.e_occ, c.e_unocc, c.omega))
+    ref = oracle.chi0_herk(c.M, w)
+    got = run_chi0(ctx, c.M, w, spexhip.PREC_OZAKI_INT8)
+    mx, fro = normwise(got, ref)
+    print(f"ozaki 660 tiles: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < TOL_SPLIT
+
+
 def test_ozaki_positive_weights_and_large_k(ctx):
     """All-positive weights (sign +1) and K spanning several int32 flushes."""
     c = synth.chi0_case(nk=4, nocc=8, nunocc=300, nb=72, nw=2, seed=4)  # n = 9600 > 4096

@@ -79,3 +79,26 @@ def test_pipeline_reports_not_pd(ctx, crystal):
         job.wait()
     assert e.value.code == spexhip.E_INVALID
     job.free()
+
+
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-13), (spexhip.PREC_OZAKI_INT8, 1e-8)])
+def test_k_packets_match_one_packet(ctx, crystal, precision, tol):
+    """kpacket = 3 of 8 k-points (packets 3, 3, 2; buffers sized for 3) gives
+    the one-packet result up to the summation order of the packets."""
+    one = QJob(ctx, crystal, nocc=3, nunocc=6, nw=2, precision=precision, seed=21)
+    one.launch()
+    one.wait()
+    ref = one.pipe.result()
+    d = one.desc
+    d.kpacket = 3
+    pk = spexhip.QPipe(ctx, d, [])
+    assert spexhip.qpipe_footprint(d) < one.footprint
+    pk.launch()
+    pk.wait()
+    got = pk.result()
+    e1, e2 = normwise(got[0], ref[0]), normwise(got[1], ref[1])
+    print(f"packets prec={precision}: chi0 {e1} eps^-1 {e2}")
+    assert max(e1) < tol and max(e2) < 100 * tol
+    pk.destroy()
+    one.free()
+
