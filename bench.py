@@ -57,6 +57,8 @@ def parse():
                    help="CPU baseline sample: all pairs among this many columns")
     p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                    help="only run 5 plain device copies of MIB MiB (byte-counter calibration)")
+    p.add_argument("--only", choices=[f for f, _ in FORMS], default=None,
+                   help="profiling aid: run one chi0 form and print its timings only")
     p.add_argument("--no-batch", action="store_true", help="skip the 32-q batched leg")
     p.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     p.add_argument("--serve-seconds", type=float, default=20.0,
@@ -300,6 +302,8 @@ def main():
     res, out = {}, {}
     flops = chi0_flops(a.nw, pipe.n, a.nb)
     for name, pname in FORMS:
+        if a.only and name != a.only:
+            continue
         prec = getattr(spexhip, pname)
         r = run_form(pipe, prec, a.steps, a.warmup)
         log(f"{name}: {r['ms_per_step']:.1f} ms/step (chi0 {r['chi0_ms']:.1f} ms)")
@@ -311,6 +315,11 @@ def main():
             prep, rank, groups = ctx.chi0_phase_ms()
             r.update(prep_ms=prep, rank_ms=rank, groups=groups)
         res[name] = r
+    if a.only:
+        print(json.dumps({"only": a.only, "forms": res}), flush=True)
+        pipe.free()
+        ctx.close()
+        return
     errs_ = {k: {"chi0": errs(out[k][0], out["fp64"][0]), "eps_inv": errs(out[k][1], out["fp64"][1])}
              for k in ("split_fp32", "ozaki_int8")}
 
