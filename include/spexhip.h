@@ -383,6 +383,21 @@ int spexhip_dielectric_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ld
  * wrapper.f:2362-2374).  SPEXHIP_E_NOTPD if a pivot is not positive.      */
 int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx);
 
+/* Screened interaction W = v^1/2 eps^-1 v^1/2 of nw full device matrices
+ * (correlation.f:868-877, p_diagmatdiag(sqcoul, dielec)); vkind 0: vhalf =
+ * sqrt(coul) (nb doubles), 1: dense nb x nb complex.  W must not alias
+ * eps^-1.                                                                  */
+int spexhip_screened_w_dev(spexhip_ctx *ctx, int nb, int nw, const void *einv_dev, int lde,
+                           int vkind, const void *vhalf_dev, void *W_dev, int ldw);
+/* RPA correlation-energy integrand of nw frequencies (correlation.f:
+ * 680-693): out[i] = weight * sum over the eigenvalues e of eps_i of
+ * (log e + 1 - e), with weight = count(kptp==ikpt)/nkpt.  Evaluated as
+ * weight * (log det eps_i + nb - tr eps_i), log det from the pivots of an
+ * FP64 Gauss-Jordan elimination of a copy (eps is left unchanged).
+ * out: host, nw doubles.  SPEXHIP_E_NOTPD if eps is not positive definite. */
+int spexhip_rpa_integrand_dev(spexhip_ctx *ctx, int nb, int nw, const void *eps_dev, int lde,
+                              double weight, double *out);
+
 /* Host drop-in for correlation.f:612-747: chi (packed, nb(nb+1)/2 x nw, the
  * reference susceph after the spin factor) -> eps^-1 (packed, same layout).
  * vhalf is sqrt(coul) (vkind 0, nb doubles) or dense (vkind 1, nb x nb

@@ -169,6 +169,27 @@ def inverse_hpd_packed(packed):
     return m
 
 
+def screened_w(einv_packed, vhalf):
+    """W = v^1/2 eps^-1 v^1/2, packed upper (correlation.f:868-877:
+    p_diagmatdiag(sqcoul, dielec) in the Coulomb eigenbasis).  vhalf: real
+    vector sqrt(coul) or a dense nb x nb matrix."""
+    vhalf = np.asarray(vhalf)
+    nb = vhalf.shape[0]
+    e = unpack_herm(np.asarray(einv_packed, np.complex128), nb)
+    vh = np.diag(vhalf.astype(np.complex128)) if vhalf.ndim == 1 else vhalf.astype(np.complex128)
+    return pack_upper(vh @ e @ vh)
+
+
+def rpa_integrand(eps_packed, weight):
+    """weight * sum(log(e) + 1 - e) over the eigenvalues e of eps, the RPA
+    correlation-energy integrand of one frequency (correlation.f:686-690:
+    Mdiagonalize, then sum(log(eig) + 1 - eig) * count(kptp==ikpt)/nkpt)."""
+    e = np.asarray(eps_packed, np.complex128)
+    nb = int(round((np.sqrt(8 * e.size + 1) - 1) / 2))
+    eig = np.linalg.eigvalsh(unpack_herm(e, nb))
+    return weight * float(np.sum(np.log(eig) + 1.0 - eig))
+
+
 # ------------------------------------------------------------------ gaunt
 def wigner3j(l1, l2, l3, m1, m2, m3):
     return lib().or_wigner3j(l1, l2, l3, m1, m2, m3)

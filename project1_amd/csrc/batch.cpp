@@ -129,7 +129,7 @@ int enqueue(spexhip_ctx *ctx, spexhip_batch *b) {
     const spexhip_qdesc &d = q.d;
     hipStream_t st = b->st[q.s];
     rc = spx_dielectric(ctx, st, d.nb, d.nw, q.X, d.nb, d.vkind, d.vhalf_dev, q.E, d.nb, q.ws);
-    if (!rc) rc = spx_inverse(ctx, st, d.nb, d.nw, q.E, d.nb, q.ws, b->flags + i);
+    if (!rc) rc = spx_inverse(ctx, st, d.nb, d.nw, q.E, d.nb, q.ws, b->flags + i, nullptr);
   }
   if (!rc) rc = join_all(ctx, b);
   return rc;

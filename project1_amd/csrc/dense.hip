@@ -81,11 +81,16 @@ constexpr int GB = 64;  // panel width
 
 // P = inverse of the GB x GB diagonal block at (k0,k0) (rows/cols beyond nb
 // are treated as identity).  One workgroup per matrix (blockIdx.y).
+// logdet (optional): logdet[blockIdx.y] = sum of log(pivot) over this block
+// -- the pivots of Gauss-Jordan on an HPD matrix are the Schur-complement
+// diagonals, so summed over all blocks they give log det.
 __global__ __launch_bounds__(NT) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
                                                      long ldx, long sX, int k0,
-                                                     double2 *__restrict__ P, int *__restrict__ flag) {
+                                                     double2 *__restrict__ P, int *__restrict__ flag,
+                                                     double *__restrict__ logdet) {
   __shared__ double ar[GB][GB + 1], ai[GB][GB + 1];
   __shared__ double pr_s;
+  double ld = 0.0;
   const int tid = threadIdx.x;
   const double2 *Xb = X + sX * blockIdx.y;
   for (int e = tid; e < GB * GB; e += NT) {
@@ -102,6 +107,7 @@ __global__ __launch_bounds__(NT) void gj_diag_kernel(int nb, const double2 *__re
       const double d = ar[p][p];
       if (!(d > 0.0)) flag[0] = 1;
       pr_s = 1.0 / d;
+      ld += log(d);
     }
     __syncthreads();
     const double pr = pr_s;
@@ -140,6 +146,47 @@ __global__ __launch_bounds__(NT) void gj_diag_kernel(int nb, const double2 *__re
     const int i = e % GB, j = e / GB;
     Pb[i + GB * j] = make_double2(ar[i][j], ai[i][j]);
   }
+  if (logdet && tid == 0) logdet[blockIdx.y] = ld;
+}
+
+// W = diag(s) X diag(s) (correlation.f:868-872, p_diagmatdiag with sqcoul).
+__global__ void scale_diag_kernel(int nb, const double *__restrict__ sq,
+                                  const double2 *__restrict__ X, long ldx, long sX,
+                                  double2 *__restrict__ W, long ldw, long sW) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)nb * nb) return;
+  const int j = (int)(g / nb), i = (int)(g % nb);
+  const double2 v = X[sX * blockIdx.y + i + ldx * (long)j];
+  const double f = sq[i] * sq[j];
+  W[sW * blockIdx.y + i + ldw * (long)j] = make_double2(f * v.x, f * v.y);
+}
+
+// tr[w] = sum_i Re X(i,i,w), one workgroup per matrix, fixed-order tree.
+__global__ __launch_bounds__(NT) void trace_kernel(int nb, const double2 *__restrict__ X, long ldx,
+                                                   long sX, double *__restrict__ tr) {
+  __shared__ double s[NT];
+  const double2 *Xb = X + sX * blockIdx.y;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nb; i += NT) a += Xb[i + ldx * (long)i].x;
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = NT / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) s[threadIdx.x] += s[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) tr[blockIdx.y] = s[0];
+}
+
+// out[w] = weight * (log det eps + nb - tr eps) = weight * sum(log(e) + 1 - e)
+// over the eigenvalues e of eps (correlation.f:686-690).
+__global__ void rpa_sum_kernel(int nblk, int nw, int nb, const double *__restrict__ logdet,
+                               const double *__restrict__ tr, double weight,
+                               double *__restrict__ out) {
+  const int w = threadIdx.x;
+  if (w >= nw) return;
+  double ld = 0.0;
+  for (int k = 0; k < nblk; k++) ld += logdet[(long)k * nw + w];
+  out[w] = weight * (ld + (double)nb - tr[w]);
 }
 
 // Copy the old column panel X(:, k0:k0+GB) into Cp (nb x GB, ld nbp).
@@ -233,7 +280,7 @@ size_t spx_inverse_ws(int nb, int nw) {
 // In-place inverse of nw HPD matrices (full storage).  ws: spx_inverse_ws
 // bytes; flag: device int, set non-zero if a pivot is not positive.
 int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, void *ws,
-                int *flag) {
+                int *flag, double *logdet_parts) {
   const long nbp = ((nb + GB - 1) / GB) * GB;
   double2 *P = (double2 *)ws;
   double2 *R = P + (long)nw * GB * GB;        // GB x nbp per matrix (ld GB)
@@ -243,7 +290,7 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
   for (int kb = 0; kb < nt; kb++) {
     const int k0 = kb * GB;
     hipLaunchKernelGGL(gj_diag_kernel, dim3(1, nw), dim3(NT), 0, st, nb, (const double2 *)X, ldx,
-                       sX, k0, P, flag);
+                       sX, k0, P, flag, logdet_parts ? logdet_parts + (long)kb * nw : nullptr);
     SPX_LAUNCHED(ctx);
     // R = P * X(K, :)   (GB x nb), batched
     int rc = spx_zgemm(ctx, st, GB, nb, (nb - k0 < GB ? nb - k0 : GB), 1.0, P, GB, (long)GB * GB,
@@ -282,7 +329,81 @@ int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, lo
   return spx_zgemm(ctx, st, nb, nb, nb, -1.0, vh, nb, 0, tmp, nb, nn, 0.0, E, lde, lde * (long)nb, 1.0, nw);
 }
 
+// W = Vh Einv Vh (correlation.f:868-877).  vkind 0: vh = sqrt(coul)
+// vector; 1: dense nb x nb.  tmp: nb*nb*nw complex (vkind 1); W != Einv.
+int spx_screen(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, const void *Einv, long lde,
+               int vkind, const void *vh, void *W, long ldw, void *tmp) {
+  const long nn = (long)nb * nb;
+  if (vkind == 0) {
+    hipLaunchKernelGGL(scale_diag_kernel, dim3((nn + 255) / 256, nw), dim3(256), 0, st, nb,
+                       (const double *)vh, (const double2 *)Einv, lde, lde * (long)nb,
+                       (double2 *)W, ldw, ldw * (long)nb);
+    SPX_LAUNCHED(ctx);
+    return SPEXHIP_OK;
+  }
+  int rc = spx_zgemm(ctx, st, nb, nb, nb, 1.0, Einv, lde, lde * (long)nb, vh, nb, 0, 0.0, tmp, nb,
+                     nn, 0.0, nw);
+  if (rc) return rc;
+  return spx_zgemm(ctx, st, nb, nb, nb, 1.0, vh, nb, 0, tmp, nb, nn, 0.0, W, ldw, ldw * (long)nb,
+                   0.0, nw);
+}
+
 extern "C" {
+
+int spexhip_screened_w_dev(spexhip_ctx *ctx, int nb, int nw, const void *einv, int lde, int vkind,
+                           const void *vhalf, void *W, int ldw) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && lde >= nb && ldw >= nb, "screened W: bad sizes");
+  SPX_CHECK(ctx, vkind == 0 || vkind == 1, "screened W: vkind must be 0 (diagonal) or 1 (dense)");
+  SPX_CHECK(ctx, einv != W, "screened W: W must not alias eps^-1");
+  SPX_CHECK(ctx, spx_owns(ctx, einv, 16 * (size_t)lde * nb * nw) && spx_owns(ctx, W, 16 * (size_t)ldw * nb * nw),
+            "screened W: eps^-1/W not context allocations of the stated shape");
+  SPX_CHECK(ctx, spx_owns(ctx, vhalf, vkind ? 16 * (size_t)nb * nb : 8 * (size_t)nb),
+            "screened W: vhalf not a context allocation of the stated shape");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *tmp = nullptr;
+  if (vkind) {
+    int rc = spx_scratch(ctx, 16 * (size_t)nb * nb * nw, &tmp);
+    if (rc) return rc;
+  }
+  return spx_screen(ctx, ctx->stream, nb, nw, einv, lde, vkind, vhalf, W, ldw, tmp);
+}
+
+int spexhip_rpa_integrand_dev(spexhip_ctx *ctx, int nb, int nw, const void *eps, int lde,
+                              double weight, double *out) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && nw <= 1024 && lde >= nb && out, "rpa integrand: bad arguments");
+  SPX_CHECK(ctx, spx_owns(ctx, eps, 16 * (size_t)lde * nb * nw),
+            "rpa integrand: eps not a context allocation of the stated shape");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const int nblk = (nb + GB - 1) / GB;
+  const size_t cb = 16 * (size_t)nb * nb * nw, ib = spx_inverse_ws(nb, nw);
+  const size_t tail = 8 * ((size_t)nblk * nw + 2 * (size_t)nw) + 256;
+  void *ws;
+  int rc = spx_scratch(ctx, cb + ib + tail, &ws);
+  if (rc) return rc;
+  double2 *E = (double2 *)ws;  // copy of eps: the inversion is destructive
+  double *parts = (double *)((char *)ws + cb + ib);
+  double *tr = parts + (size_t)nblk * nw, *res = tr + nw;
+  int *flag = (int *)(res + nw);
+  SPX_HIP(ctx, hipMemcpy2DAsync(E, 16 * (size_t)nb, eps, 16 * (size_t)lde, 16 * (size_t)nb,
+                                (size_t)nb * nw, hipMemcpyDeviceToDevice, ctx->stream));
+  SPX_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(trace_kernel, dim3(1, nw), dim3(NT), 0, ctx->stream, nb, (const double2 *)E,
+                     (long)nb, (long)nb * nb, tr);
+  SPX_LAUNCHED(ctx);
+  rc = spx_inverse(ctx, ctx->stream, nb, nw, E, nb, (char *)ws + cb, flag, parts);
+  if (rc) return rc;
+  hipLaunchKernelGGL(rpa_sum_kernel, dim3(1), dim3(1024), 0, ctx->stream, nblk, nw, nb, parts, tr,
+                     weight, res);
+  SPX_LAUNCHED(ctx);
+  int h = 0;
+  SPX_HIP(ctx, hipMemcpyAsync(out, res, 8 * (size_t)nw, hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h) return spx_seterr(ctx, SPEXHIP_E_NOTPD, "rpa integrand: eps not positive definite");
+  return SPEXHIP_OK;
+}
 
 int spexhip_dielectric_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx, int vkind,
                            const void *vhalf, void *E, int lde) {
@@ -313,7 +434,7 @@ int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx) 
   if (rc) return rc;
   int *flag = (int *)((char *)ws + spx_inverse_ws(nb, nw) - 256);
   SPX_HIP(ctx, hipMemsetAsync(flag, 0, sizeof(int), ctx->stream));
-  rc = spx_inverse(ctx, ctx->stream, nb, nw, X, ldx, ws, flag);
+  rc = spx_inverse(ctx, ctx->stream, nb, nw, X, ldx, ws, flag, nullptr);
   if (rc) return rc;
   int h = 0;
   SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -33,7 +33,9 @@ int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, lo
                    const void *vh, void *E, long lde, void *tmp);
 size_t spx_inverse_ws(int nb, int nw);
 int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, void *ws,
-                int *flag);
+                int *flag, double *logdet_parts);
+int spx_screen(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, const void *Einv, long lde,
+               int vkind, const void *vh, void *W, long ldw, void *tmp);
 // packing and fills
 int spx_pack_upper(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, const void *X, long ldx,
                    void *P);

@@ -107,7 +107,7 @@ int enqueue(spexhip_ctx *ctx, spexhip_qpipe *q) {
   }
   rc = spx_dielectric(ctx, st, d.nb, d.nw, q->X, d.nb, d.vkind, d.vhalf_dev, q->E, d.nb, q->ws);
   if (rc) return rc;
-  return spx_inverse(ctx, st, d.nb, d.nw, q->E, d.nb, q->ws, q->flags);
+  return spx_inverse(ctx, st, d.nb, d.nw, q->E, d.nb, q->ws, q->flags, nullptr);
 }
 
 void destroy(spexhip_ctx *ctx, spexhip_qpipe *q) {

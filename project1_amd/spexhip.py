@@ -358,6 +358,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_wavefproducts2_pw": (I, [_vp, _vp, _vp, I, I, _vp, _vp, _vp, I, I, I, I, _vp]),
         "spexhip_dielectric_dev": (I, [_vp, I, I, _vp, I, I, _vp, _vp, I]),
         "spexhip_inverse_hpd_dev": (I, [_vp, I, I, _vp, I]),
+        "spexhip_screened_w_dev": (I, [_vp, I, I, _vp, I, I, _vp, _vp, I]),
+        "spexhip_rpa_integrand_dev": (I, [_vp, I, I, _vp, I, C.c_double, _vp]),
         "spexhip_dielectric_inverse": (I, [_vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_inverse_hpd_packed": (I, [_vp, I, I, _vp]),
     }
@@ -685,6 +687,18 @@ class Context:
                        vhalf: DevBuf, E: DevBuf, lde: int) -> None:
         self._check(lib().spexhip_dielectric_dev(self.h, nb, nw, X.ptr, ldx, vkind,
                                                  vhalf.ptr, E.ptr, lde))
+
+    def screened_w_dev(self, Einv: DevBuf, nb: int, nw: int, lde: int, vkind: int,
+                       vhalf: DevBuf, W: DevBuf, ldw: int) -> None:
+        self._check(lib().spexhip_screened_w_dev(self.h, nb, nw, Einv.ptr, lde, vkind, vhalf.ptr,
+                                                 W.ptr, ldw))
+
+    def rpa_integrand_dev(self, E: DevBuf, nb: int, nw: int, lde: int,
+                          weight: float = 1.0) -> np.ndarray:
+        out = np.zeros(nw, np.float64)
+        self._check(lib().spexhip_rpa_integrand_dev(self.h, nb, nw, E.ptr, lde, weight,
+                                                    _ptr(out)))
+        return out
 
     def inverse_hpd_dev(self, X: DevBuf, nb: int, nw: int, ldx: int) -> None:
         self._check(lib().spexhip_inverse_hpd_dev(self.h, nb, nw, X.ptr, ldx))

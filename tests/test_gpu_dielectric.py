@@ -74,3 +74,55 @@ def test_dielectric_inverse_dense_v(ctx, nb):
         mx, fro = normwise(got[f], ref)
         print(f"eps^-1 dense nb={nb} f={f}: max-rel {mx:.2e} fro-rel {fro:.2e}")
         assert fro < 1e-11, (f, mx, fro)
+
+
+@pytest.mark.parametrize("nb", [1, 64, 130])
+def test_rpa_integrand_vs_oracle(ctx, nb):
+    """log det from the Gauss-Jordan pivots (+ nb - trace) against the
+    eigenvalue sum of the reference algorithm; 3 frequencies, weight 1/8."""
+    g = np.random.default_rng(nb + 7)
+    mats = []
+    for _ in range(3):
+        a = g.standard_normal((nb, nb)) + 1j * g.standard_normal((nb, nb))
+        mats.append(oracle.pack_upper(a @ a.conj().T / nb + np.eye(nb)))
+    p = np.stack(mats)
+    dE = ctx.alloc(16 * nb * nb * 3)
+    ctx.unpack(p, nb, 3, dE, nb)
+    got = ctx.rpa_integrand_dev(dE, nb, 3, nb, 0.125)
+    want = np.array([oracle.rpa_integrand(p[f], 0.125) for f in range(3)])
+    # the sum cancels: compare against the size of its two halves
+    scale = np.array([abs(np.log(np.linalg.eigvalsh(oracle.unpack_herm(p[f], nb)))).sum() + nb
+                      for f in range(3)]) * 0.125
+    print(f"rpa nb={nb}: got {got} want {want}")
+    assert np.all(np.abs(got - want) < 1e-12 * scale)
+    # eps itself is left unchanged
+    back = ctx.pack_upper(dE, nb, 3, nb)
+    assert np.array_equal(back, p)
+    dE.free()
+
+
+@pytest.mark.parametrize("vkind", [0, 1])
+def test_screened_w_vs_oracle(ctx, vkind):
+    nb, nw = 70, 2
+    c, chi = chi_case(nb, nw, seed=12)
+    g = np.random.default_rng(12)
+    if vkind:
+        _, vh, _ = synth.coulomb(g, nb)
+        vbuf = np.asfortranarray(vh.astype(np.complex128))
+        einv = np.stack([oracle.inverse_hpd_packed(oracle.dielectric_dense(chi[f], vh.astype(complex)))
+                         for f in range(nw)])
+    else:
+        vh = np.sqrt(np.exp(g.uniform(np.log(1e-3), np.log(1e3), nb)))
+        vbuf = vh
+        einv = np.stack([oracle.inverse_hpd_packed(oracle.dielectric_diag(chi[f], vh)) for f in range(nw)])
+    want = np.stack([oracle.screened_w(einv[f], vh) for f in range(nw)])
+    dE, dW, dV = ctx.alloc(16 * nb * nb * nw), ctx.alloc(16 * nb * nb * nw), ctx.to_dev(vbuf)
+    ctx.unpack(einv, nb, nw, dE, nb)
+    ctx.screened_w_dev(dE, nb, nw, nb, vkind, dV, dW, nb)
+    got = ctx.pack_upper(dW, nb, nw, nb)
+    mx, fro = normwise(got, want)
+    print(f"W vkind={vkind}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < 1e-13
+    for b in (dE, dW, dV):
+        b.free()
+
