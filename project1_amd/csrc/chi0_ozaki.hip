@@ -49,6 +49,7 @@ constexpr int TT = 64;     // output tile edge
 constexpr int NT = 256;
 constexpr int FLUSH = 64;  // chunks between int32 -> FP64 flushes
 constexpr int PFD = 1;     // L2 prefetch distance in chunks (1: none beyond the double buffer)
+constexpr int OZK_WAVES = 8;  // waves per rank-k workgroup (4 or 8)
 constexpr int NPL = 2 * NS;  // planes per chunk
 
 __device__ inline long blk_off(long c, int pl, int ib, int nib) {
@@ -221,25 +222,34 @@ __device__ inline void tile_of(long t0, int T, int nbt, int &wg, int &bi, int &b
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Rank-k update on int8 slices: 64x64 tile per workgroup, 4 waves (2x2) of
-// 32x32 (2x2 blocks of 16x16).  Per chunk the workgroup stages 8 planes x
-// 4 row blocks + 8 planes x 4 column blocks (64 KB) into one of two LDS
-// buffers; each wave issues 16 global_load_lds_dwordx4 (1 KB each).
+// Rank-k update on int8 slices: 64x64 tile per workgroup of NWV waves, a
+// 2 x NWV/2 grid of 32-row x 64/(NWV/2)-column wave tiles (16x16 blocks).
+// Per chunk the workgroup stages 8 planes x 4 row blocks + 8 planes x 4
+// column blocks (64 KB) into one of two LDS buffers; each wave issues
+// 64/NWV global_load_lds_dwordx4 (1 KB each).  With NWV = 8 two waves share
+// each SIMD, so one wave's barrier and LDS waits hide behind the other's
+// MFMAs.
 //
 // Persistent: one workgroup per CU (G of them, a multiple of 8) walks the
 // tiles in rounds of G.  Every tile streams the same K, so the tiles of a
 // round run side by side, and within a round XCD x (workgroups x, x+8, ...)
 // holds G/8 consecutive tiles of the logical order: tiles that share row
 // blocks read them through one L2 at nearly the same time.
-__global__ __launch_bounds__(NT) void ozk_herk_kernel(
+template <int NWV>
+__global__ __launch_bounds__(NWV * 64) void ozk_herk_kernel(
     long nchunk, int nb, int nbp, int ntile, int ngroup, const unsigned char *__restrict__ S,
     long sbytes, int w0, const double *__restrict__ scale, const double *__restrict__ sgn,
     double2 *__restrict__ chi, long ldc, double alpha, double beta) {
+  constexpr int WCN = NWV / 2;     // waves across the columns
+  constexpr int BB = 4 / WCN;      // 16-column blocks per wave
+  constexpr int EPW = 64 / NWV;    // 1 KB copies per wave per chunk
+  constexpr int PFI = EPW / 8;     // prefetch instructions per wave per chunk
+  static_assert(NWV == 4 || NWV == 8, "4 or 8 waves");
   // [buf][panel][plane][block of 4][1 KB], then 256 B of prefetch scratch
   constexpr int LPF = 2 * 2 * NPL * 4 * 1024;
   __shared__ __attribute__((aligned(1024))) unsigned char Ls[LPF + 256];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = wv >> 1, wc = wv & 1;
+  const int wr = wv / WCN, wc = wv % WCN;
   const int nib = nbp / 16, nbt = nbp / TT;
   const long ntot = (long)ntile * ngroup, G = gridDim.x;
   const long slot = (long)(blockIdx.x % 8) * (G / 8) + blockIdx.x / 8;
@@ -247,12 +257,12 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
   int bi = 0, bj = 0, wg = 0, w = 0;
   const unsigned char *Sw = S;
 
-  // wave wv loads the 16 blocks e = wv*16 .. wv*16+15 of the 64 per chunk:
+  // wave wv loads the blocks e = wv*EPW .. of the 64 per chunk:
   // e -> panel = e/32, plane = (e/4)%8, block = e%4
   auto issue = [&](long c, int buf) {
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-      const int e = wv * 16 + r;
+    for (int r = 0; r < EPW; r++) {
+      const int e = wv * EPW + r;
       const int panel = e >> 5, pl = (e >> 2) & 7, blk = e & 3;
       const int ibk = (panel ? bj : bi) * 4 + blk;
       const unsigned char *src = Sw + blk_off(c, pl, ibk, nib) + 16 * lane;
@@ -261,14 +271,14 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     }
   };
   // L2 prefetch of chunk c: one dword of each 128-byte line of this wave's
-  // 16 blocks (128 lines, two instructions), dropped into a scratch LDS word
-  // per lane.  The double buffer alone gives a copy one chunk of MFMA work
-  // to land, less than an HBM round trip under load; with the prefetch
-  // PFD-1 chunks ahead the copies hit L2.
+  // blocks (8 lines per block, 64 per instruction), dropped into a scratch
+  // LDS word per lane.  The double buffer alone gives a copy one chunk of
+  // MFMA work to land; with the prefetch PFD-1 chunks ahead the copies hit
+  // L2.
   auto prefetch = [&](long c) {
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int e = wv * 16 + h * 8 + (lane >> 3);
+    for (int h = 0; h < PFI; h++) {
+      const int e = wv * EPW + h * 8 + (lane >> 3);
       const int panel = e >> 5, pl = (e >> 2) & 7, blk = e & 3;
       const int ibk = (panel ? bj : bi) * 4 + blk;
       const unsigned char *src = Sw + blk_off(c, pl, ibk, nib) + 128 * (lane & 7);
@@ -279,7 +289,9 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
   // issued before the last n prefetch instructions have landed), lgkmcnt 0.
   auto wait_copies = [&](int n) {
     if (n >= 4) __builtin_amdgcn_s_waitcnt(0x74);
-    else if (n >= 2) __builtin_amdgcn_s_waitcnt(0x72);
+    else if (n == 3) __builtin_amdgcn_s_waitcnt(0x73);
+    else if (n == 2) __builtin_amdgcn_s_waitcnt(0x72);
+    else if (n == 1) __builtin_amdgcn_s_waitcnt(0x71);
     else __builtin_amdgcn_s_waitcnt(0x70);
   };
   auto opnd = [&](int buf, int panel, int pl, int blk) -> i4_t {
@@ -290,12 +302,12 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
   // for Im per 16x16 block; Im = Ai.Br + (-Ar).Bi.  r4: Re of the pair of
   // third slices (0-based p = q = 2), the one level-4 term that does not
   // average out (on the diagonal it is a sum of squares).
-  i4_t re[NS][2][2], im[NS][2][2], r4[2][2];
+  i4_t re[NS][2][BB], im[NS][2][BB], r4[2][BB];
   auto zero_acc = [&]() {
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
+      for (int b = 0; b < BB; b++) {
         r4[a][b] = i4_t{0, 0, 0, 0};
 #pragma unroll
         for (int l = 0; l < NS; l++) {
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
   double sg = 0.0;
   auto flush = [&](bool first) {
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < BB; b++) {
       const int J = J0 + b * 16;
       if (J >= nb) continue;
       const double sJ = sg * scale[(long)w * nb + J];
@@ -360,8 +372,8 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     // this wave's copies of chunk c have landed (prefetches issued after
     // them may still be in flight) ...
     wait_copies(PFD <= 1 ? 0
-                : c == 0 ? 2 * (int)(nchunk - 1 < PFD - 1 ? nchunk - 1 : PFD - 1)
-                         : (c - 1 + PFD < nchunk ? 2 : 0));
+                : c == 0 ? PFI * (int)(nchunk - 1 < PFD - 1 ? nchunk - 1 : PFD - 1)
+                         : (c - 1 + PFD < nchunk ? PFI : 0));
     // ... and every other wave's (a plain barrier: the wait above already
     // made this wave's copies visible, and a fence would drain the
     // prefetches too)
@@ -370,13 +382,13 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     if (PFD > 1 && c + PFD < nchunk) prefetch(c + PFD);
     // every column-side operand of the chunk stays in registers; the row
     // side is read once per slice
-    i4_t Br[NS][2], Bi[NS][2];
+    i4_t Br[NS][BB], Bi[NS][BB];
 #pragma unroll
     for (int q = 0; q < NS; q++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        Br[q][b] = opnd(buf, 1, q, wc * 2 + b);
-        Bi[q][b] = opnd(buf, 1, NS + q, wc * 2 + b);
+      for (int b = 0; b < BB; b++) {
+        Br[q][b] = opnd(buf, 1, q, wc * BB + b);
+        Bi[q][b] = opnd(buf, 1, NS + q, wc * BB + b);
       }
 #pragma unroll
     for (int p = 0; p < NS; p++) {
@@ -395,14 +407,14 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
-          for (int b = 0; b < 2; b++) {
+          for (int b = 0; b < BB; b++) {
             re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[q][b], z ? z4 : re[l][a][b], 0, 0, 0);
             im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Br[q][b], z ? z4 : im[l][a][b], 0, 0, 0);
           }
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
-          for (int b = 0; b < 2; b++) {
+          for (int b = 0; b < BB; b++) {
             re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[q][b], re[l][a][b], 0, 0, 0);
             im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(An[a], Bi[q][b], im[l][a][b], 0, 0, 0);
           }
@@ -411,7 +423,7 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
 #pragma unroll
         for (int a = 0; a < 2; a++)
 #pragma unroll
-          for (int b = 0; b < 2; b++) {
+          for (int b = 0; b < BB; b++) {
             r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Br[2][b], FIRST ? z4 : r4[a][b], 0, 0, 0);
             r4[a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[2][b], r4[a][b], 0, 0, 0);
           }
@@ -424,7 +436,7 @@ __global__ __launch_bounds__(NT) void ozk_herk_kernel(
     w = w0 + wg;
     Sw = S + (long)wg * sbytes;
     I0 = bi * TT + wr * 32 + 4 * lk;
-    J0 = bj * TT + wc * 32 + lr;
+    J0 = bj * TT + wc * (16 * BB) + lr;
     C = chi + (long)w * ldc * nb;
     sg = sgn[w] * alpha;
     __syncthreads();  // the previous tile's LDS reads are done before new copies land
@@ -553,8 +565,9 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
     // persistent grid: one workgroup per CU (128 KB of LDS each), a multiple of 8
     const long want = ((long)ntile * g + 7) / 8 * 8;
     const int grid = (int)std::min<long>(want, std::max(8, ozk_ncu(ctx) / 8 * 8));
-    hipLaunchKernelGGL(ozk_herk_kernel, dim3(grid), dim3(NT), 0, st, nchunk, nb, nbp, ntile, g,
-                       S, (long)sb, w0, scale, sgn, (double2 *)chi, (long)ldchi, alpha, beta);
+    hipLaunchKernelGGL(ozk_herk_kernel<OZK_WAVES>, dim3(grid), dim3(OZK_WAVES * 64), 0, st,
+                       nchunk, nb, nbp, ntile, g, S, (long)sb, w0, scale, sgn, (double2 *)chi,
+                       (long)ldchi, alpha, beta);
     SPX_LAUNCHED(ctx);
   }
   if (timed) {
