@@ -221,32 +221,34 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def batch_leg(ctx, a, budget=160 << 30):
+def batch_leg(ctx, a, precision="split_fp32", budget=160 << 30):
     """32 uneven q-points (Nk=64, Nocc=16, Nunocc ~ U[200,600], Nb ~ U[600,1600],
-    Nw=12) in split-FP32: per k packet one grouped chi0 launch over all q
-    with the autotuned tile configuration; the fork/join pipeline is
-    captured into one HIP graph.  Two q are checked against their FP64
-    per-q pipelines."""
+    Nw=12).  split_fp32 / fp64: per k packet one grouped chi0 launch over all
+    q with the autotuned tile configuration; ozaki_int8: per-q int8 rank-k
+    launches on the q streams.  The fork/join pipeline is captured into one
+    HIP graph.  Two q are checked against their FP64 per-q pipelines."""
     from project1_amd import serving, spexhip, synth
 
+    prec = {"fp64": spexhip.PREC_FP64, "split_fp32": spexhip.PREC_SPLIT_FP32,
+            "ozaki_int8": spexhip.PREC_OZAKI_INT8}[precision]
     lat = serving.Lattice(ctx, nocc=16, nunocc=600, seed=a.seed + 5)
     g = synth.rng(a.seed + 6)
     specs = [serving.Spec(nmesh=4, nb=int(g.integers(600, 1601)), nw=12, seed=a.seed * 10 + i,
                           nunocc=int(g.integers(200, 601)), iq=1 + i) for i in range(32)]
-    reqs = [serving.Request(lat, sp, spexhip.PREC_SPLIT_FP32) for sp in specs]
+    reqs = [serving.Request(lat, sp, prec) for sp in specs]
     descs = [r.desc for r in reqs]
-    kp = next(k for k in (64, 32, 16, 8, 4) if spexhip.Batch.footprint(descs, k) <= budget)
+    kp = next(k for k in (64, 32, 16, 8, 4, 2, 1) if spexhip.Batch.footprint(descs, k) <= budget)
     t0 = time.perf_counter()
     b = spexhip.Batch(ctx, descs, kpacket=kp, config=-1, tune_k=16384)
     t_create = time.perf_counter() - t0
     cfg, npacket, tune = b.config()
-    log(f"batch: {len(reqs)} q, kpacket {kp}, config {cfg}, created in {t_create:.1f} s")
+    log(f"batch {precision}: {len(reqs)} q, kpacket {kp}, config {cfg}, created in {t_create:.1f} s")
     b.launch(graph=False)
     ms_eager = b.wait()
-    log(f"batch: eager run {ms_eager:.0f} ms")
+    log(f"batch {precision}: eager run {ms_eager:.0f} ms")
     b.launch(graph=True)  # captured once, then replayed
     ms_graph = b.wait()
-    log(f"batch: graph run {ms_graph:.0f} ms")
+    log(f"batch {precision}: graph run {ms_graph:.0f} ms")
     flops = sum(chi0_flops(r.spec.nw, r.n, r.spec.nb) + eps_flops(r.spec.nw, r.spec.nb)
                 for r in reqs)
     check = {}
@@ -264,14 +266,16 @@ def batch_leg(ctx, a, budget=160 << 30):
     for r in reqs:
         r.free()
     lat.free()
-    tm, kc, pc = spexhip.chi0_configs()[cfg]
-    return {"workload": "32 q, Nk=64, Nocc=16, Nunocc~U[200,600], Nb~U[600,1600], Nw=12, split-FP32",
-            "kpacket": kp, "packets": npacket,
-            "config": {"index": cfg, "tile": tm, "kc": kc, "wg_per_cu": pc},
-            "autotune_ms": [round(t, 3) for t in tune], "create_s": round(t_create, 2),
-            "ms_eager": round(ms_eager, 1), "ms_graph": round(ms_graph, 1),
-            "qpts_per_s": round(32e3 / ms_graph, 3), "gflops": round(flops / ms_graph / 1e6, 1),
-            "err_vs_fp64_per_q": check}
+    out = {"workload": f"32 q, Nk=64, Nocc=16, Nunocc~U[200,600], Nb~U[600,1600], Nw=12, {precision}",
+           "kpacket": kp, "packets": npacket, "create_s": round(t_create, 2),
+           "ms_eager": round(ms_eager, 1), "ms_graph": round(ms_graph, 1),
+           "qpts_per_s": round(32e3 / ms_graph, 3), "gflops": round(flops / ms_graph / 1e6, 1),
+           "err_vs_fp64_per_q": check}
+    if cfg >= 0:
+        tm, kc, pc = spexhip.chi0_configs()[cfg]
+        out["config"] = {"index": cfg, "tile": tm, "kc": kc, "wg_per_cu": pc}
+        out["autotune_ms"] = [round(t, 3) for t in tune]
+    return out
 
 
 PEAK_I8 = 5000.0  # TOPS dense int8 matrix, MI355X
@@ -376,7 +380,7 @@ def main():
         line["cpu_baseline"] = cpu_baseline(pipe, a)
     pipe.free()
     if not a.no_batch:
-        line["batch"] = batch_leg(ctx, a)
+        line["batch"] = {f: batch_leg(ctx, a, f) for f in ("split_fp32", "ozaki_int8")}
         log(f"batch {json.dumps(line['batch'])}")
     if not a.no_serve:
         from project1_amd import serving

@@ -323,7 +323,9 @@ int spexhip_qpipe_destroy(spexhip_ctx *ctx, spexhip_qpipe *qp);
  * products and weights run on one of up to 8 streams, then ONE persistent
  * grouped chi0 launch covers all q (chi0 += packet); after the last packet
  * each q's eps and eps^-1 run on its stream.  Descriptors as for
- * spexhip_qpipe (products from plans; all FP64 or all SPLIT_FP32).
+ * spexhip_qpipe (products from plans; one precision for all q).  In the
+ * Ozaki form each q's chi0 += packet runs on its stream right after its
+ * products (no grouped launch; config is reported as -1).
  * config < 0: the tile configuration is chosen by timing every
  * configuration on packet 0 with all K scaled by one factor so that the
  * longest K is tune_k (tune_k <= 0: unscaled).                           */

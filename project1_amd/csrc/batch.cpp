@@ -3,7 +3,10 @@
 // q (tiles in longest-first order, chi0 += packet contribution); after the
 // last packet every q's eps = 1 - v^1/2 chi0 v^1/2 and eps^-1 run on their
 // streams again.  The fork/join structure is captured once into a HIP graph
-// and replayed.
+// and replayed.  In the Ozaki form each q's chi0 += packet runs right after
+// its products on its own stream (the int8 rank-k kernel is persistent and
+// fills the device by itself); the q of one stream share one slice
+// workspace.
 //
 // Reference flow per q: correlation.f:404-747 (susceptibility -> dielectric
 // matrix -> Minverse); the k packets are the reference's (nk, no, nu)
@@ -12,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "internal.hpp"
@@ -62,7 +66,9 @@ struct spexhip_batch {
   hipStream_t main = nullptr;
   hipStream_t st[kMaxStreams] = {};
   hipEvent_t fork = nullptr, join[kMaxStreams] = {}, ev0 = nullptr, ev1 = nullptr;
-  int *flags = nullptr;  // per q: eps not positive definite
+  int *flags = nullptr;  // per q: [0, nq) eps not positive definite, [nq, 2nq) mixed signs
+  void *ozws[kMaxStreams] = {};  // Ozaki slice workspace per stream
+  size_t ozwsb[kMaxStreams] = {};
   std::vector<void *> owned;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -109,16 +115,24 @@ int packet_products(spexhip_ctx *ctx, spexhip_batch *b, BatchQ &q, int k0, int c
 }
 
 int enqueue(spexhip_ctx *ctx, spexhip_batch *b) {
-  int rc = spx_fill_zero(ctx, b->main, b->flags, 4 * b->q.size());
+  const bool ozaki = b->precision == SPEXHIP_PREC_OZAKI_INT8;
+  const int nq = (int)b->q.size();
+  int rc = spx_fill_zero(ctx, b->main, b->flags, 8 * (size_t)nq);
   if (!rc) rc = fork_all(ctx, b);
   for (auto &q : b->q)
     if (!rc) rc = spx_fill_zero(ctx, b->st[q.s], q.X, 16 * (size_t)q.d.nb * q.d.nb * q.d.nw);
   for (int p = 0; p < b->npacket && !rc; p++) {
     if (p > 0) rc = fork_all(ctx, b);
-    for (auto &q : b->q) {
+    for (int i = 0; i < nq && !rc; i++) {
+      BatchQ &q = b->q[i];
       const int k0 = p * b->kpacket;
       const int cnt = std::min(b->kpacket, q.d.nk - k0);
-      if (!rc && cnt > 0) rc = packet_products(ctx, b, q, k0, cnt);
+      if (cnt <= 0) continue;
+      rc = packet_products(ctx, b, q, k0, cnt);
+      if (!rc && ozaki)
+        rc = spx_chi0_ozaki_async(ctx, b->st[q.s], (long)cnt * q.d.nocc * q.d.nunocc, q.d.nb, q.M,
+                                  q.d.nb, 0, q.d.nw, q.D, q.d.alpha, 1.0, q.X, q.d.nb,
+                                  b->ozws[q.s], b->ozwsb[q.s], b->flags + nq + i);
     }
     if (!rc) rc = join_all(ctx, b);
     if (!rc && b->groups[p]) rc = spx_group_launch(ctx, b->main, b->groups[p]);
@@ -182,11 +196,20 @@ extern "C" {
 
 int spexhip_batch_footprint(int nq, const spexhip_qdesc *descs, int kpacket, size_t *bytes) {
   if (nq <= 0 || !descs || kpacket <= 0 || !bytes) return SPEXHIP_E_INVALID;
-  size_t t = 256 + 4 * (size_t)nq;
+  size_t t = 256 + 8 * (size_t)nq;
+  size_t ozs[kMaxStreams] = {};
+  const int nstream = std::min(nq, kMaxStreams);
   for (int i = 0; i < nq; i++) {
-    const Sizes s = sizes_of(descs[i], kpacket);
+    const spexhip_qdesc &d = descs[i];
+    const Sizes s = sizes_of(d, kpacket);
     t += s.M + s.D + 2 * s.X + s.ws + s.tab;
+    if (d.precision == SPEXHIP_PREC_OZAKI_INT8) {
+      const long n = (long)std::min(kpacket, d.nk) * d.nocc * d.nunocc;
+      ozs[i % nstream] = std::max(ozs[i % nstream],
+                                  1024 + spx_ozaki_ws_bytes(n, d.nb, d.nw, d.ozaki_group > 0 ? std::min(d.ozaki_group, d.nw) : d.nw));
+    }
   }
+  for (int s = 0; s < nstream; s++) t += ozs[s];
   *bytes = t;
   return SPEXHIP_OK;
 }
@@ -197,8 +220,8 @@ int spexhip_batch_create(spexhip_ctx *ctx, int nq, const spexhip_qdesc *descs, i
   *out = nullptr;
   SPX_CHECK(ctx, nq > 0 && descs && kpacket > 0, "batch: bad arguments");
   const int prec = descs[0].precision;
-  SPX_CHECK(ctx, prec == SPEXHIP_PREC_FP64 || prec == SPEXHIP_PREC_SPLIT_FP32,
-            "batch: precision must be FP64 or SPLIT_FP32 (the grouped kernel's forms)");
+  SPX_CHECK(ctx, prec >= 0 && prec <= 2, "batch: unknown precision");
+  const bool ozaki = prec == SPEXHIP_PREC_OZAKI_INT8;
   int cfgs = 16;
   SPX_CHECK(ctx, spexhip_chi0_configs(&cfgs, nullptr, nullptr, nullptr) == 0 && config < cfgs,
             "batch: config %d out of range", config);
@@ -231,7 +254,7 @@ int spexhip_batch_create(spexhip_ctx *ctx, int nq, const spexhip_qdesc *descs, i
     if (!rc) b->owned.push_back(*p);
   };
   void *fl = nullptr;
-  grab(256 + 4 * (size_t)nq, &fl);
+  grab(256 + 8 * (size_t)nq, &fl);
   b->flags = (int *)fl;
   int maxnk = 0;
   for (int i = 0; i < nq && !rc; i++) {
@@ -277,9 +300,26 @@ int spexhip_batch_create(spexhip_ctx *ctx, int nq, const spexhip_qdesc *descs, i
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "batch: stream/event creation failed");
   }
-  // tile configuration: given, or timed on packet 0 with every q's K scaled
-  // down by one common factor (so the longest-first balance is preserved)
-  if (!rc && config < 0) {
+  // Ozaki: one slice workspace per stream, sized for its largest packet
+  for (int s = 0; s < b->nstream && !rc && ozaki; s++) {
+    size_t need = 0;
+    for (const auto &q : b->q)
+      if (q.s == s) {
+        const long n = (long)std::min(kpacket, q.d.nk) * q.d.nocc * q.d.nunocc;
+        need = std::max(need, spx_ozaki_ws_bytes(n, q.d.nb, q.d.nw,
+                                                 q.d.ozaki_group > 0 ? std::min(q.d.ozaki_group, q.d.nw) : q.d.nw));
+      }
+    void *raw = nullptr;
+    grab(need + 1024, &raw);  // the slice kernels want a 1 KB aligned workspace
+    b->ozws[s] = (void *)(((uintptr_t)raw + 1023) & ~(uintptr_t)1023);
+    b->ozwsb[s] = need;
+  }
+  // tile configuration (grouped kernel): given, or timed on packet 0 with
+  // every q's K scaled down by one common factor (so the longest-first
+  // balance is preserved)
+  if (!rc && ozaki) {
+    b->config = -1;
+  } else if (!rc && config < 0) {
     std::vector<spexhip_chi0_problem> pr = packet_problems(b, 0);
     long nmax = 1;
     for (auto &x : pr) nmax = std::max(nmax, x.n);
@@ -292,7 +332,8 @@ int spexhip_batch_create(spexhip_ctx *ctx, int nq, const spexhip_qdesc *descs, i
   for (int p = 0; p < b->npacket && !rc; p++) {
     std::vector<spexhip_chi0_problem> pr = packet_problems(b, p);
     spexhip_group *g = nullptr;
-    if (!pr.empty()) rc = spexhip_group_create(ctx, (int)pr.size(), pr.data(), prec, b->config, &g);
+    if (!pr.empty() && !ozaki)
+      rc = spexhip_group_create(ctx, (int)pr.size(), pr.data(), prec, b->config, &g);
     b->groups.push_back(g);
   }
   if (rc) {
@@ -349,9 +390,12 @@ int spexhip_batch_wait(spexhip_ctx *ctx, spexhip_batch *b, float *ms) {
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   SPX_HIP(ctx, hipEventSynchronize(b->ev1));
   if (ms) SPX_HIP(ctx, hipEventElapsedTime(ms, b->ev0, b->ev1));
-  std::vector<int> f(b->q.size());
+  const size_t nq = b->q.size();
+  std::vector<int> f(2 * nq);
   SPX_HIP(ctx, hipMemcpy(f.data(), b->flags, 4 * f.size(), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < f.size(); i++)
+  for (size_t i = 0; i < nq; i++)
+    if (f[nq + i]) return spx_seterr(ctx, SPEXHIP_E_INVALID, "batch: chi0 weights of q %zu have mixed signs (Ozaki form)", i);
+  for (size_t i = 0; i < nq; i++)
     if (f[i]) return spx_seterr(ctx, SPEXHIP_E_NOTPD, "batch: eps of q %zu not positive definite", i);
   return SPEXHIP_OK;
 }

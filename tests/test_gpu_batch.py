@@ -89,12 +89,23 @@ def test_batch_vs_oracle_and_graph_replays(lattice):
         r.free()
 
 
-def test_batch_rejects_ozaki_and_reports_footprint(lattice):
-    reqs = uneven(lattice, spexhip.PREC_OZAKI_INT8, nq=2)
-    with pytest.raises(spexhip.SpexError) as e:
-        spexhip.Batch(lattice.ctx, [r.desc for r in reqs], kpacket=4)
-    assert e.value.code == spexhip.E_INVALID
+def test_batch_ozaki_matches_per_q_and_footprint(lattice):
+    """Ozaki form: per-q int8 rank-k launches on the q streams, one slice
+    workspace per stream; against the FP64 per-q pipelines at 1e-8."""
+    reqs = uneven(lattice, spexhip.PREC_OZAKI_INT8, nq=5, seed=11)
+    ref = per_q(lattice, [serving.Request(lattice, r.spec, spexhip.PREC_FP64) for r in reqs])
     descs = [r.desc for r in reqs]
     assert spexhip.Batch.footprint(descs, 2) < spexhip.Batch.footprint(descs, 8)
+    b = spexhip.Batch(lattice.ctx, descs, kpacket=3)
+    cfg, npacket, _ = b.config()
+    assert cfg == -1 and npacket == 3
+    b.launch(graph=True)
+    b.wait()
+    for i, r in enumerate(reqs):
+        chi, einv = b.result(i)
+        e1 = normwise(chi, ref[i][0])
+        print(f"ozaki batch q{i}: chi0 {e1}")
+        assert max(e1) < 1e-8
+    b.destroy()
     for r in reqs:
         r.free()
