@@ -399,6 +399,12 @@ int spexhip_screened_w_dev(spexhip_ctx *ctx, int nb, int nw, const void *einv_de
  * out: host, nw doubles.  SPEXHIP_E_NOTPD if eps is not positive definite. */
 int spexhip_rpa_integrand_dev(spexhip_ctx *ctx, int nb, int nw, const void *eps_dev, int lde,
                               double weight, double *out);
+/* Host drop-ins of the two above on packed upper storage (nb(nb+1)/2 x nw
+ * complex, the reference's dielec / screen arrays).                       */
+int spexhip_screened_w_packed(spexhip_ctx *ctx, int nb, int nw, const double *einv_packed,
+                              int vkind, const double *vhalf, double *w_packed);
+int spexhip_rpa_integrand_packed(spexhip_ctx *ctx, int nb, int nw, const double *eps_packed,
+                                 double weight, double *out);
 
 /* Host drop-in for correlation.f:612-747: chi (packed, nb(nb+1)/2 x nw, the
  * reference susceph after the spin factor) -> eps^-1 (packed, same layout).

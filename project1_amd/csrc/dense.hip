@@ -405,6 +405,44 @@ int spexhip_rpa_integrand_dev(spexhip_ctx *ctx, int nb, int nw, const void *eps,
   return SPEXHIP_OK;
 }
 
+// Host drop-ins on packed upper storage (the reference dielec / screen).
+int spexhip_screened_w_packed(spexhip_ctx *ctx, int nb, int nw, const double *einv_packed,
+                              int vkind, const double *vhalf, double *w_packed) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && einv_packed && vhalf && w_packed && (vkind == 0 || vkind == 1),
+            "screened W (packed): bad arguments");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t mb = 16 * (size_t)nb * nb * nw, vb = vkind ? 16 * (size_t)nb * nb : 8 * (size_t)nb;
+  void *E = nullptr, *W = nullptr, *V = nullptr;
+  int rc = spx_alloc(ctx, mb, &E);
+  if (!rc) rc = spx_alloc(ctx, mb, &W);
+  if (!rc) rc = spx_alloc(ctx, vb, &V);
+  if (!rc && hipMemcpy(V, vhalf, vb, hipMemcpyHostToDevice) != hipSuccess)
+    rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "screened W: upload failed");
+  if (!rc) rc = spexhip_unpack_h2d(ctx, nb, nw, einv_packed, E, nb);
+  if (!rc) rc = spexhip_screened_w_dev(ctx, nb, nw, E, nb, vkind, V, W, nb);
+  if (!rc) rc = spexhip_pack_upper_d2h(ctx, nb, nw, W, nb, w_packed, 0);
+  (void)hipStreamSynchronize(ctx->stream);
+  spx_free(ctx, E);
+  spx_free(ctx, W);
+  spx_free(ctx, V);
+  return rc;
+}
+
+int spexhip_rpa_integrand_packed(spexhip_ctx *ctx, int nb, int nw, const double *eps_packed,
+                                 double weight, double *out) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && eps_packed && out, "rpa integrand (packed): bad arguments");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *E = nullptr;
+  int rc = spx_alloc(ctx, 16 * (size_t)nb * nb * nw, &E);
+  if (!rc) rc = spexhip_unpack_h2d(ctx, nb, nw, eps_packed, E, nb);
+  if (!rc) rc = spexhip_rpa_integrand_dev(ctx, nb, nw, E, nb, weight, out);
+  (void)hipStreamSynchronize(ctx->stream);
+  spx_free(ctx, E);
+  return rc;
+}
+
 int spexhip_dielectric_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx, int vkind,
                            const void *vhalf, void *E, int lde) {
   if (!ctx) return SPEXHIP_E_INVALID;

@@ -79,6 +79,38 @@ program fdriver
     write(*, '(A,ES10.3)') 'fortran eps^-1 * eps - 1 max: ', maxval(abs(prod))
     if (maxval(abs(prod)) > 1d-10) stop 4
   end block
+  ! screened interaction: screen = sqcoul einv sqcoul
+  block
+    complex(c_double_complex) :: scr(nb*(nb+1)/2, nw)
+    real(c_double) :: d
+    call spexhip_screen(ctx, nb, nw, einv, sq, scr)
+    d = 0
+    ij = 0
+    do j = 1, nb
+      do i = 1, j
+        ij = ij + 1
+        d = max(d, abs(scr(ij, 1) - sq(i) * einv(ij, 1) * sq(j)))
+      end do
+    end do
+    write(*, '(A,ES10.3)') 'fortran screen max error: ', d / maxval(abs(scr(:, 1)))
+    if (d / maxval(abs(scr(:, 1))) > 1d-13) stop 5
+  end block
+  ! RPA integrand of a diagonal dielectric matrix: sum(log d + 1 - d) / 8
+  block
+    complex(c_double_complex) :: dg(nb*(nb+1)/2, 1)
+    real(c_double) :: rint(1), want, dd
+    dg = 0
+    want = 0
+    do j = 1, nb
+      dd = 1 + real(j, c_double) / nb
+      dg(j*(j+1)/2, 1) = dd
+      want = want + (log(dd) + 1 - dd) / 8
+    end do
+    rint = 0
+    call spexhip_rpa_int(ctx, nb, 1, dg, 0.125d0, rint)
+    write(*, '(A,ES10.3)') 'fortran rpa_int rel error: ', abs(rint(1) - want) / abs(want)
+    if (abs(rint(1) - want) > 1d-12 * abs(want)) stop 6
+  end block
   rc = spexhip_destroy(ctx)
   write(*, '(A)') 'fortran driver OK'
 end program fdriver

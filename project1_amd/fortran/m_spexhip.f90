@@ -4,6 +4,8 @@
 !   susceptibility.f:1159-1251  pair loop (HILBERT NONE)   -> spexhip_pairsum
 !   susceptibility.f:778,790    wavefproducts2_mt/_pw      -> spexhip_wfprod2_mt/_pw
 !   correlation.f:612-747       dielectric matrix, Minverse -> spexhip_dielec_inverse
+!   correlation.f:868-872       screened interaction        -> spexhip_screen
+!   correlation.f:684-690       RPA correlation integrand   -> spexhip_rpa_int
 ! A non-zero status stops the run with the library message, like the
 ! reference's Error(...) -> spex_error (util.f:480-505).
 module m_spexhip
@@ -37,7 +39,7 @@ module m_spexhip
   public :: spexhip_create, spexhip_destroy, spexhip_mt_plan_create, spexhip_pw_plan_create
   public :: spexhip_plan_destroy
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
-  public :: spexhip_minverse
+  public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -96,6 +98,19 @@ module m_spexhip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, chi, vhalf, einv
       integer(c_int), value :: nb, nw, vkind
+    end function
+    integer(c_int) function c_screened_w_packed(ctx, nb, nw, einv, vkind, vhalf, w) &
+        bind(C, name='spexhip_screened_w_packed')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, einv, vhalf, w
+      integer(c_int), value :: nb, nw, vkind
+    end function
+    integer(c_int) function c_rpa_integrand_packed(ctx, nb, nw, eps, weight, out) &
+        bind(C, name='spexhip_rpa_integrand_packed')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, eps, out
+      integer(c_int), value :: nb, nw
+      real(c_double), value :: weight
     end function
     integer(c_int) function c_inverse_hpd_packed(ctx, nb, count, packed) &
         bind(C, name='spexhip_inverse_hpd_packed')
@@ -186,6 +201,32 @@ contains
     complex(c_double_complex), intent(out), target :: dielec(nbas*(nbas+1)/2, nfreq)
     call check(ctx, c_dielectric_inverse(ctx, int(nbas, c_int), int(nfreq, c_int), c_loc(suscep), &
                0_c_int, c_loc(sqcoul), c_loc(dielec)), 'dielectric inverse')
+  end subroutine
+
+  ! correlation.f:868-872: screen = v^1/2 dielec v^1/2 per frequency, with
+  ! dielec already inverted; sqcoul = sqrt(coul) (Coulomb eigenbasis).
+  subroutine spexhip_screen(ctx, nbas, nfreq, dielec, sqcoul, screen)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nbas, nfreq
+    complex(c_double_complex), intent(in), target :: dielec(nbas*(nbas+1)/2, nfreq)
+    real(c_double), intent(in), target :: sqcoul(nbas)
+    complex(c_double_complex), intent(out), target :: screen(nbas*(nbas+1)/2, nfreq)
+    call check(ctx, c_screened_w_packed(ctx, int(nbas, c_int), int(nfreq, c_int), c_loc(dielec), &
+               0_c_int, c_loc(sqcoul), c_loc(screen)), 'screened interaction')
+  end subroutine
+
+  ! correlation.f:684-690: rpa_int(i) += sum(log(eig) + 1 - eig) of dielec(:,i)
+  ! times weight = count(kptp==ikpt)/nkpt.
+  subroutine spexhip_rpa_int(ctx, nbas, nfreq, dielec, weight, rpa_int)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nbas, nfreq
+    complex(c_double_complex), intent(in), target :: dielec(nbas*(nbas+1)/2, nfreq)
+    real(c_double), intent(in) :: weight
+    real(c_double), intent(inout) :: rpa_int(nfreq)
+    real(c_double), target :: add(nfreq)
+    call check(ctx, c_rpa_integrand_packed(ctx, int(nbas, c_int), int(nfreq, c_int), c_loc(dielec), &
+               weight, c_loc(add)), 'rpa integrand')
+    rpa_int = rpa_int + add
   end subroutine
 
   ! Minverse(dielec(:,i)) on packed Hermitian positive definite matrices.

@@ -360,6 +360,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_inverse_hpd_dev": (I, [_vp, I, I, _vp, I]),
         "spexhip_screened_w_dev": (I, [_vp, I, I, _vp, I, I, _vp, _vp, I]),
         "spexhip_rpa_integrand_dev": (I, [_vp, I, I, _vp, I, C.c_double, _vp]),
+        "spexhip_screened_w_packed": (I, [_vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_rpa_integrand_packed": (I, [_vp, I, I, _vp, C.c_double, _vp]),
         "spexhip_dielectric_inverse": (I, [_vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_inverse_hpd_packed": (I, [_vp, I, I, _vp]),
     }
@@ -698,6 +700,28 @@ class Context:
         out = np.zeros(nw, np.float64)
         self._check(lib().spexhip_rpa_integrand_dev(self.h, nb, nw, E.ptr, lde, weight,
                                                     _ptr(out)))
+        return out
+
+    def screened_w_packed(self, einv_packed: np.ndarray, vhalf: np.ndarray) -> np.ndarray:
+        """W = v^1/2 eps^-1 v^1/2 on packed storage (nw, nb(nb+1)/2)."""
+        e = np.ascontiguousarray(np.atleast_2d(einv_packed), np.complex128)
+        nw, npack = e.shape
+        nb = int(round((np.sqrt(8 * npack + 1) - 1) / 2))
+        vkind = 1 if np.asarray(vhalf).ndim == 2 else 0
+        v = (np.asfortranarray(vhalf, np.complex128) if vkind else
+             np.ascontiguousarray(vhalf, np.float64))
+        out = np.zeros_like(e)
+        self._check(lib().spexhip_screened_w_packed(self.h, nb, nw, _ptr(e), vkind, _ptr(v),
+                                                    _ptr(out)))
+        return out
+
+    def rpa_integrand_packed(self, eps_packed: np.ndarray, weight: float = 1.0) -> np.ndarray:
+        e = np.ascontiguousarray(np.atleast_2d(eps_packed), np.complex128)
+        nw, npack = e.shape
+        nb = int(round((np.sqrt(8 * npack + 1) - 1) / 2))
+        out = np.zeros(nw, np.float64)
+        self._check(lib().spexhip_rpa_integrand_packed(self.h, nb, nw, _ptr(e), weight,
+                                                       _ptr(out)))
         return out
 
     def inverse_hpd_dev(self, X: DevBuf, nb: int, nw: int, ldx: int) -> None:

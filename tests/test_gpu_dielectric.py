@@ -126,3 +126,20 @@ def test_screened_w_vs_oracle(ctx, vkind):
     for b in (dE, dW, dV):
         b.free()
 
+
+def test_packed_dropins_w_and_rpa(ctx):
+    """Host drop-ins on packed storage: screen = v^1/2 dielec v^1/2 with the
+    reference sqcoul vector, and rpa_int from packed dielec."""
+    nb, nw = 50, 3
+    c, chi = chi_case(nb, nw, seed=5)
+    sq = np.sqrt(np.exp(np.random.default_rng(5).uniform(np.log(1e-2), np.log(1e2), nb)))
+    eps = np.stack([oracle.dielectric_diag(chi[f], sq) for f in range(nw)])
+    einv = np.stack([oracle.inverse_hpd_packed(e) for e in eps])
+    w = ctx.screened_w_packed(einv, sq)
+    want = np.stack([oracle.screened_w(einv[f], sq) for f in range(nw)])
+    assert max(normwise(w, want)) < 1e-13
+    r = ctx.rpa_integrand_packed(eps, 1.0 / 8)
+    want_r = np.array([oracle.rpa_integrand(eps[f], 1.0 / 8) for f in range(nw)])
+    scale = np.abs(want_r).max() + 1e-300
+    assert np.max(np.abs(r - want_r)) < 1e-11 * max(scale, 1.0)
+
