@@ -36,6 +36,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("χ0+ε⁻¹ q-points/s & GFLOP/s, split-FP32 vs FP64 at "
           "≤1e-8 rel; p50/p99 latency")
 PEAK_F64 = 78.6  # TFLOP/s dense FP64 matrix, MI355X
+WORKLOAD = ("single q, Nk=64 Nocc=16 Nunocc=400 Nb=1200 (294 MT + 906 IPW) Nw=12, 36^3 FFT "
+            "grid, dense v^1/2; products->chi0->eps^-1")
 PEAK_F32 = 157.3  # TFLOP/s dense FP32 matrix, MI355X
 
 
@@ -217,6 +219,20 @@ def cpu_baseline(pipe, a):
                 seconds_per_q_extrapolated=t_q)
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` at `workload`, as measured offline with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a copy-kernel
+    calibration (profiles/traffic.json, written from those summaries); None
+    when no measurement of this kernel at this workload is recorded."""
+    path = ROOT / "profiles" / "traffic.json"
+    if not path.exists():
+        return None
+    rec = json.loads(path.read_text()).get(kernel)
+    if not rec or rec.get("workload") != workload:
+        return None
+    return rec["hbm_bytes_per_launch"]
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -337,7 +353,7 @@ def main():
         i8 = 10.5 * flops / (head["rank_ms"] * 1e9)  # 84 int8 ops per complex MAC
         roof = {"bound": "mfma", "kernel": "ozk_herk_kernel", "achieved": round(i8, 1),
                 "peak": PEAK_I8, "unit": "TFLOP/s (int8 ops)", "frac": round(i8 / PEAK_I8, 4),
-                "traffic": None,
+                "traffic": measured_traffic("ozk_herk_kernel", WORKLOAD),
                 "note": "achieved = 84 int8 ops per complex multiply-add of chi0 (10 slice "
                         "pairs x 4 real products + 2 for the (3,3) pair, 2 ops each) / "
                         "rank-k kernel time"}
@@ -364,8 +380,7 @@ def main():
         "data": ("synthetic (seeded wave functions -> products on device each step, random SPD "
                  "Coulomb matrix)" if pipe.products else
                  "synthetic (seeded Gaussian products on device, random SPD Coulomb matrix)"),
-        "config": {"workload": ("single q, Nk=64 Nocc=16 Nunocc=400 Nb=1200 (294 MT + 906 IPW) "
-                                "Nw=12, 36^3 FFT grid, dense v^1/2; products->chi0->eps^-1"),
+        "config": {"workload": WORKLOAD,
                    "nk": a.nk, "nocc": a.nocc, "nunocc": a.nunocc, "nb": a.nb, "nw": a.nw,
                    "parallelism": "1 GPU", "headline_form": head_name},
         "gflops": round(head["gflops"], 1),

@@ -15,9 +15,10 @@
 // below 2.4e-10 * s.  Products of slices with p+q <= 5, plus the real
 // part of the pair (3,3), are accumulated exactly in int32 on
 // v_mfma_i32_16x16x64_i8 -- one int32 accumulator per level l = p+q, since
-// w_p w_q depends on p+q only -- and flushed into FP64 accumulators (weight
-// w_l) every 4096 transitions, far below the int32 overflow bound
-// (4 pairs * 2 products * 127^2 * 4096 < 2^31).  Of the dropped pairs
+// w_p w_q depends on p+q only -- and flushed into FP64 (weight w_l) every
+// 16384 transitions, within the int32 bound: a level gathers at most
+// 4 pairs x 2 products of magnitude <= 127^2 per transition, and
+// 8 * 127^2 * 16384 = 2.114e9 < 2^31 - 1.  Of the dropped pairs
 // (p+q >= 6), those with p != q are products of independent digits and
 // average out over the transitions; (3,3) is kept because on the diagonal
 // (and for correlated columns) it is a sum of squares, a bias of about
@@ -47,7 +48,8 @@ constexpr int NS = 4;      // slices
 constexpr int TK = 64;     // transitions per chunk (MFMA K)
 constexpr int TT = 64;     // output tile edge
 constexpr int NT = 256;
-constexpr int FLUSH = 64;  // chunks between int32 -> FP64 flushes
+constexpr int FLUSH = 256;  // chunks (of 64 transitions) between int32 -> FP64 flushes
+static_assert(8LL * 127 * 127 * 64 * FLUSH <= 2147483647LL, "int32 level accumulators overflow");
 constexpr int PFD = 1;     // L2 prefetch distance in chunks (1: none beyond the double buffer)
 constexpr int OZK_WAVES = 8;  // waves per rank-k workgroup (4 or 8)
 constexpr int NPL = 2 * NS;  // planes per chunk

@@ -168,8 +168,8 @@ def test_ozaki_many_tiles_several_rounds(ctx):
 
 
 def test_ozaki_positive_weights_and_large_k(ctx):
-    """All-positive weights (sign +1) and K spanning several int32 flushes."""
-    c = synth.chi0_case(nk=4, nocc=8, nunocc=300, nb=72, nw=2, seed=4)  # n = 9600 > 4096
+    """All-positive weights (sign +1) and K spanning int32 flushes."""
+    c = synth.chi0_case(nk=4, nocc=8, nunocc=1100, nb=72, nw=2, seed=4)  # n = 35200 > 2 x 16384
     w = -np.array(oracle.weights_none(c.e_occ, c.e_unocc, c.omega))
     ref = oracle.chi0_herk(c.M, np.asfortranarray(w))
     got = run_chi0(ctx, c.M, np.asfortranarray(w), spexhip.PREC_OZAKI_INT8)
