@@ -74,44 +74,39 @@ __global__ void ozk_sqd_kernel(long n, const double *__restrict__ D, double *__r
 }
 
 // scale[w][I] = max_t sqd[w][t] * max(|Re M|, |Im M|), via atomicMax on the
-// bit pattern (non-negative doubles order like unsigned integers).  One
-// workgroup = 64 columns (one per lane) x tchunk transitions, the four waves
-// taking every fourth transition; the waves' maxima meet in LDS and one
-// atomic per column and frequency leaves the workgroup.
+// bit pattern (non-negative doubles order like unsigned integers).
 template <bool COLMAJOR>
-__global__ __launch_bounds__(256) void ozk_scale_kernel(long n, int nb, const double2 *__restrict__ M,
-                                                        long ldm, int nw,
-                                                        const double *__restrict__ sqd, long tchunk,
-                                                        unsigned long long *__restrict__ scale) {
-  __shared__ double red[4][16][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int I = blockIdx.x * 64 + lane;
+__global__ void ozk_scale_kernel(long n, int nb, const double2 *__restrict__ M, long ldm, int nw,
+                                 const double *__restrict__ sqd, long tchunk,
+                                 unsigned long long *__restrict__ scale) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
   const long t0 = (long)blockIdx.y * tchunk;
   const long t1 = t0 + tchunk < n ? t0 + tchunk : n;
   double m[16];
 #pragma unroll
   for (int w = 0; w < 16; w++) m[w] = 0.0;
-  if (I < nb) {
 #pragma unroll 4
-    for (long t = t0 + wv; t < t1; t += 4) {
-      const double2 v = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
-      const double a = fmax(fabs(v.x), fabs(v.y));
-#pragma unroll
-      for (int w = 0; w < 16; w++)
-        if (w < nw) m[w] = fmax(m[w], a * sqd[(long)w * n + t]);
-    }
-  }
-#pragma unroll
-  for (int w = 0; w < 16; w++) red[wv][w][lane] = m[w];
-  __syncthreads();
-  if (wv == 0 && I < nb) {
+  for (long t = t0; t < t1; t++) {
+    const double2 v = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
+    const double a = fmax(fabs(v.x), fabs(v.y));
 #pragma unroll
     for (int w = 0; w < 16; w++)
-      if (w < nw) {
-        const double r = fmax(fmax(red[0][w][lane], red[1][w][lane]),
-                              fmax(red[2][w][lane], red[3][w][lane]));
-        atomicMax(scale + (long)w * nb + I, (unsigned long long)__double_as_longlong(r));
-      }
+      if (w < nw) m[w] = fmax(m[w], a * sqd[(long)w * n + t]);
+  }
+#pragma unroll
+  for (int w = 0; w < 16; w++)
+    if (w < nw) atomicMax(scale + (long)w * nb + I, (unsigned long long)__double_as_longlong(m[w]));
+}
+
+__device__ inline void slice4(double u, int a[NS]) {
+  // u in [-127, 127]
+  double r = u;
+#pragma unroll
+  for (int p = 0; p < NS; p++) {
+    const double q = rint(r);
+    a[p] = (int)q;
+    r = (r - q) * 254.0;
   }
 }
 
@@ -145,32 +140,32 @@ __global__ __launch_bounds__(NT) void ozk_slice_kernel(long n, int nb, int nbp,
     const int w = w0 + wg;
     const double s = I < nb ? scale[(long)w * nb + I] : 0.0;
     const double inv = s > 0.0 ? 127.0 / s : 0.0;
+    unsigned int q[NPL][4];
+#pragma unroll
+    for (int pl = 0; pl < NPL; pl++)
+#pragma unroll
+      for (int d = 0; d < 4; d++) q[pl][d] = 0u;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const long t = tb + j;
+      const double f = t < n ? sqd[(long)w * n + t] * inv : 0.0;
+      int a[NS];
+      slice4(v[j].x * f, a);
+#pragma unroll
+      for (int p = 0; p < NS; p++) q[p][j >> 2] |= ((unsigned)a[p] & 0xFFu) << (8 * (j & 3));
+      slice4(v[j].y * f, a);
+#pragma unroll
+      for (int p = 0; p < NS; p++) q[NS + p][j >> 2] |= ((unsigned)a[p] & 0xFFu) << (8 * (j & 3));
+    }
     unsigned char *base = S + (long)wg * sbytes;
-    // one part (Re, Im) at a time and one slice plane at a time: u/s*127 in
-    // [-127, 127] = sum_p a_p 254^-p + r, each a_p rounded to nearest; the
-    // 16 residuals are carried from plane to plane and each plane is stored
-    // as soon as it is packed
 #pragma unroll
-    for (int part = 0; part < 2; part++) {
-      double r[16];
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const long t = tb + j;
-        const double f = t < n ? sqd[(long)w * n + t] * inv : 0.0;
-        r[j] = (part ? v[j].y : v[j].x) * f;
-      }
-#pragma unroll
-      for (int p = 0; p < NS; p++) {
-        unsigned int q[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const double a = rint(r[j]);
-          q[j >> 2] |= ((unsigned)(int)a & 0xFFu) << (8 * (j & 3));
-          r[j] = (r[j] - a) * 254.0;
-        }
-        *(uint4 *)(base + blk_off(c, part * NS + p, ib, nib) + 16 * l) =
-            make_uint4(q[0], q[1], q[2], q[3]);
-      }
+    for (int pl = 0; pl < NPL; pl++) {
+      uint4 o;
+      o.x = q[pl][0];
+      o.y = q[pl][1];
+      o.z = q[pl][2];
+      o.w = q[pl][3];
+      *(uint4 *)(base + blk_off(c, pl, ib, nib) + 16 * l) = o;
     }
   }
 }
@@ -544,8 +539,8 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
   hipLaunchKernelGGL(ozk_sign_kernel, dim3(1), dim3(64), 0, st, nw, flags, sgn, err_dev);
   SPX_LAUNCHED(ctx);
   {
-    const long tchunk = 2048;  // enough workgroups to keep HBM busy; few atomics per column
-    dim3 grid((nb + 63) / 64, (n + tchunk - 1) / tchunk);
+    const long tchunk = 1024;  // enough workgroups to keep HBM busy; few atomics per column
+    dim3 grid((nb + 255) / 256, (n + tchunk - 1) / tchunk);
     if (colmajor)
       hipLaunchKernelGGL(ozk_scale_kernel<true>, grid, dim3(256), 0, st, n, nb, (const double2 *)M,
                          ldm, nw, sqd, tchunk, (unsigned long long *)scale);
