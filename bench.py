@@ -388,6 +388,12 @@ def main():
         "forms": {k: {kk: round(v, 4) for kk, v in r.items()} for k, r in res.items()},
         "speedup_vs_fp64": {k: round(res["fp64"]["ms_per_step"] / res[k]["ms_per_step"], 3)
                             for k in ("split_fp32", "ozaki_int8")},
+        # the same ratio over chi0 + eps^-1 alone (the products stage is the
+        # same FP64 code in every form)
+        "speedup_vs_fp64_chi0_eps": {
+            k: round((res["fp64"]["chi0_ms"] + res["fp64"]["eps_ms"])
+                     / (res[k]["chi0_ms"] + res[k]["eps_ms"]), 3)
+            for k in ("split_fp32", "ozaki_int8")},
         "err_vs_fp64": errs_,
         "roofline": roof,
     }
