@@ -26,6 +26,7 @@ __global__ __launch_bounds__(NT) void zgemm_kernel(ZgemmArgs a) {
   __shared__ GemmLds s;
   const int b = blockIdx.z;
   const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TM;
+  if (a.upper && i0 > j0) return;  // tile wholly below the diagonal
   TileAcc acc;
   tile_zero(acc);
   gemm_tile(a.A + (b / a.dA) * a.sA, a.lda, a.B + (b / a.dB) * a.sB, a.ldb, a.m, a.n, a.k, i0,
@@ -80,73 +81,81 @@ __global__ void eps_diag_kernel(int nb, const double *__restrict__ sq,
 constexpr int GB = 64;  // panel width
 
 // P = inverse of the GB x GB diagonal block at (k0,k0) (rows/cols beyond nb
-// are treated as identity).  One workgroup per matrix (blockIdx.y).
+// are treated as identity).  One workgroup of four waves per matrix
+// (blockIdx.y): wave v keeps columns 16v .. 16v+15 of the block in
+// registers, lane = row.  Pivot step p publishes column p (its owner wave)
+// and row p (lane p of every wave) through LDS, double-buffered so that one
+// barrier per step suffices: a buffer is rewritten two steps later, after
+// every wave has passed the barrier that follows its reads.
 // logdet (optional): logdet[blockIdx.y] = sum of log(pivot) over this block
 // -- the pivots of Gauss-Jordan on an HPD matrix are the Schur-complement
 // diagonals, so summed over all blocks they give log det.
-__global__ __launch_bounds__(NT) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
-                                                     long ldx, long sX, int k0,
-                                                     double2 *__restrict__ P, int *__restrict__ flag,
-                                                     double *__restrict__ logdet) {
-  __shared__ double ar[GB][GB + 1], ai[GB][GB + 1];
-  __shared__ double pr_s;
-  double ld = 0.0;
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
+                                                      long ldx, long sX, int k0,
+                                                      double2 *__restrict__ P, int *__restrict__ flag,
+                                                      double *__restrict__ logdet) {
+  static_assert(GB == 64, "one lane per row, four waves of 16 columns");
+  __shared__ double2 colb[2][GB], rowb[2][GB];
+  __shared__ double piv[GB];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double2 *Xb = X + sX * blockIdx.y;
-  for (int e = tid; e < GB * GB; e += NT) {
-    const int i = e % GB, j = e / GB;
-    double2 v = make_double2(i == j ? 1.0 : 0.0, 0.0);
-    if (k0 + i < nb && k0 + j < nb) v = Xb[(k0 + i) + ldx * (long)(k0 + j)];
-    ar[i][j] = v.x;
-    ai[i][j] = v.y;
+  double2 a[16];
+#pragma unroll
+  for (int c = 0; c < 16; c++) {
+    const int j = wv * 16 + c;
+    double2 v = make_double2(lane == j ? 1.0 : 0.0, 0.0);
+    if (k0 + lane < nb && k0 + j < nb) v = Xb[(k0 + lane) + ldx * (long)(k0 + j)];
+    a[c] = v;
   }
-  __syncthreads();
-  for (int p = 0; p < GB; p++) {
-    if (tid == 0) {
+  bool bad = false;
+  for (int pb = 0; pb < 4; pb++) {
+#pragma unroll
+    for (int pc = 0; pc < 16; pc++) {
+      const int p = pb * 16 + pc, buf = p & 1;
+      if (wv == pb) colb[buf][lane] = a[pc];
+      if (lane == p) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) rowb[buf][wv * 16 + c] = a[c];
+      }
+      __syncthreads();
+      const double2 f = colb[buf][lane];
       // Hermitian positive definite: the pivot is real and positive.
-      const double d = ar[p][p];
-      if (!(d > 0.0)) flag[0] = 1;
-      pr_s = 1.0 / d;
-      ld += log(d);
-    }
-    __syncthreads();
-    const double pr = pr_s;
-    // scale row p
-    for (int j = tid; j < GB; j += NT)
-      if (j != p) {
-        const double xr = ar[p][j], xi = ai[p][j];
-        ar[p][j] = xr * pr;
-        ai[p][j] = xi * pr;
+      const double d = colb[buf][p].x;
+      bad |= !(d > 0.0);
+      if (threadIdx.x == 0) piv[p] = d;
+      const double pr = 1.0 / d;
+#pragma unroll
+      for (int c = 0; c < 16; c++) {
+        const int j = wv * 16 + c;
+        if (j == p) {
+          a[c] = lane == p ? make_double2(pr, 0.0) : make_double2(-f.x * pr, -f.y * pr);
+        } else {
+          const double2 r = rowb[buf][j];
+          const double xr = r.x * pr, xi = r.y * pr;
+          if (lane == p) {
+            a[c] = make_double2(xr, xi);
+          } else {
+            a[c].x -= f.x * xr - f.y * xi;
+            a[c].y -= f.x * xi + f.y * xr;
+          }
+        }
       }
-    __syncthreads();
-    // eliminate column p from the other rows
-    for (int e = tid; e < GB * GB; e += NT) {
-      const int i = e % GB, j = e / GB;
-      if (i == p || j == p) continue;
-      const double fr = ar[i][p], fi = ai[i][p];
-      const double xr = ar[p][j], xi = ai[p][j];
-      ar[i][j] -= fr * xr - fi * xi;
-      ai[i][j] -= fr * xi + fi * xr;
     }
-    __syncthreads();
-    for (int i = tid; i < GB; i += NT)
-      if (i != p) {
-        const double fr = ar[i][p], fi = ai[i][p];
-        ar[i][p] = -fr * pr;
-        ai[i][p] = -fi * pr;
-      }
-    if (tid == 0) {
-      ar[p][p] = pr;
-      ai[p][p] = 0.0;
-    }
-    __syncthreads();
   }
   double2 *Pb = P + (long)GB * GB * blockIdx.y;
-  for (int e = tid; e < GB * GB; e += NT) {
-    const int i = e % GB, j = e / GB;
-    Pb[i + GB * j] = make_double2(ar[i][j], ai[i][j]);
+#pragma unroll
+  for (int c = 0; c < 16; c++) Pb[lane + GB * (wv * 16 + c)] = a[c];
+  if (threadIdx.x == 0 && bad) flag[0] = 1;
+  if (logdet) {
+    __syncthreads();
+    if (wv == 0) piv[lane] = log(piv[lane]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double ld = 0.0;
+      for (int p = 0; p < GB; p++) ld += piv[p];
+      logdet[blockIdx.y] = ld;
+    }
   }
-  if (logdet && tid == 0) logdet[blockIdx.y] = ld;
 }
 
 // W = diag(s) X diag(s) (correlation.f:868-872, p_diagmatdiag with sqcoul).
@@ -323,10 +332,16 @@ int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, lo
     SPX_LAUNCHED(ctx);
     return SPEXHIP_OK;
   }
-  // tmp = chi * Vh ; E = I - Vh * tmp
+  // tmp = chi * Vh ; E = I - Vh * tmp on the tiles on and above the
+  // diagonal (E is Hermitian), then the lower triangle from the upper
   rc = spx_zgemm(ctx, st, nb, nb, nb, 1.0, X, ldx, ldx * (long)nb, vh, nb, 0, 0.0, tmp, nb, nn, 0.0, nw);
   if (rc) return rc;
-  return spx_zgemm(ctx, st, nb, nb, nb, -1.0, vh, nb, 0, tmp, nb, nn, 0.0, E, lde, lde * (long)nb, 1.0, nw);
+  ZgemmArgs e{nb, nb, nb, make_double2(-1.0, 0.0), (const double2 *)vh, nb, 0, 1,
+              (const double2 *)tmp, nb, nn, 1, 0.0, (double2 *)E, lde, lde * (long)nb, 1, 1.0};
+  e.upper = 1;
+  rc = spx_zgemm_ex(ctx, st, e, nw);
+  if (rc) return rc;
+  return spx_hermitize(ctx, st, nb, nw, E, lde, 0);
 }
 
 // W = Vh Einv Vh (correlation.f:868-877).  vkind 0: vh = sqrt(coul)

@@ -131,6 +131,7 @@ struct ZgemmArgs {
   double beta;
   double2 *C; long ldc, sC; int dC;
   double diag;
+  int upper;  // non-zero: only tiles that touch the upper triangle (C Hermitian)
 };
 
 }  // namespace spxg

@@ -323,13 +323,22 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
     int ngptm, double2 *__restrict__ M, long ldm, long t0, int col0) {
   static_assert(N1 % 4 == 0 && N2 % 4 == 0 && N1 >= PW_BOX && N2 >= PW_BOX, "grid");
   constexpr int G = 16 / gcd16(N2), RB = G * N2 / 16, KX = N1 / 4, KY = N2 / 4;
-  __shared__ double2 tz[64][PW_BOX];
+  // z twiddles, N3 x PW_BOX (dynamic: 6 KB for N3 = 24 instead of a 64-plane
+  // table, so four workgroups fit in a CU's LDS)
+  extern __shared__ double2 tz[];
   __shared__ double2 Qs[NT / 64][PW_BOX * PW_BOX];
   __shared__ double2 F1s[KX][64], F2s[KY][64];
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
+  // Workgroup -> (unoccupied state b2, group of four occupied states): the
+  // ngrp workgroups of one b2 get ids that are equal mod 8, so they run on
+  // one XCD (ids go round-robin over the 8) and read psi2 through one L2.
+  const int ngrp = (nocc + 3) / 4;
+  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int b2 = (r / ngrp) * 8 + xcd, grp = r % ngrp;
+  if (b2 >= nunocc) return;  // whole workgroup, before any barrier
   for (int e = tid; e < N3 * PW_BOX; e += NT) {
     const int z = e / PW_BOX, g = e % PW_BOX;
-    tz[z][g] = twid((long)(g - PW_OFF) * z, N3, -1.0);
+    tz[e] = twid((long)(g - PW_OFF) * z, N3, -1.0);
   }
   for (int e = tid; e < KX * 64; e += NT) {
     const int kk = e >> 6, ln = e & 63;
@@ -340,8 +349,7 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
     F2s[kk][ln] = twid((long)((ln & 15) - PW_OFF) * (4 * kk + (ln >> 4)), N2, -1.0);
   }
   __syncthreads();
-  const int b2 = blockIdx.x;
-  const int b1 = blockIdx.y * (NT / 64) + wv;
+  const int b1 = grp * (NT / 64) + wv;
   if (b1 >= nocc) return;  // whole wave; no workgroup barrier after this point
   const long NN = (long)N1 * N2 * N3;
   const double2 *p1 = grids + (long)b1 * NN;
@@ -395,7 +403,7 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
 #pragma unroll
         for (int j = 0; j < PW_FOUT; j++) {
           if (ob[j] >= 0) {
-            const double2 t = tz[z][ob[j] >> 8];
+            const double2 t = tz[z * PW_BOX + (ob[j] >> 8)];
             const double2 v = Q[ob[j] & 255];
             acc[j].x += t.x * v.x - t.y * v.y;
             acc[j].y += t.x * v.y + t.y * v.x;
@@ -471,14 +479,16 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
                        p->step, grids);
     SPX_LAUNCHED(ctx);
     if (p->ngptm <= 64 * PW_FOUT && N1 == N2 && (N1 == 16 || N1 == 20 || N1 == 24 || N1 == 32)) {
-      const dim3 grid(nunocc, (nocc + 3) / 4);
+      const int ngrp = (nocc + 3) / 4;
+      const dim3 grid((unsigned)(((nunocc + 7) / 8) * 8 * ngrp));
+      const size_t tzb = (size_t)N3 * PW_BOX * sizeof(double2);
       const int *ox = oidx_dev + (long)ik * p->ngptm;
       const long tt = (long)ik * nocc * nunocc;
       switch (N1) {
-        case 16: hipLaunchKernelGGL((pw_pair_joint_kernel<16, 16>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 20: hipLaunchKernelGGL((pw_pair_joint_kernel<20, 20>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 24: hipLaunchKernelGGL((pw_pair_joint_kernel<24, 24>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        default: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), 0, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 16: hipLaunchKernelGGL((pw_pair_joint_kernel<16, 16>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 20: hipLaunchKernelGGL((pw_pair_joint_kernel<20, 20>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 24: hipLaunchKernelGGL((pw_pair_joint_kernel<24, 24>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        default: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
       }
       SPX_LAUNCHED(ctx);
       continue;
