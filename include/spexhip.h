@@ -117,6 +117,21 @@ int spexhip_weights_none_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
                              int nkpt, int nsym1, const void *nkpts_dev,
                              int nw, const void *omega_dev, void *wght_dev);
 
+/* ---- frequency weights: HILBERT mesh ------------------------------------ */
+/* Replaces the spectral-function form of the pair loop for a Hilbert mesh:
+ * the wtetra index and threshold (susceptibility.f:1066-1077) and
+ * spectral_function (:1793-1849), which per pair sum re(t)*wtetra(t,s) over
+ * t and then S(s)*wghtS(s,w) over s.  Summed over s first, the weights
+ * become per transition,
+ *   D(t,w) = sum over s with wtetra(t,s) > wghtthr of wtetra(t,s)*wghtS(s,w),
+ * and spexhip_chi0_dev takes D as its wght.  wtetra_dev (n x nfrqs, ld ldwt,
+ * t fastest: the reference wtetra(n,nfrqs) before indexing), wghtS_dev
+ * (nfrqs x nfrq, ld ldws: wghtSr or wghtSi of getwghtS, hilbert.f:193-300),
+ * D_dev (n x nfrq, ld ldd).  At most 32 frequencies per call.             */
+int spexhip_weights_hilbert_dev(spexhip_ctx *ctx, long n, int nfrqs, const void *wtetra_dev,
+                                long ldwt, double wghtthr, int nfrq, const void *wghtS_dev,
+                                int ldws, void *D_dev, long ldd);
+
 /* ---- chi0 pair sum ----------------------------------------------------- */
 /* Device form of the pair loop susceptibility.f:1159-1251 with
  * spectral_function HILBERT NONE (:1780-1788):
@@ -129,6 +144,16 @@ int spexhip_chi0_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev,
                      long ldm, int colmajor, int nw, const void *wght_dev,
                      int precision, double alpha, double beta, void *chi_dev,
                      int ldchi);
+
+/* Real-frequency form (suscepc, susceptibility.f:1231-1236): complex weights
+ * Dr + i*Di (from wghtSr and wghtSi of the real frequencies) and the whole
+ * matrix, chi(I,J,w) = alpha * sum_t (Dr + i*Di)(t,w) M(t,I) conj(M(t,J))
+ * for all I, J (chi is overwritten).  Dr_dev, Di_dev (n x nw, t fastest).
+ * Two Hermitian rank-k updates (FP64 or split-FP32) and one pass that
+ * writes both triangles; the context scratch holds the second.            */
+int spexhip_chi0_cplx_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev, long ldm,
+                          int colmajor, int nw, const void *Dr_dev, const void *Di_dev,
+                          int precision, double alpha, void *chi_dev, int ldchi);
 
 /* ---- grouped chi0 over many q-points (persistent launch) ---------------- */
 /* One q-point's pair sum (susceptibility.f:1159-1251) as a problem of a

@@ -13,7 +13,8 @@ module m_spexhip
   implicit none
   private
 
-  integer(c_int), parameter, public :: SPEXHIP_PREC_FP64 = 0, SPEXHIP_PREC_SPLIT_FP32 = 1
+  integer(c_int), parameter, public :: SPEXHIP_PREC_FP64 = 0, SPEXHIP_PREC_SPLIT_FP32 = 1, &
+                                        SPEXHIP_PREC_OZAKI_INT8 = 2
 
   type, bind(C), public :: spexhip_mt_type
     integer(c_int) :: lcut

@@ -303,6 +303,10 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_weights_none_dev": (I, [_vp, I, I, I, _vp, _vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_chi0_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
                                  C.c_double, C.c_double, _vp, I]),
+        "spexhip_weights_hilbert_dev": (I, [_vp, _c_long, I, _vp, _c_long, C.c_double, I, _vp,
+                                            I, _vp, _c_long]),
+        "spexhip_chi0_cplx_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, _vp, I,
+                                      C.c_double, _vp, I]),
         "spexhip_pack_upper_d2h": (I, [_vp, I, I, _vp, I, _vp, I]),
         "spexhip_unpack_h2d": (I, [_vp, I, I, _vp, _vp, I]),
         "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
@@ -516,7 +520,22 @@ class Context:
             self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, nkpt, nsym1,
             nkpts.ptr, nw, omega.ptr, out.ptr))
 
+    def weights_hilbert_dev(self, n: int, nfrqs: int, wtetra: DevBuf, ldwt: int,
+                            wghtthr: float, nfrq: int, wghtS: DevBuf, ldws: int, out: DevBuf,
+                            ldd: int) -> None:
+        """D(t,w) = sum over wtetra(t,s) > wghtthr of wtetra(t,s) wghtS(s,w)."""
+        self._check(lib().spexhip_weights_hilbert_dev(
+            self.h, n, nfrqs, wtetra.ptr, ldwt, wghtthr, nfrq, wghtS.ptr, ldws, out.ptr, ldd))
+
     # -- chi0 -------------------------------------------------------------
+    def chi0_cplx_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int,
+                      Dr: DevBuf, Di: DevBuf, precision: int, chi: DevBuf, ldchi: int,
+                      alpha: float = 1.0) -> None:
+        """Whole matrix chi(I,J,w) = alpha sum_t (Dr + i Di)(t,w) M(t,I) conj(M(t,J))."""
+        self._check(lib().spexhip_chi0_cplx_dev(
+            self.h, n, nb, M.ptr, ldm, int(colmajor), nw, Dr.ptr, Di.ptr, precision, alpha,
+            chi.ptr, ldchi))
+
     def chi0_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool,
                  nw: int, wght: DevBuf, precision: int, chi: DevBuf, ldchi: int,
                  alpha: float = 1.0, beta: float = 0.0) -> None:

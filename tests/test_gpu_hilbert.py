@@ -1,0 +1,115 @@
+"""GPU parity of the HILBERT-mesh path (per-transition weights folded from
+tetrahedron weights and Hilbert weights) and of the real-frequency,
+complex-weighted chi0, against the oracle (C ABI)."""
+import numpy as np
+import pytest
+
+from oracle import hilbert
+from project1_amd import spexhip
+from tests.conftest import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _f64_dev(ctx, a):
+    """(rows, cols) float64 in column-major (rows fastest) on the device."""
+    return ctx.to_dev(np.ascontiguousarray(np.asarray(a, np.float64).T))
+
+
+def _case(seed, n, nfrqs_mesh=(0.005, 1.02), freqs=None):
+    g = np.random.default_rng(seed)
+    f = hilbert.hilbert_mesh(nfrqs_mesh, 0.05, 3.0)
+    freqs = freqs if freqs is not None else 1j * np.linspace(0.05, 4.0, 12)
+    wr, wi = hilbert.hilbert_weights(freqs, f, 0.0, 0, 0)
+    # tetrahedron-like weights: each transition touches a few neighbouring
+    # mesh points, a few weights fall below the threshold
+    wt = np.zeros((n, len(f)))
+    s0 = g.integers(0, len(f) - 4, n)
+    for k in range(4):
+        wt[np.arange(n), s0 + k] = g.uniform(0, 1, n)
+    wt[g.uniform(0, 1, wt.shape) < 0.05] = 1e-9
+    return g, f, wr, wi, wt
+
+
+@pytest.mark.parametrize("n,nfrq", [(5000, 12), (257, 1), (1000, 32)])
+def test_weights_fold_vs_oracle(ctx, n, nfrq):
+    g, f, wr, _, wt = _case(1, n, freqs=1j * np.linspace(0.1, 3.0, nfrq))
+    assert len(f) > 64  # several LDS stages
+    thr = 1e-6
+    dWt, dH = _f64_dev(ctx, wt), _f64_dev(ctx, wr)
+    dD = ctx.alloc(8 * n * nfrq)
+    ctx.weights_hilbert_dev(n, len(f), dWt, n, thr, nfrq, dH, len(f), dD, n)
+    got = dD.download(np.float64, (nfrq, n)).T
+    want = hilbert.fold_weights(wt, wr, thr)
+    mx, fro = normwise(got, want)
+    print(f"fold n={n} nfrq={nfrq}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < 1e-13
+    for b in (dWt, dH, dD):
+        b.free()
+
+
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-12),
+                                           (spexhip.PREC_SPLIT_FP32, 4e-8)])
+def test_hilbert_chi0_vs_reference_order(ctx, precision, tol):
+    """Fold on the GPU, then the rank-k pair sum, against the reference's
+    order (spectral function per pair, then the Hilbert sum)."""
+    n, nb, nfrq = 600, 40, 3
+    g, f, wr, _, wt = _case(2, n, freqs=np.array([0.2j, 1.1j, 3.0j]))
+    M = (g.standard_normal((n, nb)) + 1j * g.standard_normal((n, nb))) / np.sqrt(nb)
+    thr = 1e-6
+    dWt, dH = _f64_dev(ctx, wt), _f64_dev(ctx, wr)
+    dD = ctx.alloc(8 * n * nfrq)
+    ctx.weights_hilbert_dev(n, len(f), dWt, n, thr, nfrq, dH, len(f), dD, n)
+    dM = ctx.to_dev(np.ascontiguousarray(M))
+    dC = ctx.alloc(16 * nb * nb * nfrq).zero()
+    ctx.chi0_dev(n, nb, dM, nb, False, nfrq, dD, precision, dC, nb)
+    got = ctx.pack_upper(dC, nb, nfrq, nb).T
+    ref = hilbert.chi0_hilbert_pairsum(M, wt, wr, thr)
+    for w in range(nfrq):
+        want = np.array([ref[i, j, w] for j in range(nb) for i in range(j + 1)])  # packed upper
+        mx, fro = normwise(got[:, w], want)
+        print(f"hilbert chi0 prec={precision} w={w}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+        assert mx < tol
+    for b in (dWt, dH, dD, dM, dC):
+        b.free()
+
+
+# Real-frequency weights change sign, so the sums cancel: the split form's
+# FP32 rounding, set by sum |D| |M|^2, is a larger fraction of the result
+# than with one-signed weights (small n: no averaging down either).
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-12),
+                                           (spexhip.PREC_SPLIT_FP32, 1e-7)])
+@pytest.mark.parametrize("colmajor", [False, True])
+def test_chi0_complex_weights_vs_oracle(ctx, precision, tol, colmajor):
+    """Real frequencies: whole matrix with complex weights (suscepc)."""
+    n, nb, nw = 700, 33, 3
+    g, f, _, _, wt = _case(3, n)
+    wr, wi = hilbert.hilbert_weights(np.array([0.37, 1.21, 2.5]), f, 0.01, 0, 0)
+    Dr = hilbert.fold_weights(wt, wr)
+    Di = hilbert.fold_weights(wt, wi)
+    assert (Dr > 0).any() and (Dr < 0).any()
+    M = (g.standard_normal((n, nb)) + 1j * g.standard_normal((n, nb))) / np.sqrt(nb)
+    dM = ctx.to_dev(np.asfortranarray(M).T if colmajor else np.ascontiguousarray(M))
+    dR, dI = _f64_dev(ctx, Dr), _f64_dev(ctx, Di)
+    dC = ctx.alloc(16 * nb * nb * nw)
+    ctx.chi0_cplx_dev(n, nb, dM, n if colmajor else nb, colmajor, nw, dR, dI, precision, dC, nb,
+                      alpha=2.0)
+    got = dC.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)  # (I, J, w), column-major
+    want = 2.0 * hilbert.chi0_complex_weights(M, Dr, Di)
+    for w in range(nw):
+        mx, fro = normwise(got[:, :, w], want[:, :, w])
+        print(f"cplx prec={precision} colmajor={colmajor} w={w}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+        assert mx < tol
+    for b in (dM, dR, dI, dC):
+        b.free()
+
+
+def test_chi0_complex_weights_rejects_ozaki(ctx):
+    n, nb, nw = 64, 8, 1
+    dM = ctx.alloc(16 * n * nb).zero()
+    dR = ctx.alloc(8 * n * nw).zero()
+    dC = ctx.alloc(16 * nb * nb * nw)
+    with pytest.raises(spexhip.SpexError):
+        ctx.chi0_cplx_dev(n, nb, dM, nb, False, nw, dR, dR, spexhip.PREC_OZAKI_INT8, dC, nb)
+    for b in (dM, dR, dC):
+        b.free()
