@@ -213,6 +213,20 @@ int spexhip_susceptibility_none(spexhip_ctx *ctx, long n, int nb,
                                 const double *wght, int precision,
                                 double *susceph);
 
+/* Host drop-in for the HILBERT pair sum of one memory packet: the wtetra
+ * index and threshold (susceptibility.f:1066-1077), the pair loop with
+ * spectral_function (:1159-1251, :1793-1849) and the accumulation into
+ * susceph and suscepc (:1213-1236).  Host arrays in the reference layouts:
+ * cprod (n x nb, ld ldc), wtetra (n x nfrqs, ld ldwt), wghtSr and wghtSi
+ * (nfrqs x (nfrqh+nfrqc); wghtSi is read for the real frequencies only),
+ * susceph (packed nb(nb+1)/2 x nfrqh) and suscepc (nb x nb x nfrqc), both
+ * ACCUMULATED.  No spin factor (applied once by the caller, :1336).       */
+int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const double *cprod,
+                                   long ldc, int nfrqs, const double *wtetra, long ldwt,
+                                   double wghtthr, int nfrqh, int nfrqc, const double *wghtSr,
+                                   const double *wghtSi, int precision, double *susceph,
+                                   double *suscepc);
+
 /* ---- wave-function products -------------------------------------------- */
 /* The reference reads the basis and wave functions from module global
  * (cmt, cpw, gpt, gptm, nfft, imati, integral, gaunt -- wavefproducts.f:

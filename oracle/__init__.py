@@ -8,8 +8,8 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 ``bench.py`` may import this package, and only as the checker.  The product
 library never links or calls it.
 
-Parity status of each function (what pins it) is listed in
-``oracle/PINNING.md``.
+What pins each function (golden fixtures, closed forms, independent
+algorithms) is stated by the tests that check it, tests/test_oracle_*.py.
 """
 from __future__ import annotations
 

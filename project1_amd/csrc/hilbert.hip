@@ -152,3 +152,92 @@ int spexhip_chi0_cplx_dev(spexhip_ctx *ctx, long n, int nb, const void *M, long 
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Host drop-in for the HILBERT pair sum of one memory packet: the index
+// and threshold (susceptibility.f:1066-1077), the pair loop with
+// spectral_function (:1159-1251, :1793-1849) and its accumulation into
+// susceph (imaginary frequencies, packed upper) and suscepc (real
+// frequencies, whole matrix, :1231-1236).  All arrays are host arrays in the
+// reference layouts; susceph and suscepc are accumulated.
+int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const double *cprod,
+                                   long ldc, int nfrqs, const double *wtetra, long ldwt,
+                                   double wghtthr, int nfrqh, int nfrqc, const double *wghtSr,
+                                   const double *wghtSi, int precision, double *susceph,
+                                   double *suscepc) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, n >= 0 && nb > 0 && nfrqs >= 1 && nfrqh >= 0 && nfrqc >= 0 && ldc >= n &&
+                     ldwt >= n,
+            "susceptibility_hilbert: bad arguments");
+  SPX_CHECK(ctx, (nfrqh == 0 || susceph) && (nfrqc == 0 || (suscepc && wghtSi)),
+            "susceptibility_hilbert: missing output or wghtSi");
+  if (n == 0 || nfrqh + nfrqc == 0) return SPEXHIP_OK;
+  SPX_CHECK(ctx, cprod && wtetra && wghtSr, "susceptibility_hilbert: null input");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const int nf = nfrqh + nfrqc;
+  const int FB = HB_MAXF;  // frequencies per fold / pair-sum batch
+  const size_t mb = 16 * (size_t)n * nb, tb = 8 * (size_t)n * nfrqs, sb = 8 * (size_t)nfrqs * nf,
+               db = 8 * (size_t)n * FB, cb = 16 * (size_t)nb * nb * FB;
+  void *dM = nullptr, *dT = nullptr, *dSr = nullptr, *dSi = nullptr, *dDr = nullptr,
+       *dDi = nullptr, *dC = nullptr;
+  int rc = spx_alloc(ctx, mb, &dM);
+  if (!rc) rc = spx_alloc(ctx, tb, &dT);
+  if (!rc) rc = spx_alloc(ctx, sb, &dSr);
+  if (!rc && nfrqc) rc = spx_alloc(ctx, sb, &dSi);
+  if (!rc) rc = spx_alloc(ctx, db, &dDr);
+  if (!rc && nfrqc) rc = spx_alloc(ctx, db, &dDi);
+  if (!rc) rc = spx_alloc(ctx, cb, &dC);
+  std::vector<double> full;
+  if (!rc) {
+    hipError_t e = hipMemcpy2DAsync(dM, 16 * n, cprod, 16 * ldc, 16 * n, nb,
+                                    hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpy2DAsync(dT, 8 * n, wtetra, 8 * ldwt, 8 * n, nfrqs, hipMemcpyHostToDevice,
+                           ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dSr, wghtSr, sb, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && nfrqc)
+      e = hipMemcpyAsync(dSi, wghtSi, sb, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "susceptibility_hilbert: upload failed: %s",
+                      hipGetErrorString(e));
+  }
+  // imaginary frequencies: fold, Hermitian pair sum, packed accumulate
+  for (int f0 = 0; !rc && f0 < nfrqh; f0 += FB) {
+    const int k = nfrqh - f0 < FB ? nfrqh - f0 : FB;
+    rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k,
+                                     (const double *)dSr + (size_t)nfrqs * f0, nfrqs, dDr, n);
+    if (!rc) rc = spx_chi0_launch(ctx, ctx->stream, n, nb, dM, n, 1, k, dDr, precision, 1.0, 0.0, dC, nb);
+    if (!rc)
+      rc = spexhip_pack_upper_d2h(ctx, nb, k, dC, nb, susceph + 2 * (size_t)nb * (nb + 1) / 2 * f0, 1);
+  }
+  // real frequencies: complex weights, whole matrix
+  for (int f0 = 0; !rc && f0 < nfrqc; f0 += FB) {
+    const int k = nfrqc - f0 < FB ? nfrqc - f0 : FB;
+    const size_t col = (size_t)nfrqs * (nfrqh + f0);
+    rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k, (const double *)dSr + col,
+                                     nfrqs, dDr, n);
+    if (!rc)
+      rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k, (const double *)dSi + col,
+                                       nfrqs, dDi, n);
+    if (!rc)
+      rc = spexhip_chi0_cplx_dev(ctx, n, nb, dM, n, 1, k, dDr, dDi, precision, 1.0, dC, nb);
+    if (!rc) {
+      full.resize(2 * (size_t)nb * nb * k);
+      hipError_t e = hipMemcpyAsync(full.data(), dC, 16 * (size_t)nb * nb * k,
+                                    hipMemcpyDeviceToHost, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+      if (e != hipSuccess)
+        rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "susceptibility_hilbert: download failed: %s",
+                        hipGetErrorString(e));
+      double *dst = suscepc + 2 * (size_t)nb * nb * f0;
+      for (size_t i = 0; !rc && i < full.size(); i++) dst[i] += full[i];
+    }
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : {dM, dT, dSr, dSi, dDr, dDi, dC})
+    if (p) spx_free(ctx, p);
+  return rc;
+}
+
+}  // extern "C"

@@ -318,7 +318,7 @@ constexpr int gcd16(int n) { return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n %
 // (N2*zz + 4kq)/16, register ((N2*zz + 4kq) % 16)/4 -- the accumulator
 // layout (rows lane/16 + 4r) is again the operand layout.
 template <int N1, int N2>
-__global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 3))) void pw_pair_joint_kernel(
     const double2 *__restrict__ grids, int nocc, int nunocc, int N3, const int *__restrict__ oidx,
     int ngptm, double2 *__restrict__ M, long ldm, long t0, int col0) {
   static_assert(N1 % 4 == 0 && N2 % 4 == 0 && N1 >= PW_BOX && N2 >= PW_BOX, "grid");
@@ -328,6 +328,9 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
   extern __shared__ double2 tz[];
   __shared__ double2 Qs[NT / 64][PW_BOX * PW_BOX];
   __shared__ double2 F1s[KX][64], F2s[KY][64];
+  // output box indices in LDS rather than 16 registers per lane: with the
+  // register cap of three waves per SIMD the kernel then fits without spills
+  __shared__ int obs[64 * PW_FOUT];
   const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63, lr = l & 15, lk = l >> 4;
   // Workgroup -> (unoccupied state b2, group of four occupied states): the
   // ngrp workgroups of one b2 get ids that are equal mod 8, so they run on
@@ -340,6 +343,7 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
     const int z = e / PW_BOX, g = e % PW_BOX;
     tz[e] = twid((long)(g - PW_OFF) * z, N3, -1.0);
   }
+  for (int e = tid; e < 64 * PW_FOUT; e += NT) obs[e] = e < ngptm ? oidx[e] : -1;
   for (int e = tid; e < KX * 64; e += NT) {
     const int kk = e >> 6, ln = e & 63;
     F1s[kk][ln] = twid((long)((ln & 15) - PW_OFF) * (4 * kk + (ln >> 4)), N1, -1.0);
@@ -354,14 +358,9 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
   const long NN = (long)N1 * N2 * N3;
   const double2 *p1 = grids + (long)b1 * NN;
   const double2 *p2 = grids + (long)(nocc + b2) * NN;
-  int ob[PW_FOUT];
   double2 acc[PW_FOUT];
 #pragma unroll
-  for (int j = 0; j < PW_FOUT; j++) {
-    const int ig = l + 64 * j;
-    ob[j] = ig < ngptm ? oidx[ig] : -1;
-    acc[j] = make_double2(0.0, 0.0);
-  }
+  for (int j = 0; j < PW_FOUT; j++) acc[j] = make_double2(0.0, 0.0);
   double2 *Q = Qs[wv];
   for (int z0 = 0; z0 < N3; z0 += G) {
     d4_t xr[RB], xi[RB];
@@ -402,9 +401,10 @@ __global__ __launch_bounds__(NT) void pw_pair_joint_kernel(
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 #pragma unroll
         for (int j = 0; j < PW_FOUT; j++) {
-          if (ob[j] >= 0) {
-            const double2 t = tz[z * PW_BOX + (ob[j] >> 8)];
-            const double2 v = Q[ob[j] & 255];
+          const int o = obs[l + 64 * j];
+          if (o >= 0) {
+            const double2 t = tz[z * PW_BOX + (o >> 8)];
+            const double2 v = Q[o & 255];
             acc[j].x += t.x * v.x - t.y * v.y;
             acc[j].y += t.x * v.y + t.y * v.x;
           }

@@ -1,6 +1,7 @@
 ! Fortran host exercising the ISO_C_BINDING boundary: one packet of the
 ! HILBERT NONE pair sum and the dielectric inverse, compared with a plain
-! Fortran pair loop written the way susceptibility.f:1183-1240 does it.
+! Fortran pair loop written the way susceptibility.f:1183-1240 does it; the
+! screened interaction, the RPA integrand and the HILBERT pair sum likewise.
 ! Exit status 0 on agreement (normwise relative error < 1e-12).
 program fdriver
   use, intrinsic :: iso_c_binding
@@ -110,6 +111,68 @@ program fdriver
     call spexhip_rpa_int(ctx, nb, 1, dg, 0.125d0, rint)
     write(*, '(A,ES10.3)') 'fortran rpa_int rel error: ', abs(rint(1) - want) / abs(want)
     if (abs(rint(1) - want) > 1d-12 * abs(want)) stop 6
+  end block
+  ! HILBERT branch: spectral function per pair on a mesh of nfs points, then
+  ! the Hilbert sums (susceptibility.f:1793-1849), into susceph (imaginary
+  ! frequencies) and suscepc (real ones, :1231-1236)
+  block
+    integer, parameter :: nfs = 10, nfh = 2, nfc = 2
+    real(c_double), target :: wt(n, nfs), wsr(nfs, nfh+nfc), wsi(nfs, nfh+nfc)
+    complex(c_double_complex), target :: sh(npack, nfh), sc(nb, nb, nfc)
+    complex(c_double_complex) :: rh(npack, nfh), rcx(nb, nb, nfc), cdum, cdum1
+    real(c_double) :: reS(nfs), imS(nfs), s1, s2, c1, c2
+    integer :: s
+    do s = 1, nfs
+      do k = 1, n
+        wt(k, s) = 0
+        if (mod(k + s, 4) == 0) wt(k, s) = 0.1d0 + 0.01d0 * mod(3*k + s, 17)
+      end do
+      do f = 1, nfh + nfc
+        wsr(s, f) = -1d0 / (0.2d0*s + 0.5d0*f)
+        wsi(s, f) = 0.05d0 * sin(0.7d0*s + f)
+      end do
+    end do
+    rh = 0
+    rcx = 0
+    do j = 1, nb
+      do i = 1, j
+        do k = 1, n
+          re(k) = real(cprod(k, i) * conjg(cprod(k, j)))
+          im(k) = aimag(cprod(k, i) * conjg(cprod(k, j)))
+        end do
+        do s = 1, nfs
+          reS(s) = 0
+          imS(s) = 0
+          do k = 1, n
+            if (wt(k, s) > 1d-6) then
+              reS(s) = reS(s) + re(k) * wt(k, s)
+              imS(s) = imS(s) + im(k) * wt(k, s)
+            end if
+          end do
+        end do
+        ij = j*(j-1)/2 + i
+        do f = 1, nfh
+          rh(ij, f) = rh(ij, f) + cmplx(dot_product(reS, wsr(:, f)), dot_product(imS, wsr(:, f)), &
+                                        kind=c_double_complex)
+        end do
+        do f = 1, nfc
+          s1 = dot_product(reS, wsr(:, nfh+f))
+          s2 = dot_product(imS, wsr(:, nfh+f))
+          c1 = dot_product(reS, wsi(:, nfh+f))
+          c2 = dot_product(imS, wsi(:, nfh+f))
+          cdum = cmplx(s1, c1, kind=c_double_complex)
+          cdum1 = cmplx(-c2, s2, kind=c_double_complex)
+          rcx(i, j, f) = rcx(i, j, f) + cdum + cdum1
+          if (i /= j) rcx(j, i, f) = rcx(j, i, f) + cdum - cdum1
+        end do
+      end do
+    end do
+    sh = 0
+    sc = 0
+    call spexhip_pairsum_hilbert(ctx, n, nb, cprod, nfs, wt, 1d-6, nfh, nfc, wsr, wsi, sh, sc)
+    err = max(maxval(abs(sh - rh)) / maxval(abs(rh)), maxval(abs(sc - rcx)) / maxval(abs(rcx)))
+    write(*, '(A,ES10.3)') 'fortran pairsum_hilbert max-rel error: ', err
+    if (err > 1d-12) stop 7
   end block
   rc = spexhip_destroy(ctx)
   write(*, '(A)') 'fortran driver OK'

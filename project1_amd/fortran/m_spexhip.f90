@@ -40,7 +40,7 @@ module m_spexhip
   public :: spexhip_create, spexhip_destroy, spexhip_mt_plan_create, spexhip_pw_plan_create
   public :: spexhip_plan_destroy
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
-  public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int
+  public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -65,6 +65,14 @@ module m_spexhip
       type(c_ptr), value :: ctx, cprod, wght, sus
       integer(c_long), value :: n, ldc
       integer(c_int), value :: nb, nw, prec
+    end function
+    integer(c_int) function c_susceptibility_hilbert(ctx, n, nb, cprod, ldc, nfrqs, wtetra, ldwt, &
+        wghtthr, nfrqh, nfrqc, wghtSr, wghtSi, prec, sush, susc) bind(C, name='spexhip_susceptibility_hilbert')
+      import :: c_int, c_long, c_ptr, c_double
+      type(c_ptr), value :: ctx, cprod, wtetra, wghtSr, wghtSi, sush, susc
+      integer(c_long), value :: n, ldc, ldwt
+      integer(c_int), value :: nb, nfrqs, nfrqh, nfrqc, prec
+      real(c_double), value :: wghtthr
     end function
     integer(c_int) function spexhip_mt_plan_create(ctx, ntype, types, cent, maxlmindx, plan) &
         bind(C, name='spexhip_mt_plan_create')
@@ -157,6 +165,29 @@ contains
     if (present(precision)) prec = precision
     call check(ctx, c_susceptibility_none(ctx, int(n, c_long), int(dim, c_int), c_loc(cprod), &
                int(n, c_long), int(nfrqh, c_int), c_loc(wghtSr), prec, c_loc(susceph)), 'pairsum')
+  end subroutine
+
+  ! HILBERT branch of the pair loop (susceptibility.f:1066-1251): wtetra(n,nfrqs)
+  ! as filled by tetrahedron5_init (before indexing), wghtSr/wghtSi(nfrqs,
+  ! nfrqh+nfrqc) from getwghtS; susceph and suscepc are accumulated.
+  subroutine spexhip_pairsum_hilbert(ctx, n, dim, cprod, nfrqs, wtetra, wghtthr, nfrqh, nfrqc, &
+                                     wghtSr, wghtSi, susceph, suscepc, precision)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: n, dim, nfrqs, nfrqh, nfrqc
+    complex(c_double_complex), intent(in), target :: cprod(n, dim)
+    real(c_double), intent(in), target :: wtetra(n, nfrqs), wghtSr(nfrqs, nfrqh+nfrqc), &
+                                          wghtSi(nfrqs, nfrqh+nfrqc)
+    real(c_double), intent(in) :: wghtthr
+    complex(c_double_complex), intent(inout), target :: susceph(dim*(dim+1)/2, max(1,nfrqh)), &
+                                                        suscepc(dim, dim, max(1,nfrqc))
+    integer(c_int), intent(in), optional :: precision
+    integer(c_int) :: prec
+    prec = SPEXHIP_PREC_FP64
+    if (present(precision)) prec = precision
+    call check(ctx, c_susceptibility_hilbert(ctx, int(n, c_long), int(dim, c_int), c_loc(cprod), &
+               int(n, c_long), int(nfrqs, c_int), c_loc(wtetra), int(n, c_long), wghtthr, &
+               int(nfrqh, c_int), int(nfrqc, c_int), c_loc(wghtSr), c_loc(wghtSi), prec, &
+               c_loc(susceph), c_loc(suscepc)), 'pairsum_hilbert')
   end subroutine
 
   ! wavefproducts2_mt(cprod,k1,nk,ik0,ispin1,ispin2,b1low,b1up,b2low,b2up)

@@ -310,6 +310,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_pack_upper_d2h": (I, [_vp, I, I, _vp, I, _vp, I]),
         "spexhip_unpack_h2d": (I, [_vp, I, I, _vp, _vp, I]),
         "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
+        "spexhip_susceptibility_hilbert": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long,
+                                               C.c_double, I, I, _vp, _vp, I, _vp, _vp]),
         "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
         "spexhip_synth_normal_rows_dev": (I, [_vp, _vp, _c_long, _c_long, _c_long,
                                               C.c_ulonglong, _vp]),
@@ -615,6 +617,29 @@ class Context:
             self.h, n, nb, cp.ctypes.data, n, nw, w.ctypes.data, precision,
             susceph.ctypes.data))
         return susceph
+
+    def susceptibility_hilbert(self, cprod: np.ndarray, wtetra: np.ndarray, wghtthr: float,
+                               wghtSr: np.ndarray, wghtSi: np.ndarray, nfrqh: int,
+                               susceph: np.ndarray, suscepc: np.ndarray,
+                               precision: int = PREC_FP64) -> None:
+        """Accumulate one packet's HILBERT pair sum: ``susceph`` (npack, nfrqh)
+        and ``suscepc`` (nb, nb, nfrqc), both Fortran-ordered complex128.
+        ``wtetra`` (n, nfrqs), ``wghtSr``/``wghtSi`` (nfrqs, nfrqh+nfrqc)
+        Fortran-ordered real.  Mirrors ``susceptibility.f:1066-1251``."""
+        cp = np.asfortranarray(cprod, dtype=np.complex128)
+        n, nb = cp.shape
+        wt = np.asfortranarray(wtetra, dtype=np.float64)
+        sr = np.asfortranarray(wghtSr, dtype=np.float64)
+        si = np.asfortranarray(wghtSi, dtype=np.float64)
+        nfrqs, nf = sr.shape
+        nfrqc = nf - nfrqh
+        assert wt.shape == (n, nfrqs) and si.shape == sr.shape and 0 <= nfrqh <= nf
+        assert susceph.dtype == np.complex128 and susceph.size == nfrqh * nb * (nb + 1) // 2
+        assert suscepc.dtype == np.complex128 and suscepc.size == nfrqc * nb * nb
+        assert suscepc.flags.f_contiguous and (susceph.flags.f_contiguous or susceph.flags.c_contiguous)
+        self._check(lib().spexhip_susceptibility_hilbert(
+            self.h, n, nb, cp.ctypes.data, n, nfrqs, wt.ctypes.data, n, wghtthr, nfrqh, nfrqc,
+            sr.ctypes.data, si.ctypes.data, precision, susceph.ctypes.data, suscepc.ctypes.data))
 
     # -- wave-function products -------------------------------------------
     def mt_plan(self, types: list, cent: np.ndarray, maxlmindx: int) -> Plan:

@@ -113,3 +113,32 @@ def test_chi0_complex_weights_rejects_ozaki(ctx):
         ctx.chi0_cplx_dev(n, nb, dM, nb, False, nw, dR, dR, spexhip.PREC_OZAKI_INT8, dC, nb)
     for b in (dM, dR, dC):
         b.free()
+
+
+def test_host_dropin_accumulates_both_axes(ctx):
+    """spexhip_susceptibility_hilbert: imaginary frequencies into packed
+    susceph, real ones into the whole suscepc, accumulated over two calls
+    (two memory packets of the same transitions)."""
+    n, nb = 400, 24
+    g, f, _, _, wt = _case(4, n)
+    freqs = np.concatenate([1j * np.array([0.1, 0.9, 2.5]), np.array([0.45, 1.7])])
+    nfrqh, nfrqc = 3, 2
+    wr, wi = hilbert.hilbert_weights(freqs, f, 0.01, 0, 0)
+    M = (g.standard_normal((n, nb)) + 1j * g.standard_normal((n, nb))) / np.sqrt(nb)
+    thr = 1e-6
+    npack = nb * (nb + 1) // 2
+    sush = np.zeros((npack, nfrqh), np.complex128, order="F")
+    susc = np.zeros((nb, nb, nfrqc), np.complex128, order="F")
+    for _ in range(2):
+        ctx.susceptibility_hilbert(M, wt, thr, wr, wi, nfrqh, sush, susc)
+    ref_h = hilbert.chi0_hilbert_pairsum(M, wt, wr[:, :nfrqh], thr)
+    Dr = hilbert.fold_weights(wt, wr[:, nfrqh:], thr)
+    Di = hilbert.fold_weights(wt, wi[:, nfrqh:], thr)
+    ref_c = hilbert.chi0_complex_weights(M, Dr, Di)
+    for w in range(nfrqh):
+        want = 2 * np.array([ref_h[i, j, w] for j in range(nb) for i in range(j + 1)])
+        mx, _ = normwise(sush[:, w], want)
+        assert mx < 1e-12, (w, mx)
+    for w in range(nfrqc):
+        mx, _ = normwise(susc[:, :, w], 2 * ref_c[:, :, w])
+        assert mx < 1e-12, (w, mx)
