@@ -407,6 +407,9 @@ int spexhip_server_result(spexhip_server *srv, long id, double *chi_packed, doub
 int spexhip_server_release(spexhip_server *srv, long id);
 int spexhip_server_stats(spexhip_server *srv, long *admitted, long *completed,
                          size_t *peak_used, int *peak_inflight);
+/* Stops the scheduler and frees every request (in-flight pipelines are
+ * drained first).  spexhip_destroy does this for servers still alive; their
+ * handles are then invalid.                                               */
 int spexhip_server_destroy(spexhip_server *srv);
 
 /* ---- dielectric step and inverse (FP64) -------------------------------- */

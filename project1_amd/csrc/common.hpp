@@ -8,6 +8,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "../../include/spexhip.h"
 
@@ -36,6 +37,9 @@ struct spexhip_ctx {
   // first frequency group | rank-k updates and later groups)
   hipEvent_t phase_ev[3] = {nullptr, nullptr, nullptr};
   int phase_groups = 0;
+  // live request servers (each runs a scheduler thread that uses the
+  // context); spexhip_destroy stops them before it frees anything
+  std::vector<struct spexhip_server *> servers;
 };
 
 int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...);
@@ -57,6 +61,8 @@ int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...);
 
 // Allocation tracked against the context budget.
 int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p);
+// Bytes spx_alloc reserves for a request of `bytes` (its size class).
+size_t spx_alloc_bytes(size_t bytes);
 int spx_free(spexhip_ctx *ctx, void *p);
 // Context-owned scratch of at least `bytes` (contents undefined).
 int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p);

@@ -28,6 +28,8 @@ static size_t size_class(size_t b) {
   return (b + step - 1) / step * step;
 }
 
+size_t spx_alloc_bytes(size_t bytes) { return size_class(bytes); }
+
 static void trim_cache(spexhip_ctx *ctx) {
   std::multimap<size_t, void *> blocks;
   {
@@ -146,8 +148,19 @@ int spexhip_create(int device, size_t budget, spexhip_ctx **out) {
 
 int spexhip_destroy(spexhip_ctx *ctx) {
   if (!ctx) return SPEXHIP_OK;
+  // servers still alive: stop their schedulers and drain their pipelines
+  for (;;) {
+    spexhip_server *sv = nullptr;
+    {
+      std::lock_guard<std::mutex> g(ctx->mu);
+      if (ctx->servers.empty()) break;
+      sv = ctx->servers.back();
+    }
+    spexhip_server_destroy(sv);  // unregisters itself
+  }
   (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->stream);
+  // work on other streams (pipelines, batches) may still read context memory
+  (void)hipDeviceSynchronize();
   for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
   ctx->allocs.clear();
   trim_cache(ctx);

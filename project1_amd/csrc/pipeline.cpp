@@ -63,6 +63,14 @@ Sizes sizes_of(const spexhip_qdesc &d) {
   return s;
 }
 
+// Device bytes create() reserves: each buffer at its allocator size class.
+size_t footprint_of(const Sizes &s) {
+  size_t b = 0;
+  for (size_t x : {s.M, s.D, s.X, s.X, s.ws, s.tab})
+    if (x) b += spx_alloc_bytes(x);
+  return b;
+}
+
 // The k range runs in packets (susceptibility.f:614-713): per packet the
 // products, weights and chi0 += packet; eps and eps^-1 after the last one.
 int enqueue(spexhip_ctx *ctx, spexhip_qpipe *q) {
@@ -127,8 +135,7 @@ extern "C" {
 
 int spexhip_qpipe_footprint(const spexhip_qdesc *d, size_t *bytes) {
   if (!d || !bytes) return SPEXHIP_E_INVALID;
-  const Sizes s = sizes_of(*d);
-  *bytes = s.M + s.D + 2 * s.X + s.ws + s.tab;
+  *bytes = footprint_of(sizes_of(*d));
   return SPEXHIP_OK;
 }
 
@@ -163,7 +170,7 @@ int spexhip_qpipe_create(spexhip_ctx *ctx, const spexhip_qdesc *desc, spexhip_qp
     if (d.pw_plan) q->g2.assign(d.g2, d.g2 + 3 * d.nk);
   }
   const Sizes s = sizes_of(d);
-  q->footprint = s.M + s.D + 2 * s.X + s.ws + s.tab;
+  q->footprint = footprint_of(s);
   int rc = SPEXHIP_OK;
   auto grab = [&](size_t bytes, void **p) {
     if (rc) return;

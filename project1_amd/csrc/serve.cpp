@@ -15,6 +15,7 @@
 // Every request computes exactly what it would compute alone: pipelines
 // share no buffers, and each kernel's reduction order does not depend on
 // what else runs.
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -148,6 +149,10 @@ int spexhip_server_create(spexhip_ctx *ctx, int max_inflight, size_t hbm_budget,
   s->max_inflight = max_inflight;
   s->budget = hbm_budget;
   s->backfill = backfill;
+  {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    ctx->servers.push_back(s);
+  }
   s->th = std::thread(scheduler, s);
   *out = s;
   return SPEXHIP_OK;
@@ -252,6 +257,11 @@ int spexhip_server_stats(spexhip_server *s, long *admitted, long *completed, siz
 
 int spexhip_server_destroy(spexhip_server *s) {
   if (!s) return SPEXHIP_OK;
+  {
+    std::lock_guard<std::mutex> g(s->ctx->mu);
+    auto &v = s->ctx->servers;
+    v.erase(std::remove(v.begin(), v.end(), s), v.end());
+  }
   s->stop.store(true);
   s->cv.notify_all();
   if (s->th.joinable()) s->th.join();

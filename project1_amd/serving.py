@@ -210,8 +210,14 @@ class Request:
         eo, eu = synth.energies(g, nk, lat.nocc, nunocc)
         om, _ = synth.gl_frequencies(nw)
         vh = np.asfortranarray(lat.vhalf[:nb, :nb].astype(np.complex128))
-        self.bufs = [ctx.to_dev(eo), ctx.to_dev(eu), ctx.to_dev(np.ones(nk, np.int32)),
-                     ctx.to_dev(om), ctx.to_dev(vh)]
+        self.bufs = []
+        try:
+            for a in (eo, eu, np.ones(nk, np.int32), om, vh):
+                self.bufs.append(ctx.to_dev(a))
+        except spexhip.SpexError:
+            for b in self.bufs:  # nothing half-staged stays allocated
+                b.free()
+            raise
         d = spexhip.QDesc()
         d.nk, d.nocc, d.nunocc, d.nb, d.nw = nk, lat.nocc, nunocc, nb, nw
         d.precision, d.vkind, d.alpha, d.ozaki_group = precision, 1, 2.0, ozaki_group
@@ -343,6 +349,20 @@ def summarize(records: list[dict], t0: float, stats: dict) -> dict:
     }
 
 
+def _stage(lat: Lattice, spec: Spec, precision: int, patience_s: float = 120.0):
+    """A request's inputs on the device.  When the device is momentarily full
+    (pipelines in flight plus the allocator's cache), wait for completions
+    to release memory and try again rather than fail the stream."""
+    t_end = time.perf_counter() + patience_s
+    while True:
+        try:
+            return Request(lat, spec, precision)
+        except spexhip.SpexError as e:
+            if e.code != spexhip.E_NOMEM or time.perf_counter() > t_end:
+                raise
+            time.sleep(0.05)
+
+
 def run_closed(lat: Lattice, specs: list[Spec], precision: int, max_inflight: int,
                hbm_budget: int, keep_results=False, log=None):
     """All requests submitted at once (backlogged server): capacity."""
@@ -350,9 +370,11 @@ def run_closed(lat: Lattice, specs: list[Spec], precision: int, max_inflight: in
     lat.ctx.sync()
     drv = Driver(lat, precision, max_inflight, hbm_budget, keep_results=keep_results, log=log)
     t0 = time.perf_counter()
-    for i, r in enumerate(reqs):
-        drv.submit(i, r)
-    stats = drv.close()
+    try:
+        for i, r in enumerate(reqs):
+            drv.submit(i, r)
+    finally:
+        stats = drv.close()  # drains the server even when a submission fails
     return summarize(drv.records, t0, stats), drv.results
 
 
@@ -371,13 +393,15 @@ def run_poisson(lat: Lattice, specs: list[Spec], precision: int, rate: float, se
         arrivals.append(t)
     drv = Driver(lat, precision, max_inflight, hbm_budget, log=log)
     t0 = time.perf_counter()
-    for i, ta in enumerate(arrivals):
-        req = Request(lat, specs[i % len(specs)], precision)
-        dt = t0 + ta - time.perf_counter()
-        if dt > 0:
-            time.sleep(dt)
-        drv.submit(i, req)
-    stats = drv.close()
+    try:
+        for i, ta in enumerate(arrivals):
+            req = _stage(lat, specs[i % len(specs)], precision)
+            dt = t0 + ta - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            drv.submit(i, req)
+    finally:
+        stats = drv.close()  # drains the server even when staging fails
     out = summarize(drv.records, t0, stats)
     out.update(offered_rate=round(rate, 4), window_s=seconds, submitted=len(arrivals))
     return out

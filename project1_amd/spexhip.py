@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -222,6 +223,9 @@ class Server:
                                                C.byref(h)))
         self.h = h.value
         self._desc = {}
+        if not hasattr(ctx, "_servers"):
+            ctx._servers = weakref.WeakSet()
+        ctx._servers.add(self)
 
     def submit(self, desc: QDesc) -> int:
         v = C.c_long()
@@ -482,6 +486,9 @@ class Context:
 
     def close(self) -> None:
         if self.h:
+            # the library stops and frees servers still alive; their handles die with it
+            for srv in list(getattr(self, "_servers", ())):
+                srv.h = None
             lib().spexhip_destroy(self.h)
             self.h = None
 

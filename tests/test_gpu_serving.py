@@ -91,3 +91,21 @@ def test_device_generator_matches_host_mirror(ctx):
                                rtol=1e-13, atol=1e-13)
     for b in (ds, out, plain):
         b.free()
+
+
+def test_context_close_drains_a_live_server():
+    """Closing a context whose server still has requests queued and running
+    stops the scheduler and drains the pipelines before any memory is freed
+    (regression: the context used to be freed under a live scheduler
+    thread).  Uses a context of its own."""
+    ctx2 = spexhip.Context(0)
+    lat = serving.Lattice(ctx2, nocc=4, nunocc=12, lcut=2, nrad=2, gcut=3.1, nfft=(16, 16, 16),
+                          max_atoms=3, nb_max=100, ipw_target=40, seed=6)
+    srv = spexhip.Server(ctx2, 2, 64 << 30)
+    reqs = [serving.Request(lat, s, spexhip.PREC_FP64) for s in small_mix(6, seed=9)]
+    ids = [srv.submit(r.desc) for r in reqs]
+    assert srv.state(ids[0]) in (spexhip.Server.QUEUED, spexhip.Server.RUNNING,
+                                 spexhip.Server.DONE)
+    ctx2.close()
+    assert srv.h is None
+    srv.destroy()  # a no-op once the context has taken the server down
