@@ -364,6 +364,16 @@ def main():
     roof["fp64_kernel"] = {"kernel": "chi0_f64_kernel", "achieved": round(res["fp64"]["chi0_tflops"], 2),
                            "peak": PEAK_F64, "unit": "TFLOP/s",
                            "frac": round(res["fp64"]["chi0_tflops"] / PEAK_F64, 4)}
+    if pipe.products:
+        # products in HBM terms: M written once plus the wave functions read once
+        pb = 16 * pipe.n * a.nb + pipe.sys.cmt.nbytes + pipe.sys.cpw.nbytes
+        gbs = pb / (head["products_ms"] * 1e6)
+        roof["products"] = {
+            "kernel": "wave-function products (MT GEMMs + pw_pair_joint_kernel)",
+            "achieved": round(gbs, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(gbs / 8000.0, 4), "bytes_per_step": int(pb),
+            "note": "algorithmic bytes over the products time; the step is bound by its FP64 "
+                    "MFMA DFT passes and load latency, not by HBM"}
     line = {
         "metric": METRIC,
         "value": round(head["qpts_per_s"], 4),
