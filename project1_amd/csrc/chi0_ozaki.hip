@@ -430,7 +430,6 @@ __global__ __launch_bounds__(NWV * 64) void ozk_herk_kernel(
     // made this wave's copies visible, and a fence would drain the
     // prefetches too)
     __builtin_amdgcn_s_barrier();
-    if (c + 1 < nchunk) issue(c + 1, buf ^ 1);  // buffer buf^1 was last read before the barrier
     if (PFD > 1 && c + PFD < nchunk) prefetch(c + PFD);
     // every column-side operand of the chunk stays in registers; the row
     // side is read once per slice
@@ -471,6 +470,12 @@ __global__ __launch_bounds__(NWV * 64) void ozk_herk_kernel(
             im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(An[a], Bi[q][b], im[l][a][b], 0, 0, 0);
           }
       }
+      // copies of chunk c+1 into buffer buf^1 (last read before the barrier),
+      // issued once the first slice's MFMAs are queued: right after the
+      // barrier they would hold back the MFMA pipe while both waves of a
+      // SIMD issue their copies; later than this they have too little time
+      // to land (measured)
+      if (p == 0 && c + 1 < nchunk) issue(c + 1, buf ^ 1);
       if (p == 2) {
 #pragma unroll
         for (int a = 0; a < 2; a++)
