@@ -318,11 +318,15 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
 }
 
 // eps = 1 - Vh chi Vh.  vkind 0: vh = real vector sqrt(coul); 1: dense
-// complex nb x nb (column-major, ld nb).  chi is Hermitized first.  E may
-// alias X only for vkind 0.  tmp: nb*nb*nw complex (vkind 1).
+// complex nb x nb (column-major, ld nb).  chi is Hermitized first, with a
+// real diagonal: the reference's diagonal is real by construction (re(t) =
+// |cprod|^2, im(t) = 0 for i = j, susceptibility.f:1195-1201), while a split
+// form leaves rounding in Im chi(I,I) (about 1e-10 of max|chi| in FP32
+// pieces) that eps^-1 would amplify by its condition number.  E may alias X
+// only for vkind 0.  tmp: nb*nb*nw complex (vkind 1).
 int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, int vkind,
                    const void *vh, void *E, long lde, void *tmp) {
-  int rc = spx_hermitize(ctx, st, nb, nw, X, ldx, 0);
+  int rc = spx_hermitize(ctx, st, nb, nw, X, ldx, 1);
   if (rc) return rc;
   const long nn = (long)nb * nb;
   if (vkind == 0) {
