@@ -52,6 +52,8 @@ constexpr int FLUSH = 256;  // chunks (of 64 transitions) between int32 -> FP64 
 static_assert(8LL * 127 * 127 * 64 * FLUSH <= 2147483647LL, "int32 level accumulators overflow");
 constexpr int PFD = 1;     // L2 prefetch distance in chunks (1: none beyond the double buffer)
 constexpr int OZK_WAVES = 8;  // waves per rank-k workgroup (4 or 8)
+constexpr int OZK_KSPLIT = 2;  // transition parts: slicing part i+1 overlaps the rank-k of part i
+static_assert(OZK_KSPLIT >= 1 && OZK_KSPLIT <= 8, "one context event per part");
 constexpr int NPL = 2 * NS;  // planes per chunk
 
 __device__ inline long blk_off(long c, int pl, int ib, int nib) {
@@ -160,9 +162,9 @@ __device__ inline void slice4(double u, int a[NS]) {
   }
 }
 
-// Slices of frequencies w0 .. w0+nwg-1 (nwg arrays of sbytes each).  One
-// wave per (chunk, 16-column block); lane l = row l%16, transitions
-// 64c + 16(l/16) + 0..15.
+// Slices of frequencies w0 .. w0+nwg-1 (nwg arrays of sbytes each) for the
+// chunks c0, c0+1, ...  One wave per (chunk, 16-column block); lane l = row
+// l%16, transitions 64c + 16(l/16) + 0..15.
 template <bool COLMAJOR>
 __global__ __launch_bounds__(NT) void ozk_slice_kernel(long n, int nb, int nbp,
                                                        const double2 *__restrict__ M, long ldm,
@@ -170,11 +172,11 @@ __global__ __launch_bounds__(NT) void ozk_slice_kernel(long n, int nb, int nbp,
                                                        const double *__restrict__ sqd,
                                                        const double *__restrict__ scale,
                                                        unsigned char *__restrict__ S, long sbytes,
-                                                       long nwaves) {
+                                                       long c0, long nwaves) {
   const long wid = (long)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
   if (wid >= nwaves) return;
   const int nib = nbp / 16;
-  const long c = wid / nib;
+  const long c = c0 + wid / nib;
   const int ib = (int)(wid % nib);
   const int l = threadIdx.x & 63, row = l & 15, g = l >> 4;
   const int I = ib * 16 + row;
@@ -618,27 +620,63 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
     SPX_LAUNCHED(ctx);
   }
   const int nbt = nbp / TT;
-  const long nwaves = nchunk * (nbp / 16);
-  if (timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[0], st));
-  for (int w0 = 0; w0 < nw; w0 += nwg) {
-    const int g = nw - w0 < nwg ? nw - w0 : nwg;
-    dim3 sgrid((nwaves + NT / 64 - 1) / (NT / 64));
+  const int nib = nbp / 16;
+  const int ntile = nbt * (nbt + 1) / 2;
+  const size_t cbytes = (size_t)NPL * nib * 1024;  // slice bytes of one chunk, one frequency
+  auto slice = [&](hipStream_t s2, int w0, int g, long c0, long c1) {
+    const long nwaves = (c1 - c0) * nib;
+    dim3 sgrid((unsigned)((nwaves + NT / 64 - 1) / (NT / 64)));
     if (colmajor)
-      hipLaunchKernelGGL(ozk_slice_kernel<true>, sgrid, dim3(NT), 0, st, n, nb, nbp,
-                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, nwaves);
+      hipLaunchKernelGGL(ozk_slice_kernel<true>, sgrid, dim3(NT), 0, s2, n, nb, nbp,
+                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, c0, nwaves);
     else
-      hipLaunchKernelGGL(ozk_slice_kernel<false>, sgrid, dim3(NT), 0, st, n, nb, nbp,
-                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, nwaves);
-    SPX_LAUNCHED(ctx);
-    if (w0 == 0 && timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
-    const int ntile = nbt * (nbt + 1) / 2;
+      hipLaunchKernelGGL(ozk_slice_kernel<false>, sgrid, dim3(NT), 0, s2, n, nb, nbp,
+                         (const double2 *)M, ldm, w0, g, sqd, scale, S, (long)sb, c0, nwaves);
+  };
+  auto herk = [&](int w0, int g, long c0, long c1, double b) {
     // persistent grid: one workgroup per CU (128 KB of LDS each), a multiple of 8
     const long want = ((long)ntile * g + 7) / 8 * 8;
     const int grid = (int)std::min<long>(want, std::max(8, ozk_ncu(ctx) / 8 * 8));
     hipLaunchKernelGGL(ozk_herk_kernel<OZK_WAVES>, dim3(grid), dim3(OZK_WAVES * 64), 0, st,
-                       nchunk, nb, nbp, ntile, g, S, (long)sb, w0, scale, sgn, (double2 *)chi,
-                       (long)ldchi, alpha, beta);
-    SPX_LAUNCHED(ctx);
+                       c1 - c0, nb, nbp, ntile, g, S + c0 * cbytes, (long)sb, w0, scale, sgn,
+                       (double2 *)chi, (long)ldchi, alpha, b);
+  };
+  if (timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[0], st));
+  // One frequency group on the context stream: the transitions run in
+  // OZK_KSPLIT parts, and the slices of part i+1 (HBM-bound) are formed on a
+  // second stream while the rank-k of part i (MFMA-bound) runs; every part
+  // after the first accumulates (beta = 1).  Measured at 409,600 transitions:
+  // the exposed slicing halves (13.8 -> 6.9 ms) but the overlapped half slows
+  // the rank-k by 5.6 ms, a net 2.3 ms; three or four parts gain no more.
+  const int parts = (nwg >= nw && st == ctx->stream && nchunk >= 2 * OZK_KSPLIT) ? OZK_KSPLIT : 1;
+  if (parts > 1) {
+    int rc = spx_aux_stream(ctx);
+    if (rc) return rc;
+    hipStream_t aux = ctx->aux;
+    SPX_HIP(ctx, hipEventRecord(ctx->aux_ev[0], st));  // scales done
+    SPX_HIP(ctx, hipStreamWaitEvent(aux, ctx->aux_ev[0], 0));
+    for (int i = 0; i < parts; i++) {
+      const long c0 = nchunk * i / parts, c1 = nchunk * (i + 1) / parts;
+      slice(aux, 0, nw, c0, c1);
+      SPX_LAUNCHED(ctx);
+      SPX_HIP(ctx, hipEventRecord(ctx->aux_ev[1 + i], aux));
+    }
+    for (int i = 0; i < parts; i++) {
+      const long c0 = nchunk * i / parts, c1 = nchunk * (i + 1) / parts;
+      SPX_HIP(ctx, hipStreamWaitEvent(st, ctx->aux_ev[1 + i], 0));
+      if (i == 0 && timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
+      herk(0, nw, c0, c1, i == 0 ? beta : 1.0);
+      SPX_LAUNCHED(ctx);
+    }
+  } else {
+    for (int w0 = 0; w0 < nw; w0 += nwg) {
+      const int g = nw - w0 < nwg ? nw - w0 : nwg;
+      slice(st, w0, g, 0, nchunk);
+      SPX_LAUNCHED(ctx);
+      if (w0 == 0 && timed) SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[1], st));
+      herk(w0, g, 0, nchunk, beta);
+      SPX_LAUNCHED(ctx);
+    }
   }
   if (timed) {
     SPX_HIP(ctx, hipEventRecord(ctx->phase_ev[2], st));

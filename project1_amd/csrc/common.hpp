@@ -37,6 +37,11 @@ struct spexhip_ctx {
   // first frequency group | rank-k updates and later groups)
   hipEvent_t phase_ev[3] = {nullptr, nullptr, nullptr};
   int phase_groups = 0;
+  // second stream of the context and its events (created on first use):
+  // the Ozaki form slices the next transition part on it while the rank-k
+  // of the current part runs on the context stream
+  hipStream_t aux = nullptr;
+  hipEvent_t aux_ev[9] = {};
   // live request servers (each runs a scheduler thread that uses the
   // context); spexhip_destroy stops them before it frees anything
   std::vector<struct spexhip_server *> servers;
@@ -63,6 +68,8 @@ int spx_seterr(spexhip_ctx *ctx, int code, const char *fmt, ...);
 int spx_alloc(spexhip_ctx *ctx, size_t bytes, void **p);
 // Bytes spx_alloc reserves for a request of `bytes` (its size class).
 size_t spx_alloc_bytes(size_t bytes);
+// Creates ctx->aux and ctx->aux_ev on first use (default priority).
+int spx_aux_stream(spexhip_ctx *ctx);
 int spx_free(spexhip_ctx *ctx, void *p);
 // Context-owned scratch of at least `bytes` (contents undefined).
 int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p);

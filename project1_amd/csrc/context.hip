@@ -105,6 +105,13 @@ bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes) {
   return false;
 }
 
+int spx_aux_stream(spexhip_ctx *ctx) {
+  if (ctx->aux) return SPEXHIP_OK;
+  SPX_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking));
+  for (auto &e : ctx->aux_ev) SPX_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return SPEXHIP_OK;
+}
+
 int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p) {
   if (ctx->scratch_bytes >= bytes) {
     *p = ctx->scratch;
@@ -166,6 +173,9 @@ int spexhip_destroy(spexhip_ctx *ctx) {
   trim_cache(ctx);
   for (auto &e : ctx->phase_ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : ctx->aux_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->aux) (void)hipStreamDestroy(ctx->aux);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return SPEXHIP_OK;
