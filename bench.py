@@ -356,7 +356,9 @@ def main():
                 "traffic": measured_traffic("ozk_herk_kernel", WORKLOAD),
                 "note": "achieved = 84 int8 ops per complex multiply-add of chi0 (10 slice "
                         "pairs x 4 real products + 2 for the (3,3) pair, 2 ops each) / "
-                        "rank-k kernel time"}
+                        "rank-k phase time (HIP events on the context stream around its two "
+                        "launches, one per half of the transitions; the slicing of the second "
+                        "half runs beside the first launch)"}
     else:
         roof = {"bound": "mfma", "kernel": "chi0_split_kernel",
                 "achieved": round(head["chi0_tflops"], 2), "peak": PEAK_F32, "unit": "TFLOP/s",
