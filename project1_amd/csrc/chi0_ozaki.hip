@@ -24,7 +24,7 @@
 // (and for correlated columns) it is a sum of squares, a bias of about
 // 254^-4/3 (s/rms)^2 relative to chi.
 //
-//     Re chi = sum Br_I Br_J + Bi_I Bi_J,  Im chi = sum Bi_I Br_J + (-Br_I) Bi_J,
+//     Re chi = sum Br_I Br_J + Bi_I Bi_J,  Im chi = sum Bi_I Br_J + Br_I (-Bi_J),
 // then chi = sgn * s_I s_J * (...).
 //
 // Slice layout (one frequency), "MFMA-ready": for 64-transition chunk c,
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(NWV * 64) void ozk_herk_kernel(
   };
 
   // level l = p+q (0-based slices) -> one int32 accumulator for Re and one
-  // for Im per 16x16 block; Im = Ai.Br + (-Ar).Bi.  r4: Re of the pair of
+  // for Im per 16x16 block; Im = Ai.Br + Ar.(-Bi).  r4: Re of the pair of
   // third slices (0-based p = q = 2), the one level-4 term that does not
   // average out (on the diagonal it is a sum of squares).
   i4_t re[NS][2][BB], im[NS][2][BB], r4[2][BB];
@@ -435,22 +435,24 @@ __global__ __launch_bounds__(NWV * 64) void ozk_herk_kernel(
     if (PFD > 1 && c + PFD < nchunk) prefetch(c + PFD);
     // every column-side operand of the chunk stays in registers; the row
     // side is read once per slice
-    i4_t Br[NS][BB], Bi[NS][BB];
+    // Im = Ai.Br + Ar.(-Bi): the bytewise negation is done once per chunk on
+    // the column side (NS blocks) instead of once per slice on the row side
+    i4_t Br[NS][BB], Bi[NS][BB], Bn[NS][BB];
 #pragma unroll
     for (int q = 0; q < NS; q++)
 #pragma unroll
       for (int b = 0; b < BB; b++) {
         Br[q][b] = opnd(buf, 1, q, wc * BB + b);
         Bi[q][b] = opnd(buf, 1, NS + q, wc * BB + b);
+        Bn[q][b] = neg_i8x16(Bi[q][b]);
       }
 #pragma unroll
     for (int p = 0; p < NS; p++) {
-      i4_t Ar[2], Ai[2], An[2];
+      i4_t Ar[2], Ai[2];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
         Ar[a] = opnd(buf, 0, p, wr * 2 + a);
         Ai[a] = opnd(buf, 0, NS + p, wr * 2 + a);
-        An[a] = neg_i8x16(Ar[a]);
       }
 #pragma unroll
       for (int q = 0; q < NS - p; q++) {
@@ -469,7 +471,7 @@ __global__ __launch_bounds__(NWV * 64) void ozk_herk_kernel(
 #pragma unroll
           for (int b = 0; b < BB; b++) {
             re[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ai[a], Bi[q][b], re[l][a][b], 0, 0, 0);
-            im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(An[a], Bi[q][b], im[l][a][b], 0, 0, 0);
+            im[l][a][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Ar[a], Bn[q][b], im[l][a][b], 0, 0, 0);
           }
       }
       // copies of chunk c+1 into buffer buf^1 (last read before the barrier),
