@@ -237,6 +237,46 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def gamma_leg(pipe, a, reps=5):
+    """The q = 0 head and wings (headwing_kernel) on this step's M and
+    weights, with a synthetic momentum matrix, and the head/wing update of
+    eps^-1: HIP events on the context stream around `reps` launches each.
+    Not part of the timed step (the workload is q != 0)."""
+    ctx, n, nb, nw = pipe.ctx, pipe.n, a.nb, a.nw
+    dP = ctx.alloc(16 * 3 * n)
+    ctx.synth_normal(dP, 6 * n, a.seed + 7, 1.0)
+    dH, dWg = ctx.alloc(16 * 9 * nw), ctx.alloc(16 * 3 * nb * nw)
+    hw = lambda: ctx.headwings_dev(n, nb, pipe.dM, nb, False, dP, n, nw, pipe.dW, n, dH, dWg,
+                                   alpha=2.0)
+    hw()
+    e0, e1, e2 = ctx.event(), ctx.event(), ctx.event()
+    e0.record()
+    for _ in range(reps):
+        hw()
+    e1.record()
+    for _ in range(reps):
+        ctx.gamma_inverse_dev(nb, nw, pipe.dE, nb, dH, dWg)
+    e2.record()
+    ctx.sync()
+    ms = e1.ms_since(e0) / reps
+    inv_ms = e2.ms_since(e1) / reps
+    for b in (dP, dH, dWg):
+        b.free()
+    for e in (e0, e1, e2):
+        e.destroy()
+    # algorithmic bytes per launch: M once per pass (16 B per element), the
+    # momentum matrix (48 B per transition) and the weights (8 B each)
+    passes = -(-nw // 16)
+    byts = passes * (16 * n * nb + 48 * n) + 8 * n * nw
+    gbs = byts / (ms * 1e6)
+    return {"kernel": "headwing_kernel", "ms": round(ms, 3), "achieved": round(gbs, 1),
+            "peak": 8000.0, "unit": "GB/s", "frac": round(gbs / 8000.0, 4),
+            "bytes_per_launch": int(byts), "inverse_update_ms": round(inv_ms, 3),
+            "note": "q = 0 head + wings of chi0 on this step's M (synthetic momentum matrix), "
+                    "and the head/wing update of eps^-1 (gamma_wing + gamma_head kernels); "
+                    "outside the timed step"}
+
+
 def batch_leg(ctx, a, precision="split_fp32", budget=160 << 30):
     """32 uneven q-points (Nk=64, Nocc=16, Nunocc ~ U[200,600], Nb ~ U[600,1600],
     Nw=12).  split_fp32 / fp64: per k packet one grouped chi0 launch over all
@@ -409,6 +449,8 @@ def main():
         "err_vs_fp64": errs_,
         "roofline": roof,
     }
+    line["gamma_headwings"] = gamma_leg(pipe, a)
+    log(f"gamma {json.dumps(line['gamma_headwings'])}")
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pipe, a)
     pipe.free()

@@ -227,6 +227,78 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
                                    const double *wghtSi, int precision, double *susceph,
                                    double *suscepc);
 
+/* ---- Gamma point (q = 0): head and wings -------------------------------- */
+/* Replaces the momentum-matrix scaling of susceptibility.f:858-873
+ * (multdiff_ = 0): dcprod = dcprod / sqrt(vol) / (e_unocc - e_occ), zero
+ * where |e_unocc - e_occ| <= 1e-8, in place.  P_dev (n x 3 complex, t
+ * fastest, ld ldp) holds the momentum matrix <u|p|o> of the transitions
+ * t = (k*nocc + b1)*nunocc + b2; e_occ_dev [nk][nocc], e_unocc_dev
+ * [nk][nunocc] (energies at k).                                           */
+int spexhip_momentum_scale_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                               const void *e_occ_dev, const void *e_unocc_dev, double vol,
+                               void *P_dev, long ldp);
+
+/* Head and wings of chi0 at q = 0 (susceptibility.f:1092-1127, imaginary
+ * frequencies):
+ *   head(k1,k2,w) = beta*head + alpha * sum_t Dh(t,w) P(t,k1) conj(P(t,k2)),
+ *   wing(k,i,w)   = beta*wing + alpha * sum_t D(t,w)  M(t,i)  conj(P(t,k)).
+ * M_dev as for spexhip_chi0_dev (transition-major, or colmajor=1 for the
+ * reference cprod); P_dev = dcprod (n x 3, ld ldp); D_dev and Dh_dev (n x
+ * nw, ld ldd) weights of the wings and of the head (Dh_dev NULL: D; they
+ * differ under MULTDIFF, where the Hilbert weights of the head and the
+ * wings carry 1/frqs^2 and 1/frqs, :1097, :1130, :1817-1824).  head_dev: 3
+ * x 3 x nw complex, full (k1 + 3*k2 + 9*w); wing_dev: 3 x nb x nw complex
+ * (the reference wingh layout).                                           */
+int spexhip_headwings_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev, long ldm,
+                          int colmajor, const void *P_dev, long ldp, int nw, const void *D_dev,
+                          const void *Dh_dev, long ldd, double alpha, double beta,
+                          void *head_dev, void *wing_dev);
+/* Real-frequency form (headc, wingc: susceptibility.f:1112-1125, :1144-1151)
+ * with complex weights Dr + i Di:
+ *   head(k1,k2,w)  = sum_t Dhc P(t,k1) conj(P(t,k2)),
+ *   wing(k,i,0,w)  = sum_t Dc  M(t,i) conj(P(t,k)),
+ *   wing(k,i,1,w)  = sum_t Dc  conj(M(t,i)) P(t,k)
+ * (with beta, alpha as above); wing_dev 3 x nb x 2 x nw complex (wingc).
+ * Dhr_dev, Dhi_dev both NULL: the wing weights.                           */
+int spexhip_headwings_cplx_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev, long ldm,
+                               int colmajor, const void *P_dev, long ldp, int nw,
+                               const void *Dr_dev, const void *Di_dev, const void *Dhr_dev,
+                               const void *Dhi_dev, long ldd, double alpha, double beta,
+                               void *head_dev, void *wing_dev);
+/* Host drop-in of the two above for one memory packet: cprod (n x nb, ld
+ * ldc), dcprod (n x 3, ld lddc), weights (n x nfrqh: wh of the wings, whh
+ * of the head or NULL; n x nfrqc: wcr, wci, and wchr, wchi or NULL).  headh
+ * (6 x nfrqh packed upper), wingh (3 x nb x nfrqh), headc (3 x 3 x nfrqc),
+ * wingc (3 x nb x 2 x nfrqc) are ACCUMULATED.  No spin factor.            */
+int spexhip_susceptibility_headwings(spexhip_ctx *ctx, long n, int nb, const double *cprod,
+                                     long ldc, const double *dcprod, long lddc, int nfrqh,
+                                     const double *wh, const double *whh, int nfrqc,
+                                     const double *wcr, const double *wci, const double *wchr,
+                                     const double *wchi, double *headh, double *wingh,
+                                     double *headc, double *wingc);
+
+/* Dielectric side at q = 0, imaginary axis.  Before spexhip_dielectric_dev
+ * (correlation.f:570-629): chi row and column 0 and wing(:,0,:) set to
+ * zero, head = 1 - 4 pi head, wing(:,i,:) = -sqrt(4 pi) sqcoul(i)
+ * wing(:,i,:).  X_dev: nw full chi matrices (ld ldx); head_dev 3 x 3 x nw,
+ * wing_dev 3 x nb x nw (as above); sqcoul_dev: nb doubles.                */
+int spexhip_gamma_prepare_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx,
+                              void *head_dev, void *wing_dev, const void *sqcoul_dev);
+/* After spexhip_inverse_hpd_dev (correlation.f:749-759): wing = -wing
+ * conj(eps^-1), head = head + conj(wing_old) wing^T.  Einv_dev: nw full
+ * Hermitian matrices (ld lde), both triangles valid.                      */
+int spexhip_gamma_inverse_dev(spexhip_ctx *ctx, int nb, int nw, const void *Einv_dev, int lde,
+                              void *head_dev, void *wing_dev);
+/* Host drop-in for correlation.f:570-759 at q = 0 on the imaginary axis:
+ * chi (packed upper, nb(nb+1)/2 x nw) -> eps^-1 (packed), with head
+ * (packed 6 x nw) and wing (3 x nb x nw) replaced in place by those of the
+ * inverse; v^1/2 = sqcoul (nb doubles).  SPEXHIP_E_INVALID if the updated
+ * head deviates from Hermitian by more than 1e-6 (packmat's test,
+ * w_packmat.inc:52-73).                                                   */
+int spexhip_dielectric_inverse_gamma(spexhip_ctx *ctx, int nb, int nw, const double *chi_packed,
+                                     const double *sqcoul, double *head, double *wing,
+                                     double *einv_packed);
+
 /* ---- wave-function products -------------------------------------------- */
 /* The reference reads the basis and wave functions from module global
  * (cmt, cpw, gpt, gptm, nfft, imati, integral, gaunt -- wavefproducts.f:

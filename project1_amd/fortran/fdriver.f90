@@ -1,7 +1,8 @@
 ! Fortran host exercising the ISO_C_BINDING boundary: one packet of the
 ! HILBERT NONE pair sum and the dielectric inverse, compared with a plain
 ! Fortran pair loop written the way susceptibility.f:1183-1240 does it; the
-! screened interaction, the RPA integrand and the HILBERT pair sum likewise.
+! screened interaction, the RPA integrand, the HILBERT pair sum and the
+! Gamma-point head and wings likewise.
 ! Exit status 0 on agreement (normwise relative error < 1e-12).
 program fdriver
   use, intrinsic :: iso_c_binding
@@ -173,6 +174,63 @@ program fdriver
     err = max(maxval(abs(sh - rh)) / maxval(abs(rh)), maxval(abs(sc - rcx)) / maxval(abs(rcx)))
     write(*, '(A,ES10.3)') 'fortran pairsum_hilbert max-rel error: ', err
     if (err > 1d-12) stop 7
+  end block
+
+  ! Gamma point: head and wings of the packet against plain loops written the
+  ! way susceptibility.f:1096-1127 does them (HILBERT NONE), then the head and
+  ! wings of the inverse: its eps^-1 must equal the plain inverse of the chi
+  ! whose first row and column are zero (correlation.f:597-603).
+  block
+    complex(c_double_complex), target :: dc(n, 3), hh(6, nw), wh(3, nb, nw), rhh(6, nw), &
+        rwh(3, nb, nw), hc(3, 3, 1), wc(3, nb, 2, 1), chz(npack, nw), e1(npack, nw), e2(npack, nw)
+    real(c_double), target :: dummy(n, 1)
+    integer :: k1, k2
+    do k2 = 1, 3
+      do k = 1, n
+        x = 0.21d0*k + 0.9d0*k2
+        dc(k, k2) = cmplx(cos(x), sin(0.6d0*x), kind=c_double_complex)
+      end do
+    end do
+    rhh = 0
+    rwh = 0
+    do k2 = 1, 3
+      do k1 = 1, k2
+        re = real(dc(:, k1) * conjg(dc(:, k2)))
+        im = aimag(dc(:, k1) * conjg(dc(:, k2)))
+        ij = (k2-1)*k2/2 + k1
+        do f = 1, nw
+          rhh(ij, f) = rhh(ij, f) + cmplx(dot_product(re, wght(:, f)), dot_product(im, wght(:, f)), &
+                                          kind=c_double_complex)
+        end do
+      end do
+      do i = 1, nb
+        re = real(cprod(:, i) * conjg(dc(:, k2)))
+        im = aimag(cprod(:, i) * conjg(dc(:, k2)))
+        do f = 1, nw
+          rwh(k2, i, f) = rwh(k2, i, f) + cmplx(dot_product(re, wght(:, f)), &
+                                                dot_product(im, wght(:, f)), kind=c_double_complex)
+        end do
+      end do
+    end do
+    hh = 0
+    wh = 0
+    dummy = 0
+    call spexhip_headwings(ctx, n, nb, cprod, dc, nw, wght, wght, 0, dummy, dummy, dummy, dummy, &
+                           hh, wh, hc, wc)
+    err = max(maxval(abs(hh - rhh)) / maxval(abs(rhh)), maxval(abs(wh - rwh)) / maxval(abs(rwh)))
+    write(*, '(A,ES10.3)') 'fortran headwings max-rel error: ', err
+    if (err > 1d-12) stop 8
+    chz = sus
+    do i = 1, nb
+      chz(1 + i*(i-1)/2, :) = 0
+    end do
+    call spexhip_dielec_inverse(ctx, nb, nw, chz, sq, e1)
+    hh = hh * 1d-2
+    wh = wh * 1d-2
+    call spexhip_dielec_inverse_gamma(ctx, nb, nw, sus, sq, hh, wh, e2)
+    err = maxval(abs(e2 - e1)) / maxval(abs(e1))
+    write(*, '(A,ES10.3)') 'fortran dielec_inverse_gamma body max-rel error: ', err
+    if (err > 1d-12 .or. any(wh(:, 1, :) /= 0)) stop 9
   end block
   rc = spexhip_destroy(ctx)
   write(*, '(A)') 'fortran driver OK'

@@ -41,6 +41,7 @@ module m_spexhip
   public :: spexhip_plan_destroy
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
+  public :: spexhip_headwings, spexhip_dielec_inverse_gamma
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -73,6 +74,21 @@ module m_spexhip
       integer(c_long), value :: n, ldc, ldwt
       integer(c_int), value :: nb, nfrqs, nfrqh, nfrqc, prec
       real(c_double), value :: wghtthr
+    end function
+    integer(c_int) function c_susceptibility_headwings(ctx, n, nb, cprod, ldc, dcprod, lddc, &
+        nfrqh, wh, whh, nfrqc, wcr, wci, wchr, wchi, headh, wingh, headc, wingc) &
+        bind(C, name='spexhip_susceptibility_headwings')
+      import :: c_int, c_long, c_ptr
+      type(c_ptr), value :: ctx, cprod, dcprod, wh, whh, wcr, wci, wchr, wchi, headh, wingh, &
+                            headc, wingc
+      integer(c_long), value :: n, ldc, lddc
+      integer(c_int), value :: nb, nfrqh, nfrqc
+    end function
+    integer(c_int) function c_dielectric_inverse_gamma(ctx, nb, nw, chi, sqcoul, head, wing, &
+        einv) bind(C, name='spexhip_dielectric_inverse_gamma')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, chi, sqcoul, head, wing, einv
+      integer(c_int), value :: nb, nw
     end function
     integer(c_int) function spexhip_mt_plan_create(ctx, ntype, types, cent, maxlmindx, plan) &
         bind(C, name='spexhip_mt_plan_create')
@@ -188,6 +204,42 @@ contains
                int(n, c_long), int(nfrqs, c_int), c_loc(wtetra), int(n, c_long), wghtthr, &
                int(nfrqh, c_int), int(nfrqc, c_int), c_loc(wghtSr), c_loc(wghtSi), prec, &
                c_loc(susceph), c_loc(suscepc)), 'pairsum_hilbert')
+  end subroutine
+
+  ! Gamma-point head and wings of one memory packet (susceptibility.f:1092-1157):
+  ! dcprod(n,3) after the momentum scaling (:858-873), weights per transition
+  ! (wh of the wings, whh of the head, n x nfrqh; wcr/wci and wchr/wchi for the
+  ! real frequencies); headh(6,nfrqh), wingh(3,dim,nfrqh), headc(3,3,nfrqc) and
+  ! wingc(3,dim,2,nfrqc) are accumulated.
+  subroutine spexhip_headwings(ctx, n, dim, cprod, dcprod, nfrqh, wh, whh, nfrqc, wcr, wci, &
+                               wchr, wchi, headh, wingh, headc, wingc)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: n, dim, nfrqh, nfrqc
+    complex(c_double_complex), intent(in), target :: cprod(n, dim), dcprod(n, 3)
+    real(c_double), intent(in), target :: wh(n, max(1,nfrqh)), whh(n, max(1,nfrqh)), &
+                                          wcr(n, max(1,nfrqc)), wci(n, max(1,nfrqc)), &
+                                          wchr(n, max(1,nfrqc)), wchi(n, max(1,nfrqc))
+    complex(c_double_complex), intent(inout), target :: headh(6, max(1,nfrqh)), &
+        wingh(3, dim, max(1,nfrqh)), headc(3, 3, max(1,nfrqc)), wingc(3, dim, 2, max(1,nfrqc))
+    call check(ctx, c_susceptibility_headwings(ctx, int(n, c_long), int(dim, c_int), c_loc(cprod), &
+               int(n, c_long), c_loc(dcprod), int(n, c_long), int(nfrqh, c_int), c_loc(wh), &
+               c_loc(whh), int(nfrqc, c_int), c_loc(wcr), c_loc(wci), c_loc(wchr), c_loc(wchi), &
+               c_loc(headh), c_loc(wingh), c_loc(headc), c_loc(wingc)), 'headwings')
+  end subroutine
+
+  ! correlation.f:570-759 at the Gamma point (imaginary axis): suscep (packed,
+  ! after the spin factor) -> dielec = eps^-1 (packed); head(6,nfreq) and
+  ! wing(3,nbas,nfreq) replaced by those of the inverse; sqcoul = sqrt(coul).
+  subroutine spexhip_dielec_inverse_gamma(ctx, nbas, nfreq, suscep, sqcoul, head, wing, dielec)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nbas, nfreq
+    complex(c_double_complex), intent(in), target :: suscep(nbas*(nbas+1)/2, nfreq)
+    real(c_double), intent(in), target :: sqcoul(nbas)
+    complex(c_double_complex), intent(inout), target :: head(6, nfreq), wing(3, nbas, nfreq)
+    complex(c_double_complex), intent(out), target :: dielec(nbas*(nbas+1)/2, nfreq)
+    call check(ctx, c_dielectric_inverse_gamma(ctx, int(nbas, c_int), int(nfreq, c_int), &
+               c_loc(suscep), c_loc(sqcoul), c_loc(head), c_loc(wing), c_loc(dielec)), &
+               'dielec_inverse_gamma')
   end subroutine
 
   ! wavefproducts2_mt(cprod,k1,nk,ik0,ispin1,ispin2,b1low,b1up,b2low,b2up)

@@ -316,6 +316,18 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
         "spexhip_susceptibility_hilbert": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long,
                                                C.c_double, I, I, _vp, _vp, I, _vp, _vp]),
+        "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
+        "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
+                                      _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
+        "spexhip_headwings_cplx_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I,
+                                           _vp, _vp, _vp, _vp, _c_long, C.c_double, C.c_double,
+                                           _vp, _vp]),
+        "spexhip_susceptibility_headwings": (I, [_vp, _c_long, I, _vp, _c_long, _vp, _c_long, I,
+                                                 _vp, _vp, I, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                 _vp]),
+        "spexhip_gamma_prepare_dev": (I, [_vp, I, I, _vp, I, _vp, _vp, _vp]),
+        "spexhip_gamma_inverse_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_dielectric_inverse_gamma": (I, [_vp, I, I, _vp, _vp, _vp, _vp, _vp]),
         "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
         "spexhip_synth_normal_rows_dev": (I, [_vp, _vp, _c_long, _c_long, _c_long,
                                               C.c_ulonglong, _vp]),
@@ -647,6 +659,91 @@ class Context:
         self._check(lib().spexhip_susceptibility_hilbert(
             self.h, n, nb, cp.ctypes.data, n, nfrqs, wt.ctypes.data, n, wghtthr, nfrqh, nfrqc,
             sr.ctypes.data, si.ctypes.data, precision, susceph.ctypes.data, suscepc.ctypes.data))
+
+    # -- Gamma point: head and wings --------------------------------------
+    def momentum_scale_dev(self, e_occ: DevBuf, e_unocc: DevBuf, nk: int, nocc: int,
+                           nunocc: int, vol: float, P: DevBuf, ldp: int) -> None:
+        """dcprod = P / sqrt(vol) / (e_unocc - e_occ) in place, zero where the
+        difference is at most 1e-8 in size."""
+        self._check(lib().spexhip_momentum_scale_dev(
+            self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, vol, P.ptr, ldp))
+
+    def headwings_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, P: DevBuf,
+                      ldp: int, nw: int, D: DevBuf, ldd: int, head: DevBuf, wing: DevBuf,
+                      Dh: DevBuf | None = None, alpha: float = 1.0, beta: float = 0.0) -> None:
+        """head (3 x 3 x nw, full) and wing (3 x nb x nw) of chi0 at q = 0."""
+        self._check(lib().spexhip_headwings_dev(
+            self.h, n, nb, M.ptr, ldm, int(colmajor), P.ptr, ldp, nw, D.ptr,
+            Dh.ptr if Dh is not None else None, ldd, alpha, beta, head.ptr, wing.ptr))
+
+    def headwings_cplx_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool,
+                           P: DevBuf, ldp: int, nw: int, Dr: DevBuf, Di: DevBuf, ldd: int,
+                           head: DevBuf, wing: DevBuf, Dhr: DevBuf | None = None,
+                           Dhi: DevBuf | None = None, alpha: float = 1.0,
+                           beta: float = 0.0) -> None:
+        """Real-frequency head (3 x 3 x nw) and wings (3 x nb x 2 x nw)."""
+        self._check(lib().spexhip_headwings_cplx_dev(
+            self.h, n, nb, M.ptr, ldm, int(colmajor), P.ptr, ldp, nw, Dr.ptr, Di.ptr,
+            Dhr.ptr if Dhr is not None else None, Dhi.ptr if Dhi is not None else None, ldd,
+            alpha, beta, head.ptr, wing.ptr))
+
+    def susceptibility_headwings(self, cprod: np.ndarray, dcprod: np.ndarray,
+                                 wh: np.ndarray | None, headh: np.ndarray | None,
+                                 wingh: np.ndarray | None, wcr: np.ndarray | None = None,
+                                 wci: np.ndarray | None = None, headc: np.ndarray | None = None,
+                                 wingc: np.ndarray | None = None, whh: np.ndarray | None = None,
+                                 wchr: np.ndarray | None = None,
+                                 wchi: np.ndarray | None = None) -> None:
+        """Accumulate one packet's head and wings (reference layouts, all
+        Fortran-ordered): headh (6, nfrqh), wingh (3, nb, nfrqh), headc
+        (3, 3, nfrqc), wingc (3, nb, 2, nfrqc).  Weights (n, nfrq)."""
+        cp = np.asfortranarray(cprod, dtype=np.complex128)
+        n, nb = cp.shape
+        dp = np.asfortranarray(dcprod, dtype=np.complex128)
+        assert dp.shape == (n, 3)
+        f = lambda a: None if a is None else np.asfortranarray(a, dtype=np.float64)
+        wh, whh, wcr, wci, wchr, wchi = map(f, (wh, whh, wcr, wci, wchr, wchi))
+        nfrqh = 0 if wh is None else wh.shape[1]
+        nfrqc = 0 if wcr is None else wcr.shape[1]
+        for a in (headh, wingh, headc, wingc):
+            assert a is None or (a.dtype == np.complex128 and a.flags.f_contiguous)
+        if nfrqh:
+            assert headh.shape == (6, nfrqh) and wingh.shape == (3, nb, nfrqh)
+        if nfrqc:
+            assert headc.shape == (3, 3, nfrqc) and wingc.shape == (3, nb, 2, nfrqc)
+        ptr = lambda a: None if a is None else a.ctypes.data
+        self._check(lib().spexhip_susceptibility_headwings(
+            self.h, n, nb, cp.ctypes.data, n, dp.ctypes.data, n, nfrqh, ptr(wh), ptr(whh),
+            nfrqc, ptr(wcr), ptr(wci), ptr(wchr), ptr(wchi), ptr(headh), ptr(wingh),
+            ptr(headc), ptr(wingc)))
+
+    def gamma_prepare_dev(self, nb: int, nw: int, X: DevBuf, ldx: int, head: DevBuf,
+                          wing: DevBuf, sqcoul: DevBuf) -> None:
+        self._check(lib().spexhip_gamma_prepare_dev(self.h, nb, nw, X.ptr, ldx, head.ptr,
+                                                    wing.ptr, sqcoul.ptr))
+
+    def gamma_inverse_dev(self, nb: int, nw: int, einv: DevBuf, lde: int, head: DevBuf,
+                          wing: DevBuf) -> None:
+        self._check(lib().spexhip_gamma_inverse_dev(self.h, nb, nw, einv.ptr, lde, head.ptr,
+                                                    wing.ptr))
+
+    def dielectric_inverse_gamma(self, chi_packed: np.ndarray, sqcoul: np.ndarray,
+                                 head: np.ndarray, wing: np.ndarray) -> np.ndarray:
+        """correlation.f:570-759 at q = 0: returns eps^-1 packed (npack, nw)
+        and replaces head (6, nw) and wing (3, nb, nw) (Fortran-ordered
+        complex128) in place by those of the inverse."""
+        sq = np.ascontiguousarray(sqcoul, np.float64)
+        nb = sq.size
+        c = np.asfortranarray(chi_packed, dtype=np.complex128)
+        nw = c.shape[1]
+        assert c.shape[0] == nb * (nb + 1) // 2
+        assert head.dtype == np.complex128 and head.flags.f_contiguous and head.shape == (6, nw)
+        assert wing.dtype == np.complex128 and wing.flags.f_contiguous and wing.shape == (3, nb, nw)
+        out = np.zeros_like(c, order="F")
+        self._check(lib().spexhip_dielectric_inverse_gamma(
+            self.h, nb, nw, c.ctypes.data, sq.ctypes.data, head.ctypes.data, wing.ctypes.data,
+            out.ctypes.data))
+        return out
 
     # -- wave-function products -------------------------------------------
     def mt_plan(self, types: list, cent: np.ndarray, maxlmindx: int) -> Plan:

@@ -1,0 +1,171 @@
+"""Pinning of the Gamma-point head/wing oracle (oracle/gamma.py): the head
+and wing sums are entries of the chi0 pair sum of the augmented matrix
+[cprod | dcprod] (an independent, golden-pinned oracle); the reference's
+head/wing update of the inverse is the Schur complement of the bordered
+matrix; the scaling and dielectric steps have closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import gamma, hilbert
+
+
+def _rand(g, n, nb):
+    cp = (g.standard_normal((n, nb)) + 1j * g.standard_normal((n, nb))) / math.sqrt(nb)
+    dp = g.standard_normal((n, 3)) + 1j * g.standard_normal((n, 3))
+    return cp, dp
+
+
+def _packed_index(i, j):
+    """zero-based packed-upper index of (i, j), i <= j."""
+    return j * (j + 1) // 2 + i
+
+
+def test_headwings_are_entries_of_the_augmented_pair_sum():
+    g = np.random.default_rng(11)
+    n, nb, nw = 300, 9, 3
+    cp, dp = _rand(g, n, nb)
+    D = -g.uniform(0.1, 2.0, (n, nw))
+    head, wing = gamma.headwings_imag(cp, dp, D)
+    aug = oracle.chi0_pairsum(np.concatenate([cp, dp], axis=1), D)  # (npack, nw)
+    for k2 in range(3):
+        for k1 in range(k2 + 1):
+            want = aug[_packed_index(nb + k1, nb + k2)]
+            np.testing.assert_allclose(head[_packed_index(k1, k2)], want, rtol=1e-12, atol=1e-12)
+        for i in range(nb):
+            np.testing.assert_allclose(wing[k2, i], aug[_packed_index(i, nb + k2)], rtol=1e-12,
+                                       atol=1e-12)
+
+
+def test_separate_head_weights_leave_the_wings_alone():
+    g = np.random.default_rng(12)
+    n, nb, nw = 120, 4, 2
+    cp, dp = _rand(g, n, nb)
+    D = g.standard_normal((n, nw))
+    Dh = g.standard_normal((n, nw))
+    h1, w1 = gamma.headwings_imag(cp, dp, D, Dh)
+    h2, _ = gamma.headwings_imag(cp, dp, Dh)
+    _, w2 = gamma.headwings_imag(cp, dp, D)
+    np.testing.assert_array_equal(h1, h2)
+    np.testing.assert_array_equal(w1, w2)
+
+
+def test_real_frequency_headwings_are_the_complex_weighted_augmented_matrix():
+    g = np.random.default_rng(13)
+    n, nb, nw = 200, 6, 2
+    cp, dp = _rand(g, n, nb)
+    Dr, Di = g.standard_normal((n, nw)), g.standard_normal((n, nw))
+    head, wing = gamma.headwings_real(cp, dp, Dr, Di)
+    C = hilbert.chi0_complex_weights(np.concatenate([cp, dp], axis=1), Dr, Di)
+    np.testing.assert_allclose(head, C[nb:, nb:], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(wing[:, :, 0], C[:nb, nb:].transpose(1, 0, 2), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(wing[:, :, 1], C[nb:, :nb], rtol=1e-12, atol=1e-12)
+
+
+def test_imaginary_head_is_hermitian_and_real_weights_give_conjugate_wings():
+    """With Di = 0 the second real-frequency wing is the conjugate of the
+    first, and the imaginary-axis head equals the packed upper of the full
+    real-frequency head."""
+    g = np.random.default_rng(14)
+    n, nb, nw = 150, 5, 2
+    cp, dp = _rand(g, n, nb)
+    D = g.standard_normal((n, nw))
+    hc, wc = gamma.headwings_real(cp, dp, D, np.zeros_like(D))
+    hh, wh = gamma.headwings_imag(cp, dp, D)
+    np.testing.assert_allclose(wc[:, :, 1], np.conj(wc[:, :, 0]), rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(wc[:, :, 0], wh, rtol=1e-13, atol=1e-13)
+    for f in range(nw):
+        np.testing.assert_allclose(gamma.unpack_head(hh[:, f]), hc[:, :, f], rtol=1e-13, atol=1e-13)
+
+
+def test_momentum_scale_closed_form_and_zero_rule():
+    nk, nocc, nunocc = 2, 2, 3
+    eo = np.array([[-1.0, -0.5], [-0.25, -0.125]])
+    eu = np.array([[0.5, 1.0, -0.5 + 1e-8], [2.0, -0.125 - 5e-9, 0.75]])
+    n = nk * nocc * nunocc
+    mom = (np.arange(n * 3) + 1.0).reshape(n, 3) * (1 - 2j)
+    vol = 4.0
+    got = gamma.momentum_scale(mom, eo, eu, vol)
+    k, b1, b2 = gamma.transitions(nk, nocc, nunocc)
+    assert list(zip(k[:4], b1[:4], b2[:4])) == [(0, 0, 0), (0, 0, 1), (0, 0, 2), (0, 1, 0)]
+    for t in range(n):
+        d = eu[k[t], b2[t]] - eo[k[t], b1[t]]
+        if abs(d) > 1e-8:
+            np.testing.assert_allclose(got[t], mom[t] / 2.0 / d, rtol=1e-15)
+        else:
+            assert (got[t] == 0).all(), t
+    # the (0,1,2) and (1,1,1) differences are exactly at / below the threshold
+    assert (got[(0 * nocc + 1) * nunocc + 2] == 0).all()
+    assert (got[(1 * nocc + 1) * nunocc + 1] == 0).all()
+
+
+def test_gamma_prepare_closed_form():
+    g = np.random.default_rng(15)
+    nb, nw = 6, 2
+    A = g.standard_normal((nb, nb)) + 1j * g.standard_normal((nb, nb))
+    chi = -(A @ A.conj().T)
+    chi_p = np.stack([oracle.pack_upper(chi), oracle.pack_upper(2 * chi)], axis=1)
+    sq = g.uniform(0.5, 2.0, nb)
+    head = g.standard_normal((6, nw)) + 1j * g.standard_normal((6, nw))
+    wing = g.standard_normal((3, nb, nw)) + 1j * g.standard_normal((3, nb, nw))
+    eps, h, w = gamma.gamma_prepare(chi_p, head, wing, sq)
+    for f, s in enumerate((1, 2)):
+        c = s * chi.copy()
+        c[0, :] = 0
+        c[:, 0] = 0
+        want = np.eye(nb) - sq[:, None] * c * sq[None, :]
+        np.testing.assert_allclose(oracle.unpack_herm(eps[:, f], nb), want, rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(h[[0, 2, 5]], 1 - 4 * math.pi * head[[0, 2, 5]], rtol=1e-15)
+    np.testing.assert_allclose(h[[1, 3, 4]], -4 * math.pi * head[[1, 3, 4]], rtol=1e-15)
+    assert (w[:, 0, :] == 0).all()
+    np.testing.assert_allclose(w[:, 1:], -math.sqrt(4 * math.pi) * sq[None, 1:, None] * wing[:, 1:],
+                               rtol=1e-15)
+
+
+@pytest.mark.parametrize("seed", [16, 17])
+def test_gamma_inverse_is_the_bordered_matrix_inverse(seed):
+    """The reference keeps S = H - conj(w) B^-1 w^T as the head and
+    -w conj(B^-1) as the wings: the Schur complement and the scaled
+    off-diagonal blocks of the Hermitian bordered matrix
+    E = [[H, conj(w)], [w^T, B]], whose inverse has S^-1 as its head block,
+    w_new^T S^-1 below and S^-1 conj(w_new) beside it."""
+    g = np.random.default_rng(seed)
+    nb, nw = 8, 2
+    heads, wings, einvs, Es = [], [], [], []
+    for _ in range(nw):
+        A = g.standard_normal((nb + 3, nb + 3)) + 1j * g.standard_normal((nb + 3, nb + 3))
+        E = A @ A.conj().T + (nb + 3) * np.eye(nb + 3)
+        H, X, B = E[:3, :3], E[:3, 3:], E[3:, 3:]
+        heads.append(oracle.pack_upper(H))
+        wings.append(np.conj(X))
+        einvs.append(oracle.pack_upper(np.linalg.inv(B)))
+        Es.append(E)
+    h, w = gamma.gamma_inverse(np.stack(einvs, 1), np.stack(heads, 1), np.stack(wings, 2))
+    for f in range(nw):
+        Ei = np.linalg.inv(Es[f])
+        S = gamma.unpack_head(h[:, f])
+        Si = np.linalg.inv(S)
+        np.testing.assert_allclose(Si, Ei[:3, :3], rtol=1e-11, atol=1e-13)
+        np.testing.assert_allclose(w[:, :, f].T @ Si, Ei[3:, :3], rtol=1e-11, atol=1e-13)
+        np.testing.assert_allclose(Si @ np.conj(w[:, :, f]), Ei[:3, 3:], rtol=1e-11, atol=1e-13)
+
+
+def test_multdiff_spectral_function_is_a_fold_with_scaled_hilbert_weights():
+    """spectral_function with mm = 1, 2 equals the per-transition fold with
+    wghtS(s, :) / frqs(s)**mm (s with frqs(s) = 0 left undivided)."""
+    g = np.random.default_rng(18)
+    n, nfrqs, nw = 80, 12, 3
+    frqs = np.concatenate([[0.0], np.sort(g.uniform(0.05, 3.0, nfrqs - 1))])
+    wt = g.uniform(0, 1, (n, nfrqs))
+    wt[g.uniform(0, 1, wt.shape) < 0.2] = 1e-9
+    wS = g.standard_normal((nfrqs, nw))
+    cp, dp = _rand(g, n, 2)
+    c = cp[:, 0] * np.conj(dp[:, 1])
+    for mm in (0, 1, 2):
+        scale = np.where(frqs > 0, frqs, 1.0) ** mm
+        D = hilbert.fold_weights(wt, wS / scale[:, None], 1e-6)
+        want = c.real @ D + 1j * (c.imag @ D)
+        got = gamma.spectral_function_mm(c, wt, frqs, wS, mm, 1e-6)
+        np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
