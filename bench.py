@@ -237,6 +237,46 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def hilbert_leg(pipe, a, nfrqs=64, reps=3):
+    """Tetrahedron weights on the step's energies (4x4x4 mesh, insulator)
+    over an exponential Hilbert mesh of `nfrqs` points, then their fold into
+    per-transition weights of the Nw frequencies: HIP events on the context
+    stream.  Not part of the timed step (HILBERT NONE there)."""
+    ctx, n, nw = pipe.ctx, pipe.n, a.nw
+    m = round(a.nk ** (1 / 3))
+    frq = 0.05 + 0.01 * (1.1 ** np.arange(nfrqs) - 1)
+    dWt = ctx.alloc(8 * n * nfrqs)
+    dS = ctx.to_dev(np.ascontiguousarray(np.outer(1.0 / (1.0 + frq), 1.0 / (1.0 + pipe.omega)).T))
+    dD = ctx.alloc(8 * n * nw)
+    tw = lambda: ctx.tetra_weights_dev((m, m, m), np.eye(3), pipe.deo, pipe.deu, a.nocc,
+                                       a.nunocc, 0.0, frq, dWt, n)
+    fold = lambda: ctx.weights_hilbert_dev(n, nfrqs, dWt, n, 1e-10, nw, dS, nfrqs, dD, n)
+    tw()
+    fold()
+    e0, e1, e2 = ctx.event(), ctx.event(), ctx.event()
+    e0.record()
+    for _ in range(reps):
+        ctx.tetra_weights_dev((m, m, m), np.eye(3), pipe.deo, pipe.deu, a.nocc, a.nunocc, 0.0,
+                              frq, dWt, n)
+    e1.record()
+    for _ in range(reps):
+        fold()
+    e2.record()
+    ctx.sync()
+    t_ms, f_ms = e1.ms_since(e0) / reps, e2.ms_since(e1) / reps
+    for b in (dWt, dS, dD):
+        b.free()
+    for e in (e0, e1, e2):
+        e.destroy()
+    wbytes = 8 * n * nfrqs
+    return {"kernels": "tetra_weights_kernel, weights_fold_kernel", "nfrqs": nfrqs,
+            "tetra_ms": round(t_ms, 3), "fold_ms": round(f_ms, 3),
+            "tetra_GBps_written": round(wbytes / (t_ms * 1e6), 1),
+            "fold_GBps_read": round(wbytes / (f_ms * 1e6), 1),
+            "note": "each tetra_weights call synchronises on its metal check; the events "
+                    "bracket the calls"}
+
+
 def gamma_leg(pipe, a, reps=5):
     """The q = 0 head and wings (headwing_kernel) on this step's M and
     weights, with a synthetic momentum matrix, and the head/wing update of
@@ -450,6 +490,8 @@ def main():
         "roofline": roof,
     }
     line["gamma_headwings"] = gamma_leg(pipe, a)
+    line["hilbert_weights"] = hilbert_leg(pipe, a)
+    log(f"hilbert {json.dumps(line['hilbert_weights'])}")
     log(f"gamma {json.dumps(line['gamma_headwings'])}")
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(pipe, a)

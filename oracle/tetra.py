@@ -1,0 +1,201 @@
+"""Tetrahedron weights of the Hilbert mesh -- TEST INFRASTRUCTURE, part of
+the oracle; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg may import it.
+
+Restates, in plain numpy/Python:
+- get_tetra (tetrahedron.f:219-271, mode 1): the six tetrahedra of a k-mesh
+  cube around its shortest main diagonal;
+- cuttetraN (tetrahedron.f:347-412): the corner weights of the part of a
+  tetrahedron where a linear function lies below a threshold;
+- tetrahedron5_init (tetrahedron.f:903-1332) for a full cubic k mesh
+  without symmetry reduction (kpt1p the identity, kflag = 1) and bands that
+  the Fermi energy does not cut (insulators: case (c0) of the reference):
+  the weight of each transition t = (k, b1, b2) in the frequency bin
+  [w(i) - dw(i), w(i) + dw(i)] of the Hilbert mesh, divided by 2 dw(i).
+  Degenerate-state averaging (:1233-1273) is not part of it.
+
+k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
+(i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
+(0,1,0), (1,1,0), (0,0,1), (1,0,1), (0,1,1), (1,1,1) offsets, as kcube in
+tetrahedron.f:1014-1021.
+
+What pins each function is in tests/test_oracle_tetra.py: exact
+integration of the barycentric coordinates over the cut tetrahedron
+(independent subdivision quadrature), the volume and sum rules, flat bands
+(all weight in one bin), and the total weight of the Brillouin zone.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ETA = 1e-14  # cuttetraN's tolerance (tetrahedron.f:355)
+
+# the four decompositions of get_tetra, 1-based corner numbers, one row of
+# four corners per tetrahedron (tetrahedron.f:253-268)
+_DECOMP = [
+    [[1, 2, 3, 6], [5, 7, 3, 6], [1, 5, 3, 6], [2, 4, 3, 6], [4, 8, 3, 6], [7, 8, 3, 6]],  # 3-6
+    [[5, 6, 2, 7], [1, 5, 2, 7], [1, 3, 2, 7], [8, 6, 2, 7], [4, 3, 2, 7], [8, 4, 2, 7]],  # 2-7
+    [[2, 6, 1, 8], [2, 4, 1, 8], [3, 4, 1, 8], [3, 7, 1, 8], [5, 7, 1, 8], [5, 6, 1, 8]],  # 1-8
+    [[2, 6, 4, 5], [1, 2, 4, 5], [1, 3, 4, 5], [3, 7, 4, 5], [7, 8, 4, 5], [6, 8, 4, 5]],  # 4-5
+]
+
+
+def tetra_choice(rlat, nkpt3):
+    """Index (0..3) of the decomposition get_tetra(mode 1) picks: the
+    shortest of the diagonals 3-6, 2-7, 1-8, 4-5 of the mesh cell spanned by
+    rlat(:,i)/nkpt3(i) (first on ties, like minloc)."""
+    rl = np.asarray(rlat, float) / np.asarray(nkpt3, float)[None, :]
+    r1, r2, r3 = rl[:, 0], rl[:, 1], rl[:, 2]
+    diag = [np.dot(v, v) for v in (r1 + r3 - r2, r2 + r3 - r1, r1 + r2 + r3, r1 + r2 - r3)]
+    return int(np.argmin(diag))
+
+
+def get_tetra(rlat, nkpt3):
+    """(6, 4) zero-based corner indices of the six tetrahedra."""
+    return np.array(_DECOMP[tetra_choice(rlat, nkpt3)]) - 1
+
+
+def cuttetraN(etetra_in, ef):
+    """(wtetra(4), vtetra(3), ntetra) of cuttetraN (tetrahedron.f:347-412):
+    the corner weights (barycentric coordinates integrated over the part of
+    the tetrahedron where the linear function with corner values etetra_in
+    is below ef, scaled so that the whole tetrahedron gives 1 per corner)."""
+    e_in = np.asarray(etetra_in, float)
+    # pnt: the reference's size ordering (minloc picks the first minimum)
+    pnt = np.full(4, 4)
+    pnt[int(np.argmin(e_in))] = 1
+    m = np.where(pnt > 1, e_in, np.inf)
+    pnt[int(np.argmin(m))] = 2
+    m = np.where(pnt > 2, e_in, np.inf)
+    pnt[int(np.argmin(m))] = 3
+    # pnt(pnt) = [1,2,3,4]: invert the rank map
+    order = np.empty(4, int)
+    order[pnt - 1] = np.arange(4)
+    e = e_in[order]
+    p = order  # p[j] = position (0-based) of the (j+1)-th smallest value
+    w = np.zeros(4)
+    v = np.zeros(3)
+    if ef >= e[3] - ETA:
+        return np.ones(4), np.array([1.0, 0.0, 0.0]), 1
+    if ef >= e[2]:
+        f = (ef - e[:3]) / (e[3] - e[:3])
+        v[0] = f[1]
+        v[1] = f[0] * (1 - f[1]) * (1 - f[2])
+        v[2] = f[2] * (1 - f[1])
+        w[p[0]] = v[0] + v[1] * (2 - f[0]) + v[2]
+        w[p[1]] = v[0] + (v[0] + v[1] + v[2]) * (1 - f[1])
+        w[p[2]] = v[0] + v[1] * (1 - f[2]) + v[2] * (2 - f[2])
+        w[p[3]] = v[0] * f[1] + v[1] * f[0] + (v[1] + v[2]) * (f[1] + f[2])
+        return w, v, 3
+    if ef > e[1]:
+        f = np.empty(4)
+        f[:2] = (ef - e[0]) / (e[2:4] - e[0])
+        f[2:] = (ef - e[1]) / (e[2:4] - e[1])
+        v[0] = f[0] * f[1]
+        v[1] = f[0] * (1 - f[1]) * f[3]
+        v[2] = f[2] * (1 - f[0]) * f[3]
+        w[p[0]] = v[0] * (3 - f[0] - f[1]) + v[1] * (2 - f[0] - f[1]) + v[2] * (1 - f[0])
+        w[p[1]] = v[0] + v[1] * (2 - f[3]) + v[2] * (3 - f[2] - f[3])
+        w[p[2]] = (v[0] + v[1] + v[2]) * f[0] + v[2] * f[2]
+        w[p[3]] = (v[0] + v[1]) * f[1] + (v[1] + v[2]) * f[3]
+        return w, v, 3
+    if ef >= e[0] + ETA:
+        f = (ef - e[0]) / (e[1:4] - e[0])
+        v[0] = f[0] * f[1] * f[2]
+        w[p[0]] = v[0] * (4 - f[0] - f[1] - f[2])
+        w[p[1]] = v[0] * f[0]
+        w[p[2]] = v[0] * f[1]
+        w[p[3]] = v[0] * f[2]
+        return w, v, 1
+    return w, v, 0
+
+
+def hilbert_bins(frq):
+    """(frq1, dw) of tetrahedron5_init (tetrahedron.f:951-975): the upper
+    bin edges frq1(i) = w(i) + dw(i) and the half widths dw(i) of an
+    exponential mesh (a checked constant, dw(i) = a (w(i+1) - w(i)))."""
+    frq = np.asarray(frq, float)
+    nfrq = frq.size
+    if nfrq < 2:
+        raise ValueError("Number of Hilbert frequencies must be larger than one.")
+    if np.any(np.diff(frq) <= 0):
+        raise ValueError("input frequency mesh not properly ordered.")
+    rdum = 0.0
+    for i in range(1, nfrq - 1):
+        a = (frq[i] - frq[i - 1]) / (frq[i + 1] - frq[i - 1])
+        rdum += a
+        if abs(rdum / i - a) > 1e-10:
+            raise ValueError("Hilbert mesh not exponential.")
+    a = rdum / (nfrq - 2) if nfrq > 2 else 0.5
+    frq1 = np.empty(nfrq)
+    frq1[:-1] = frq[:-1] + a * (frq[1:] - frq[:-1])
+    frq1[-1] = frq[-1] + (1 - a) * (frq[-1] - frq[-2])
+    return frq1, frq1 - frq
+
+
+def mesh_index(nkpt3):
+    n1, n2, n3 = nkpt3
+
+    def pk(i1, i2, i3):
+        return ((i1 % n1) * n2 + (i2 % n2)) * n3 + (i3 % n3)
+
+    return pk
+
+
+def tetrahedron5_weights(nkpt3, rlat, e_occ, e_unocc, efermi, frq):
+    """wintgr5 of tetrahedron5_init for the whole packet, as (n, nfrqs) with
+    t = (k*nocc + b1)*nunocc + b2 fastest -- the wtetra(n, nfrqs) of
+    susceptibility.f:936-948.  e_occ (nk, nocc) at k, e_unocc (nk, nunocc)
+    at k + q, indexed by k.  Literal loop order of the reference (cubes k,
+    then n2, n1, frequencies; tetrahedra in order within a cube)."""
+    e_occ = np.asarray(e_occ, float)
+    e_unocc = np.asarray(e_unocc, float)
+    nk, nocc = e_occ.shape
+    nunocc = e_unocc.shape[1]
+    n1m, n2m, n3m = nkpt3
+    assert n1m * n2m * n3m == nk
+    if e_occ.max() > efermi or e_unocc.min() < efermi:
+        raise ValueError("a band is cut by the Fermi energy (metal): not covered")
+    tetra = get_tetra(rlat, nkpt3)
+    frq1, dw = hilbert_bins(frq)
+    nfrq = frq1.size
+    pk = mesh_index(nkpt3)
+    W = np.zeros((nk, nocc, nunocc, nfrq))
+    wadd0 = np.array([np.count_nonzero(tetra == i) for i in range(8)], float)
+    for i1 in range(n1m):
+        for i2 in range(n2m):
+            for i3 in range(n3m):
+                kcube = [pk(i1 + d1, i2 + d2, i3 + d3)
+                         for d3 in (0, 1) for d2 in (0, 1) for d1 in (0, 1)]
+                for b2 in range(nunocc):
+                    ecube2 = e_unocc[kcube, b2]
+                    for b1 in range(nocc):
+                        ecube1 = e_occ[kcube, b1]
+                        dcube = ecube2 - ecube1
+                        dmin, dmax = dcube.min(), dcube.max()
+                        for ifr in range(nfrq):
+                            f = frq1[ifr]
+                            if f < dmin:
+                                continue
+                            if f > dmax:
+                                wadd = wadd0
+                            else:
+                                wadd = np.zeros(8)
+                                for it in range(6):
+                                    dt = dcube[tetra[it]]
+                                    if np.all(dt >= f):
+                                        continue
+                                    if np.all(dt < f):
+                                        wadd[tetra[it]] += 1
+                                    else:
+                                        wadd[tetra[it]] += cuttetraN(dt, f)[0]
+                            for c in range(8):
+                                W[kcube[c], b1, b2, ifr] += wadd[c]
+    rdum = 4.0 * 6 * nk
+    out = np.empty_like(W)
+    for ifr in range(nfrq - 1, 0, -1):
+        out[..., ifr] = (W[..., ifr] - W[..., ifr - 1]) / (2 * (frq1[ifr] - frq[ifr]) * rdum)
+    out[..., 0] = W[..., 0] / (2 * (frq1[0] - frq[0]) * rdum)
+    if frq[0] == 0:
+        out[..., 0] = 0
+    return out.reshape(nk * nocc * nunocc, nfrq)

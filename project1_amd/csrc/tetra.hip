@@ -1,0 +1,269 @@
+// Tetrahedron weights of the Hilbert mesh (the wtetra of the HILBERT pair
+// sum) for a full cubic k mesh and bands that the Fermi energy does not cut.
+//
+// Reference: tetrahedron5_init (tetrahedron.f:903-1332).  Every mesh cube
+// is split into six tetrahedra around its shortest diagonal (get_tetra,
+// :219-271); the band-energy difference d = e_unocc(k+q) - e_occ(k) is
+// interpolated linearly in each tetrahedron, and for every upper bin edge
+// frq1(i) of the Hilbert mesh the cube adds to each of its 8 corner k points
+// the integral of that corner's barycentric coordinate over {d < frq1(i)}
+// (cuttetraN, :347-412; 1 per tetrahedron when the whole tetrahedron lies
+// below).  Differences of consecutive edges over 2 dw(i) * 24 nkpt give the
+// weight of bin i (:1224-1231).  The reference loops over cubes and
+// scatters to corners; here one thread owns one transition (k, b1, b2) and
+// gathers from the 8 cubes that have k as a corner.  Their corners are the
+// 27 mesh points k + {-1,0,1}^3, whose d values the thread loads once; only
+// edges between the smallest and largest of them need the tetrahedron
+// cuts (below: nothing, above: the full 24).  The output is written once,
+// transition-fastest (coalesced): n x nfrqs doubles, the HBM bound.
+#include "common.hpp"
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace {
+
+constexpr int TW_NT = 64;  // one wave: its 27 x 64 LDS values (13.5 KB) leave room for 11 waves per CU
+constexpr double TW_ETA = 1e-14;  // cuttetraN's tolerance
+
+// the four decompositions of get_tetra (tetrahedron.f:253-268), corners
+// 0..7, one row of four corners per tetrahedron; DEC is the index of the
+// shortest diagonal (3-6, 2-7, 1-8, 4-5)
+__device__ constexpr int kDecomp[4][6][4] = {
+    {{0, 1, 2, 5}, {4, 6, 2, 5}, {0, 4, 2, 5}, {1, 3, 2, 5}, {3, 7, 2, 5}, {6, 7, 2, 5}},
+    {{4, 5, 1, 6}, {0, 4, 1, 6}, {0, 2, 1, 6}, {7, 5, 1, 6}, {3, 2, 1, 6}, {7, 3, 1, 6}},
+    {{1, 5, 0, 7}, {1, 3, 0, 7}, {2, 3, 0, 7}, {2, 6, 0, 7}, {4, 6, 0, 7}, {4, 5, 0, 7}},
+    {{1, 5, 3, 4}, {0, 1, 3, 4}, {0, 2, 3, 4}, {2, 6, 3, 4}, {6, 7, 3, 4}, {5, 7, 3, 4}}};
+
+// position of corner p in tetrahedron it of decomposition DEC, or -1
+template <int DEC>
+__device__ constexpr int corner_pos(int it, int p) {
+  return kDecomp[DEC][it][0] == p   ? 0
+         : kDecomp[DEC][it][1] == p ? 1
+         : kDecomp[DEC][it][2] == p ? 2
+         : kDecomp[DEC][it][3] == p ? 3
+                                    : -1;
+}
+
+// Weight of corner position `q` (0..3) of one tetrahedron with corner values
+// e_in and threshold ef: cuttetraN's wtetra(q) (tetrahedron.f:347-412).
+__device__ double cut_corner(const double e_in[4], double ef, int q) {
+  // rank of each corner, ties to the lower index (minloc picks the first)
+  int r[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    int k = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) k += (e_in[b] < e_in[a]) || (e_in[b] == e_in[a] && b < a);
+    r[a] = k;
+  }
+  double e[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if (r[b] == a) e[a] = e_in[b];
+  }
+  const int rq = r[q];  // the sorted position of corner q
+  if (ef >= e[3] - TW_ETA) return 1.0;
+  if (ef >= e[2]) {
+    const double f0 = (ef - e[0]) / (e[3] - e[0]), f1 = (ef - e[1]) / (e[3] - e[1]),
+                 f2 = (ef - e[2]) / (e[3] - e[2]);
+    const double v0 = f1, v1 = f0 * (1 - f1) * (1 - f2), v2 = f2 * (1 - f1);
+    switch (rq) {
+      case 0: return v0 + v1 * (2 - f0) + v2;
+      case 1: return v0 + (v0 + v1 + v2) * (1 - f1);
+      case 2: return v0 + v1 * (1 - f2) + v2 * (2 - f2);
+      default: return v0 * f1 + v1 * f0 + (v1 + v2) * (f1 + f2);
+    }
+  }
+  if (ef > e[1]) {
+    const double f0 = (ef - e[0]) / (e[2] - e[0]), f1 = (ef - e[0]) / (e[3] - e[0]),
+                 f2 = (ef - e[1]) / (e[2] - e[1]), f3 = (ef - e[1]) / (e[3] - e[1]);
+    const double v0 = f0 * f1, v1 = f0 * (1 - f1) * f3, v2 = f2 * (1 - f0) * f3;
+    switch (rq) {
+      case 0: return v0 * (3 - f0 - f1) + v1 * (2 - f0 - f1) + v2 * (1 - f0);
+      case 1: return v0 + v1 * (2 - f3) + v2 * (3 - f2 - f3);
+      case 2: return (v0 + v1 + v2) * f0 + v2 * f2;
+      default: return (v0 + v1) * f1 + (v1 + v2) * f3;
+    }
+  }
+  if (ef >= e[0] + TW_ETA) {
+    const double f0 = (ef - e[0]) / (e[1] - e[0]), f1 = (ef - e[0]) / (e[2] - e[0]),
+                 f2 = (ef - e[0]) / (e[3] - e[0]);
+    const double v0 = f0 * f1 * f2;
+    switch (rq) {
+      case 0: return v0 * (4 - f0 - f1 - f2);
+      case 1: return v0 * f0;
+      case 2: return v0 * f1;
+      default: return v0 * f2;
+    }
+  }
+  return 0.0;
+}
+
+// The 27 d values of a thread live in LDS (d[a][thread]), so the loops over
+// cubes and tetrahedra index them at run time without scratch memory.
+template <int DEC>
+__global__ __launch_bounds__(TW_NT) void tetra_weights_kernel(
+    int n1, int n2, int n3, int nocc, int nunocc, const double *__restrict__ eo,
+    const double *__restrict__ eu, double efermi, int nfrq, const double *__restrict__ frq1,
+    const double *__restrict__ den, int zero_first, double *__restrict__ W, long ldw,
+    int *__restrict__ flag) {
+  __shared__ double ds[27][TW_NT];
+  const long nk = (long)n1 * n2 * n3;
+  const long n = nk * nocc * nunocc;
+  const int tid = threadIdx.x;
+  const long t = (long)blockIdx.x * TW_NT + tid;
+  if (t >= n) return;  // no barrier below: each thread reads only its own column of ds
+  const int b2 = (int)(t % nunocc);
+  const long r = t / nunocc;
+  const int b1 = (int)(r % nocc);
+  const int k = (int)(r / nocc);
+  const int i3 = k % n3, i2 = (k / n3) % n2, i1 = k / (n2 * n3);
+  if (eo[(long)k * nocc + b1] > efermi || eu[(long)k * nunocc + b2] < efermi) atomicOr(flag, 1);
+  // d at the 27 points k + (a1,a2,a3), a in {-1,0,1}: index (a1+1) + 3(a2+1) + 9(a3+1)
+  double dlo = 1e300, dhi = -1e300;
+  for (int a = 0; a < 27; a++) {
+    const int j1 = (i1 + a % 3 - 1 + n1) % n1, j2 = (i2 + (a / 3) % 3 - 1 + n2) % n2,
+              j3 = (i3 + a / 9 - 1 + n3) % n3;
+    const long kk = ((long)j1 * n2 + j2) * n3 + j3;
+    const double v = eu[kk * nunocc + b2] - eo[kk * nocc + b1];
+    ds[a][tid] = v;
+    dlo = v < dlo ? v : dlo;
+    dhi = v > dhi ? v : dhi;
+  }
+  double prev = 0.0;
+  for (int i = 0; i < nfrq; i++) {
+    const double f = frq1[i];
+    double w = 0.0;
+    if (f > dhi) {
+      w = 24.0;  // every tetrahedron of the 8 cubes below f: 24 corner shares
+    } else if (f >= dlo) {
+      // cube with origin k - (s1,s2,s3); this k is its corner p = s
+      for (int s = 0; s < 8; s++) {
+        const int s1 = s & 1, s2 = (s >> 1) & 1, s3 = s >> 2;
+        double cmin = 1e300, cmax = -1e300;
+        for (int c = 0; c < 8; c++) {
+          const double v = ds[((c & 1) - s1 + 1) + 3 * (((c >> 1) & 1) - s2 + 1) +
+                              9 * ((c >> 2) - s3 + 1)][tid];
+          cmin = v < cmin ? v : cmin;
+          cmax = v > cmax ? v : cmax;
+        }
+        if (f < cmin) continue;  // (a) the cube adds nothing
+        for (int it = 0; it < 6; it++) {
+          const int q = corner_pos<DEC>(it, s);
+          if (q < 0) continue;
+          if (f > cmax) {  // (b) the whole cube: wadd0
+            w += 1.0;
+            continue;
+          }
+          double et[4];
+          bool all_ge = true, all_lt = true;
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            const int c = kDecomp[DEC][it][j];
+            et[j] = ds[((c & 1) - s1 + 1) + 3 * (((c >> 1) & 1) - s2 + 1) +
+                       9 * ((c >> 2) - s3 + 1)][tid];
+            all_ge = all_ge && et[j] >= f;
+            all_lt = all_lt && et[j] < f;
+          }
+          if (all_ge) continue;
+          w += all_lt ? 1.0 : cut_corner(et, f, q);
+        }
+      }
+    }
+    const double out = (i == 0) ? (zero_first ? 0.0 : w / den[0]) : (w - prev) / den[i];
+    W[t + ldw * i] = out;
+    prev = w;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
+                              int nunocc, const void *e_occ, const void *e_unocc, double efermi,
+                              int nfrqs, const double *frqs, void *wtetra, long ldwt) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nkpt3 && rlat && frqs, "tetra_weights: null argument");
+  SPX_CHECK(ctx, nkpt3[0] > 0 && nkpt3[1] > 0 && nkpt3[2] > 0 && nocc > 0 && nunocc > 0,
+            "tetra_weights: bad sizes");
+  const long nk = (long)nkpt3[0] * nkpt3[1] * nkpt3[2];
+  const long n = nk * nocc * nunocc;
+  SPX_CHECK(ctx, ldwt >= n, "tetra_weights: ldwt < n");
+  // the Hilbert mesh: ascending, exponential, at least two points
+  // (tetrahedron.f:951-975); upper bin edges frq1 and the denominators
+  // 2 (frq1 - frq) * 24 nkpt of the bin differences (:1224-1231)
+  SPX_CHECK(ctx, nfrqs >= 2, "tetra_weights: Number of Hilbert frequencies must be larger than one");
+  for (int i = 1; i < nfrqs; i++)
+    SPX_CHECK(ctx, frqs[i] > frqs[i - 1], "tetra_weights: input frequency mesh not properly ordered");
+  double rdum = 0.0, a = 0.5;
+  for (int i = 1; i < nfrqs - 1; i++) {
+    const double ai = (frqs[i] - frqs[i - 1]) / (frqs[i + 1] - frqs[i - 1]);
+    rdum += ai;
+    SPX_CHECK(ctx, std::fabs(rdum / i - ai) <= 1e-10, "tetra_weights: Hilbert mesh not exponential");
+  }
+  if (nfrqs > 2) a = rdum / (nfrqs - 2);
+  std::vector<double> host(2 * (size_t)nfrqs);
+  double *f1 = host.data(), *den = f1 + nfrqs;
+  for (int i = 0; i < nfrqs - 1; i++) f1[i] = frqs[i] + a * (frqs[i + 1] - frqs[i]);
+  f1[nfrqs - 1] = frqs[nfrqs - 1] + (1 - a) * (frqs[nfrqs - 1] - frqs[nfrqs - 2]);
+  const double norm = 4.0 * 6 * (double)nk;
+  for (int i = 0; i < nfrqs; i++) den[i] = (2 * (f1[i] - frqs[i])) * norm;
+  // get_tetra (mode 1): the decomposition around the shortest diagonal of
+  // the mesh cell spanned by rlat(:,i)/nkpt3(i), first on ties
+  double rl[3][3];
+  for (int j = 0; j < 3; j++)
+    for (int x = 0; x < 3; x++) rl[j][x] = rlat[x + 3 * j] / nkpt3[j];
+  double diag[4];
+  for (int dsel = 0; dsel < 4; dsel++) {
+    static const int sg[4][3] = {{1, -1, 1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1}};
+    double s = 0.0;
+    for (int x = 0; x < 3; x++) {
+      const double v = sg[dsel][0] * rl[0][x] + sg[dsel][1] * rl[1][x] + sg[dsel][2] * rl[2][x];
+      s += v * v;
+    }
+    diag[dsel] = s;
+  }
+  int idmin = 0;
+  for (int dsel = 1; dsel < 4; dsel++)
+    if (diag[dsel] < diag[idmin]) idmin = dsel;
+  SPX_CHECK(ctx, spx_owns(ctx, e_occ, 8 * (size_t)nk * nocc) &&
+                     spx_owns(ctx, e_unocc, 8 * (size_t)nk * nunocc) &&
+                     spx_owns(ctx, wtetra, 8 * ((size_t)ldwt * (nfrqs - 1) + n)),
+            "tetra_weights: arrays are not context allocations of the stated shapes");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *ws;
+  int rc = spx_scratch(ctx, 16 * (size_t)nfrqs + 256, &ws);
+  if (rc) return rc;
+  double *dfrq = (double *)ws;
+  int *flag = (int *)(dfrq + 2 * (size_t)nfrqs);
+  SPX_HIP(ctx, hipMemcpyAsync(dfrq, host.data(), 16 * (size_t)nfrqs, hipMemcpyHostToDevice,
+                              ctx->stream));
+  rc = spx_fill_zero(ctx, ctx->stream, flag, sizeof(int));
+  if (rc) return rc;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)((n + TW_NT - 1) / TW_NT)), dim3(TW_NT), 0,
+                       ctx->stream, nkpt3[0], nkpt3[1], nkpt3[2], nocc, nunocc,
+                       (const double *)e_occ, (const double *)e_unocc, efermi, nfrqs, dfrq,
+                       dfrq + nfrqs, frqs[0] == 0.0 ? 1 : 0, (double *)wtetra, ldwt, flag);
+  };
+  switch (idmin) {
+    case 0: go(tetra_weights_kernel<0>); break;
+    case 1: go(tetra_weights_kernel<1>); break;
+    case 2: go(tetra_weights_kernel<2>); break;
+    default: go(tetra_weights_kernel<3>); break;
+  }
+  SPX_LAUNCHED(ctx);
+  int h = 0;
+  SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h)
+    return spx_seterr(ctx, SPEXHIP_E_INVALID,
+                      "tetra_weights: a band is cut by the Fermi energy (metals are not covered)");
+  return SPEXHIP_OK;
+}
+
+}  // extern "C"

@@ -316,6 +316,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
         "spexhip_susceptibility_hilbert": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long,
                                                C.c_double, I, I, _vp, _vp, I, _vp, _vp]),
+        "spexhip_tetra_weights_dev": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp, _vp,
+                                          _c_long]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
@@ -547,6 +549,18 @@ class Context:
         """D(t,w) = sum over wtetra(t,s) > wghtthr of wtetra(t,s) wghtS(s,w)."""
         self._check(lib().spexhip_weights_hilbert_dev(
             self.h, n, nfrqs, wtetra.ptr, ldwt, wghtthr, nfrq, wghtS.ptr, ldws, out.ptr, ldd))
+
+    def tetra_weights_dev(self, nkpt3, rlat: np.ndarray, e_occ: DevBuf, e_unocc: DevBuf,
+                          nocc: int, nunocc: int, efermi: float, frqs: np.ndarray, out: DevBuf,
+                          ldwt: int) -> None:
+        """wtetra (n x nfrqs) of the tetrahedron method on a full cubic mesh
+        (insulators); rlat columns are the reciprocal lattice vectors."""
+        k3 = np.ascontiguousarray(nkpt3, np.int32)
+        rl = np.asfortranarray(rlat, np.float64)
+        fr = np.ascontiguousarray(frqs, np.float64)
+        self._check(lib().spexhip_tetra_weights_dev(
+            self.h, _ptr(k3), rl.ctypes.data, nocc, nunocc, e_occ.ptr, e_unocc.ptr, efermi,
+            fr.size, _ptr(fr), out.ptr, ldwt))
 
     # -- chi0 -------------------------------------------------------------
     def chi0_cplx_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int,

@@ -1,0 +1,166 @@
+"""Pinning of the tetrahedron-weight oracle (oracle/tetra.py): cuttetraN
+against exact integration over the clipped tetrahedron (a convex polytope
+triangulated by scipy), the volume rule, the decomposition choice, and for
+the whole mesh: flat bands (all weight in one bin), the Brillouin-zone sum
+rule, and an independent assembly from the clipped-polytope integrals."""
+import math
+
+import numpy as np
+import pytest
+from scipy.spatial import ConvexHull, Delaunay
+
+from oracle import tetra
+
+# unit tetrahedron: corner j at these Cartesian points; barycentric
+# coordinates of a point x are lambda = B^-1 (x - v0) completed to sum 1
+V = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+
+
+def _bary(x):
+    lam = x - V[0]  # for this tetrahedron lambda_1..3 are the coordinates
+    return np.column_stack([1 - lam.sum(1), lam])
+
+
+def clipped_corner_weights(d, f):
+    """Exact integral of each barycentric coordinate over {x in T: d(x) < f}
+    for the linear d with corner values d, scaled so that the whole
+    tetrahedron gives 1 per corner (4/vol * integral)."""
+    d = np.asarray(d, float)
+    pts = [V[j] for j in range(4) if d[j] < f]
+    for a in range(4):
+        for b in range(a + 1, 4):
+            if (d[a] < f) != (d[b] < f):
+                s = (f - d[a]) / (d[b] - d[a])
+                pts.append(V[a] + s * (V[b] - V[a]))
+    pts = np.array(pts)
+    if len(pts) < 4:
+        return np.zeros(4)
+    try:
+        if ConvexHull(pts).volume < 1e-15:
+            return np.zeros(4)
+    except Exception:
+        return np.zeros(4)
+    tri = Delaunay(pts)
+    vol_t = 1.0 / 6.0
+    w = np.zeros(4)
+    for s in tri.simplices:
+        p = pts[s]
+        vol = abs(np.linalg.det(p[1:] - p[0])) / 6.0
+        w += vol * _bary(p).mean(0)
+    return 4.0 * w / vol_t
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_cuttetraN_is_the_exact_clipped_integral(seed):
+    g = np.random.default_rng(seed)
+    d = g.uniform(-1, 1, 4)
+    ds = np.sort(d)
+    # thresholds in each regime: below, (d1,d2), (d2,d3), (d3,d4), above
+    fs = [ds[0] - 0.1, *(0.5 * (ds[:-1] + ds[1:])), 0.3 * ds[1] + 0.7 * ds[2], ds[3] + 0.1]
+    for f in fs:
+        w, v, nt = tetra.cuttetraN(d, f)
+        want = clipped_corner_weights(d, f)
+        np.testing.assert_allclose(w, want, rtol=1e-9, atol=1e-12)
+        # volume rule: the pieces' volumes add up to the mean corner weight
+        np.testing.assert_allclose(v[:nt].sum() if nt else 0.0, w.sum() / 4, rtol=1e-12, atol=1e-15)
+
+
+def test_cuttetraN_order_of_equal_values_and_limits():
+    w, _, nt = tetra.cuttetraN([0.5, 0.5, 0.5, 0.5], 0.5 + 1e-15)
+    assert nt == 1 and (w == 1).all()  # within eta of the top: whole tetrahedron
+    w, _, nt = tetra.cuttetraN([0.2, 0.4, 0.6, 0.8], 0.2)
+    assert nt == 0 and (w == 0).all()  # at the bottom: nothing (strict, eta)
+
+
+def test_tetra_choice_shortest_diagonal():
+    cubic = np.eye(3)
+    assert tetra.tetra_choice(cubic, (4, 4, 4)) == 0  # all equal: the first (3-6)
+    # fcc reciprocal lattice (bcc vectors): a1 + a2 + a3 is the short diagonal 1-8? check by lengths
+    rl = np.array([[-1.0, 1.0, 1.0], [1.0, -1.0, 1.0], [1.0, 1.0, -1.0]]).T
+    r1, r2, r3 = rl.T
+    lens = [np.dot(v, v) for v in (r1 + r3 - r2, r2 + r3 - r1, r1 + r2 + r3, r1 + r2 - r3)]
+    assert tetra.tetra_choice(rl, (2, 2, 2)) == int(np.argmin(lens))
+    t = tetra.get_tetra(rl, (2, 2, 2))
+    assert t.shape == (6, 4) and set(np.unique(t)) == set(range(8))
+    # every tetrahedron holds both ends of the chosen diagonal
+    ends = [(2, 5), (1, 6), (0, 7), (3, 4)][tetra.tetra_choice(rl, (2, 2, 2))]
+    assert all(ends[0] in row and ends[1] in row for row in t)
+
+
+def _mesh_case(g, nkpt3, nocc, nunocc):
+    nk = int(np.prod(nkpt3))
+    eo = -g.uniform(0.2, 1.0, (nk, nocc))
+    eu = g.uniform(0.1, 1.2, (nk, nunocc))
+    return eo, eu
+
+
+def test_flat_bands_put_all_weight_in_one_bin():
+    nkpt3 = (3, 2, 2)
+    nk = 12
+    frq = 0.05 * (1.3 ** np.arange(12) - 1)  # exponential mesh starting at 0
+    frq1, dw = tetra.hilbert_bins(frq)
+    d0 = 0.5 * (frq1[5] + frq1[6])  # inside bin 6
+    eo = np.full((nk, 1), -d0 / 2)
+    eu = np.full((nk, 2), d0 / 2)
+    w = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    want = np.zeros(len(frq))
+    want[6] = 1.0 / (nk * 2 * dw[6])
+    for t in range(w.shape[0]):
+        np.testing.assert_allclose(w[t], want, rtol=1e-13, atol=1e-15)
+
+
+def test_brillouin_zone_sum_rule():
+    g = np.random.default_rng(3)
+    nkpt3 = (2, 3, 2)
+    eo, eu = _mesh_case(g, nkpt3, 2, 3)
+    frq = 0.02 * (1.25 ** np.arange(24) - 1) + 0.01
+    frq1, dw = tetra.hilbert_bins(frq)
+    assert frq1[-1] > (eu.max() - eo.min())
+    w = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    nk, nocc, nunocc = eo.shape[0], 2, 3
+    tot = (w * (2 * dw)[None, :]).reshape(nk, nocc, nunocc, -1).sum(axis=(0, 3))
+    np.testing.assert_allclose(tot, 1.0, rtol=1e-12)
+
+
+def test_mesh_weights_from_clipped_integrals():
+    """Independent assembly: per cube and tetrahedron the exact clipped
+    integrals at every bin edge, scattered to the corner k points, then the
+    bin differences -- against the oracle's cuttetraN path."""
+    g = np.random.default_rng(5)
+    nkpt3 = (2, 2, 2)
+    rl = np.array([[1.0, 0.1, 0.0], [0.0, 1.2, 0.0], [0.2, 0.0, 0.9]]).T
+    eo, eu = _mesh_case(g, nkpt3, 1, 2)
+    frq = 0.1 * (1.4 ** np.arange(10) - 1) + 0.2
+    frq1, dw = tetra.hilbert_bins(frq)
+    got = tetra.tetrahedron5_weights(nkpt3, rl, eo, eu, 0.0, frq)
+    tt = tetra.get_tetra(rl, nkpt3)
+    pk = tetra.mesh_index(nkpt3)
+    nk = 8
+    cum = np.zeros((nk, 1, 2, len(frq)))
+    for i1 in range(2):
+        for i2 in range(2):
+            for i3 in range(2):
+                kc = [pk(i1 + a, i2 + b, i3 + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                for b2 in range(2):
+                    dc = eu[kc, b2] - eo[kc, 0]
+                    for j, f in enumerate(frq1):
+                        for row in tt:
+                            w = clipped_corner_weights(dc[row], f)
+                            for c, wc in zip(row, w):
+                                cum[kc[c], 0, b2, j] += wc
+    want = np.empty_like(cum)
+    want[..., 1:] = np.diff(cum, axis=-1) / (2 * dw[1:] * 24 * nk)
+    want[..., 0] = cum[..., 0] / (2 * dw[0] * 24 * nk)
+    np.testing.assert_allclose(got, want.reshape(got.shape), rtol=1e-8, atol=1e-10)
+
+
+def test_metals_and_bad_meshes_are_rejected():
+    nkpt3 = (2, 2, 2)
+    eo = np.full((8, 1), -0.1)
+    eu = np.full((8, 1), 0.2)
+    eu[3, 0] = -0.05  # an unoccupied band below the Fermi energy at one k
+    frq = 0.1 * (1.3 ** np.arange(6) - 1)
+    with pytest.raises(ValueError):
+        tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    with pytest.raises(ValueError):
+        tetra.hilbert_bins(np.array([0.0, 0.1, 0.15, 0.4]))  # not exponential
