@@ -133,8 +133,9 @@ int spexhip_weights_hilbert_dev(spexhip_ctx *ctx, long n, int nfrqs, const void 
                                 int ldws, void *D_dev, long ldd);
 
 /* Replaces tetrahedron5_init (tetrahedron.f:903-1332) for a full cubic k
- * mesh without symmetry reduction (kflag 1) and bands that the Fermi energy
- * does not cut (insulators): wtetra_dev (n x nfrqs, ld ldwt, t fastest,
+ * mesh without symmetry reduction (kflag 1), insulators and metals (bands
+ * cut by efermi are restricted to their occupied / unoccupied parts):
+ * wtetra_dev (n x nfrqs, ld ldwt, t fastest,
  * t = (k*nocc + b1)*nunocc + b2) receives the tetrahedron weight of each
  * transition in the bin [w(i) - dw(i), w(i) + dw(i)] of the Hilbert mesh
  * frqs (host array, ascending, exponential), divided by 2 dw(i) -- the
@@ -142,9 +143,9 @@ int spexhip_weights_hilbert_dev(spexhip_ctx *ctx, long n, int nfrqs, const void 
  * for the mesh point (i1,i2,i3)/nkpt3; rlat (host, 3 x 3 column-major, the
  * reciprocal lattice vectors) selects the decomposition (get_tetra,
  * :219-271).  e_occ_dev [nk][nocc] at k, e_unocc_dev [nk][nunocc] at k+q.
- * SPEXHIP_E_INVALID if an occupied energy lies above efermi or an
- * unoccupied one below it (metals), or if the mesh is not exponential.
- * Degenerate-state averaging (:1233-1273) is not applied.                  */
+ * SPEXHIP_E_INVALID if the mesh is not ascending and exponential.
+ * Degenerate-state averaging (:1233-1273) is not applied.  Returns after
+ * the weights are written (it reads the host frequency table).            */
 int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
                               int nunocc, const void *e_occ_dev, const void *e_unocc_dev,
                               double efermi, int nfrqs, const double *frqs, void *wtetra_dev,

@@ -12,7 +12,11 @@ Restates, in plain numpy/Python:
   the Fermi energy does not cut (insulators: case (c0) of the reference):
   the weight of each transition t = (k, b1, b2) in the frequency bin
   [w(i) - dw(i), w(i) + dw(i)] of the Hilbert mesh, divided by 2 dw(i).
-  Degenerate-state averaging (:1233-1273) is not part of it.
+  Degenerate-state averaging (:1233-1273) is not part of it;
+- the same with bands cut by the Fermi energy (metals, case (c1)): each
+  tetrahedron restricted to its occupied (e_occ < efermi) and unoccupied
+  (e_unocc > efermi) part through cuttetraN's sub-tetrahedra (:1045-1129),
+  then cut by d < frq1(i) (:1176-1202).
 
 k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
 (i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
@@ -108,6 +112,162 @@ def cuttetraN(etetra_in, ef):
         w[p[3]] = v[0] * f[2]
         return w, v, 1
     return w, v, 0
+
+
+def cuttetraN_sub(etetra_in, ef):
+    """cuttetraN with its optional tetra output (tetrahedron.f:347-412):
+    (wtetra(4), vtetra(3), ntetra, T(ntetra, 4, 4)) where T[m][i, j] is the
+    barycentric coordinate, on input corner i, of corner j of sub-tetrahedron
+    m of the part below ef, and vtetra(m) its volume fraction."""
+    e_in = np.asarray(etetra_in, float)
+    w, v, nt = cuttetraN(e_in, ef)
+    pnt = np.full(4, 4)
+    pnt[int(np.argmin(e_in))] = 1
+    m = np.where(pnt > 1, e_in, np.inf)
+    pnt[int(np.argmin(m))] = 2
+    m = np.where(pnt > 2, e_in, np.inf)
+    pnt[int(np.argmin(m))] = 3
+    order = np.empty(4, int)
+    order[pnt - 1] = np.arange(4)
+    e = e_in[order]
+    if nt == 0:
+        return w, v, 0, np.zeros((0, 4, 4))
+    if ef >= e[3] - ETA:
+        cols = [[[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]]]
+    elif ef >= e[2]:
+        f1, f2, f3 = (ef - e[:3]) / (e[3] - e[:3])
+        cols = [[[1, 0, 0, 0], [0, 1, 0, 0], [0, 0, 1, 0], [0, 1 - f2, 0, f2]],
+                [[1, 0, 0, 0], [1 - f1, 0, 0, f1], [0, 1 - f2, 0, f2], [0, 0, 1 - f3, f3]],
+                [[1, 0, 0, 0], [0, 0, 1, 0], [0, 1 - f2, 0, f2], [0, 0, 1 - f3, f3]]]
+    elif ef > e[1]:
+        f1, f2 = (ef - e[0]) / (e[2:4] - e[0])
+        f3, f4 = (ef - e[1]) / (e[2:4] - e[1])
+        cols = [[[1, 0, 0, 0], [0, 1, 0, 0], [1 - f1, 0, f1, 0], [1 - f2, 0, 0, f2]],
+                [[0, 1, 0, 0], [1 - f1, 0, f1, 0], [1 - f2, 0, 0, f2], [0, 1 - f4, 0, f4]],
+                [[0, 1, 0, 0], [1 - f1, 0, f1, 0], [0, 1 - f4, 0, f4], [0, 1 - f3, f3, 0]]]
+    else:
+        f1, f2, f3 = (ef - e[0]) / (e[1:4] - e[0])
+        cols = [[[1, 0, 0, 0], [1 - f1, f1, 0, 0], [1 - f2, 0, f2, 0], [1 - f3, 0, 0, f3]]]
+    T = np.zeros((nt, 4, 4))
+    for mm in range(nt):
+        X = np.array(cols[mm], float).T  # X[i_sorted, j]: column j lists sorted-corner coords
+        T[mm][order, :] = X  # tetra(pnt, :, m) = X
+    return w, v, nt, T
+
+
+def _restricted(e1t, e2t, ef, cut1, cut2):
+    """The part of one tetrahedron with occupied e1 < ef and unoccupied
+    e2 > ef as (wtetra_(4), [(v, T)]) -- the tetra_/vtetra_/wtetra_ of
+    tetrahedron5_init (:1100-1129).  cut1/cut2: the cube's bands are cut."""
+    if cut1 and not cut2:
+        w, v, nt, T = cuttetraN_sub(e1t, ef)
+        return w, [(v[m], T[m]) for m in range(nt)]
+    if cut2 and not cut1:
+        w, v, nt, T = cuttetraN_sub(-np.asarray(e2t), -ef)
+        return w, [(v[m], T[m]) for m in range(nt)]
+    _, v2, nt2, T2 = cuttetraN_sub(-np.asarray(e2t), -ef)
+    w_ = np.zeros(4)
+    subs = []
+    for m2 in range(nt2):
+        e1s = np.asarray(e1t) @ T2[m2]
+        w, v, nt, T1 = cuttetraN_sub(e1s, ef)
+        w_ += v2[m2] * (T2[m2] @ w)
+        for m1 in range(nt):
+            subs.append((v2[m2] * v[m1], T2[m2] @ T1[m1]))
+    return w_, subs
+
+
+def corner_weights_at(dt, part, f):
+    """One tetrahedron's contribution to its 4 corners at the bin edge f
+    (tetrahedron.f:1160-1202): dt the band-energy differences at its
+    corners, part the (wtetra_, sub-tetrahedra) of its occupied-unoccupied
+    part when the cube's bands are cut (None: not cut, the whole
+    tetrahedron, case (c0))."""
+    dt = np.asarray(dt, float)
+    if part is None:
+        if np.all(dt >= f):
+            return np.zeros(4)
+        if np.all(dt < f):
+            return np.ones(4)
+        return cuttetraN(dt, f)[0]
+    w_, subs = part
+    if not subs or np.all(dt >= f):
+        return np.zeros(4)
+    if np.all(dt < f):
+        return w_
+    out = np.zeros(4)
+    for v, T in subs:
+        ds_ = dt @ T
+        if np.all(ds_ >= f):
+            continue
+        if np.all(ds_ < f):
+            out += v * T.sum(1)
+        else:
+            out += v * (T @ cuttetraN(ds_, f)[0])
+    return out
+
+
+def tetrahedron5_weights_metal(nkpt3, rlat, e_occ, e_unocc, efermi, frq):
+    """tetrahedron5_init with bands cut by the Fermi energy (cases (c0) and
+    (c1), tetrahedron.f:1045-1213), full cubic mesh, kflag = 1, no symmetry;
+    same output layout as tetrahedron5_weights.  Literal loop order."""
+    e_occ = np.asarray(e_occ, float)
+    e_unocc = np.asarray(e_unocc, float)
+    nk, nocc = e_occ.shape
+    nunocc = e_unocc.shape[1]
+    n1m, n2m, n3m = nkpt3
+    tetra = get_tetra(rlat, nkpt3)
+    frq1, _ = hilbert_bins(frq)
+    nfrq = frq1.size
+    pk = mesh_index(nkpt3)
+    W = np.zeros((nk, nocc, nunocc, nfrq))
+    cnt = np.array([np.count_nonzero(tetra == i) for i in range(8)], float)
+    for i1 in range(n1m):
+        for i2 in range(n2m):
+            for i3 in range(n3m):
+                kcube = [pk(i1 + d1, i2 + d2, i3 + d3)
+                         for d3 in (0, 1) for d2 in (0, 1) for d1 in (0, 1)]
+                for b2 in range(nunocc):
+                    ecube2 = e_unocc[kcube, b2]
+                    if np.all(ecube2 < efermi):
+                        continue
+                    cut2 = ecube2.min() < efermi
+                    for b1 in range(nocc):
+                        ecube1 = e_occ[kcube, b1]
+                        if np.all(ecube1 > efermi):
+                            continue
+                        cut1 = ecube1.max() > efermi
+                        dcube = ecube2 - ecube1
+                        dmin, dmax = dcube.min(), dcube.max()
+                        if cut1 or cut2:
+                            parts = [_restricted(ecube1[tetra[it]], ecube2[tetra[it]], efermi,
+                                                 cut1, cut2) for it in range(6)]
+                            wadd0 = np.zeros(8)
+                            for it in range(6):
+                                wadd0[tetra[it]] += parts[it][0]
+                        else:
+                            wadd0 = cnt
+                        for ifr in range(nfrq):
+                            f = frq1[ifr]
+                            if f < dmin:
+                                continue
+                            if f > dmax:
+                                wadd = wadd0
+                            else:
+                                wadd = np.zeros(8)
+                                for it in range(6):
+                                    wadd[tetra[it]] += corner_weights_at(
+                                        dcube[tetra[it]], parts[it] if (cut1 or cut2) else None, f)
+                            for c in range(8):
+                                W[kcube[c], b1, b2, ifr] += wadd[c]
+    rdum = 4.0 * 6 * nk
+    out = np.empty_like(W)
+    for ifr in range(nfrq - 1, 0, -1):
+        out[..., ifr] = (W[..., ifr] - W[..., ifr - 1]) / (2 * (frq1[ifr] - frq[ifr]) * rdum)
+    out[..., 0] = W[..., 0] / (2 * (frq1[0] - frq[0]) * rdum)
+    if frq[0] == 0:
+        out[..., 0] = 0
+    return out.reshape(nk * nocc * nunocc, nfrq)
 
 
 def hilbert_bins(frq):

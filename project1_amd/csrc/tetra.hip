@@ -1,5 +1,5 @@
 // Tetrahedron weights of the Hilbert mesh (the wtetra of the HILBERT pair
-// sum) for a full cubic k mesh and bands that the Fermi energy does not cut.
+// sum) for a full cubic k mesh, insulators and metals.
 //
 // Reference: tetrahedron5_init (tetrahedron.f:903-1332).  Every mesh cube
 // is split into six tetrahedra around its shortest diagonal (get_tetra,
@@ -14,8 +14,11 @@
 // gathers from the 8 cubes that have k as a corner.  Their corners are the
 // 27 mesh points k + {-1,0,1}^3, whose d values the thread loads once; only
 // edges between the smallest and largest of them need the tetrahedron
-// cuts (below: nothing, above: the full 24).  The output is written once,
-// transition-fastest (coalesced): n x nfrqs doubles, the HBM bound.
+// cuts (below: nothing, above: the full 24).  In cubes whose bands the Fermi
+// energy cuts (metals, case (c1)) each tetrahedron is first restricted to
+// its occupied-unoccupied part, as cuttetraN sub-tetrahedra (nested when
+// both bands are cut), and each of those is cut by d < frq1(i).  The output
+// is written once, transition-fastest (coalesced): n x nfrqs doubles.
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -102,14 +105,188 @@ __device__ double cut_corner(const double e_in[4], double ef, int q) {
   return 0.0;
 }
 
+// Linear index of the mesh point k + (a%3-1, (a/3)%3-1, a/9-1), periodic.
+__device__ inline long nbr(int i1, int i2, int i3, int a, int n1, int n2, int n3) {
+  const int j1 = (i1 + a % 3 - 1 + n1) % n1, j2 = (i2 + (a / 3) % 3 - 1 + n2) % n2,
+            j3 = (i3 + a / 9 - 1 + n3) % n3;
+  return ((long)j1 * n2 + j2) * n3 + j3;
+}
+
 // The 27 d values of a thread live in LDS (d[a][thread]), so the loops over
 // cubes and tetrahedra index them at run time without scratch memory.
+// Sorted ranks of four values (ties to the lower index, as the reference's
+// minloc ordering) and the sorted values.
+__device__ inline void rank4(const double e_in[4], int r[4], double e[4]) {
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    int k = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) k += (e_in[b] < e_in[a]) || (e_in[b] == e_in[a] && b < a);
+    r[a] = k;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+      if (r[b] == a) e[a] = e_in[b];
+  }
+}
+
+// Number of sub-tetrahedra of the part of a tetrahedron below ef
+// (cuttetraN's ntetra).
+__device__ inline int sub_count(const double e_in[4], double ef) {
+  int r[4];
+  double e[4];
+  rank4(e_in, r, e);
+  if (ef >= e[3] - TW_ETA) return 1;
+  if (ef >= e[2] || ef > e[1]) return 3;
+  if (ef >= e[0] + TW_ETA) return 1;
+  return 0;
+}
+
+// Sub-tetrahedron m of the part below ef (cuttetraN's tetra(:,:,m) and
+// vtetra(m), tetrahedron.f:366-407): T[i][j] = barycentric coordinate on
+// input corner i of sub-corner j.  m < sub_count(e_in, ef).
+__device__ inline double sub_tetra(const double e_in[4], double ef, int m, double T[4][4]) {
+  int r[4];
+  double e[4];
+  rank4(e_in, r, e);
+  double X[4][4] = {};  // X[s][j]: coordinate on the s-th smallest corner of sub-corner j
+  double v;
+  if (ef >= e[3] - TW_ETA) {
+#pragma unroll
+    for (int j = 0; j < 4; j++) X[j][j] = 1.0;
+    v = 1.0;
+  } else if (ef >= e[2]) {
+    const double f0 = (ef - e[0]) / (e[3] - e[0]), f1 = (ef - e[1]) / (e[3] - e[1]),
+                 f2 = (ef - e[2]) / (e[3] - e[2]);
+    X[0][0] = 1.0;
+    if (m == 0) {
+      X[1][1] = 1.0; X[2][2] = 1.0; X[1][3] = 1 - f1; X[3][3] = f1;
+      v = f1;
+    } else if (m == 1) {
+      X[0][1] = 1 - f0; X[3][1] = f0; X[1][2] = 1 - f1; X[3][2] = f1; X[2][3] = 1 - f2; X[3][3] = f2;
+      v = f0 * (1 - f1) * (1 - f2);
+    } else {
+      X[2][1] = 1.0; X[1][2] = 1 - f1; X[3][2] = f1; X[2][3] = 1 - f2; X[3][3] = f2;
+      v = f2 * (1 - f1);
+    }
+  } else if (ef > e[1]) {
+    const double g0 = (ef - e[0]) / (e[2] - e[0]), g1 = (ef - e[0]) / (e[3] - e[0]),
+                 g2 = (ef - e[1]) / (e[2] - e[1]), g3 = (ef - e[1]) / (e[3] - e[1]);
+    if (m == 0) {
+      X[0][0] = 1.0; X[1][1] = 1.0; X[0][2] = 1 - g0; X[2][2] = g0; X[0][3] = 1 - g1; X[3][3] = g1;
+      v = g0 * g1;
+    } else if (m == 1) {
+      X[1][0] = 1.0; X[0][1] = 1 - g0; X[2][1] = g0; X[0][2] = 1 - g1; X[3][2] = g1;
+      X[1][3] = 1 - g3; X[3][3] = g3;
+      v = g0 * (1 - g1) * g3;
+    } else {
+      X[1][0] = 1.0; X[0][1] = 1 - g0; X[2][1] = g0; X[1][2] = 1 - g3; X[3][2] = g3;
+      X[1][3] = 1 - g2; X[2][3] = g2;
+      v = g2 * (1 - g0) * g3;
+    }
+  } else {
+    const double h0 = (ef - e[0]) / (e[1] - e[0]), h1 = (ef - e[0]) / (e[2] - e[0]),
+                 h2 = (ef - e[0]) / (e[3] - e[0]);
+    X[0][0] = 1.0; X[0][1] = 1 - h0; X[1][1] = h0; X[0][2] = 1 - h1; X[2][2] = h1;
+    X[0][3] = 1 - h2; X[3][3] = h2;
+    v = h0 * h1 * h2;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      double x = 0.0;
+#pragma unroll
+      for (int s2 = 0; s2 < 4; s2++) x = r[i] == s2 ? X[s2][j] : x;
+      T[i][j] = x;
+    }
+  return v;
+}
+
+// One sub-tetrahedron's share for corner q at the bin edge f: rowq = row q of
+// its matrix on the original corners, ds = d at its corners.
+__device__ inline double sub_share(const double rowq[4], const double ds[4], double v, double f) {
+  bool all_ge = true, all_lt = true;
+  double sr = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    all_ge = all_ge && ds[j] >= f;
+    all_lt = all_lt && ds[j] < f;
+    sr += rowq[j];
+  }
+  if (all_ge) return 0.0;
+  if (all_lt) return v * sr;
+  double acc = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) acc += rowq[j] * cut_corner(ds, f, j);
+  return v * acc;
+}
+
+// Corner q's weight of one tetrahedron of a cube whose bands the Fermi
+// energy cuts (case (c1), tetrahedron.f:1100-1129 and :1176-1202): the part
+// with e1 < ef (occupied) and e2 > ef (unoccupied) as cuttetraN
+// sub-tetrahedra (nested when both bands are cut), each then cut by d < f.
+__device__ double metal_corner(const double e1[4], const double e2[4], double ef, double f,
+                               bool cut1, bool cut2, int q) {
+  double dt[4], m2[4];
+  bool all_ge = true;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    dt[j] = e2[j] - e1[j];
+    all_ge = all_ge && dt[j] >= f;
+    m2[j] = -e2[j];
+  }
+  if (all_ge) return 0.0;
+  double w = 0.0, T[4][4], rowq[4], ds[4];
+  if (!cut2) {  // occupied part only
+    const int nt = sub_count(e1, ef);
+    for (int m = 0; m < nt; m++) {
+      const double v = sub_tetra(e1, ef, m, T);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        rowq[j] = T[q][j];
+        ds[j] = dt[0] * T[0][j] + dt[1] * T[1][j] + dt[2] * T[2][j] + dt[3] * T[3][j];
+      }
+      w += sub_share(rowq, ds, v, f);
+    }
+    return w;
+  }
+  const int nt2 = sub_count(m2, -ef);
+  for (int a = 0; a < nt2; a++) {  // unoccupied part (e2 > ef)
+    const double v2 = sub_tetra(m2, -ef, a, T);
+    double e1s[4], dt2[4], r2[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      e1s[j] = e1[0] * T[0][j] + e1[1] * T[1][j] + e1[2] * T[2][j] + e1[3] * T[3][j];
+      dt2[j] = dt[0] * T[0][j] + dt[1] * T[1][j] + dt[2] * T[2][j] + dt[3] * T[3][j];
+      r2[j] = T[q][j];
+    }
+    if (!cut1) {
+      w += sub_share(r2, dt2, v2, f);
+      continue;
+    }
+    const int nt1 = sub_count(e1s, ef);
+    for (int b = 0; b < nt1; b++) {  // its occupied part
+      double T1[4][4];
+      const double v1 = sub_tetra(e1s, ef, b, T1);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        rowq[j] = r2[0] * T1[0][j] + r2[1] * T1[1][j] + r2[2] * T1[2][j] + r2[3] * T1[3][j];
+        ds[j] = dt2[0] * T1[0][j] + dt2[1] * T1[1][j] + dt2[2] * T1[2][j] + dt2[3] * T1[3][j];
+      }
+      w += sub_share(rowq, ds, v2 * v1, f);
+    }
+  }
+  return w;
+}
+
 template <int DEC>
 __global__ __launch_bounds__(TW_NT) void tetra_weights_kernel(
     int n1, int n2, int n3, int nocc, int nunocc, const double *__restrict__ eo,
     const double *__restrict__ eu, double efermi, int nfrq, const double *__restrict__ frq1,
-    const double *__restrict__ den, int zero_first, double *__restrict__ W, long ldw,
-    int *__restrict__ flag) {
+    const double *__restrict__ den, int zero_first, double *__restrict__ W, long ldw) {
   __shared__ double ds[27][TW_NT];
   const long nk = (long)n1 * n2 * n3;
   const long n = nk * nocc * nunocc;
@@ -121,28 +298,47 @@ __global__ __launch_bounds__(TW_NT) void tetra_weights_kernel(
   const int b1 = (int)(r % nocc);
   const int k = (int)(r / nocc);
   const int i3 = k % n3, i2 = (k / n3) % n2, i1 = k / (n2 * n3);
-  if (eo[(long)k * nocc + b1] > efermi || eu[(long)k * nunocc + b2] < efermi) atomicOr(flag, 1);
-  // d at the 27 points k + (a1,a2,a3), a in {-1,0,1}: index (a1+1) + 3(a2+1) + 9(a3+1)
+  // d at the 27 points k + (a1,a2,a3), a in {-1,0,1}: index (a1+1) + 3(a2+1) + 9(a3+1);
+  // bit a of occ_hi / unocc_lo: the occupied energy lies above efermi / the
+  // unoccupied one below it there
   double dlo = 1e300, dhi = -1e300;
+  unsigned occ_hi = 0, unocc_lo = 0;
   for (int a = 0; a < 27; a++) {
-    const int j1 = (i1 + a % 3 - 1 + n1) % n1, j2 = (i2 + (a / 3) % 3 - 1 + n2) % n2,
-              j3 = (i3 + a / 9 - 1 + n3) % n3;
-    const long kk = ((long)j1 * n2 + j2) * n3 + j3;
-    const double v = eu[kk * nunocc + b2] - eo[kk * nocc + b1];
+    const long kk = nbr(i1, i2, i3, a, n1, n2, n3);
+    const double e1 = eo[kk * nocc + b1], e2 = eu[kk * nunocc + b2];
+    const double v = e2 - e1;
     ds[a][tid] = v;
     dlo = v < dlo ? v : dlo;
     dhi = v > dhi ? v : dhi;
+    occ_hi |= (e1 > efermi ? 1u : 0u) << a;
+    unocc_lo |= (e2 < efermi ? 1u : 0u) << a;
+  }
+  // per cube s (origin k - (s1,s2,s3)): its corners' bits in the 27-point masks
+  unsigned cube_bits[8];
+  bool any_cut = false;
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    unsigned b = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      b |= 1u << (((c & 1) - (s & 1) + 1) + 3 * (((c >> 1) & 1) - ((s >> 1) & 1) + 1) +
+                  9 * ((c >> 2) - (s >> 2) + 1));
+    cube_bits[s] = b;
+    any_cut = any_cut || (occ_hi & b) || (unocc_lo & b);
   }
   double prev = 0.0;
   for (int i = 0; i < nfrq; i++) {
     const double f = frq1[i];
     double w = 0.0;
-    if (f > dhi) {
+    if (f > dhi && !any_cut) {
       w = 24.0;  // every tetrahedron of the 8 cubes below f: 24 corner shares
     } else if (f >= dlo) {
       // cube with origin k - (s1,s2,s3); this k is its corner p = s
       for (int s = 0; s < 8; s++) {
         const int s1 = s & 1, s2 = (s >> 1) & 1, s3 = s >> 2;
+        const unsigned cb = cube_bits[s];
+        if ((unocc_lo & cb) == cb || (occ_hi & cb) == cb) continue;  // no occupied-unoccupied part
+        const bool cut1 = (occ_hi & cb) != 0, cut2 = (unocc_lo & cb) != 0;
         double cmin = 1e300, cmax = -1e300;
         for (int c = 0; c < 8; c++) {
           const double v = ds[((c & 1) - s1 + 1) + 3 * (((c >> 1) & 1) - s2 + 1) +
@@ -154,6 +350,21 @@ __global__ __launch_bounds__(TW_NT) void tetra_weights_kernel(
         for (int it = 0; it < 6; it++) {
           const int q = corner_pos<DEC>(it, s);
           if (q < 0) continue;
+          if (cut1 || cut2) {  // (c1): the energies at the tetrahedron's corners
+            double e1[4], e2[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const int c = kDecomp[DEC][it][j];
+              const long kk = nbr(i1, i2, i3,
+                                  ((c & 1) - s1 + 1) + 3 * (((c >> 1) & 1) - s2 + 1) +
+                                      9 * ((c >> 2) - s3 + 1),
+                                  n1, n2, n3);
+              e1[j] = eo[kk * nocc + b1];
+              e2[j] = eu[kk * nunocc + b2];
+            }
+            w += metal_corner(e1, e2, efermi, f, cut1, cut2, q);
+            continue;
+          }
           if (f > cmax) {  // (b) the whole cube: wadd0
             w += 1.0;
             continue;
@@ -236,19 +447,16 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
             "tetra_weights: arrays are not context allocations of the stated shapes");
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   void *ws;
-  int rc = spx_scratch(ctx, 16 * (size_t)nfrqs + 256, &ws);
+  int rc = spx_scratch(ctx, 16 * (size_t)nfrqs, &ws);
   if (rc) return rc;
   double *dfrq = (double *)ws;
-  int *flag = (int *)(dfrq + 2 * (size_t)nfrqs);
   SPX_HIP(ctx, hipMemcpyAsync(dfrq, host.data(), 16 * (size_t)nfrqs, hipMemcpyHostToDevice,
                               ctx->stream));
-  rc = spx_fill_zero(ctx, ctx->stream, flag, sizeof(int));
-  if (rc) return rc;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)((n + TW_NT - 1) / TW_NT)), dim3(TW_NT), 0,
                        ctx->stream, nkpt3[0], nkpt3[1], nkpt3[2], nocc, nunocc,
                        (const double *)e_occ, (const double *)e_unocc, efermi, nfrqs, dfrq,
-                       dfrq + nfrqs, frqs[0] == 0.0 ? 1 : 0, (double *)wtetra, ldwt, flag);
+                       dfrq + nfrqs, frqs[0] == 0.0 ? 1 : 0, (double *)wtetra, ldwt);
   };
   switch (idmin) {
     case 0: go(tetra_weights_kernel<0>); break;
@@ -257,12 +465,8 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
     default: go(tetra_weights_kernel<3>); break;
   }
   SPX_LAUNCHED(ctx);
-  int h = 0;
-  SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  // the host frequency table is read by the copy above: wait before returning
   SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (h)
-    return spx_seterr(ctx, SPEXHIP_E_INVALID,
-                      "tetra_weights: a band is cut by the Fermi energy (metals are not covered)");
   return SPEXHIP_OK;
 }
 

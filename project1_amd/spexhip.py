@@ -554,7 +554,8 @@ class Context:
                           nocc: int, nunocc: int, efermi: float, frqs: np.ndarray, out: DevBuf,
                           ldwt: int) -> None:
         """wtetra (n x nfrqs) of the tetrahedron method on a full cubic mesh
-        (insulators); rlat columns are the reciprocal lattice vectors."""
+        (insulators and metals); rlat columns are the reciprocal lattice
+        vectors."""
         k3 = np.ascontiguousarray(nkpt3, np.int32)
         rl = np.asfortranarray(rlat, np.float64)
         fr = np.ascontiguousarray(frqs, np.float64)

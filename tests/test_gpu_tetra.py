@@ -1,5 +1,6 @@
 """GPU parity of the tetrahedron weights (spexhip_tetra_weights_dev) against
-the oracle's literal tetrahedron5_init, through the C ABI; the chain
+the oracle's literal tetrahedron5_init (insulators and metals), through the
+C ABI; the chain
 tetrahedron -> Hilbert fold -> chi0 against the reference order; and at a
 larger mesh the Brillouin-zone sum rule and flat bands."""
 import math
@@ -58,13 +59,30 @@ def test_tetra_weights_vs_oracle(ctx, nkpt3, rl, frq0):
     assert mx < 1e-12
 
 
-def test_tetra_rejects_metals_and_bad_meshes(ctx):
+@pytest.mark.parametrize("nkpt3,rl", [((3, 2, 2), np.eye(3)), ((3, 3, 3), RL_BCC),
+                                       ((2, 4, 2), RL_SKEW)])
+def test_tetra_weights_metal_vs_oracle(ctx, nkpt3, rl):
+    """Bands crossing the Fermi energy (partially occupied bands appear in
+    both the occupied and the unoccupied window): case (c1)."""
+    g = np.random.default_rng(50 + sum(nkpt3))
+    nk = int(np.prod(nkpt3))
+    kx = np.arange(nk)
+    band = 0.5 * np.cos(2 * np.pi * (kx % nkpt3[2]) / nkpt3[2]) + 0.3 * np.sin(kx * 1.7)
+    eo = np.stack([band - 0.6, band], 1) + 0.02 * g.standard_normal((nk, 2))
+    eu = np.stack([band, band + 0.4, band + 0.9], 1) + 0.02 * g.standard_normal((nk, 3))
+    eu[:, 0] = eo[:, 1]  # the partially occupied band in both windows
+    ef = 0.05
+    assert (eo > ef).any() and (eu < ef).any()
+    frq = 0.01 * (1.2 ** np.arange(30) - 1)
+    got = _run(ctx, nkpt3, rl, eo, eu, frq, efermi=ef)
+    want = tetra.tetrahedron5_weights_metal(nkpt3, rl, eo, eu, ef, frq)
+    mx, fro = normwise(got, want)
+    print(f"tetra metal {nkpt3}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < 1e-12
+
+
+def test_tetra_rejects_bad_meshes(ctx):
     g = np.random.default_rng(1)
-    eo, eu = _energies(g, 8, 1, 2)
-    eu[5, 1] = -0.01  # an unoccupied band below the Fermi energy
-    frq = 0.05 * (1.3 ** np.arange(8) - 1)
-    with pytest.raises(spexhip.SpexError):
-        _run(ctx, (2, 2, 2), np.eye(3), eo, eu, frq)
     eo, eu = _energies(g, 8, 1, 2)
     with pytest.raises(spexhip.SpexError):
         _run(ctx, (2, 2, 2), np.eye(3), eo, eu, np.array([0.0, 0.1, 0.15, 0.4]))

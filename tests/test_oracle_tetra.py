@@ -164,3 +164,105 @@ def test_metals_and_bad_meshes_are_rejected():
         tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
     with pytest.raises(ValueError):
         tetra.hilbert_bins(np.array([0.0, 0.1, 0.15, 0.4]))  # not exponential
+
+
+def clipped_weights_multi(halfspaces):
+    """Exact 4/vol * integral of each barycentric coordinate over the part of
+    the unit tetrahedron where every (values, c, sign) half-space holds:
+    sign * (linear function with corner values `values`) < sign * c."""
+    pts = [V[j] for j in range(4)]
+    for vals, c, sgn in halfspaces:
+        vals = np.asarray(vals, float)
+        lam = _bary(np.array(pts))
+        g = sgn * (lam @ vals - c)  # < 0 inside
+        keep = [p for p, gi in zip(pts, g) if gi < 0]
+        for a in range(len(pts)):
+            for b in range(a + 1, len(pts)):
+                if (g[a] < 0) != (g[b] < 0):
+                    s = g[a] / (g[a] - g[b])
+                    keep.append(pts[a] + s * (pts[b] - pts[a]))
+        pts = keep
+        if len(pts) < 4:
+            return np.zeros(4)
+    pts = np.unique(np.round(np.array(pts), 15), axis=0)
+    if len(pts) < 4:
+        return np.zeros(4)
+    try:
+        if ConvexHull(pts).volume < 1e-14:
+            return np.zeros(4)
+    except Exception:
+        return np.zeros(4)
+    tri = Delaunay(pts)
+    w = np.zeros(4)
+    for s in tri.simplices:
+        p = pts[s]
+        vol = abs(np.linalg.det(p[1:] - p[0])) / 6.0
+        w += vol * _bary(p).mean(0)
+    return 4.0 * w * 6.0
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("mode", ["occ", "unocc", "both"])
+def test_metal_corner_weights_are_exact_clipped_integrals(seed, mode):
+    """The (c1) corner weights (occupied/unoccupied part through cuttetraN's
+    sub-tetrahedra, then the cut d < f) against the exact integral over
+    {e_occ < ef, e_unocc > ef, d < f}."""
+    g = np.random.default_rng(100 + seed)
+    ef = 0.0
+    e1 = g.uniform(-0.6, 0.3, 4) if mode != "unocc" else g.uniform(-0.6, -0.05, 4)
+    e2 = g.uniform(-0.3, 0.8, 4) if mode != "occ" else g.uniform(0.05, 0.8, 4)
+    cut1, cut2 = e1.max() > ef, e2.min() < ef
+    part = tetra._restricted(e1, e2, ef, cut1, cut2)
+    d = e2 - e1
+    for f in np.linspace(d.min() - 0.05, d.max() + 0.05, 9):
+        got = tetra.corner_weights_at(d, part, f)
+        want = clipped_weights_multi([(e1, ef, 1), (e2, ef, -1), (d, f, 1)])
+        np.testing.assert_allclose(got, want, rtol=1e-8, atol=1e-11)
+
+
+def test_metal_path_without_cuts_is_the_insulator_path():
+    g = np.random.default_rng(21)
+    nkpt3 = (2, 3, 2)
+    eo, eu = _mesh_case(g, nkpt3, 2, 3)
+    frq = 0.02 * (1.25 ** np.arange(20) - 1) + 0.01
+    a = tetra.tetrahedron5_weights(nkpt3, RL_T, eo, eu, 0.0, frq)
+    b = tetra.tetrahedron5_weights_metal(nkpt3, RL_T, eo, eu, 0.0, frq)
+    assert np.array_equal(a, b)
+
+
+RL_T = np.array([[1.0, 0.1, 0.0], [0.0, 1.2, 0.0], [0.2, 0.0, 0.9]]).T
+
+
+def test_metal_zone_sum_is_the_occupied_unoccupied_volume():
+    """Summed over k and bins, each (b1, b2) carries the volume fraction of
+    {e_occ(k) < ef < e_unocc(k+q)} under linear interpolation in every
+    tetrahedron -- computed here by exact clipping, independently."""
+    g = np.random.default_rng(22)
+    nkpt3 = (2, 2, 3)
+    nk = 12
+    kx = np.arange(nk)
+    band = 0.4 * np.cos(2 * np.pi * (kx % 3) / 3) + 0.2 * ((kx // 6) - 0.5)
+    eo = np.stack([band - 0.05, band - 0.6], 1) + 0.02 * g.standard_normal((nk, 2))
+    eu = np.stack([band + 0.1, band + 0.7], 1) + 0.02 * g.standard_normal((nk, 2))
+    ef = 0.05
+    frq = 0.02 * (1.3 ** np.arange(20) - 1) + 0.001
+    frq1, dw = tetra.hilbert_bins(frq)
+    assert frq1[-1] > (eu.max() - eo.min())
+    w = tetra.tetrahedron5_weights_metal(nkpt3, RL_T, eo, eu, ef, frq)
+    tot = (w * (2 * dw)[None, :]).reshape(nk, 2, 2, -1).sum(axis=(0, 3))
+    tt = tetra.get_tetra(RL_T, nkpt3)
+    pk = tetra.mesh_index(nkpt3)
+    want = np.zeros((2, 2))
+    for i1 in range(2):
+        for i2 in range(2):
+            for i3 in range(3):
+                kc = [pk(i1 + a, i2 + b, i3 + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                for b1 in range(2):
+                    for b2 in range(2):
+                        for row in tt:
+                            e1 = eo[np.array(kc)[row], b1]
+                            e2 = eu[np.array(kc)[row], b2]
+                            cw = clipped_weights_multi([(e1, ef, 1), (e2, ef, -1)])
+                            want[b1, b2] += cw.sum() / 4 / 6 / nk
+    assert ((want > 0.01) & (want < 0.99)).any()  # some pairs are cut by the Fermi energy
+    np.testing.assert_allclose(tot, want, rtol=1e-8, atol=1e-12)
