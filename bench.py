@@ -273,8 +273,8 @@ def hilbert_leg(pipe, a, nfrqs=64, reps=3):
             "tetra_ms": round(t_ms, 3), "fold_ms": round(f_ms, 3),
             "tetra_GBps_written": round(wbytes / (t_ms * 1e6), 1),
             "fold_GBps_read": round(wbytes / (f_ms * 1e6), 1),
-            "note": "each tetra_weights call synchronises on its metal check; the events "
-                    "bracket the calls"}
+            "note": "each tetra_weights call returns after its weights are written (it "
+                    "reads a host frequency table); the events bracket the calls"}
 
 
 def gamma_leg(pipe, a, reps=5):

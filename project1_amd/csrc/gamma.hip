@@ -18,7 +18,9 @@
 // from HBM once per pass of up to 16 (6 complex) frequencies: the kernel is
 // HBM-bound (16 B per M element per pass against 3 complex products and
 // 3 x nw weighted sums).  Partial sums per K-split are reduced in a fixed
-// order (deterministic).
+// order (deterministic).  The head (three columns, from P alone) has its own
+// kernel: one thread per transition, the six products p_k1 conj(p_k2), a
+// shuffle and LDS reduction per block and a fixed-order sum over blocks.
 //
 // Real frequencies (headc, wingc, :1112-1125, :1144-1151) carry complex
 // weights Dr + i Di: wingc(k2,i,1) = sum Dc M(t,i) conj(P(t,k2)) and
@@ -35,6 +37,7 @@
 #include "common.hpp"
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -63,7 +66,8 @@ __global__ void momentum_scale_kernel(int nk, int nocc, int nunocc, const double
   }
 }
 
-// Partial sums of one K-split: part[((s*NWB + w)*3 + k)*NV + v][c], NV = 1
+// Partial sums of one K-split: part[((s*NWB + w)*3 + k)*NV + v][c] (row
+// stride pstride >= ncolM), NV = 1
 // (real weights) or 2 (complex: the M conj(P) and conj(M) P forms), for
 // the ncolM columns of M.  Each stage of HW_TT transitions is fetched into
 // registers (branch-free, indices clamped into the arrays, so the loads of
@@ -74,7 +78,7 @@ template <bool CPLX, int NWB, bool CM>
 __global__ __launch_bounds__(HW_NT) void headwing_kernel(
     long n, int ncolM, const double2 *__restrict__ M, long ldm, const double2 *__restrict__ P,
     long ldp, int nw, int w0, const double *__restrict__ Dr, const double *__restrict__ Di,
-    long ldd, long kchunk, double2 *__restrict__ part) {
+    long ldd, long kchunk, long pstride, double2 *__restrict__ part) {
   constexpr int NV = CPLX ? 2 : 1;
   constexpr int ND = CPLX ? 2 : 1;
   constexpr int NDW = HW_TT * ND * NWB;  // weights per stage
@@ -190,31 +194,28 @@ __global__ __launch_bounds__(HW_NT) void headwing_kernel(
     __syncthreads();
   }
   if (c >= ncolM) return;
-  const long ncolp = (long)gridDim.x * HW_NT;
-  double2 *ps = part + (long)blockIdx.y * NWB * 3 * NV * ncolp;
+  double2 *ps = part + (long)blockIdx.y * NWB * 3 * NV * pstride;
 #pragma unroll
   for (int w = 0; w < NWB; w++)
 #pragma unroll
     for (int k = 0; k < 3; k++)
 #pragma unroll
-      for (int v = 0; v < NV; v++) ps[(((long)w * 3 + k) * NV + v) * ncolp + c] = acc[w][k][v];
+      for (int v = 0; v < NV; v++) ps[(((long)w * 3 + k) * NV + v) * pstride + c] = acc[w][k][v];
 #undef HW_CLAMPT
 #undef HW_FETCH
 #undef HW_COMMIT
 }
 
-// out = beta*out + alpha*sum_s part (fixed order).  Wing mode (the columns
-// are those of M): real weights wing[k + 3*(c + ncolM*w)], complex
-// wing[k + 3*(c + ncolM*(v + 2*w))].  Head mode (the columns are those of
-// P, k1 = c, k2 = k, form v = 0): head[c + 3*k + 9*w].
-__global__ void headwing_reduce_kernel(int ncolM, int head_mode, int nv, int nwb, int nwv, int w0,
-                                       int nsplit, long ncolp, const double2 *__restrict__ part,
-                                       double alpha, double beta, double2 *__restrict__ out) {
+// out = beta*out + alpha*sum_s part (fixed order): real weights
+// wing[k + 3*(c + ncolM*w)], complex wing[k + 3*(c + ncolM*(v + 2*w))].
+__global__ void headwing_reduce_kernel(int ncolM, int nv, int nwb, int nwv, int w0, int nsplit,
+                                       long ncolp, const double2 *__restrict__ part, double alpha,
+                                       double beta, double2 *__restrict__ out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;  // (w*3 + k)*nv + v
   if (c >= ncolM) return;
   const int v = r % nv, k = (r / nv) % 3, w = r / (3 * nv);
-  if (w >= nwv || (head_mode && v)) return;
+  if (w >= nwv) return;
   const long stride = (long)nwb * 3 * nv * ncolp;
   double sx = 0.0, sy = 0.0;
   for (int s = 0; s < nsplit; s++) {
@@ -223,8 +224,7 @@ __global__ void headwing_reduce_kernel(int ncolM, int head_mode, int nv, int nwb
     sy += q.y;
   }
   const long wg = w0 + w;
-  double2 *o = head_mode ? out + c + 3 * k + 9 * wg
-                         : out + k + 3 * ((long)c + (long)ncolM * (nv == 2 ? v + 2 * wg : wg));
+  double2 *o = out + k + 3 * ((long)c + (long)ncolM * (nv == 2 ? v + 2 * wg : wg));
   double2 val = make_double2(alpha * sx, alpha * sy);
   if (beta != 0.0) {
     const double2 old = *o;
@@ -232,6 +232,134 @@ __global__ void headwing_reduce_kernel(int ncolM, int head_mode, int nv, int nwb
     val.y += beta * old.y;
   }
   *o = val;
+}
+
+// The head: sums over t of D(t,w) z_j(t) for the six pairs j = (k1 <= k2),
+// z = P(t,k1) conj(P(t,k2)).  One thread per transition (grid-stride),
+// NWH frequencies per launch; per block the sums are reduced across each
+// wave with shuffles, then across the four waves in LDS, into
+// part[block][(w*6 + j)*NV + v].
+constexpr int HD_NT = 256, HD_NWH = 4;
+
+template <bool CPLX>
+__global__ __launch_bounds__(HD_NT) void head_partial_kernel(
+    long n, const double2 *__restrict__ P, long ldp, int nw, int w0,
+    const double *__restrict__ Dr, const double *__restrict__ Di, long ldd,
+    double2 *__restrict__ part) {
+  constexpr int NV = CPLX ? 2 : 1;
+  constexpr int NR = HD_NWH * 6 * NV;
+  __shared__ double2 red[HD_NT / 64][NR];
+  double2 acc[HD_NWH][6][NV];
+#pragma unroll
+  for (int w = 0; w < HD_NWH; w++)
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) acc[w][j][v] = make_double2(0.0, 0.0);
+  for (long t = (long)blockIdx.x * HD_NT + threadIdx.x; t < n; t += (long)gridDim.x * HD_NT) {
+    const double2 p0 = P[t], p1 = P[t + ldp], p2 = P[t + 2 * ldp];
+    const double2 pk[3] = {p0, p1, p2};
+    double2 z[6];
+    int j = 0;
+#pragma unroll
+    for (int k2 = 0; k2 < 3; k2++)
+#pragma unroll
+      for (int k1 = 0; k1 <= k2; k1++, j++)
+        z[j] = make_double2(pk[k1].x * pk[k2].x + pk[k1].y * pk[k2].y,
+                            pk[k1].y * pk[k2].x - pk[k1].x * pk[k2].y);  // p_k1 conj(p_k2)
+#pragma unroll
+    for (int w = 0; w < HD_NWH; w++) {
+      const bool in = w0 + w < nw;
+      const long off = t + ldd * (long)(in ? w0 + w : w0);
+      const double dr = in ? Dr[off] : 0.0;
+      if constexpr (CPLX) {
+        const double di = in ? Di[off] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          acc[w][q][0].x += dr * z[q].x - di * z[q].y;  // (dr + i di) z
+          acc[w][q][0].y += dr * z[q].y + di * z[q].x;
+          acc[w][q][1].x += dr * z[q].x + di * z[q].y;  // (dr + i di) conj(z)
+          acc[w][q][1].y += di * z[q].x - dr * z[q].y;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+          acc[w][q][0].x += dr * z[q].x;
+          acc[w][q][0].y += dr * z[q].y;
+        }
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int w = 0; w < HD_NWH; w++)
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+#pragma unroll
+      for (int v = 0; v < NV; v++) {
+        double x = acc[w][q][v].x, y = acc[w][q][v].y;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          x += __shfl_down(x, o, 64);
+          y += __shfl_down(y, o, 64);
+        }
+        if (lane == 0) red[wv][(w * 6 + q) * NV + v] = make_double2(x, y);
+      }
+  __syncthreads();
+  if (threadIdx.x < NR) {
+    double2 sum = red[0][threadIdx.x];
+#pragma unroll
+    for (int r = 1; r < HD_NT / 64; r++) {
+      sum.x += red[r][threadIdx.x].x;
+      sum.y += red[r][threadIdx.x].y;
+    }
+    part[(long)blockIdx.x * NR + threadIdx.x] = sum;
+  }
+}
+
+// head(k1,k2,w0+w) = beta*head + alpha*sum over blocks (fixed order), and
+// the partner (k2,k1): the conjugate (real weights) or the conj(z) form.
+// One wave per value r = (w*6 + j)*nv + v: lane l sums blocks l, l+64, ...,
+// then the lanes are combined by shuffles (a fixed order).
+__global__ __launch_bounds__(64) void head_reduce_kernel(int nv, int nwv, int w0, int nblk,
+                                                         const double2 *__restrict__ part,
+                                                         double alpha, double beta,
+                                                         double2 *__restrict__ head) {
+  const int r = blockIdx.x;
+  const int nr = HD_NWH * 6 * nv;
+  const int v = r % nv, j = (r / nv) % 6, w = r / (6 * nv);
+  double sx = 0.0, sy = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64) {
+    const double2 q = part[(long)b * nr + r];
+    sx += q.x;
+    sy += q.y;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sx += __shfl_down(sx, o, 64);
+    sy += __shfl_down(sy, o, 64);
+  }
+  if (threadIdx.x != 0 || w >= nwv) return;
+  static constexpr int K1[6] = {0, 0, 1, 0, 1, 2}, K2[6] = {0, 1, 1, 2, 2, 2};
+  const int k1 = K1[j], k2 = K2[j];
+  auto put = [&](int a, int b, double x, double y) {
+    double2 *o = head + a + 3 * b + 9 * (long)(w0 + w);
+    double2 val = make_double2(alpha * x, alpha * y);
+    if (beta != 0.0) {
+      const double2 old = *o;
+      val.x += beta * old.x;
+      val.y += beta * old.y;
+    }
+    *o = val;
+  };
+  if (nv == 1) {
+    put(k1, k2, sx, sy);
+    if (k1 != k2) put(k2, k1, sx, -sy);
+  } else if (v == 0) {
+    put(k1, k2, sx, sy);
+  } else if (k1 != k2) {
+    put(k2, k1, sx, sy);
+  }
 }
 
 // Row and column 0 of each chi to zero, wing(:,0,:) to zero, then head =
@@ -352,7 +480,7 @@ void launch_hw(dim3 grid, hipStream_t st, long n, int ncolM, const void *M, long
                void *part) {
   hipLaunchKernelGGL((headwing_kernel<CPLX, NWB, CM>), grid, dim3(HW_NT), 0, st, n, ncolM,
                      (const double2 *)M, ldm, (const double2 *)P, ldp, nw, w0, (const double *)Dr,
-                     (const double *)Di, ldd, kchunk, (double2 *)part);
+                     (const double *)Di, ldd, kchunk, (long)ncolM, (double2 *)part);
 }
 
 template <bool CPLX, int NWB>
@@ -374,15 +502,15 @@ int nwb_for(bool cplx, int rem) {
   return rem <= 4 ? 4 : rem <= 8 ? 8 : 12;
 }
 
-// One sum over the columns of M against P (wing mode) or of P against
-// itself (head mode), all frequency passes, into out.
+// The wing sums over the columns of M against P, all frequency passes,
+// into out.
 int hw_sums(spexhip_ctx *ctx, long n, int ncolM, const void *M, long ldm, bool cm, const void *P,
             long ldp, int nw, const void *Dr, const void *Di, long ldd, double alpha, double beta,
-            bool head_mode, void *out) {
+            void *out) {
   const bool cplx = Di != nullptr;
   const int nv = cplx ? 2 : 1;
   const int ncb = (ncolM + HW_NT - 1) / HW_NT;
-  const long ncolp = (long)ncb * HW_NT;
+  const long ncolp = ncolM;  // row stride of the partial sums
   // two workgroups fit a CU (LDS): one round of 2 x 256 workgroups, split
   // evenly over the column blocks, each split at least 4 stages long
   long nsplit = 512 / ncb;
@@ -425,7 +553,7 @@ int hw_sums(spexhip_ctx *ctx, long n, int ncolM, const void *M, long ldm, bool c
     }
     hipLaunchKernelGGL(headwing_reduce_kernel,
                        dim3((unsigned)((ncolM + 255) / 256), (unsigned)(nwb * 3 * nv)), dim3(256),
-                       0, ctx->stream, ncolM, (int)head_mode, nv, nwb, rem, w0,
+                       0, ctx->stream, ncolM, nv, nwb, rem, w0,
                        (int)(n > 0 ? nsplit : 1), ncolp, (const double2 *)part, alpha, beta,
                        (double2 *)out);
     SPX_LAUNCHED(ctx);
@@ -434,8 +562,7 @@ int hw_sums(spexhip_ctx *ctx, long n, int ncolM, const void *M, long ldm, bool c
 }
 
 // Shared by the real- and complex-weight entry points: the wings as the
-// sums over the columns of M, the head as the same sums over the three
-// columns of P (P is n x 3, t fastest: the colmajor layout with ld ldp).
+// sums over the columns of M, the head from P alone.
 int headwings(spexhip_ctx *ctx, long n, int nb, const void *M, long ldm, int colmajor,
               const void *P, long ldp, int nw, const void *Dr, const void *Di, const void *Dhr,
               const void *Dhi, long ldd, double alpha, double beta, void *head, void *wing) {
@@ -460,12 +587,37 @@ int headwings(spexhip_ctx *ctx, long n, int nb, const void *M, long ldm, int col
               "headwings: weights are not context allocations of n x nw (ld ldd)");
   }
   SPX_HIP(ctx, hipSetDevice(ctx->device));
-  int rc = hw_sums(ctx, n, nb, M, ldm, colmajor != 0, P, ldp, nw, Dr, Di, ldd, alpha, beta, false,
+  int rc = hw_sums(ctx, n, nb, M, ldm, colmajor != 0, P, ldp, nw, Dr, Di, ldd, alpha, beta,
                    wing);
-  if (!rc)
-    rc = hw_sums(ctx, n, 3, P, ldp, true, P, ldp, nw, Dhr, cplx ? Dhi : nullptr, ldd, alpha, beta,
-                 true, head);
-  return rc;
+  if (rc) return rc;
+  // the head: its own two-stage reduction (a second scratch region, after
+  // the wing sums on the same stream)
+  const int nblk = (int)std::min<long>(1024, std::max<long>(1, (n + HD_NT - 1) / HD_NT));
+  const int nr = HD_NWH * 6 * nv;
+  void *hpart;
+  rc = spx_scratch(ctx, 16 * (size_t)nblk * nr, &hpart);
+  if (rc) return rc;
+  for (int w0 = 0; w0 < nw; w0 += HD_NWH) {
+    if (n > 0) {
+      if (cplx)
+        hipLaunchKernelGGL(head_partial_kernel<true>, dim3(nblk), dim3(HD_NT), 0, ctx->stream, n,
+                           (const double2 *)P, ldp, nw, w0, (const double *)Dhr,
+                           (const double *)Dhi, ldd, (double2 *)hpart);
+      else
+        hipLaunchKernelGGL(head_partial_kernel<false>, dim3(nblk), dim3(HD_NT), 0, ctx->stream, n,
+                           (const double2 *)P, ldp, nw, w0, (const double *)Dhr, nullptr, ldd,
+                           (double2 *)hpart);
+      SPX_LAUNCHED(ctx);
+    } else {
+      rc = spx_fill_zero(ctx, ctx->stream, hpart, 16 * (size_t)nr);
+      if (rc) return rc;
+    }
+    const int nwv = nw - w0 < HD_NWH ? nw - w0 : HD_NWH;
+    hipLaunchKernelGGL(head_reduce_kernel, dim3(nr), dim3(64), 0, ctx->stream, nv, nwv, w0,
+                       n > 0 ? nblk : 1, (const double2 *)hpart, alpha, beta, (double2 *)head);
+    SPX_LAUNCHED(ctx);
+  }
+  return SPEXHIP_OK;
 }
 
 }  // namespace
