@@ -308,6 +308,14 @@ int spexhip_gamma_prepare_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int
  * Hermitian matrices (ld lde), both triangles valid.                      */
 int spexhip_gamma_inverse_dev(spexhip_ctx *ctx, int nb, int nw, const void *Einv_dev, int lde,
                               void *head_dev, void *wing_dev);
+/* The head's scalar in the (1,1) element of each eps (correlation.f:651-665
+ * and :683, before the RPA diagonalisation or the dielectric output):
+ * E(0,0,w) = trace(head(:,:,w))/3 (vec NULL: angular average), or
+ * vec^T head vec for a unit vector vec (host, 3 doubles: the direction of
+ * an additional k vector, lkptadd).  head_dev as produced by
+ * spexhip_gamma_prepare_dev (1 - 4 pi head); no Drude term.               */
+int spexhip_gamma_head_scalar_dev(spexhip_ctx *ctx, int nb, int nw, void *E_dev, int lde,
+                                  const void *head_dev, const double *vec);
 /* Host drop-in for correlation.f:570-759 at q = 0 on the imaginary axis:
  * chi (packed upper, nb(nb+1)/2 x nw) -> eps^-1 (packed), with head
  * (packed 6 x nw) and wing (3 x nb x nw) replaced in place by those of the

@@ -16,8 +16,9 @@ Restates, in plain numpy/Python:
   divided by frqs(s)**mm where frqs(s) > 0;
 - the Gamma-point dielectric step, correlation.f:570-646 (body row and
   column of the first basis function set to zero, wing(:,1,:) = 0,
-  head = 1 - 4 pi head, wing = -sqrt(4 pi) sqcoul wing), and the head and
-  wings of the inverse, correlation.f:749-759.
+  head = 1 - 4 pi head, wing = -sqrt(4 pi) sqcoul wing), the head's scalar
+  in eps(1,1) (:651-665, :683), and the head and wings of the inverse,
+  correlation.f:749-759.
 
 What pins each function is in tests/test_oracle_gamma.py: the chi0 pair sum
 of the augmented matrix [cprod | dcprod] (whose head and wing entries these
@@ -192,6 +193,17 @@ def gamma_inverse(einv_packed, head, wing):
         m = w2 @ w[:, :, f].T
         h[:, f] += np.array([m[k1, k2] for k1, k2 in iu])
     return h, w
+
+
+def head_scalar(head_packed, vec=None):
+    """The head's scalar for eps(1,1) (correlation.f:656-665, :683):
+    trace(head)/3, or dotprod(vec, matvec(head, vec)) for a unit vec.
+    head_packed (6,) packed upper."""
+    h = unpack_head(head_packed)
+    if vec is None:
+        return np.trace(h) / 3
+    vec = np.asarray(vec, float)
+    return vec @ h @ vec
 
 
 def unpack_head(hp):

@@ -620,9 +620,49 @@ int headwings(spexhip_ctx *ctx, long n, int nb, const void *M, long ldm, int col
   return SPEXHIP_OK;
 }
 
+// E(0,0,w) = trace(head(:,:,w))/3, or vec^T head vec with a unit vector.
+__global__ void gamma_head_scalar_kernel(int nw, double2 *__restrict__ E, long lde, long nb,
+                                         const double2 *__restrict__ head, int use_vec, double v0,
+                                         double v1, double v2) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nw) return;
+  const double2 *h = head + 9 * (long)w;
+  double2 val;
+  if (use_vec) {
+    const double v[3] = {v0, v1, v2};
+    double x = 0.0, y = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+      for (int b = 0; b < 3; b++) {
+        x += v[a] * h[a + 3 * b].x * v[b];
+        y += v[a] * h[a + 3 * b].y * v[b];
+      }
+    val = make_double2(x, y);
+  } else {
+    val = make_double2((h[0].x + h[4].x + h[8].x) / 3, (h[0].y + h[4].y + h[8].y) / 3);
+  }
+  E[lde * nb * (long)w] = val;
+}
+
 }  // namespace
 
 extern "C" {
+
+int spexhip_gamma_head_scalar_dev(spexhip_ctx *ctx, int nb, int nw, void *E, int lde,
+                                  const void *head, const double *vec) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && lde >= nb, "gamma_head_scalar: bad sizes");
+  SPX_CHECK(ctx, spx_owns(ctx, E, 16 * (size_t)lde * nb * nw) &&
+                     spx_owns(ctx, head, 16 * 9 * (size_t)nw),
+            "gamma_head_scalar: arrays are not context allocations of the stated shapes");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(gamma_head_scalar_kernel, dim3((nw + 63) / 64), dim3(64), 0, ctx->stream, nw,
+                     (double2 *)E, (long)lde, (long)nb, (const double2 *)head, vec ? 1 : 0,
+                     vec ? vec[0] : 0.0, vec ? vec[1] : 0.0, vec ? vec[2] : 0.0);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
 
 int spexhip_momentum_scale_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const void *e_occ,
                                const void *e_unocc, double vol, void *P, long ldp) {

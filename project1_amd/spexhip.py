@@ -329,6 +329,7 @@ def _declare(L: C.CDLL) -> None:
                                                  _vp]),
         "spexhip_gamma_prepare_dev": (I, [_vp, I, I, _vp, I, _vp, _vp, _vp]),
         "spexhip_gamma_inverse_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_gamma_head_scalar_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_dielectric_inverse_gamma": (I, [_vp, I, I, _vp, _vp, _vp, _vp, _vp]),
         "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
         "spexhip_synth_normal_rows_dev": (I, [_vp, _vp, _c_long, _c_long, _c_long,
@@ -741,6 +742,13 @@ class Context:
                           wing: DevBuf) -> None:
         self._check(lib().spexhip_gamma_inverse_dev(self.h, nb, nw, einv.ptr, lde, head.ptr,
                                                     wing.ptr))
+
+    def gamma_head_scalar_dev(self, nb: int, nw: int, E: DevBuf, lde: int, head: DevBuf,
+                              vec=None) -> None:
+        """E(0,0,w) = trace(head)/3, or vec^T head vec (unit vector vec)."""
+        v = None if vec is None else np.ascontiguousarray(vec, np.float64)
+        self._check(lib().spexhip_gamma_head_scalar_dev(self.h, nb, nw, E.ptr, lde, head.ptr,
+                                                        None if v is None else _ptr(v)))
 
     def dielectric_inverse_gamma(self, chi_packed: np.ndarray, sqcoul: np.ndarray,
                                  head: np.ndarray, wing: np.ndarray) -> np.ndarray:

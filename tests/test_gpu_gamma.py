@@ -249,3 +249,36 @@ def test_headwings_rejects_bad_shapes(ctx):
         ctx.headwings_dev(64, 8, dM, 8, False, dP, 32, 1, dD, 64, dH, dW)
     for b in (dM, dP, dD, dH, dW):
         b.free()
+
+
+def test_head_scalar_and_gamma_rpa(ctx):
+    """eps(1,1) = trace(head)/3 or v^T head v (correlation.f:656-665, :683),
+    then the RPA integrand of that eps against the oracle."""
+    nb, nw = 40, 3
+    chi, head, wing, sq = _gamma_case(9, nb, nw)
+    eps_o, h_o, _ = gamma.gamma_prepare(chi, head, wing, sq)
+    dX = ctx.alloc(16 * nb * nb * nw)
+    ctx.unpack(np.ascontiguousarray(chi.T), nb, nw, dX, nb)
+    hfull = np.stack([gamma.unpack_head(head[:, f]) for f in range(nw)], 2)
+    dH = _dev_f(ctx, hfull, np.complex128)
+    dW = _dev_f(ctx, wing, np.complex128)
+    dS = ctx.to_dev(sq)
+    ctx.gamma_prepare_dev(nb, nw, dX, nb, dH, dW, dS)
+    ctx.dielectric_dev(dX, nb, nw, nb, 0, dS, dX, nb)
+    vec = np.array([0.6, 0.0, 0.8])
+    for v in (None, vec):
+        ctx.gamma_head_scalar_dev(nb, nw, dX, nb, dH, v)
+        got = ctx.pack_upper(dX, nb, nw, nb).T
+        for f in range(nw):
+            want = gamma.head_scalar(h_o[:, f], v)
+            assert abs(got[0, f] - want) <= 1e-13 * abs(want), (v, f)
+    # RPA at Gamma with the angular average in eps(1,1)
+    ctx.gamma_head_scalar_dev(nb, nw, dX, nb, dH, None)
+    got = ctx.rpa_integrand_dev(dX, nb, nw, nb, 0.25)
+    for f in range(nw):
+        e = eps_o[:, f].copy()
+        e[0] = gamma.head_scalar(h_o[:, f]).real
+        want = oracle.rpa_integrand(e, 0.25)
+        assert abs(got[f] - want) <= 1e-10 * abs(want), (f, got[f], want)
+    for b in (dX, dH, dW, dS):
+        b.free()

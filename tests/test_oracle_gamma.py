@@ -169,3 +169,19 @@ def test_multdiff_spectral_function_is_a_fold_with_scaled_hilbert_weights():
         want = c.real @ D + 1j * (c.imag @ D)
         got = gamma.spectral_function_mm(c, wt, frqs, wS, mm, 1e-6)
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_head_scalar_closed_forms():
+    """trace/3 is the angular average of v^T head v over unit vectors (the
+    mean of v v^T over the sphere is I/3), and a projection on an axis picks
+    the diagonal element."""
+    g = np.random.default_rng(19)
+    B = g.standard_normal((3, 3)) + 1j * g.standard_normal((3, 3))
+    H = np.eye(3) + B @ B.conj().T
+    hp = oracle.pack_upper(H)
+    v = g.standard_normal((20000, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    avg = np.mean([gamma.head_scalar(hp, x) for x in v[:4000]])
+    assert abs(avg - gamma.head_scalar(hp)) < 0.05 * abs(gamma.head_scalar(hp))
+    assert np.isclose(gamma.head_scalar(hp), np.trace(H) / 3)
+    assert np.isclose(gamma.head_scalar(hp, [0.0, 1.0, 0.0]), H[1, 1])
