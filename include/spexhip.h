@@ -144,12 +144,24 @@ int spexhip_weights_hilbert_dev(spexhip_ctx *ctx, long n, int nfrqs, const void 
  * reciprocal lattice vectors) selects the decomposition (get_tetra,
  * :219-271).  e_occ_dev [nk][nocc] at k, e_unocc_dev [nk][nunocc] at k+q.
  * SPEXHIP_E_INVALID if the mesh is not ascending and exponential.
- * Degenerate-state averaging (:1233-1273) is not applied.  Returns after
+ * Degenerate states are averaged by the next entry point.  Returns after
  * the weights are written (it reads the host frequency table).            */
 int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
                               int nunocc, const void *e_occ_dev, const void *e_unocc_dev,
                               double efermi, int nfrqs, const double *frqs, void *wtetra_dev,
                               long ldwt);
+
+/* The averaging of those weights over degenerate states (tetrahedron.f:
+ * 1233-1273), in place: over each group of occupied states at k, then over
+ * each group of unoccupied states at k+q.  deg_occ_dev [nk][nocc] and
+ * deg_unocc_dev [nk][nunocc] (int32, 0-based within the band windows) in
+ * SPEX's deg convention: for the first band of a group the group's last
+ * band, for the others the first band.  Groups must lie inside the band
+ * windows (the reference's extension of windows that cut a group is not
+ * done).                                                                   */
+int spexhip_tetra_average_degenerate_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                                         const void *deg_occ_dev, const void *deg_unocc_dev,
+                                         int nfrqs, void *wtetra_dev, long ldwt);
 
 /* ---- chi0 pair sum ----------------------------------------------------- */
 /* Device form of the pair loop susceptibility.f:1159-1251 with

@@ -16,7 +16,9 @@ Restates, in plain numpy/Python:
 - the same with bands cut by the Fermi energy (metals, case (c1)): each
   tetrahedron restricted to its occupied (e_occ < efermi) and unoccupied
   (e_unocc > efermi) part through cuttetraN's sub-tetrahedra (:1045-1129),
-  then cut by d < frq1(i) (:1176-1202).
+  then cut by d < frq1(i) (:1176-1202);
+- the averaging of the weights over degenerate states (:1233-1273), for
+  groups inside the band windows.
 
 k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
 (i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
@@ -268,6 +270,57 @@ def tetrahedron5_weights_metal(nkpt3, rlat, e_occ, e_unocc, efermi, frq):
     if frq[0] == 0:
         out[..., 0] = 0
     return out.reshape(nk * nocc * nunocc, nfrq)
+
+
+def _group(deg_row, b):
+    """(first, last) of band b's degenerate group from SPEX's deg convention:
+    deg(b) is the last band of the group when b is its first band, else the
+    first band (global.f deg; tetrahedron.f:1237-1240)."""
+    d = int(deg_row[b])
+    if d >= b:
+        return b, d
+    return d, int(deg_row[d])
+
+
+def average_degenerate(w, nk, nocc, nunocc, deg_occ, deg_unocc):
+    """Degenerate-state averaging of tetrahedron5_init (tetrahedron.f:
+    1233-1273) for groups inside the band windows: first over each group of
+    occupied states at k (all unoccupied b2, all bins), then over each group
+    of unoccupied states at k+q.  w (n, nfrqs) with t = (k*nocc + b1)*nunocc
+    + b2; deg_occ (nk, nocc), deg_unocc (nk, nunocc) window-relative."""
+    W = np.array(w, float).reshape(nk, nocc, nunocc, -1)
+    for k in range(nk):
+        n = _group(deg_occ[k], 0)[0]
+        while n < nocc:
+            first, last = _group(deg_occ[k], n)
+            W[k, first:last + 1] = W[k, first:last + 1].sum(0) / (last - first + 1)
+            n = last + 1
+    for k in range(nk):
+        n = _group(deg_unocc[k], 0)[0]
+        while n < nunocc:
+            first, last = _group(deg_unocc[k], n)
+            W[k, :, first:last + 1] = (W[k, :, first:last + 1].sum(1) / (last - first + 1))[:, None]
+            n = last + 1
+    return W.reshape(nk * nocc * nunocc, -1)
+
+
+def deg_from_energies(e, tol):
+    """SPEX-style deg array (window-relative) of sorted energies e (nk, nb):
+    bands within tol of their neighbour form a group."""
+    e = np.asarray(e, float)
+    nk, nb = e.shape
+    deg = np.zeros((nk, nb), int)
+    for k in range(nk):
+        b = 0
+        while b < nb:
+            c = b
+            while c + 1 < nb and abs(e[k, c + 1] - e[k, c]) <= tol:
+                c += 1
+            deg[k, b] = c
+            for x in range(b + 1, c + 1):
+                deg[k, x] = b
+            b = c + 1
+    return deg
 
 
 def hilbert_bins(frq):

@@ -19,6 +19,8 @@
 // its occupied-unoccupied part, as cuttetraN sub-tetrahedra (nested when
 // both bands are cut), and each of those is cut by d < frq1(i).  The output
 // is written once, transition-fastest (coalesced): n x nfrqs doubles.
+// The averaging over degenerate states is a separate pass (two
+// out-of-place kernels: occupied groups, then unoccupied groups).
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -390,9 +392,77 @@ __global__ __launch_bounds__(TW_NT) void tetra_weights_kernel(
   }
 }
 
+// First and last band of b's degenerate group from a deg row (SPEX's
+// convention: deg(b) is the group's last band if b is its first, else the
+// first band).
+__device__ inline void deg_group(const int *__restrict__ deg, int b, int &first, int &last) {
+  const int d = deg[b];
+  if (d >= b) {
+    first = b;
+    last = d;
+  } else {
+    first = d;
+    last = deg[d];
+  }
+}
+
+// out(t, i) = mean of in over the degenerate group of b1 (side 0, occupied
+// states at k) or of b2 (side 1, unoccupied states at k+q), for every
+// transition t = (k*nocc + b1)*nunocc + b2 and bin i (tetrahedron.f:
+// 1233-1273).  Out of place: every thread reads only `in`.
+__global__ void deg_average_kernel(int nk, int nocc, int nunocc, const int *__restrict__ deg,
+                                   int side, int nfrq, const double *__restrict__ in, long ldi,
+                                   double *__restrict__ out, long ldo) {
+  const long n = (long)nk * nocc * nunocc;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (t >= n || i >= nfrq) return;
+  const int b2 = (int)(t % nunocc);
+  const long r = t / nunocc;
+  const int b1 = (int)(r % nocc);
+  const int k = (int)(r / nocc);
+  int first, last;
+  double s = 0.0;
+  if (side == 0) {
+    deg_group(deg + (long)k * nocc, b1, first, last);
+    for (int b = first; b <= last; b++) s += in[((long)k * nocc + b) * nunocc + b2 + ldi * i];
+  } else {
+    deg_group(deg + (long)k * nunocc, b2, first, last);
+    for (int b = first; b <= last; b++) s += in[((long)k * nocc + b1) * nunocc + b + ldi * i];
+  }
+  out[t + ldo * i] = s / (last - first + 1);
+}
+
 }  // namespace
 
 extern "C" {
+
+int spexhip_tetra_average_degenerate_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                                         const void *deg_occ, const void *deg_unocc, int nfrqs,
+                                         void *wtetra, long ldwt) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0, "tetra_average_degenerate: bad sizes");
+  const long n = (long)nk * nocc * nunocc;
+  SPX_CHECK(ctx, ldwt >= n, "tetra_average_degenerate: ldwt < n");
+  SPX_CHECK(ctx, spx_owns(ctx, deg_occ, 4 * (size_t)nk * nocc) &&
+                     spx_owns(ctx, deg_unocc, 4 * (size_t)nk * nunocc) &&
+                     spx_owns(ctx, wtetra, 8 * ((size_t)ldwt * (nfrqs - 1) + n)),
+            "tetra_average_degenerate: arrays are not context allocations of the stated shapes");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *tmp;
+  int rc = spx_scratch(ctx, 8 * (size_t)n * nfrqs, &tmp);
+  if (rc) return rc;
+  const dim3 grid((unsigned)((n + 255) / 256), (unsigned)nfrqs);
+  hipLaunchKernelGGL(deg_average_kernel, grid, dim3(256), 0, ctx->stream, nk, nocc, nunocc,
+                     (const int *)deg_occ, 0, nfrqs, (const double *)wtetra, ldwt, (double *)tmp,
+                     n);
+  SPX_LAUNCHED(ctx);
+  hipLaunchKernelGGL(deg_average_kernel, grid, dim3(256), 0, ctx->stream, nk, nocc, nunocc,
+                     (const int *)deg_unocc, 1, nfrqs, (const double *)tmp, n, (double *)wtetra,
+                     ldwt);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
 
 int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
                               int nunocc, const void *e_occ, const void *e_unocc, double efermi,

@@ -318,6 +318,7 @@ def _declare(L: C.CDLL) -> None:
                                                C.c_double, I, I, _vp, _vp, I, _vp, _vp]),
         "spexhip_tetra_weights_dev": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp, _vp,
                                           _c_long]),
+        "spexhip_tetra_average_degenerate_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
@@ -563,6 +564,12 @@ class Context:
         self._check(lib().spexhip_tetra_weights_dev(
             self.h, _ptr(k3), rl.ctypes.data, nocc, nunocc, e_occ.ptr, e_unocc.ptr, efermi,
             fr.size, _ptr(fr), out.ptr, ldwt))
+
+    def tetra_average_degenerate_dev(self, nk: int, nocc: int, nunocc: int, deg_occ: DevBuf,
+                                     deg_unocc: DevBuf, nfrqs: int, w: DevBuf, ldwt: int) -> None:
+        """Average the tetrahedron weights over degenerate states in place."""
+        self._check(lib().spexhip_tetra_average_degenerate_dev(
+            self.h, nk, nocc, nunocc, deg_occ.ptr, deg_unocc.ptr, nfrqs, w.ptr, ldwt))
 
     # -- chi0 -------------------------------------------------------------
     def chi0_cplx_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int,

@@ -142,3 +142,31 @@ def test_tetra_sum_rule_and_flat_bands_at_size(ctx):
     want = np.zeros(len(frq))
     want[31] = 1.0 / (nk * 2 * dw[31])
     np.testing.assert_allclose(w, np.broadcast_to(want, w.shape), rtol=1e-13, atol=1e-13)
+
+
+def test_degenerate_averaging_vs_oracle(ctx):
+    nkpt3 = (3, 2, 2)
+    nk, nocc, nunocc = 12, 4, 5
+    g = np.random.default_rng(77)
+    eo, eu = _energies(g, nk, nocc, nunocc)
+    eo.sort(axis=1)
+    eu.sort(axis=1)
+    eo[:, 2] = eo[:, 3]          # a pair of occupied states
+    eu[:, 1] = eu[:, 2] = eu[:, 3]  # a triple of unoccupied states
+    eu[5, 4] = eu[5, 3]          # and at one k a fourth one
+    frq = 0.03 * (1.2 ** np.arange(24) - 1)
+    n = nk * nocc * nunocc
+    raw = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    # perturb so that the averaging has work to do
+    pert = raw + g.uniform(0, 1e-3, raw.shape)
+    do = tetra.deg_from_energies(eo, 1e-12).astype(np.int32)
+    du = tetra.deg_from_energies(eu, 1e-12).astype(np.int32)
+    want = tetra.average_degenerate(pert, nk, nocc, nunocc, do, du)
+    dWt = ctx.to_dev(np.ascontiguousarray(pert.T))
+    dDo, dDu = ctx.to_dev(do), ctx.to_dev(du)
+    ctx.tetra_average_degenerate_dev(nk, nocc, nunocc, dDo, dDu, len(frq), dWt, n)
+    got = dWt.download(np.float64, (len(frq), n)).T
+    mx, _ = normwise(got, want)
+    assert mx < 1e-14
+    for b in (dWt, dDo, dDu):
+        b.free()

@@ -266,3 +266,31 @@ def test_metal_zone_sum_is_the_occupied_unoccupied_volume():
                             want[b1, b2] += cw.sum() / 4 / 6 / nk
     assert ((want > 0.01) & (want < 0.99)).any()  # some pairs are cut by the Fermi energy
     np.testing.assert_allclose(tot, want, rtol=1e-8, atol=1e-12)
+
+
+def test_degenerate_averaging_rules():
+    """Averaging over degenerate groups: members end up equal, group sums
+    are kept, singleton groups change nothing, and on exactly degenerate
+    bands (two copies of one band) the raw weights already agree."""
+    g = np.random.default_rng(31)
+    nkpt3 = (2, 2, 2)
+    nk = 8
+    eo, eu = _mesh_case(g, nkpt3, 3, 4)
+    eo.sort(axis=1)
+    eu.sort(axis=1)
+    eo[:, 1] = eo[:, 2]  # occupied bands 1, 2 degenerate everywhere (still sorted)
+    eu[:, 0] = eu[:, 1]  # unoccupied 0, 1 degenerate everywhere
+    frq = 0.02 * (1.25 ** np.arange(20) - 1) + 0.01
+    w = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    do, du = tetra.deg_from_energies(eo, 1e-12), tetra.deg_from_energies(eu, 1e-12)
+    assert (do[:, 1] == 2).all() and (do[:, 2] == 1).all() and (du[:, 0] == 1).all()
+    a = tetra.average_degenerate(w, nk, 3, 4, do, du)
+    W, A = w.reshape(nk, 3, 4, -1), a.reshape(nk, 3, 4, -1)
+    np.testing.assert_allclose(A, W, rtol=1e-13, atol=1e-15)  # identical bands: no change
+    # a perturbation inside the groups is averaged out, the group sums kept
+    P = W + g.uniform(0, 1e-3, W.shape)
+    B = tetra.average_degenerate(P.reshape(w.shape), nk, 3, 4, do, du).reshape(P.shape)
+    np.testing.assert_allclose(B[:, 1], B[:, 2], rtol=1e-14)
+    np.testing.assert_allclose(B[:, :, 0], B[:, :, 1], rtol=1e-14)
+    np.testing.assert_allclose(B.sum(axis=(1, 2)), P.sum(axis=(1, 2)), rtol=1e-13)
+    np.testing.assert_array_equal(B[:, 0, 2:], P[:, 0, 2:])  # singleton groups untouched
