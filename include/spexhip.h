@@ -163,6 +163,18 @@ int spexhip_tetra_average_degenerate_dev(spexhip_ctx *ctx, int nk, int nocc, int
                                          const void *deg_occ_dev, const void *deg_unocc_dev,
                                          int nfrqs, void *wtetra_dev, long ldwt);
 
+/* Weights of a symmetry-reduced k set: tetrahedron5_init scatters a cube
+ * corner to its parent kpt1p(k) and skips corners without one (tetrahedron.f:
+ * 1022, 1092-1095, 1207-1210), so a parent's row is the sum of the
+ * full-mesh rows of its star.  w_full_dev (nk*nocc*nunocc x nfrqs, ld ldw)
+ * from spexhip_tetra_weights_dev; kpt1p (host, nk ints: 0-based parent or
+ * -1); w_red_dev (nkpt1*nocc*nunocc x nfrqs, ld ldr, t = (p*nocc + b1)*
+ * nunocc + b2).  Call before spexhip_tetra_average_degenerate_dev (with
+ * nk = nkpt1).  Returns after the sums are written.                       */
+int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, int nfrqs,
+                                     const void *w_full_dev, long ldw, const int *kpt1p,
+                                     int nkpt1, void *w_red_dev, long ldr);
+
 /* ---- chi0 pair sum ----------------------------------------------------- */
 /* Device form of the pair loop susceptibility.f:1159-1251 with
  * spectral_function HILBERT NONE (:1780-1788):

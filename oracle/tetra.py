@@ -18,7 +18,9 @@ Restates, in plain numpy/Python:
   (e_unocc > efermi) part through cuttetraN's sub-tetrahedra (:1045-1129),
   then cut by d < frq1(i) (:1176-1202);
 - the averaging of the weights over degenerate states (:1233-1273), for
-  groups inside the band windows.
+  groups inside the band windows;
+- the weights of a symmetry-reduced k set (kpt1p): the sum over the
+  full-mesh points that map to each parent.
 
 k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
 (i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
@@ -270,6 +272,21 @@ def tetrahedron5_weights_metal(nkpt3, rlat, e_occ, e_unocc, efermi, frq):
     if frq[0] == 0:
         out[..., 0] = 0
     return out.reshape(nk * nocc * nunocc, nfrq)
+
+
+def reduce_to_parents(w, nk, nocc, nunocc, kpt1p, nkpt1):
+    """The k points of a symmetry-reduced set: tetrahedron5_init scatters a
+    cube corner k to its parent kpt1p(k) and skips corners without one
+    (tetrahedron.f:1022, :1092-1095, :1207-1210), so the row of a parent is
+    the sum of the full-mesh rows of the points that map to it.  w (n_full,
+    nfrqs); kpt1p (nk,) 0-based parent or -1.  Returns (nkpt1*nocc*nunocc,
+    nfrqs), in the loop order of the reference's scatter (k ascending)."""
+    W = np.asarray(w, float).reshape(nk, nocc, nunocc, -1)
+    out = np.zeros((nkpt1, nocc, nunocc, W.shape[-1]))
+    for k in range(nk):
+        if kpt1p[k] >= 0:
+            out[kpt1p[k]] += W[k]
+    return out.reshape(nkpt1 * nocc * nunocc, -1)
 
 
 def _group(deg_row, b):

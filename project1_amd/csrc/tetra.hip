@@ -433,9 +433,68 @@ __global__ void deg_average_kernel(int nk, int nocc, int nunocc, const int *__re
   out[t + ldo * i] = s / (last - first + 1);
 }
 
+// Row of parent p = sum over its star (full-mesh points kp with
+// kpt1p(kp) = p, ascending) of the full-mesh rows; one thread per reduced
+// transition (p, b1, b2) and bin.
+__global__ void star_sum_kernel(int nkpt1, int nocc, int nunocc, const int *__restrict__ off,
+                                const int *__restrict__ mem, int nfrq,
+                                const double *__restrict__ in, long ldi, double *__restrict__ out,
+                                long ldo) {
+  const long per = (long)nocc * nunocc;
+  const long nr = (long)nkpt1 * per;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (t >= nr || i >= nfrq) return;
+  const int p = (int)(t / per);
+  const long bb = t - (long)p * per;
+  double s = 0.0;
+  for (int m = off[p]; m < off[p + 1]; m++) s += in[(long)mem[m] * per + bb + ldi * i];
+  out[t + ldo * i] = s;
+}
+
 }  // namespace
 
 extern "C" {
+
+int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, int nfrqs,
+                                     const void *w_full, long ldw, const int *kpt1p, int nkpt1,
+                                     void *w_red, long ldr) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0 && nkpt1 > 0 && kpt1p,
+            "tetra_reduce_parents: bad arguments");
+  const long n = (long)nk * nocc * nunocc, nr = (long)nkpt1 * nocc * nunocc;
+  SPX_CHECK(ctx, ldw >= n && ldr >= nr, "tetra_reduce_parents: leading dimensions too small");
+  // stars in CSR form, members ascending (the order of the reference's scatter)
+  std::vector<int> off(nkpt1 + 1, 0), mem;
+  for (int k = 0; k < nk; k++) {
+    SPX_CHECK(ctx, kpt1p[k] >= -1 && kpt1p[k] < nkpt1, "tetra_reduce_parents: kpt1p out of range");
+    if (kpt1p[k] >= 0) off[kpt1p[k] + 1]++;
+  }
+  for (int p = 0; p < nkpt1; p++) off[p + 1] += off[p];
+  mem.resize(off[nkpt1]);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int k = 0; k < nk; k++)
+    if (kpt1p[k] >= 0) mem[fill[kpt1p[k]]++] = k;
+  SPX_CHECK(ctx, spx_owns(ctx, w_full, 8 * ((size_t)ldw * (nfrqs - 1) + n)) &&
+                     spx_owns(ctx, w_red, 8 * ((size_t)ldr * (nfrqs - 1) + nr)),
+            "tetra_reduce_parents: arrays are not context allocations of the stated shapes");
+  SPX_CHECK(ctx, w_full != w_red, "tetra_reduce_parents: output must not alias the input");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *ws;
+  int rc = spx_scratch(ctx, 4 * (off.size() + mem.size()) + 16, &ws);
+  if (rc) return rc;
+  int *doff = (int *)ws, *dmem = doff + off.size();
+  SPX_HIP(ctx, hipMemcpyAsync(doff, off.data(), 4 * off.size(), hipMemcpyHostToDevice, ctx->stream));
+  if (!mem.empty())
+    SPX_HIP(ctx, hipMemcpyAsync(dmem, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(star_sum_kernel, dim3((unsigned)((nr + 255) / 256), (unsigned)nfrqs),
+                     dim3(256), 0, ctx->stream, nkpt1, nocc, nunocc, doff, dmem, nfrqs,
+                     (const double *)w_full, ldw, (double *)w_red, ldr);
+  SPX_LAUNCHED(ctx);
+  // the host star tables are read by the copies above
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SPEXHIP_OK;
+}
 
 int spexhip_tetra_average_degenerate_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
                                          const void *deg_occ, const void *deg_unocc, int nfrqs,

@@ -319,6 +319,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_tetra_weights_dev": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp, _vp,
                                           _c_long]),
         "spexhip_tetra_average_degenerate_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long]),
+        "spexhip_tetra_reduce_parents_dev": (I, [_vp, I, I, I, I, _vp, _c_long, _vp, I, _vp,
+                                                 _c_long]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
@@ -570,6 +572,14 @@ class Context:
         """Average the tetrahedron weights over degenerate states in place."""
         self._check(lib().spexhip_tetra_average_degenerate_dev(
             self.h, nk, nocc, nunocc, deg_occ.ptr, deg_unocc.ptr, nfrqs, w.ptr, ldwt))
+
+    def tetra_reduce_parents_dev(self, nk: int, nocc: int, nunocc: int, nfrqs: int,
+                                 w_full: DevBuf, ldw: int, kpt1p, nkpt1: int, w_red: DevBuf,
+                                 ldr: int) -> None:
+        """Rows of a symmetry-reduced k set: star sums of the full-mesh rows."""
+        kp = np.ascontiguousarray(kpt1p, np.int32)
+        self._check(lib().spexhip_tetra_reduce_parents_dev(
+            self.h, nk, nocc, nunocc, nfrqs, w_full.ptr, ldw, _ptr(kp), nkpt1, w_red.ptr, ldr))
 
     # -- chi0 -------------------------------------------------------------
     def chi0_cplx_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int,

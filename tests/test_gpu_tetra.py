@@ -170,3 +170,24 @@ def test_degenerate_averaging_vs_oracle(ctx):
     assert mx < 1e-14
     for b in (dWt, dDo, dDu):
         b.free()
+
+
+def test_reduce_to_parents_vs_oracle(ctx):
+    nkpt3 = (3, 2, 2)
+    nk, nocc, nunocc = 12, 2, 3
+    g = np.random.default_rng(88)
+    eo, eu = _energies(g, nk, nocc, nunocc)
+    frq = 0.03 * (1.2 ** np.arange(16) - 1)
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3, 4, 4, 4, 1])
+    nkpt1 = 5
+    n, nr = nk * nocc * nunocc, nkpt1 * nocc * nunocc
+    full = _run(ctx, nkpt3, np.eye(3), eo, eu, frq)
+    want = tetra.reduce_to_parents(full, nk, nocc, nunocc, kpt1p, nkpt1)
+    dF = ctx.to_dev(np.ascontiguousarray(full.T))
+    dR = ctx.alloc(8 * nr * len(frq))
+    ctx.tetra_reduce_parents_dev(nk, nocc, nunocc, len(frq), dF, n, kpt1p, nkpt1, dR, nr)
+    got = dR.download(np.float64, (len(frq), nr)).T
+    mx, _ = normwise(got, want)
+    assert mx < 1e-15
+    for b in (dF, dR):
+        b.free()

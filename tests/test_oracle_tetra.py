@@ -294,3 +294,39 @@ def test_degenerate_averaging_rules():
     np.testing.assert_allclose(B[:, :, 0], B[:, :, 1], rtol=1e-14)
     np.testing.assert_allclose(B.sum(axis=(1, 2)), P.sum(axis=(1, 2)), rtol=1e-13)
     np.testing.assert_array_equal(B[:, 0, 2:], P[:, 0, 2:])  # singleton groups untouched
+
+
+def test_parents_reduction_is_the_reference_scatter():
+    """A literal scatter with kcubep = kpt1p(kcube) (corners without a parent
+    skipped) equals the star sums of the full-mesh weights."""
+    g = np.random.default_rng(41)
+    nkpt3 = (2, 2, 2)
+    nk = 8
+    eo, eu = _mesh_case(g, nkpt3, 1, 2)
+    frq = 0.05 * (1.3 ** np.arange(12) - 1) + 0.02
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3])  # some stars, one point without a parent
+    nkpt1 = 4
+    full = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    got = tetra.reduce_to_parents(full, nk, 1, 2, kpt1p, nkpt1)
+    # literal scatter (cube loop, corner -> parent), then the same bin differences
+    tt = tetra.get_tetra(np.eye(3), nkpt3)
+    pk = tetra.mesh_index(nkpt3)
+    frq1, dw = tetra.hilbert_bins(frq)
+    cum = np.zeros((nkpt1, 1, 2, len(frq)))
+    for i1 in range(2):
+        for i2 in range(2):
+            for i3 in range(2):
+                kc = [pk(i1 + a, i2 + b, i3 + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                for b2 in range(2):
+                    dc = eu[kc, b2] - eo[kc, 0]
+                    for j, f in enumerate(frq1):
+                        wadd = np.zeros(8)
+                        for row in tt:
+                            wadd[row] += tetra.corner_weights_at(dc[row], None, f)
+                        for c in range(8):
+                            if kpt1p[kc[c]] >= 0:
+                                cum[kpt1p[kc[c]], 0, b2, j] += wadd[c]
+    want = np.empty_like(cum)
+    want[..., 1:] = np.diff(cum, axis=-1) / (2 * dw[1:] * 24 * nk)
+    want[..., 0] = cum[..., 0] / (2 * dw[0] * 24 * nk)
+    np.testing.assert_allclose(got, want.reshape(got.shape), rtol=1e-12, atol=1e-14)
