@@ -332,6 +332,16 @@ int spexhip_gamma_prepare_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int
  * Hermitian matrices (ld lde), both triangles valid.                      */
 int spexhip_gamma_inverse_dev(spexhip_ctx *ctx, int nb, int nw, const void *Einv_dev, int lde,
                               void *head_dev, void *wing_dev);
+/* Real frequencies at q = 0 (correlation.f:597-607, :637-646, :760-763):
+ * the same two steps around spexhip_dielectric_general_dev and
+ * spexhip_inverse_general_dev, with head_dev 3 x 3 x nw (headc) and
+ * wing_dev 3 x nb x 2 x nw (wingc).  After the inverse: wingc(:,:,1) =
+ * -wingc(:,:,1) dielecc^T, headc += wingc_old(:,:,2) wingc(:,:,1)^T,
+ * wingc(:,:,2) = -wingc(:,:,2) dielecc.                                   */
+int spexhip_gamma_prepare_cplx_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx,
+                                   void *head_dev, void *wing_dev, const void *sqcoul_dev);
+int spexhip_gamma_inverse_cplx_dev(spexhip_ctx *ctx, int nb, int nw, const void *Einv_dev,
+                                   int lde, void *head_dev, void *wing_dev);
 /* The head's scalar in the (1,1) element of each eps (correlation.f:651-665
  * and :683, before the RPA diagonalisation or the dielectric output):
  * E(0,0,w) = trace(head(:,:,w))/3 (vec NULL: angular average), or
@@ -549,6 +559,19 @@ int spexhip_dielectric_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ld
  * replaces Minverse(dielec(:,i)) (correlation.f:709 -> inverse_zp1,
  * wrapper.f:2362-2374).  SPEXHIP_E_NOTPD if a pivot is not positive.      */
 int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx);
+
+/* Real frequencies (dielecc, correlation.f:632-636 and :701-747): eps = 1 -
+ * v^1/2 chi v^1/2 of general (non-Hermitian) chi, e.g. from
+ * spexhip_chi0_cplx_dev, without Hermitization (vkind as above; E may
+ * alias X only for vkind 0), and the in-place inverse of nw general
+ * matrices.  The reference inverts with LU and partial pivoting (zgetrf/
+ * zgetri); this Gauss-Jordan does not pivot, which suits eps at real
+ * frequencies (identity minus a perturbation with a non-negative
+ * absorptive part) but not an arbitrary matrix.  SPEXHIP_E_NOTPD if a
+ * pivot is zero or not finite.                                            */
+int spexhip_dielectric_general_dev(spexhip_ctx *ctx, int nb, int nw, const void *X_dev, int ldx,
+                                   int vkind, const void *vhalf_dev, void *E_dev, int lde);
+int spexhip_inverse_general_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx);
 
 /* Screened interaction W = v^1/2 eps^-1 v^1/2 of nw full device matrices
  * (correlation.f:868-877, p_diagmatdiag(sqcoul, dielec)); vkind 0: vhalf =

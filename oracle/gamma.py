@@ -195,6 +195,39 @@ def gamma_inverse(einv_packed, head, wing):
     return h, w
 
 
+def gamma_prepare_cplx(chi, head, wing, sqcoul):
+    """Real frequencies, correlation.f:597-607 and :632-646: chi (nb, nb,
+    nw) = suscepc with row and column 1 set to zero, eps = 1 - sqcoul chi
+    sqcoul (p_diagmatdiag(-sqcoul, ., sqcoul), p_plusdiag), headc =
+    identity(3) - 4 pi headc (3, 3, nw), wingc(:,i,:,:) = -sqrt(4 pi)
+    sqcoul(i) wingc (3, nb, 2, nw) after wingc(:,1,:,:) = 0."""
+    sq = np.asarray(sqcoul, float)
+    c = np.array(chi, complex)
+    c[0, :, :] = 0
+    c[:, 0, :] = 0
+    eps = np.eye(sq.size)[:, :, None] + (-sq)[:, None, None] * c * sq[None, :, None]
+    h = np.eye(3)[:, :, None] - FOURPI * np.asarray(head, complex)
+    w = np.array(wing, complex)
+    w[:, 0] = 0
+    w = -math.sqrt(FOURPI) * sq[None, :, None, None] * w
+    return eps, h, w
+
+
+def gamma_inverse_cplx(einv, head, wing):
+    """correlation.f:760-763 per real frequency: wingc(:,:,1) =
+    -matmat(wingc(:,:,1), transpose(dielecc)); headc += matmat(wingc(:,:,2),
+    transpose(wingc(:,:,1))) (the new first wing); wingc(:,:,2) =
+    -matmat(wingc(:,:,2), dielecc).  einv (nb, nb, nw)."""
+    h = np.array(head, complex)
+    w = np.array(wing, complex)
+    for f in range(w.shape[3]):
+        e = np.asarray(einv[:, :, f], complex)
+        w[:, :, 0, f] = -w[:, :, 0, f] @ e.T
+        h[:, :, f] = h[:, :, f] + w[:, :, 1, f] @ w[:, :, 0, f].T
+        w[:, :, 1, f] = -w[:, :, 1, f] @ e
+    return h, w
+
+
 def head_scalar(head_packed, vec=None):
     """The head's scalar for eps(1,1) (correlation.f:656-665, :683):
     trace(head)/3, or dotprod(vec, matvec(head, vec)) for a unit vec.

@@ -16,6 +16,7 @@
 // one workgroup inverts the diagonal block in LDS, an MFMA GEMM forms the
 // row panel, and a rank-64 MFMA update sweeps the whole matrix.
 #include "gemm_f64.hpp"
+#include "internal.hpp"
 
 using namespace spxg;
 
@@ -90,6 +91,9 @@ constexpr int GB = 64;  // panel width
 // logdet (optional): logdet[blockIdx.y] = sum of log(pivot) over this block
 // -- the pivots of Gauss-Jordan on an HPD matrix are the Schur-complement
 // diagonals, so summed over all blocks they give log det.
+// GEN: a general complex matrix (real-frequency eps): complex pivots, no
+// pivoting; a zero or non-finite pivot sets the flag.
+template <bool GEN>
 __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
                                                       long ldx, long sX, int k0,
                                                       double2 *__restrict__ P, int *__restrict__ flag,
@@ -119,19 +123,28 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__r
       }
       __syncthreads();
       const double2 f = colb[buf][lane];
-      // Hermitian positive definite: the pivot is real and positive.
-      const double d = colb[buf][p].x;
-      bad |= !(d > 0.0);
-      if (threadIdx.x == 0) piv[p] = d;
-      const double pr = 1.0 / d;
+      double2 pr;  // 1 / pivot
+      if constexpr (GEN) {
+        const double2 d = colb[buf][p];
+        const double m2 = d.x * d.x + d.y * d.y;
+        bad |= !(m2 > 0.0) || !isfinite(m2);
+        pr = make_double2(d.x / m2, -d.y / m2);
+      } else {
+        // Hermitian positive definite: the pivot is real and positive.
+        const double d = colb[buf][p].x;
+        bad |= !(d > 0.0);
+        if (threadIdx.x == 0) piv[p] = d;
+        pr = make_double2(1.0 / d, 0.0);
+      }
 #pragma unroll
       for (int c = 0; c < 16; c++) {
         const int j = wv * 16 + c;
         if (j == p) {
-          a[c] = lane == p ? make_double2(pr, 0.0) : make_double2(-f.x * pr, -f.y * pr);
+          a[c] = lane == p ? pr
+                           : make_double2(-(f.x * pr.x - f.y * pr.y), -(f.x * pr.y + f.y * pr.x));
         } else {
           const double2 r = rowb[buf][j];
-          const double xr = r.x * pr, xi = r.y * pr;
+          const double xr = r.x * pr.x - r.y * pr.y, xi = r.x * pr.y + r.y * pr.x;
           if (lane == p) {
             a[c] = make_double2(xr, xi);
           } else {
@@ -146,7 +159,7 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__r
 #pragma unroll
   for (int c = 0; c < 16; c++) Pb[lane + GB * (wv * 16 + c)] = a[c];
   if (threadIdx.x == 0 && bad) flag[0] = 1;
-  if (logdet) {
+  if (!GEN && logdet) {
     __syncthreads();
     if (wv == 0) piv[lane] = log(piv[lane]);
     __syncthreads();
@@ -289,7 +302,7 @@ size_t spx_inverse_ws(int nb, int nw) {
 // In-place inverse of nw HPD matrices (full storage).  ws: spx_inverse_ws
 // bytes; flag: device int, set non-zero if a pivot is not positive.
 int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, void *ws,
-                int *flag, double *logdet_parts) {
+                int *flag, double *logdet_parts, int general) {
   const long nbp = ((nb + GB - 1) / GB) * GB;
   double2 *P = (double2 *)ws;
   double2 *R = P + (long)nw * GB * GB;        // GB x nbp per matrix (ld GB)
@@ -298,8 +311,13 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
   const int nt = (int)(nbp / GB);
   for (int kb = 0; kb < nt; kb++) {
     const int k0 = kb * GB;
-    hipLaunchKernelGGL(gj_diag_kernel, dim3(1, nw), dim3(NT), 0, st, nb, (const double2 *)X, ldx,
-                       sX, k0, P, flag, logdet_parts ? logdet_parts + (long)kb * nw : nullptr);
+    if (general)
+      hipLaunchKernelGGL(gj_diag_kernel<true>, dim3(1, nw), dim3(NT), 0, st, nb,
+                         (const double2 *)X, ldx, sX, k0, P, flag, nullptr);
+    else
+      hipLaunchKernelGGL(gj_diag_kernel<false>, dim3(1, nw), dim3(NT), 0, st, nb,
+                         (const double2 *)X, ldx, sX, k0, P, flag,
+                         logdet_parts ? logdet_parts + (long)kb * nw : nullptr);
     SPX_LAUNCHED(ctx);
     // R = P * X(K, :)   (GB x nb), batched
     int rc = spx_zgemm(ctx, st, GB, nb, (nb - k0 < GB ? nb - k0 : GB), 1.0, P, GB, (long)GB * GB,
@@ -346,6 +364,28 @@ int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, lo
   rc = spx_zgemm_ex(ctx, st, e, nw);
   if (rc) return rc;
   return spx_hermitize(ctx, st, nb, nw, E, lde, 0);
+}
+
+// Real-frequency eps = 1 - Vh chi Vh of a general (non-Hermitian) chi
+// (correlation.f:632-636: p_diagmatdiag(-sqcoul, suscepc, sqcoul),
+// p_plusdiag): no Hermitization, both triangles computed.  E may alias X
+// only for vkind 0.  tmp: nb*nb*nw complex (vkind 1).
+int spx_dielectric_general(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, const void *X,
+                           long ldx, int vkind, const void *vh, void *E, long lde, void *tmp) {
+  const long nn = (long)nb * nb;
+  if (vkind == 0) {
+    hipLaunchKernelGGL(eps_diag_kernel, dim3((nn + 255) / 256, nw), dim3(256), 0, st, nb,
+                       (const double *)vh, (const double2 *)X, ldx, ldx * (long)nb, (double2 *)E,
+                       lde, lde * (long)nb);
+    SPX_LAUNCHED(ctx);
+    return SPEXHIP_OK;
+  }
+  int rc = spx_zgemm(ctx, st, nb, nb, nb, 1.0, X, ldx, ldx * (long)nb, vh, nb, 0, 0.0, tmp, nb, nn,
+                     0.0, nw);
+  if (rc) return rc;
+  ZgemmArgs e{nb, nb, nb, make_double2(-1.0, 0.0), (const double2 *)vh, nb, 0, 1,
+              (const double2 *)tmp, nb, nn, 1, 0.0, (double2 *)E, lde, lde * (long)nb, 1, 1.0};
+  return spx_zgemm_ex(ctx, st, e, nw);
 }
 
 // W = Vh Einv Vh (correlation.f:868-877).  vkind 0: vh = sqrt(coul)
@@ -497,6 +537,46 @@ int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx) 
   SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
   SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (h) return spx_seterr(ctx, SPEXHIP_E_NOTPD, "inverse: matrix not positive definite");
+  return SPEXHIP_OK;
+}
+
+int spexhip_dielectric_general_dev(spexhip_ctx *ctx, int nb, int nw, const void *X, int ldx,
+                                   int vkind, const void *vhalf, void *E, int lde) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && ldx >= nb && lde >= nb, "dielectric_general: bad sizes");
+  SPX_CHECK(ctx, vkind == 0 || vkind == 1, "dielectric_general: vkind must be 0 or 1");
+  SPX_CHECK(ctx, spx_owns(ctx, X, 16 * (size_t)ldx * nb * nw) && spx_owns(ctx, E, 16 * (size_t)lde * nb * nw),
+            "dielectric_general: X/E not context allocations of the stated shape");
+  SPX_CHECK(ctx, spx_owns(ctx, vhalf, vkind ? 16 * (size_t)nb * nb : 8 * (size_t)nb),
+            "dielectric_general: vhalf not a context allocation of the stated shape");
+  SPX_CHECK(ctx, vkind == 0 || X != E, "dielectric_general: dense v^1/2 needs E != X");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *tmp = nullptr;
+  if (vkind) {
+    int rc = spx_scratch(ctx, 16 * (size_t)nb * nb * nw, &tmp);
+    if (rc) return rc;
+  }
+  return spx_dielectric_general(ctx, ctx->stream, nb, nw, X, ldx, vkind, vhalf, E, lde, tmp);
+}
+
+int spexhip_inverse_general_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && ldx >= nb, "inverse_general: bad sizes");
+  SPX_CHECK(ctx, spx_owns(ctx, X, 16 * (size_t)ldx * nb * nw),
+            "inverse_general: X not a context allocation of the stated shape");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *ws;
+  int rc = spx_scratch(ctx, spx_inverse_ws(nb, nw), &ws);
+  if (rc) return rc;
+  int *flag = (int *)((char *)ws + spx_inverse_ws(nb, nw) - 256);
+  rc = spx_fill_zero(ctx, ctx->stream, flag, sizeof(int));
+  if (rc) return rc;
+  rc = spx_inverse(ctx, ctx->stream, nb, nw, X, ldx, ws, flag, nullptr, 1);
+  if (rc) return rc;
+  int h = 0;
+  SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h) return spx_seterr(ctx, SPEXHIP_E_NOTPD, "inverse_general: zero or non-finite pivot");
   return SPEXHIP_OK;
 }
 

@@ -364,7 +364,7 @@ __global__ __launch_bounds__(64) void head_reduce_kernel(int nv, int nwv, int w0
 
 // Row and column 0 of each chi to zero, wing(:,0,:) to zero, then head =
 // 1 - 4 pi head and wing(:,i,:) = -sqrt(4 pi) sqcoul(i) wing(:,i,:).
-__global__ void gamma_prepare_kernel(int nb, int nw, double2 *__restrict__ X, long ldx,
+__global__ void gamma_prepare_kernel(int nb, int nw, int nv, double2 *__restrict__ X, long ldx,
                                      double2 *__restrict__ head, double2 *__restrict__ wing,
                                      const double *__restrict__ sq, double fourpi, double sfp) {
   const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,11 +374,13 @@ __global__ void gamma_prepare_kernel(int nb, int nw, double2 *__restrict__ X, lo
     Xw[g] = make_double2(0.0, 0.0);        // chi(g, 0)
     Xw[ldx * g] = make_double2(0.0, 0.0);  // chi(0, g)
   }
-  if (g < 3 * (long)nb) {
+  if (g < 3 * (long)nb) {  // nv wing forms per frequency (2: wingc(:,:,1:2,w))
     const int i = (int)(g / 3);
-    double2 *p = wing + g + 3 * (long)nb * w;
     const double s = i == 0 ? 0.0 : -sfp * sq[i];
-    *p = make_double2(s * p->x, s * p->y);
+    for (int v = 0; v < nv; v++) {
+      double2 *p = wing + g + 3 * (long)nb * (v + nv * (long)w);
+      *p = make_double2(s * p->x, s * p->y);
+    }
   }
   if (g < 9) {
     double2 *p = head + g + 9 * (long)w;
@@ -387,18 +389,19 @@ __global__ void gamma_prepare_kernel(int nb, int nw, double2 *__restrict__ X, lo
   }
 }
 
-// tmp(k, j, w) = -sum_i wing(k, i, w) E(j, i, w)  (E Hermitian: E(j,i) =
-// conj(E(i,j)), so this is -wing conj(E)).  64 columns j per workgroup,
+// tmp(k, j, w) = -sum_i wing(k, i, w) E(j, i, w) = -(wing E^T)(k, j): for a
+// Hermitian E that is -wing conj(E) (correlation.f:756), for the
+// real-frequency dielecc the first wing's -wingc(:,:,1) dielecc^T (:761).  64 columns j per workgroup,
 // the four waves split i and are summed in LDS.
 __global__ __launch_bounds__(256) void gamma_wing_kernel(int nb, const double2 *__restrict__ E,
                                                          long lde, const double2 *__restrict__ wing,
-                                                         double2 *__restrict__ tmp) {
+                                                         long wstr, double2 *__restrict__ tmp) {
   __shared__ double2 red[4][3][64];
   const int w = blockIdx.y;
   const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + lane;
   const double2 *Ew = E + lde * nb * (long)w;
-  const double2 *ww = wing + 3 * (long)nb * w;
+  const double2 *ww = wing + wstr * w;
   const int per = (nb + 3) / 4;
   const int i0 = q * per, i1 = i0 + per < nb ? i0 + per : nb;
   double2 a[3] = {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)};
@@ -620,6 +623,88 @@ int headwings(spexhip_ctx *ctx, long n, int nb, const void *M, long ldm, int col
   return SPEXHIP_OK;
 }
 
+// tmp(k, j, w) = -sum_i wing(k, i, w) E(i, j, w): the second real-frequency
+// wing, -wingc(:,:,2) dielecc (correlation.f:763).  One wave per column j,
+// the lanes split i, shuffles combine them (fixed order).
+__global__ __launch_bounds__(64) void gamma_wing_t_kernel(int nb, const double2 *__restrict__ E,
+                                                          long lde, const double2 *__restrict__ wing,
+                                                          long wstr, double2 *__restrict__ tmp) {
+  const int j = blockIdx.x, w = blockIdx.y, lane = threadIdx.x;
+  const double2 *Ej = E + lde * nb * (long)w + lde * (long)j;
+  const double2 *ww = wing + wstr * w;
+  double a[3][2] = {};
+  for (int i = lane; i < nb; i += 64) {
+    const double2 e = Ej[i];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double2 x = ww[k + 3 * (long)i];
+      a[k][0] += x.x * e.x - x.y * e.y;
+      a[k][1] += x.x * e.y + x.y * e.x;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) a[k][c] += __shfl_down(a[k][c], o, 64);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      tmp[k + 3 * ((long)j + (long)nb * w)] = make_double2(-a[k][0], -a[k][1]);
+  }
+}
+
+// Real frequencies (correlation.f:761-763): headc(k1,k2,w) += sum_i
+// wingc(k1,i,2,w) tmp1(k2,i,w) (the old second wing, the new first one, no
+// conjugation), then wingc(:,:,1) = tmp1 and wingc(:,:,2) = tmp2.  One
+// workgroup per frequency; each thread reads and writes only its own i.
+__global__ __launch_bounds__(256) void gamma_head_cplx_kernel(int nb, double2 *__restrict__ head,
+                                                              double2 *__restrict__ wing,
+                                                              const double2 *__restrict__ t1,
+                                                              const double2 *__restrict__ t2) {
+  __shared__ double2 red[256][9];
+  const int w = blockIdx.x;
+  double2 a[9];
+#pragma unroll
+  for (int e = 0; e < 9; e++) a[e] = make_double2(0.0, 0.0);
+  double2 *w1 = wing + 3 * (long)nb * (2 * (long)w), *w2 = w1 + 3 * (long)nb;
+  const double2 *n1 = t1 + 3 * (long)nb * w, *n2 = t2 + 3 * (long)nb * w;
+  for (int i = threadIdx.x; i < nb; i += 256) {
+    double2 o[3], u[3], v[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      o[k] = w2[k + 3 * (long)i];
+      u[k] = n1[k + 3 * (long)i];
+      v[k] = n2[k + 3 * (long)i];
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < 3; k2++)
+#pragma unroll
+      for (int k1 = 0; k1 < 3; k1++) {
+        a[k1 + 3 * k2].x += o[k1].x * u[k2].x - o[k1].y * u[k2].y;
+        a[k1 + 3 * k2].y += o[k1].x * u[k2].y + o[k1].y * u[k2].x;
+      }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      w1[k + 3 * (long)i] = u[k];
+      w2[k + 3 * (long)i] = v[k];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; e++) red[threadIdx.x][e] = a[e];
+  __syncthreads();
+  if (threadIdx.x < 9) {
+    double sx = 0.0, sy = 0.0;
+    for (int r = 0; r < 256; r++) {
+      sx += red[r][threadIdx.x].x;
+      sy += red[r][threadIdx.x].y;
+    }
+    double2 *h = head + threadIdx.x + 9 * (long)w;
+    *h = make_double2(h->x + sx, h->y + sy);
+  }
+}
+
 // E(0,0,w) = trace(head(:,:,w))/3, or vec^T head vec with a unit vector.
 __global__ void gamma_head_scalar_kernel(int nw, double2 *__restrict__ E, long lde, long nb,
                                          const double2 *__restrict__ head, int use_vec, double v0,
@@ -715,7 +800,7 @@ int spexhip_gamma_prepare_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   const double fourpi = 4 * M_PI;
   hipLaunchKernelGGL(gamma_prepare_kernel, dim3((unsigned)((3 * (long)nb + 255) / 256), nw),
-                     dim3(256), 0, ctx->stream, nb, nw, (double2 *)X, (long)ldx, (double2 *)head,
+                     dim3(256), 0, ctx->stream, nb, nw, 1, (double2 *)X, (long)ldx, (double2 *)head,
                      (double2 *)wing, (const double *)sqcoul, fourpi, std::sqrt(fourpi));
   SPX_LAUNCHED(ctx);
   return SPEXHIP_OK;
@@ -735,10 +820,56 @@ int spexhip_gamma_inverse_dev(spexhip_ctx *ctx, int nb, int nw, const void *Einv
   if (rc) return rc;
   hipLaunchKernelGGL(gamma_wing_kernel, dim3((unsigned)((nb + 63) / 64), nw), dim3(256), 0,
                      ctx->stream, nb, (const double2 *)Einv, (long)lde, (const double2 *)wing,
-                     (double2 *)tmp);
+                     3 * (long)nb, (double2 *)tmp);
   SPX_LAUNCHED(ctx);
   hipLaunchKernelGGL(gamma_head_kernel, dim3(nw), dim3(256), 0, ctx->stream, nb, (double2 *)head,
                      (double2 *)wing, (const double2 *)tmp);
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
+int spexhip_gamma_prepare_cplx_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx, void *head,
+                                   void *wing, const void *sqcoul) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && ldx >= nb, "gamma_prepare_cplx: bad sizes");
+  SPX_CHECK(ctx, spx_owns(ctx, X, 16 * (size_t)ldx * nb * nw) &&
+                     spx_owns(ctx, head, 16 * 9 * (size_t)nw) &&
+                     spx_owns(ctx, wing, 16 * 6 * (size_t)nb * nw) &&
+                     spx_owns(ctx, sqcoul, 8 * (size_t)nb),
+            "gamma_prepare_cplx: arrays are not context allocations of the stated shapes");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const double fourpi = 4 * M_PI;
+  hipLaunchKernelGGL(gamma_prepare_kernel, dim3((unsigned)((3 * (long)nb + 255) / 256), nw),
+                     dim3(256), 0, ctx->stream, nb, nw, 2, (double2 *)X, (long)ldx, (double2 *)head,
+                     (double2 *)wing, (const double *)sqcoul, fourpi, std::sqrt(fourpi));
+  SPX_LAUNCHED(ctx);
+  return SPEXHIP_OK;
+}
+
+int spexhip_gamma_inverse_cplx_dev(spexhip_ctx *ctx, int nb, int nw, const void *Einv, int lde,
+                                   void *head, void *wing) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && lde >= nb, "gamma_inverse_cplx: bad sizes");
+  SPX_CHECK(ctx, spx_owns(ctx, Einv, 16 * (size_t)lde * nb * nw) &&
+                     spx_owns(ctx, head, 16 * 9 * (size_t)nw) &&
+                     spx_owns(ctx, wing, 16 * 6 * (size_t)nb * nw),
+            "gamma_inverse_cplx: arrays are not context allocations of the stated shapes");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *tmp;
+  int rc = spx_scratch(ctx, 16 * 6 * (size_t)nb * nw, &tmp);
+  if (rc) return rc;
+  double2 *t1 = (double2 *)tmp, *t2 = t1 + 3 * (size_t)nb * nw;
+  // t1 = -wingc(:,:,1) dielecc^T ; t2 = -wingc(:,:,2) dielecc
+  hipLaunchKernelGGL(gamma_wing_kernel, dim3((unsigned)((nb + 63) / 64), nw), dim3(256), 0,
+                     ctx->stream, nb, (const double2 *)Einv, (long)lde, (const double2 *)wing,
+                     6 * (long)nb, t1);
+  SPX_LAUNCHED(ctx);
+  hipLaunchKernelGGL(gamma_wing_t_kernel, dim3((unsigned)nb, nw), dim3(64), 0, ctx->stream, nb,
+                     (const double2 *)Einv, (long)lde, (const double2 *)wing + 3 * (long)nb,
+                     6 * (long)nb, t2);
+  SPX_LAUNCHED(ctx);
+  hipLaunchKernelGGL(gamma_head_cplx_kernel, dim3(nw), dim3(256), 0, ctx->stream, nb,
+                     (double2 *)head, (double2 *)wing, (const double2 *)t1, (const double2 *)t2);
   SPX_LAUNCHED(ctx);
   return SPEXHIP_OK;
 }

@@ -32,8 +32,10 @@ int spx_wfprod_pw_async(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p,
 int spx_dielectric(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, int vkind,
                    const void *vh, void *E, long lde, void *tmp);
 size_t spx_inverse_ws(int nb, int nw);
+// general != 0: complex pivots (a general matrix, no pivoting); logdet_parts
+// only for the HPD form
 int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, void *ws,
-                int *flag, double *logdet_parts);
+                int *flag, double *logdet_parts, int general = 0);
 int spx_screen(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, const void *Einv, long lde,
                int vkind, const void *vh, void *W, long ldw, void *tmp);
 // packing and fills

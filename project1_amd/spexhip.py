@@ -333,6 +333,10 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_gamma_prepare_dev": (I, [_vp, I, I, _vp, I, _vp, _vp, _vp]),
         "spexhip_gamma_inverse_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_gamma_head_scalar_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_dielectric_general_dev": (I, [_vp, I, I, _vp, I, I, _vp, _vp, I]),
+        "spexhip_gamma_prepare_cplx_dev": (I, [_vp, I, I, _vp, I, _vp, _vp, _vp]),
+        "spexhip_gamma_inverse_cplx_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
+        "spexhip_inverse_general_dev": (I, [_vp, I, I, _vp, I]),
         "spexhip_dielectric_inverse_gamma": (I, [_vp, I, I, _vp, _vp, _vp, _vp, _vp]),
         "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
         "spexhip_synth_normal_rows_dev": (I, [_vp, _vp, _c_long, _c_long, _c_long,
@@ -760,6 +764,16 @@ class Context:
         self._check(lib().spexhip_gamma_inverse_dev(self.h, nb, nw, einv.ptr, lde, head.ptr,
                                                     wing.ptr))
 
+    def gamma_prepare_cplx_dev(self, nb: int, nw: int, X: DevBuf, ldx: int, head: DevBuf,
+                               wing: DevBuf, sqcoul: DevBuf) -> None:
+        self._check(lib().spexhip_gamma_prepare_cplx_dev(self.h, nb, nw, X.ptr, ldx, head.ptr,
+                                                         wing.ptr, sqcoul.ptr))
+
+    def gamma_inverse_cplx_dev(self, nb: int, nw: int, einv: DevBuf, lde: int, head: DevBuf,
+                               wing: DevBuf) -> None:
+        self._check(lib().spexhip_gamma_inverse_cplx_dev(self.h, nb, nw, einv.ptr, lde, head.ptr,
+                                                         wing.ptr))
+
     def gamma_head_scalar_dev(self, nb: int, nw: int, E: DevBuf, lde: int, head: DevBuf,
                               vec=None) -> None:
         """E(0,0,w) = trace(head)/3, or vec^T head vec (unit vector vec)."""
@@ -911,6 +925,15 @@ class Context:
         self._check(lib().spexhip_rpa_integrand_packed(self.h, nb, nw, _ptr(e), weight,
                                                        _ptr(out)))
         return out
+
+    def dielectric_general_dev(self, X: DevBuf, nb: int, nw: int, ldx: int, vkind: int,
+                               vhalf: DevBuf, E: DevBuf, lde: int) -> None:
+        """Real-frequency eps = 1 - v^1/2 chi v^1/2 (no Hermitization)."""
+        self._check(lib().spexhip_dielectric_general_dev(self.h, nb, nw, X.ptr, ldx, vkind,
+                                                         vhalf.ptr, E.ptr, lde))
+
+    def inverse_general_dev(self, X: DevBuf, nb: int, nw: int, ldx: int) -> None:
+        self._check(lib().spexhip_inverse_general_dev(self.h, nb, nw, X.ptr, ldx))
 
     def inverse_hpd_dev(self, X: DevBuf, nb: int, nw: int, ldx: int) -> None:
         self._check(lib().spexhip_inverse_hpd_dev(self.h, nb, nw, X.ptr, ldx))

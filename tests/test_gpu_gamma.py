@@ -282,3 +282,83 @@ def test_head_scalar_and_gamma_rpa(ctx):
         assert abs(got[f] - want) <= 1e-10 * abs(want), (f, got[f], want)
     for b in (dX, dH, dW, dS):
         b.free()
+
+
+def _absorptive_chi(g, nb, nw):
+    """A real-frequency-like chi: Hermitian part of either sign, anti-
+    Hermitian part -i C C^H (absorption), so eps has a non-negative
+    imaginary part."""
+    out = np.empty((nb, nb, nw), complex)
+    for f in range(nw):
+        A = (g.standard_normal((nb, nb)) + 1j * g.standard_normal((nb, nb))) / math.sqrt(nb)
+        Cm = (g.standard_normal((nb, nb)) + 1j * g.standard_normal((nb, nb))) / math.sqrt(nb)
+        out[:, :, f] = 0.3 * (A + A.conj().T) / 2 - 0.5j * (Cm @ Cm.conj().T)
+    return out
+
+
+@pytest.mark.parametrize("vkind", [0, 1])
+def test_general_dielectric_and_inverse_vs_numpy(ctx, vkind):
+    nb, nw = 130, 3
+    g = np.random.default_rng(30 + vkind)
+    chi = _absorptive_chi(g, nb, nw)
+    if vkind == 0:
+        vh = g.uniform(0.3, 1.2, nb)
+        V = np.diag(vh).astype(complex)
+        dV = ctx.to_dev(vh)
+    else:
+        q, _ = np.linalg.qr(g.standard_normal((nb, nb)))
+        V = (q * g.uniform(0.3, 1.2, nb)) @ q.T
+        V = V.astype(complex)
+        dV = _dev_f(ctx, V, np.complex128)
+    dX = _dev_f(ctx, chi, np.complex128)
+    dE = ctx.alloc(16 * nb * nb * nw)
+    ctx.dielectric_general_dev(dX, nb, nw, nb, vkind, dV, dE, nb)
+    eps = dE.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)
+    want_eps = np.stack([np.eye(nb) - V @ chi[:, :, f] @ V for f in range(nw)], 2)
+    assert normwise(eps, want_eps)[0] < 1e-13
+    ctx.inverse_general_dev(dE, nb, nw, nb)
+    inv = dE.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)
+    for f in range(nw):
+        mx, _ = normwise(inv[:, :, f], np.linalg.inv(want_eps[:, :, f]))
+        assert mx < 1e-12, (f, mx)
+    for b in (dV, dX, dE):
+        b.free()
+
+
+def test_general_inverse_rejects_a_zero_pivot(ctx):
+    nb = 70
+    X = np.eye(nb, dtype=complex)
+    X[0, 0] = 0.0
+    X[0, 1] = X[1, 0] = 1.0  # invertible, but the first pivot is zero
+    dX = _dev_f(ctx, X, np.complex128)
+    with pytest.raises(spexhip.SpexError):
+        ctx.inverse_general_dev(dX, nb, 1, nb)
+    dX.free()
+
+
+def test_gamma_real_frequency_chain_vs_oracle(ctx):
+    nb, nw = 96, 2
+    g = np.random.default_rng(40)
+    chi = _absorptive_chi(g, nb, nw)
+    sq = g.uniform(0.3, 1.2, nb)
+    head = (g.standard_normal((3, 3, nw)) + 1j * g.standard_normal((3, 3, nw))) / 20
+    wing = (g.standard_normal((3, nb, 2, nw)) + 1j * g.standard_normal((3, nb, 2, nw))) / 30
+    eps_o, h_o, w_o = gamma.gamma_prepare_cplx(chi, head, wing, sq)
+    einv_o = np.stack([np.linalg.inv(eps_o[:, :, f]) for f in range(nw)], 2)
+    h_o, w_o = gamma.gamma_inverse_cplx(einv_o, h_o, w_o)
+    dX = _dev_f(ctx, chi, np.complex128)
+    dH = _dev_f(ctx, head, np.complex128)
+    dW = _dev_f(ctx, wing, np.complex128)
+    dS = ctx.to_dev(sq)
+    ctx.gamma_prepare_cplx_dev(nb, nw, dX, nb, dH, dW, dS)
+    ctx.dielectric_general_dev(dX, nb, nw, nb, 0, dS, dX, nb)
+    ctx.inverse_general_dev(dX, nb, nw, nb)
+    ctx.gamma_inverse_cplx_dev(nb, nw, dX, nb, dH, dW)
+    einv = dX.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)
+    h = dH.download(np.complex128, (nw, 3, 3)).transpose(2, 1, 0)
+    w = dW.download(np.complex128, (nw, 2, nb, 3)).transpose(3, 2, 1, 0)
+    assert normwise(einv, einv_o)[0] < 1e-12
+    assert normwise(h, h_o)[0] < 1e-12
+    assert normwise(w, w_o)[0] < 1e-12
+    for b in (dX, dH, dW, dS):
+        b.free()

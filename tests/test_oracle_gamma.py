@@ -185,3 +185,47 @@ def test_head_scalar_closed_forms():
     assert abs(avg - gamma.head_scalar(hp)) < 0.05 * abs(gamma.head_scalar(hp))
     assert np.isclose(gamma.head_scalar(hp), np.trace(H) / 3)
     assert np.isclose(gamma.head_scalar(hp, [0.0, 1.0, 0.0]), H[1, 1])
+
+
+def test_real_frequency_prepare_closed_form():
+    g = np.random.default_rng(20)
+    nb, nw = 5, 2
+    chi = g.standard_normal((nb, nb, nw)) + 1j * g.standard_normal((nb, nb, nw))
+    sq = g.uniform(0.5, 2.0, nb)
+    head = g.standard_normal((3, 3, nw)) + 1j * g.standard_normal((3, 3, nw))
+    wing = g.standard_normal((3, nb, 2, nw)) + 1j * g.standard_normal((3, nb, 2, nw))
+    eps, h, w = gamma.gamma_prepare_cplx(chi, head, wing, sq)
+    for f in range(nw):
+        c = chi[:, :, f].copy()
+        c[0, :] = 0
+        c[:, 0] = 0
+        np.testing.assert_allclose(eps[:, :, f], np.eye(nb) - np.diag(sq) @ c @ np.diag(sq),
+                                   rtol=1e-14, atol=1e-14)
+    np.testing.assert_allclose(h, np.eye(3)[:, :, None] - 4 * math.pi * head, rtol=1e-15)
+    assert (w[:, 0] == 0).all()
+    np.testing.assert_allclose(w[:, 1:], -math.sqrt(4 * math.pi) * sq[None, 1:, None, None] *
+                               wing[:, 1:], rtol=1e-15)
+
+
+def test_real_frequency_inverse_is_the_bordered_matrix_inverse():
+    """Non-Hermitian case: with E = [[H, w2], [w1^T, B]] the reference's
+    update gives S = H - w2 B^-1 w1^T, w1_new^T S^-1 below the head block of
+    inv(E) and S^-1 w2_new beside it."""
+    g = np.random.default_rng(21)
+    nb, nw = 7, 2
+    heads, wings, einvs, Es = [], [], [], []
+    for _ in range(nw):
+        E = np.eye(nb + 3) * (nb + 3) + g.standard_normal((nb + 3, nb + 3)) + \
+            1j * g.standard_normal((nb + 3, nb + 3))
+        H, X, Y, B = E[:3, :3], E[:3, 3:], E[3:, :3], E[3:, 3:]
+        heads.append(H)
+        wings.append(np.stack([Y.T, X], axis=2))  # wingc(:,:,1) = Y^T, wingc(:,:,2) = X
+        einvs.append(np.linalg.inv(B))
+        Es.append(E)
+    h, w = gamma.gamma_inverse_cplx(np.stack(einvs, 2), np.stack(heads, 2), np.stack(wings, 3))
+    for f in range(nw):
+        Ei = np.linalg.inv(Es[f])
+        Si = np.linalg.inv(h[:, :, f])
+        np.testing.assert_allclose(Si, Ei[:3, :3], rtol=1e-11, atol=1e-13)
+        np.testing.assert_allclose(w[:, :, 0, f].T @ Si, Ei[3:, :3], rtol=1e-11, atol=1e-13)
+        np.testing.assert_allclose(Si @ w[:, :, 1, f], Ei[:3, 3:], rtol=1e-11, atol=1e-13)
