@@ -175,6 +175,17 @@ int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nun
                                      const void *w_full_dev, long ldw, const int *kpt1p,
                                      int nkpt1, void *w_red_dev, long ldr);
 
+/* Host drop-in for tetrahedron5_init on host arrays: e_occ [nk][nocc],
+ * e_unocc [nk][nunocc] (full mesh), frqs [nfrqs]; kpt1p [nk] (or NULL: the
+ * full mesh, nkpt1 ignored) and deg_occ [nkpt1][nocc], deg_unocc
+ * [nkpt1][nunocc] (both NULL: no averaging).  wtetra (host, nkr*nocc*
+ * nunocc x nfrqs, t fastest) is overwritten, nkr = nkpt1 or nk.           */
+int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
+                                int nunocc, const double *e_occ, const double *e_unocc,
+                                double efermi, int nfrqs, const double *frqs, const int *kpt1p,
+                                int nkpt1, const int *deg_occ, const int *deg_unocc,
+                                double *wtetra);
+
 /* ---- chi0 pair sum ----------------------------------------------------- */
 /* Device form of the pair loop susceptibility.f:1159-1251 with
  * spectral_function HILBERT NONE (:1780-1788):

@@ -600,3 +600,70 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Host drop-in for tetrahedron5_init (tetrahedron.f:903-1332) on host
+// arrays: the full-mesh weights, then (kpt1p given) the rows of the reduced
+// set and (deg arrays given) the averaging over degenerate states.
+int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
+                                int nunocc, const double *e_occ, const double *e_unocc,
+                                double efermi, int nfrqs, const double *frqs, const int *kpt1p,
+                                int nkpt1, const int *deg_occ, const int *deg_unocc,
+                                double *wtetra) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nkpt3 && e_occ && e_unocc && wtetra && nocc > 0 && nunocc > 0,
+            "tetrahedron_weights: bad arguments");
+  SPX_CHECK(ctx, (deg_occ == nullptr) == (deg_unocc == nullptr),
+            "tetrahedron_weights: give both deg arrays or neither");
+  const long nk = (long)nkpt3[0] * nkpt3[1] * nkpt3[2];
+  const int nkr = kpt1p ? nkpt1 : (int)nk;
+  SPX_CHECK(ctx, nk > 0 && nkr > 0, "tetrahedron_weights: bad sizes");
+  const long n = nk * nocc * nunocc, nr = (long)nkr * nocc * nunocc;
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *dEo = nullptr, *dEu = nullptr, *dW = nullptr, *dR = nullptr, *dDo = nullptr,
+       *dDu = nullptr;
+  int rc = spx_alloc(ctx, 8 * (size_t)nk * nocc, &dEo);
+  if (!rc) rc = spx_alloc(ctx, 8 * (size_t)nk * nunocc, &dEu);
+  if (!rc) rc = spx_alloc(ctx, 8 * (size_t)n * nfrqs, &dW);
+  if (!rc && kpt1p) rc = spx_alloc(ctx, 8 * (size_t)nr * nfrqs, &dR);
+  if (!rc && deg_occ) rc = spx_alloc(ctx, 4 * (size_t)nkr * nocc, &dDo);
+  if (!rc && deg_occ) rc = spx_alloc(ctx, 4 * (size_t)nkr * nunocc, &dDu);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(dEo, e_occ, 8 * (size_t)nk * nocc, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dEu, e_unocc, 8 * (size_t)nk * nunocc, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && deg_occ)
+      e = hipMemcpyAsync(dDo, deg_occ, 4 * (size_t)nkr * nocc, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess && deg_occ)
+      e = hipMemcpyAsync(dDu, deg_unocc, 4 * (size_t)nkr * nunocc, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "tetrahedron_weights: upload failed: %s",
+                      hipGetErrorString(e));
+  }
+  if (!rc)
+    rc = spexhip_tetra_weights_dev(ctx, nkpt3, rlat, nocc, nunocc, dEo, dEu, efermi, nfrqs, frqs,
+                                   dW, n);
+  void *res = dW;
+  if (!rc && kpt1p) {
+    rc = spexhip_tetra_reduce_parents_dev(ctx, (int)nk, nocc, nunocc, nfrqs, dW, n, kpt1p, nkr, dR,
+                                          nr);
+    res = dR;
+  }
+  if (!rc && deg_occ)
+    rc = spexhip_tetra_average_degenerate_dev(ctx, nkr, nocc, nunocc, dDo, dDu, nfrqs, res, nr);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(wtetra, res, 8 * (size_t)nr * nfrqs, hipMemcpyDeviceToHost,
+                                  ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "tetrahedron_weights: download failed: %s",
+                      hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : {dEo, dEu, dW, dR, dDo, dDu})
+    if (p) spx_free(ctx, p);
+  return rc;
+}
+
+}  // extern "C"

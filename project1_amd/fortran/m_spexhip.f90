@@ -41,7 +41,7 @@ module m_spexhip
   public :: spexhip_plan_destroy
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
-  public :: spexhip_headwings, spexhip_dielec_inverse_gamma
+  public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -83,6 +83,14 @@ module m_spexhip
                             headc, wingc
       integer(c_long), value :: n, ldc, lddc
       integer(c_int), value :: nb, nfrqh, nfrqc
+    end function
+    integer(c_int) function c_tetrahedron_weights(ctx, nkpt3, rlat, nocc, nunocc, eocc, eunocc, &
+        efermi, nfrqs, frqs, kpt1p, nkpt1, degocc, degunocc, wtetra) &
+        bind(C, name='spexhip_tetrahedron_weights')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, nkpt3, rlat, eocc, eunocc, frqs, kpt1p, degocc, degunocc, wtetra
+      integer(c_int), value :: nocc, nunocc, nfrqs, nkpt1
+      real(c_double), value :: efermi
     end function
     integer(c_int) function c_dielectric_inverse_gamma(ctx, nb, nw, chi, sqcoul, head, wing, &
         einv) bind(C, name='spexhip_dielectric_inverse_gamma')
@@ -240,6 +248,40 @@ contains
     call check(ctx, c_dielectric_inverse_gamma(ctx, int(nbas, c_int), int(nfreq, c_int), &
                c_loc(suscep), c_loc(sqcoul), c_loc(head), c_loc(wing), c_loc(dielec)), &
                'dielec_inverse_gamma')
+  end subroutine
+
+  ! tetrahedron5_init (tetrahedron.f:903-1332) for kflag = 1: ene_occ(nocc,nk) at k and
+  ! ene_unocc(nunocc,nk) at k+q on the full mesh nkpt3 (k = (i1*N2+i2)*N3+i3, 0-based
+  ! coordinates), frqs(nfrqs) the Hilbert mesh; optional kpt1p(nk) (1-based parents,
+  ! 0 = none) with nkpt1, and deg_occ(nocc,nkpt1), deg_unocc(nunocc,nkpt1) (1-based band
+  ! numbers within the windows, SPEX's deg convention).  wtetra: nkr*nocc*nunocc x nfrqs
+  ! (nkr = nkpt1 with kpt1p, else the mesh size), transition fastest.
+  subroutine spexhip_tetrahedron5(ctx, nkpt3, rlat, nocc, nunocc, ene_occ, ene_unocc, efermi, &
+                                  nfrqs, frqs, wtetra, kpt1p, nkpt1, deg_occ, deg_unocc)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nkpt3(3), nocc, nunocc, nfrqs
+    real(c_double), intent(in), target :: rlat(3,3), frqs(nfrqs), &
+                                          ene_occ(nocc, nkpt3(1)*nkpt3(2)*nkpt3(3)), &
+                                          ene_unocc(nunocc, nkpt3(1)*nkpt3(2)*nkpt3(3))
+    real(c_double), intent(in) :: efermi
+    real(c_double), intent(out), target :: wtetra(*)
+    integer, intent(in), optional :: kpt1p(:), nkpt1, deg_occ(:,:), deg_unocc(:,:)
+    integer(c_int), target :: k3(3)
+    integer(c_int), allocatable, target :: kp(:), dgo(:,:), dgu(:,:)
+    type(c_ptr) :: pk, pdo, pdu
+    integer(c_int) :: nk1
+    k3 = nkpt3
+    pk = c_null_ptr ; pdo = c_null_ptr ; pdu = c_null_ptr ; nk1 = 0
+    if (present(kpt1p)) then
+      allocate(kp(size(kpt1p))) ; kp = kpt1p - 1 ; pk = c_loc(kp) ; nk1 = nkpt1
+    end if
+    if (present(deg_occ)) then
+      allocate(dgo(size(deg_occ,1),size(deg_occ,2)), dgu(size(deg_unocc,1),size(deg_unocc,2)))
+      dgo = deg_occ - 1 ; dgu = deg_unocc - 1 ; pdo = c_loc(dgo) ; pdu = c_loc(dgu)
+    end if
+    call check(ctx, c_tetrahedron_weights(ctx, c_loc(k3), c_loc(rlat), int(nocc, c_int), &
+               int(nunocc, c_int), c_loc(ene_occ), c_loc(ene_unocc), efermi, int(nfrqs, c_int), &
+               c_loc(frqs), pk, nk1, pdo, pdu, c_loc(wtetra)), 'tetrahedron5')
   end subroutine
 
   ! wavefproducts2_mt(cprod,k1,nk,ik0,ispin1,ispin2,b1low,b1up,b2low,b2up)

@@ -321,6 +321,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_tetra_average_degenerate_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long]),
         "spexhip_tetra_reduce_parents_dev": (I, [_vp, I, I, I, I, _vp, _c_long, _vp, I, _vp,
                                                  _c_long]),
+        "spexhip_tetrahedron_weights": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp,
+                                            _vp, I, _vp, _vp, _vp]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
@@ -584,6 +586,29 @@ class Context:
         kp = np.ascontiguousarray(kpt1p, np.int32)
         self._check(lib().spexhip_tetra_reduce_parents_dev(
             self.h, nk, nocc, nunocc, nfrqs, w_full.ptr, ldw, _ptr(kp), nkpt1, w_red.ptr, ldr))
+
+    def tetrahedron_weights(self, nkpt3, rlat, e_occ: np.ndarray, e_unocc: np.ndarray,
+                            efermi: float, frqs: np.ndarray, kpt1p=None, nkpt1: int = 0,
+                            deg_occ=None, deg_unocc=None) -> np.ndarray:
+        """Host drop-in of tetrahedron5_init: returns wtetra (nkr*nocc*nunocc,
+        nfrqs), Fortran-ordered."""
+        k3 = np.ascontiguousarray(nkpt3, np.int32)
+        rl = np.asfortranarray(rlat, np.float64)
+        eo = np.ascontiguousarray(e_occ, np.float64)
+        eu = np.ascontiguousarray(e_unocc, np.float64)
+        fr = np.ascontiguousarray(frqs, np.float64)
+        nk, nocc = eo.shape
+        nunocc = eu.shape[1]
+        kp = None if kpt1p is None else np.ascontiguousarray(kpt1p, np.int32)
+        nkr = nkpt1 if kp is not None else nk
+        do = None if deg_occ is None else np.ascontiguousarray(deg_occ, np.int32)
+        du = None if deg_unocc is None else np.ascontiguousarray(deg_unocc, np.int32)
+        out = np.zeros((nkr * nocc * nunocc, fr.size), np.float64, order="F")
+        p = lambda a: None if a is None else _ptr(a)
+        self._check(lib().spexhip_tetrahedron_weights(
+            self.h, _ptr(k3), rl.ctypes.data, nocc, nunocc, _ptr(eo), _ptr(eu), efermi, fr.size,
+            _ptr(fr), p(kp), nkpt1, p(do), p(du), out.ctypes.data))
+        return out
 
     # -- chi0 -------------------------------------------------------------
     def chi0_cplx_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int,

@@ -191,3 +191,30 @@ def test_reduce_to_parents_vs_oracle(ctx):
     assert mx < 1e-15
     for b in (dF, dR):
         b.free()
+
+
+def test_host_dropin_chain_vs_oracle(ctx):
+    """Host drop-in: metal weights on the full mesh, star sums over kpt1p,
+    then degenerate averaging, against the oracle chain."""
+    nkpt3 = (3, 2, 2)
+    nk, nocc, nunocc = 12, 2, 3
+    g = np.random.default_rng(99)
+    kx = np.arange(nk)
+    band = 0.4 * np.cos(2 * np.pi * (kx % 2) / 2) + 0.3 * np.sin(kx * 1.3)
+    eo = np.stack([band - 0.7, band], 1) + 0.01 * g.standard_normal((nk, 2))
+    eu = np.stack([band, band + 0.5, band + 0.5], 1)
+    eu[:, 0] = eo[:, 1]
+    ef = 0.02
+    frq = 0.01 * (1.2 ** np.arange(28) - 1)
+    kpt1p = np.array([0, 1, 1, 2, 0, 3, 2, 3, 4, 4, 4, 1])
+    nkpt1 = 5
+    eo_r = np.stack([eo[np.where(kpt1p == p)[0][0]] for p in range(nkpt1)])
+    eu_r = np.stack([eu[np.where(kpt1p == p)[0][0]] for p in range(nkpt1)])
+    do = tetra.deg_from_energies(eo_r, 1e-12).astype(np.int32)
+    du = tetra.deg_from_energies(eu_r, 1e-12).astype(np.int32)
+    got = ctx.tetrahedron_weights(nkpt3, RL_SKEW, eo, eu, ef, frq, kpt1p, nkpt1, do, du)
+    full = tetra.tetrahedron5_weights_metal(nkpt3, RL_SKEW, eo, eu, ef, frq)
+    red = tetra.reduce_to_parents(full, nk, nocc, nunocc, kpt1p, nkpt1)
+    want = tetra.average_degenerate(red, nkpt1, nocc, nunocc, do, du)
+    mx, _ = normwise(got, want)
+    assert mx < 1e-12
