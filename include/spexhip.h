@@ -158,7 +158,8 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
  * SPEX's deg convention: for the first band of a group the group's last
  * band, for the others the first band.  Groups must lie inside the band
  * windows (the reference's extension of windows that cut a group is not
- * done).                                                                   */
+ * done).  SPEXHIP_E_INVALID if a deg row is not consistent (an index
+ * outside the window); those bands are then left unaveraged.             */
 int spexhip_tetra_average_degenerate_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
                                          const void *deg_occ_dev, const void *deg_unocc_dev,
                                          int nfrqs, void *wtetra_dev, long ldwt);

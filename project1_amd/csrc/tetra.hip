@@ -395,15 +395,26 @@ __global__ __launch_bounds__(TW_NT) void tetra_weights_kernel(
 // First and last band of b's degenerate group from a deg row (SPEX's
 // convention: deg(b) is the group's last band if b is its first, else the
 // first band).
-__device__ inline void deg_group(const int *__restrict__ deg, int b, int &first, int &last) {
+// Returns false (and b's own band as its group) when the row is not a
+// consistent deg array (an index outside the window, or first > b > last).
+__device__ inline bool deg_group(const int *__restrict__ deg, int nb, int b, int &first,
+                                 int &last) {
   const int d = deg[b];
   if (d >= b) {
     first = b;
     last = d;
-  } else {
+  } else if (d >= 0) {
     first = d;
     last = deg[d];
+  } else {
+    first = last = b;
+    return false;
   }
+  if (last >= nb || last < b || first > b) {
+    first = last = b;
+    return false;
+  }
+  return true;
 }
 
 // out(t, i) = mean of in over the degenerate group of b1 (side 0, occupied
@@ -412,7 +423,7 @@ __device__ inline void deg_group(const int *__restrict__ deg, int b, int &first,
 // 1233-1273).  Out of place: every thread reads only `in`.
 __global__ void deg_average_kernel(int nk, int nocc, int nunocc, const int *__restrict__ deg,
                                    int side, int nfrq, const double *__restrict__ in, long ldi,
-                                   double *__restrict__ out, long ldo) {
+                                   double *__restrict__ out, long ldo, int *__restrict__ flag) {
   const long n = (long)nk * nocc * nunocc;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
@@ -423,13 +434,15 @@ __global__ void deg_average_kernel(int nk, int nocc, int nunocc, const int *__re
   const int k = (int)(r / nocc);
   int first, last;
   double s = 0.0;
+  bool ok;
   if (side == 0) {
-    deg_group(deg + (long)k * nocc, b1, first, last);
+    ok = deg_group(deg + (long)k * nocc, nocc, b1, first, last);
     for (int b = first; b <= last; b++) s += in[((long)k * nocc + b) * nunocc + b2 + ldi * i];
   } else {
-    deg_group(deg + (long)k * nunocc, b2, first, last);
+    ok = deg_group(deg + (long)k * nunocc, nunocc, b2, first, last);
     for (int b = first; b <= last; b++) s += in[((long)k * nocc + b1) * nunocc + b + ldi * i];
   }
+  if (!ok && i == 0) atomicOr(flag, 1);
   out[t + ldo * i] = s / (last - first + 1);
 }
 
@@ -509,17 +522,27 @@ int spexhip_tetra_average_degenerate_dev(spexhip_ctx *ctx, int nk, int nocc, int
             "tetra_average_degenerate: arrays are not context allocations of the stated shapes");
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   void *tmp;
-  int rc = spx_scratch(ctx, 8 * (size_t)n * nfrqs, &tmp);
+  int rc = spx_scratch(ctx, 8 * (size_t)n * nfrqs + 256, &tmp);
+  if (rc) return rc;
+  int *flag = (int *)((char *)tmp + 8 * (size_t)n * nfrqs);
+  rc = spx_fill_zero(ctx, ctx->stream, flag, sizeof(int));
   if (rc) return rc;
   const dim3 grid((unsigned)((n + 255) / 256), (unsigned)nfrqs);
   hipLaunchKernelGGL(deg_average_kernel, grid, dim3(256), 0, ctx->stream, nk, nocc, nunocc,
                      (const int *)deg_occ, 0, nfrqs, (const double *)wtetra, ldwt, (double *)tmp,
-                     n);
+                     n, flag);
   SPX_LAUNCHED(ctx);
   hipLaunchKernelGGL(deg_average_kernel, grid, dim3(256), 0, ctx->stream, nk, nocc, nunocc,
                      (const int *)deg_unocc, 1, nfrqs, (const double *)tmp, n, (double *)wtetra,
-                     ldwt);
+                     ldwt, flag);
   SPX_LAUNCHED(ctx);
+  int h = 0;
+  SPX_HIP(ctx, hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h)
+    return spx_seterr(ctx, SPEXHIP_E_INVALID,
+                      "tetra_average_degenerate: a deg array is not consistent (the groups' "
+                      "bands were left unaveraged)");
   return SPEXHIP_OK;
 }
 

@@ -218,3 +218,16 @@ def test_host_dropin_chain_vs_oracle(ctx):
     want = tetra.average_degenerate(red, nkpt1, nocc, nunocc, do, du)
     mx, _ = normwise(got, want)
     assert mx < 1e-12
+
+
+def test_degenerate_averaging_rejects_inconsistent_deg(ctx):
+    nk, nocc, nunocc, nf = 2, 3, 2, 4
+    n = nk * nocc * nunocc
+    dW = ctx.to_dev(np.ones((nf, n)))
+    do = np.array([[0, 7, 1], [0, 1, 2]], np.int32)  # 7: outside the window
+    du = np.array([[0, 1], [0, 1]], np.int32)
+    dDo, dDu = ctx.to_dev(do), ctx.to_dev(du)
+    with pytest.raises(spexhip.SpexError):
+        ctx.tetra_average_degenerate_dev(nk, nocc, nunocc, dDo, dDu, nf, dW, n)
+    for b in (dW, dDo, dDu):
+        b.free()
