@@ -1,0 +1,57 @@
+/*
+ * spexcor -- the spex.cor checkpoint file of SPEX (C ABI, host side).
+ *
+ * Replaces writecor / readcor (correlation.f:1636-1952) with the file
+ * layout of COR version 1 and the HDF5 wrapper's conventions (Hwrapper.f):
+ * one group per k point named by the k vector in F12.10 format, the
+ * matrices of each frequency as datasets "matrix_<i>" (packed upper) and
+ * "matrixc_<i>" (full), complex arrays as double datasets with a leading
+ * real/imaginary dimension.  Library libspexcor.so, kept apart from
+ * libspexhip.so (it links HDF5; no device code).
+ *
+ * Arrays are Fortran-ordered host arrays, complex interleaved:
+ *   coulomb  nbasm(nbasm+1)/2 complex (packed)   coul   nbas doubles
+ *   ctrafo   nbasm x nbas complex                 cblock ncblock ints
+ *   matrix   nbas(nbas+1)/2 x nfreq complex       matrixc nbas x nbas x nfreqc
+ *   head     6 x nfreq,  wing 3 x nbas x nfreq    (Gamma point only)
+ *   headc    3 x 3 x nfreqc, wingc 3 x nbas x 2 x nfreqc
+ *   freq     nfreq doubles,  freqc nfreqc complex
+ * Return 0 on success; otherwise spexcor_last_error() holds the message.
+ */
+#ifndef SPEXCOR_H
+#define SPEXCOR_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int spexcor_last_error(char *buf, size_t len);
+
+/* writecor(typ) for typ = 1 (susceptibility), 2 (dielectric), 3 (inverse
+ * dielectric); gamma != 0 writes plasma, head and wings (ikpt = 1).  Like
+ * the reference it refuses to overwrite the matrices of a k point that is
+ * already in the file, and checks nbasm and the frequency counts of an
+ * existing file.                                                          */
+int spexcor_write(const char *fname, const double kvec[3], int typ, const char *label, int nbasm,
+                  const double *coulomb, int nbas, const double *coul, const double *ctrafo,
+                  int ncblock, const int *cblock, int oselfc, const double contour[4],
+                  int gamma, double plasma, int nfreq, const double *freq, const double *matrix,
+                  const double *head, const double *wing, int nfreqc, const double *freqc,
+                  const double *matrixc, const double *headc, const double *wingc);
+
+/* Sizes for readcor: *typ = -1 if the k point is not in the file, 0 if it
+ * holds only the Coulomb matrix, else the matrix type.                    */
+int spexcor_query(const char *fname, const double kvec[3], int *typ, int *nbasm, int *nbas,
+                  int *nfreq, int *nfreqc, int *ncblock);
+
+/* readcor for a k point with matrices; any output pointer may be NULL.
+ * label receives 6 characters (not terminated).                           */
+int spexcor_read(const char *fname, const double kvec[3], char *label, double *coulomb,
+                 double *coul, double *ctrafo, int *cblock, int *oselfc, double contour[4],
+                 double *plasma, double *freq, double *matrix, double *head, double *wing,
+                 double *freqc, double *matrixc, double *headc, double *wingc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

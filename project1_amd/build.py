@@ -92,9 +92,29 @@ def build_fortran() -> Path | None:
     return FDRIVER
 
 
+COR_LIB = PKG / "libspexcor.so"
+HDF5_ROOT = Path(os.environ.get("SPEXCOR_HDF5", "/opt/conda"))
+
+
+def build_cor() -> Path | None:
+    """The spex.cor checkpoint library (host C++ over the HDF5 C API), a
+    library of its own so that libspexhip.so does not depend on HDF5; built
+    when the HDF5 headers are present."""
+    if not (HDF5_ROOT / "include" / "hdf5.h").exists():
+        return None
+    src = PKG / "cor" / "spexcor.cpp"
+    tmp = COR_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{HDF5_ROOT / 'include'}",
+          str(src), "-o", str(tmp), f"-L{HDF5_ROOT / 'lib'}", "-lhdf5",
+          f"-Wl,-rpath,{HDF5_ROOT / 'lib'}"])
+    os.replace(tmp, COR_LIB)
+    return COR_LIB
+
+
 def build_all() -> Path:
     lib = build_lib()
     build_fortran()
+    build_cor()
     return lib
 
 

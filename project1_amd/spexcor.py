@@ -1,0 +1,112 @@
+"""ctypes binding of libspexcor.so: the spex.cor checkpoint file of SPEX
+(writecor/readcor, correlation.f:1636-1952).  Arrays follow the reference
+layouts (Fortran order, complex128)."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB = None
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+
+class CorError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = Path(__file__).resolve().parent / "libspexcor.so"
+        if not path.exists():
+            raise CorError(f"{path} is missing: build with python -m project1_amd.build")
+        L = C.CDLL(str(path))
+        I = C.c_int
+        L.spexcor_last_error.argtypes = [C.c_char_p, C.c_size_t]
+        L.spexcor_write.argtypes = [C.c_char_p, _vp, I, C.c_char_p, I, _vp, I, _vp, _vp, I, _vp, I,
+                                    _vp, I, C.c_double, I, _vp, _vp, _vp, _vp, I, _vp, _vp, _vp,
+                                    _vp]
+        L.spexcor_query.argtypes = [C.c_char_p, _vp] + [C.POINTER(I)] * 6
+        L.spexcor_read.argtypes = [C.c_char_p, _vp, C.c_char_p] + [_vp] * 15
+        for f in (L.spexcor_write, L.spexcor_query, L.spexcor_read):
+            f.restype = I
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int) -> None:
+    if rc:
+        buf = C.create_string_buffer(512)
+        lib().spexcor_last_error(buf, 512)
+        raise CorError(buf.value.decode())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f(a, dt):
+    return None if a is None else np.asfortranarray(a, dtype=dt)
+
+
+def write(fname, kvec, typ, label, coulomb, coul, ctrafo, oselfc=1, contour=(0, 0, 0, 0),
+          cblock=None, freq=None, matrix=None, freqc=None, matrixc=None, gamma=False,
+          plasma=0.0, head=None, wing=None, headc=None, wingc=None) -> None:
+    """writecor: coulomb packed (nbasm(nbasm+1)/2,), coul (nbas,), ctrafo
+    (nbasm, nbas), matrix (npack, nfreq), matrixc (nbas, nbas, nfreqc),
+    head (6, nfreq), wing (3, nbas, nfreq), headc (3, 3, nfreqc), wingc
+    (3, nbas, 2, nfreqc)."""
+    ct = _f(ctrafo, np.complex128)
+    nbasm, nbas = ct.shape
+    k = np.ascontiguousarray(kvec, np.float64)
+    cont = np.ascontiguousarray(contour, np.float64)
+    fr = _f(freq, np.float64)
+    frc = _f(freqc, np.complex128)
+    arrs = dict(coulomb=_f(coulomb, np.complex128), coul=_f(coul, np.float64), cblock=_f(cblock, np.int32),
+                matrix=_f(matrix, np.complex128), matrixc=_f(matrixc, np.complex128),
+                head=_f(head, np.complex128), wing=_f(wing, np.complex128),
+                headc=_f(headc, np.complex128), wingc=_f(wingc, np.complex128))
+    nfreq = 0 if fr is None else fr.size
+    nfreqc = 0 if frc is None else frc.size
+    ncb = 0 if arrs["cblock"] is None else arrs["cblock"].size
+    _check(lib().spexcor_write(str(fname).encode(), _p(k), typ, label.encode(), nbasm,
+                               _p(arrs["coulomb"]), nbas, _p(arrs["coul"]), _p(ct), ncb,
+                               _p(arrs["cblock"]), oselfc, _p(cont), int(gamma), plasma, nfreq,
+                               _p(fr), _p(arrs["matrix"]), _p(arrs["head"]), _p(arrs["wing"]),
+                               nfreqc, _p(frc), _p(arrs["matrixc"]), _p(arrs["headc"]),
+                               _p(arrs["wingc"])))
+
+
+def query(fname, kvec) -> dict:
+    k = np.ascontiguousarray(kvec, np.float64)
+    v = [C.c_int() for _ in range(6)]
+    _check(lib().spexcor_query(str(fname).encode(), _p(k), *[C.byref(x) for x in v]))
+    return dict(zip(("type", "nbasm", "nbas", "nfreq", "nfreqc", "ncblock"), (x.value for x in v)))
+
+
+def read(fname, kvec, gamma=False) -> dict:
+    """readcor: the arrays written by `write` (head/wings when gamma)."""
+    q = query(fname, kvec)
+    if q["type"] < 1:
+        raise CorError("no matrices for this k point")
+    nbasm, nb, nf, nfc = q["nbasm"], q["nbas"], q["nfreq"], q["nfreqc"]
+    z = lambda *s: np.zeros(s, np.complex128, order="F")
+    out = dict(coulomb=z(nbasm * (nbasm + 1) // 2), coul=np.zeros(nb), ctrafo=z(nbasm, nb),
+               cblock=np.zeros(max(q["ncblock"], 1), np.int32), contour=np.zeros(4),
+               freq=np.zeros(max(nf, 1)), matrix=z(nb * (nb + 1) // 2, max(nf, 1)),
+               freqc=z(max(nfc, 1)), matrixc=z(nb, nb, max(nfc, 1)))
+    if gamma:
+        out.update(head=z(6, max(nf, 1)), wing=z(3, nb, max(nf, 1)), headc=z(3, 3, max(nfc, 1)),
+                   wingc=z(3, nb, 2, max(nfc, 1)))
+    lab = C.create_string_buffer(7)
+    osf, pl = C.c_int(), C.c_double()
+    g = lambda key: _p(out.get(key))
+    _check(lib().spexcor_read(str(fname).encode(), _p(np.ascontiguousarray(kvec, np.float64)), lab,
+                              g("coulomb"), g("coul"), g("ctrafo"), g("cblock"), C.addressof(osf),
+                              g("contour"), C.addressof(pl), g("freq"), g("matrix"), g("head"),
+                              g("wing"), g("freqc"), g("matrixc"), g("headc"), g("wingc")))
+    out.update(q, label=lab.raw[:6].decode(), oselfc=osf.value, plasma=pl.value)
+    return out
