@@ -584,6 +584,11 @@ int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int l
 int spexhip_dielectric_general_dev(spexhip_ctx *ctx, int nb, int nw, const void *X_dev, int ldx,
                                    int vkind, const void *vhalf_dev, void *E_dev, int lde);
 int spexhip_inverse_general_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx);
+/* Host drop-in of the two above: chi (suscepc, nb x nb x nw complex,
+ * column-major) -> einv (dielecc after Minverse, same layout); vhalf as
+ * for spexhip_dielectric_inverse.                                         */
+int spexhip_dielectric_inverse_general(spexhip_ctx *ctx, int nb, int nw, const double *chi,
+                                       int vkind, const double *vhalf, double *einv);
 
 /* Screened interaction W = v^1/2 eps^-1 v^1/2 of nw full device matrices
  * (correlation.f:868-877, p_diagmatdiag(sqcoul, dielec)); vkind 0: vhalf =

@@ -580,6 +580,40 @@ int spexhip_inverse_general_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int l
   return SPEXHIP_OK;
 }
 
+// Host drop-in for the real-frequency branch (dielecc, correlation.f:
+// 632-636 and :701-747): suscepc (nb x nb x nw, full) -> its eps^-1 (same
+// layout), v^1/2 = sqcoul (vkind 0) or dense (vkind 1).
+int spexhip_dielectric_inverse_general(spexhip_ctx *ctx, int nb, int nw, const double *chi,
+                                       int vkind, const double *vhalf, double *einv) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && chi && vhalf && einv && (vkind == 0 || vkind == 1),
+            "dielectric_inverse_general: bad arguments");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const size_t xb = 16 * (size_t)nb * nb * nw;
+  const size_t vb = vkind ? 16 * (size_t)nb * nb : 8 * (size_t)nb;
+  void *X = nullptr, *E = nullptr, *V = nullptr;
+  int rc = spx_alloc(ctx, xb, &X);
+  if (!rc) rc = spx_alloc(ctx, xb, &E);
+  if (!rc) rc = spx_alloc(ctx, vb, &V);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(X, chi, xb, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(V, vhalf, vb, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "upload: %s", hipGetErrorString(e));
+  }
+  if (!rc) rc = spexhip_dielectric_general_dev(ctx, nb, nw, X, nb, vkind, V, E, nb);
+  if (!rc) rc = spexhip_inverse_general_dev(ctx, nb, nw, E, nb);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(einv, E, xb, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "download: %s", hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  spx_free(ctx, X);
+  spx_free(ctx, E);
+  spx_free(ctx, V);
+  return rc;
+}
+
 int spexhip_dielectric_inverse(spexhip_ctx *ctx, int nb, int nw, const double *chi_packed,
                                int vkind, const double *vhalf, double *einv_packed) {
   if (!ctx) return SPEXHIP_E_INVALID;

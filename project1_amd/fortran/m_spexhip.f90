@@ -42,6 +42,7 @@ module m_spexhip
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
   public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
+  public :: spexhip_dielecc_inverse
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -83,6 +84,12 @@ module m_spexhip
                             headc, wingc
       integer(c_long), value :: n, ldc, lddc
       integer(c_int), value :: nb, nfrqh, nfrqc
+    end function
+    integer(c_int) function c_dielectric_inverse_general(ctx, nb, nw, chi, vkind, vhalf, einv) &
+        bind(C, name='spexhip_dielectric_inverse_general')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, chi, vhalf, einv
+      integer(c_int), value :: nb, nw, vkind
     end function
     integer(c_int) function c_tetrahedron_weights(ctx, nkpt3, rlat, nocc, nunocc, eocc, eunocc, &
         efermi, nfrqs, frqs, kpt1p, nkpt1, degocc, degunocc, wtetra) &
@@ -248,6 +255,18 @@ contains
     call check(ctx, c_dielectric_inverse_gamma(ctx, int(nbas, c_int), int(nfreq, c_int), &
                c_loc(suscep), c_loc(sqcoul), c_loc(head), c_loc(wing), c_loc(dielec)), &
                'dielec_inverse_gamma')
+  end subroutine
+
+  ! Real frequencies (correlation.f:632-636, :701-747): dielecc = Minverse of
+  ! 1 - v^1/2 suscepc v^1/2 per frequency, v^1/2 = sqcoul.
+  subroutine spexhip_dielecc_inverse(ctx, nbas, nfreqc, suscepc, sqcoul, dielecc)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nbas, nfreqc
+    complex(c_double_complex), intent(in), target :: suscepc(nbas, nbas, nfreqc)
+    real(c_double), intent(in), target :: sqcoul(nbas)
+    complex(c_double_complex), intent(out), target :: dielecc(nbas, nbas, nfreqc)
+    call check(ctx, c_dielectric_inverse_general(ctx, int(nbas, c_int), int(nfreqc, c_int), &
+               c_loc(suscepc), 0_c_int, c_loc(sqcoul), c_loc(dielecc)), 'dielecc_inverse')
   end subroutine
 
   ! tetrahedron5_init (tetrahedron.f:903-1332) for kflag = 1: ene_occ(nocc,nk) at k and

@@ -339,6 +339,7 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_gamma_prepare_cplx_dev": (I, [_vp, I, I, _vp, I, _vp, _vp, _vp]),
         "spexhip_gamma_inverse_cplx_dev": (I, [_vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_inverse_general_dev": (I, [_vp, I, I, _vp, I]),
+        "spexhip_dielectric_inverse_general": (I, [_vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_dielectric_inverse_gamma": (I, [_vp, I, I, _vp, _vp, _vp, _vp, _vp]),
         "spexhip_synth_normal_dev": (I, [_vp, _vp, _c_long, C.c_ulonglong, C.c_double]),
         "spexhip_synth_normal_rows_dev": (I, [_vp, _vp, _c_long, _c_long, _c_long,
@@ -956,6 +957,19 @@ class Context:
         """Real-frequency eps = 1 - v^1/2 chi v^1/2 (no Hermitization)."""
         self._check(lib().spexhip_dielectric_general_dev(self.h, nb, nw, X.ptr, ldx, vkind,
                                                          vhalf.ptr, E.ptr, lde))
+
+    def dielectric_inverse_general(self, chi: np.ndarray, vhalf: np.ndarray) -> np.ndarray:
+        """Real frequencies: suscepc (nb, nb, nw) -> dielecc^-1 (nb, nb, nw),
+        Fortran-ordered complex128."""
+        c = np.asfortranarray(chi, dtype=np.complex128)
+        nb, _, nw = c.shape
+        vkind = 1 if np.asarray(vhalf).ndim == 2 else 0
+        v = (np.asfortranarray(vhalf, np.complex128) if vkind else
+             np.ascontiguousarray(vhalf, np.float64))
+        out = np.zeros_like(c, order="F")
+        self._check(lib().spexhip_dielectric_inverse_general(self.h, nb, nw, c.ctypes.data, vkind,
+                                                             _ptr(v), out.ctypes.data))
+        return out
 
     def inverse_general_dev(self, X: DevBuf, nb: int, nw: int, ldx: int) -> None:
         self._check(lib().spexhip_inverse_general_dev(self.h, nb, nw, X.ptr, ldx))

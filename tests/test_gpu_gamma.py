@@ -362,3 +362,14 @@ def test_gamma_real_frequency_chain_vs_oracle(ctx):
     assert normwise(w, w_o)[0] < 1e-12
     for b in (dX, dH, dW, dS):
         b.free()
+
+
+def test_real_frequency_host_dropin_vs_numpy(ctx):
+    nb, nw = 64, 2
+    g = np.random.default_rng(50)
+    chi = _absorptive_chi(g, nb, nw)
+    sq = g.uniform(0.3, 1.2, nb)
+    got = ctx.dielectric_inverse_general(chi, sq)
+    for f in range(nw):
+        want = np.linalg.inv(np.eye(nb) - sq[:, None] * chi[:, :, f] * sq[None, :])
+        assert normwise(got[:, :, f], want)[0] < 1e-12
