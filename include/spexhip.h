@@ -176,6 +176,22 @@ int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nun
                                      const void *w_full_dev, long ldw, const int *kpt1p,
                                      int nkpt1, void *w_red_dev, long ldr);
 
+/* The last step of tetrahedron5_init (tetrahedron.f:1282-1318): where kpt1
+ * holds k points that are equivalent under the symmetries keeping q
+ * (kptsym(ikpt, isym) == ikpt), the weights of each group are replaced by
+ * their mean weighted by how often the symmetries map onto each member;
+ * groups are found in the reference's loop order.  Host tables, 0-based:
+ * kpt1 (nkpt1 full-mesh indices), kpt1p (nk: index into kpt1 or -1),
+ * kptsym (nk x nsym, column-major: image of point k under symmetry s at
+ * k + nk*s), ikpt = q.  wtetra_dev (nkpt1*nocc*nunocc x nfrqs, ld ldwt) is
+ * updated in place, bit-exact with the unfused sum count*w member by
+ * member.  Call after spexhip_tetra_average_degenerate_dev.  Returns after
+ * the update is written.                                                  */
+int spexhip_tetra_average_equivalent_dev(spexhip_ctx *ctx, int nkpt1, int nocc, int nunocc,
+                                         int nfrqs, const int *kpt1, int nk, const int *kpt1p,
+                                         int nsym, const int *kptsym, int ikpt, void *wtetra_dev,
+                                         long ldwt);
+
 /* Host drop-in for tetrahedron5_init on host arrays: e_occ [nk][nocc],
  * e_unocc [nk][nunocc] (full mesh), frqs [nfrqs]; kpt1p [nk] (or NULL: the
  * full mesh, nkpt1 ignored) and deg_occ [nkpt1][nocc], deg_unocc

@@ -20,7 +20,9 @@ Restates, in plain numpy/Python:
 - the averaging of the weights over degenerate states (:1233-1273), for
   groups inside the band windows;
 - the weights of a symmetry-reduced k set (kpt1p): the sum over the
-  full-mesh points that map to each parent.
+  full-mesh points that map to each parent;
+- the final averaging over equivalent k points of kpt1 under the symmetries
+  that keep q (:1282-1318).
 
 k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
 (i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
@@ -319,6 +321,53 @@ def average_degenerate(w, nk, nocc, nunocc, deg_occ, deg_unocc):
             W[k, :, first:last + 1] = (W[k, :, first:last + 1].sum(1) / (last - first + 1))[:, None]
             n = last + 1
     return W.reshape(nk * nocc * nunocc, -1)
+
+
+def equivalent_groups(kpt1, kpt1p, kptsym, ikpt):
+    """The groups of tetrahedron5_init's final averaging (tetrahedron.f:
+    1282-1318), in the reference's loop order: for each k of kpt1 not yet
+    done, the symmetries that keep q (kptsym(ikpt, isym) == ikpt) map
+    kpt1(k) to points whose kpt1p indices k1 are counted; if one k1 differs
+    from k, the k1 with a count form a group (ascending) that is averaged
+    with the counts as weights and marked done.  0-based: kpt1 (nkpt1,)
+    full-mesh indices, kpt1p (nk,) index into kpt1 or -1, kptsym (nk, nsym)
+    image of each point under each symmetry.  Returns [(members, counts)]."""
+    kptsym = np.asarray(kptsym)
+    nkpt1 = len(kpt1)
+    little = [s for s in range(kptsym.shape[1]) if kptsym[ikpt, s] == ikpt]
+    done = np.zeros(nkpt1, bool)
+    groups = []
+    for k in range(nkpt1):
+        if done[k]:
+            continue
+        cnt = np.zeros(nkpt1, int)
+        moved = False
+        for s in little:
+            k1 = kpt1p[kptsym[kpt1[k], s]]
+            if k1 >= 0:
+                cnt[k1] += 1
+                moved = moved or k1 != k
+        if moved:
+            mem = np.flatnonzero(cnt > 0)
+            done[mem] = True
+            groups.append((mem, cnt[mem]))
+    return groups
+
+
+def average_equivalent(w, nocc, nunocc, kpt1, kpt1p, kptsym, ikpt):
+    """tetrahedron.f:1282-1318: the weights of each group of equivalent k
+    points replaced by their count-weighted mean, summed member by member
+    in ascending order (rdum = rdum + d1up*w, unfused), then divided by the
+    sum of the counts.  Groups are applied one after another, as the
+    reference's loop does.  w (nkpt1*nocc*nunocc, nfrqs), t = (k*nocc +
+    b1)*nunocc + b2."""
+    W = np.array(w, float).reshape(len(kpt1), nocc, nunocc, -1)
+    for mem, cnt in equivalent_groups(kpt1, kpt1p, kptsym, ikpt):
+        acc = np.zeros(W.shape[1:])
+        for m, c in zip(mem, cnt):
+            acc = acc + float(c) * W[m]
+        W[mem] = acc / float(cnt.sum())
+    return W.reshape(-1, W.shape[-1])
 
 
 def deg_from_energies(e, tol):

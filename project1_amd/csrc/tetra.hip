@@ -24,6 +24,7 @@
 #include "common.hpp"
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace {
@@ -465,6 +466,30 @@ __global__ void star_sum_kernel(int nkpt1, int nocc, int nunocc, const int *__re
   out[t + ldo * i] = s;
 }
 
+// Mean over a group of equivalent k points: one thread per (group, band
+// pair) and bin; members ascending, sum = sum + count*w unfused (the
+// reference's rdum loop), divided by the total count and written to every
+// member.  The groups of one launch are disjoint, so the update is in place.
+__global__ void equiv_average_kernel(long per, int ngroup, const int *__restrict__ off,
+                                     const int *__restrict__ mem, const int *__restrict__ cnt,
+                                     int nfrq, double *__restrict__ w, long ldw) {
+#pragma clang fp contract(off)
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (t >= (long)ngroup * per || i >= nfrq) return;
+  const int g = (int)(t / per);
+  const long bb = t - (long)g * per;
+  double *col = w + ldw * i;
+  double s = 0.0;
+  int tot = 0;
+  for (int m = off[g]; m < off[g + 1]; m++) {
+    s = s + (double)cnt[m] * col[(long)mem[m] * per + bb];
+    tot += cnt[m];
+  }
+  const double avg = s / (double)tot;
+  for (int m = off[g]; m < off[g + 1]; m++) col[(long)mem[m] * per + bb] = avg;
+}
+
 }  // namespace
 
 extern "C" {
@@ -505,6 +530,84 @@ int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nun
                      (const double *)w_full, ldw, (double *)w_red, ldr);
   SPX_LAUNCHED(ctx);
   // the host star tables are read by the copies above
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SPEXHIP_OK;
+}
+
+int spexhip_tetra_average_equivalent_dev(spexhip_ctx *ctx, int nkpt1, int nocc, int nunocc,
+                                         int nfrqs, const int *kpt1, int nk, const int *kpt1p,
+                                         int nsym, const int *kptsym, int ikpt, void *wtetra,
+                                         long ldwt) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nkpt1 > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0 && nk > 0 && nsym > 0 &&
+                     kpt1 && kpt1p && kptsym && ikpt >= 0 && ikpt < nk,
+            "tetra_average_equivalent: bad arguments");
+  const long per = (long)nocc * nunocc, n = (long)nkpt1 * per;
+  SPX_CHECK(ctx, ldwt >= n, "tetra_average_equivalent: ldwt < nkpt1*nocc*nunocc");
+  SPX_CHECK(ctx, spx_owns(ctx, wtetra, 8 * ((size_t)ldwt * (nfrqs - 1) + n)),
+            "tetra_average_equivalent: wtetra is not a context allocation of the stated shape");
+  for (int k = 0; k < nkpt1; k++)
+    SPX_CHECK(ctx, kpt1[k] >= 0 && kpt1[k] < nk, "tetra_average_equivalent: kpt1 out of range");
+  for (int k = 0; k < nk; k++)
+    SPX_CHECK(ctx, kpt1p[k] >= -1 && kpt1p[k] < nkpt1,
+              "tetra_average_equivalent: kpt1p out of range");
+  for (long e = 0; e < (long)nk * nsym; e++)
+    SPX_CHECK(ctx, kptsym[e] >= 0 && kptsym[e] < nk,
+              "tetra_average_equivalent: kptsym out of range");
+  // The groups in the reference's loop order (tetrahedron.f:1282-1318),
+  // split into batches of disjoint groups: one launch per batch keeps the
+  // reference's sequence where a later group reuses a member.
+  std::vector<int> little;
+  for (int s = 0; s < nsym; s++)
+    if (kptsym[ikpt + (long)nk * s] == ikpt) little.push_back(s);
+  std::vector<char> done(nkpt1, 0);
+  std::vector<int> count(nkpt1), off(1, 0), mem, cnt, batch(1, 0), owner(nkpt1, -1);
+  for (int k = 0; k < nkpt1; k++) {
+    if (done[k]) continue;
+    std::fill(count.begin(), count.end(), 0);
+    bool moved = false;
+    for (int s : little) {
+      const int k1 = kpt1p[kptsym[kpt1[k] + (long)nk * s]];
+      if (k1 >= 0) {
+        count[k1]++;
+        moved = moved || k1 != k;
+      }
+    }
+    if (!moved) continue;
+    const int g = (int)off.size() - 1;
+    bool clash = false;
+    for (int k1 = 0; k1 < nkpt1; k1++)
+      if (count[k1] > 0 && owner[k1] >= batch.back()) clash = true;
+    if (clash) batch.push_back(g);
+    for (int k1 = 0; k1 < nkpt1; k1++)
+      if (count[k1] > 0) {
+        mem.push_back(k1);
+        cnt.push_back(count[k1]);
+        done[k1] = 1;
+        owner[k1] = g;
+      }
+    off.push_back((int)mem.size());
+  }
+  const int ngroup = (int)off.size() - 1;
+  if (ngroup == 0) return SPEXHIP_OK;
+  batch.push_back(ngroup);
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *ws;
+  int rc = spx_scratch(ctx, 4 * (off.size() + 2 * mem.size()) + 16, &ws);
+  if (rc) return rc;
+  int *doff = (int *)ws, *dmem = doff + off.size(), *dcnt = dmem + mem.size();
+  SPX_HIP(ctx, hipMemcpyAsync(doff, off.data(), 4 * off.size(), hipMemcpyHostToDevice, ctx->stream));
+  SPX_HIP(ctx, hipMemcpyAsync(dmem, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, ctx->stream));
+  SPX_HIP(ctx, hipMemcpyAsync(dcnt, cnt.data(), 4 * cnt.size(), hipMemcpyHostToDevice, ctx->stream));
+  for (size_t b = 0; b + 1 < batch.size(); b++) {
+    const int g0 = batch[b], ng = batch[b + 1] - g0;
+    hipLaunchKernelGGL(equiv_average_kernel,
+                       dim3((unsigned)(((long)ng * per + 255) / 256), (unsigned)nfrqs), dim3(256),
+                       0, ctx->stream, per, ng, doff + g0, dmem, dcnt, nfrqs, (double *)wtetra,
+                       ldwt);
+    SPX_LAUNCHED(ctx);
+  }
+  // the host group tables are read by the copies above
   SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return SPEXHIP_OK;
 }

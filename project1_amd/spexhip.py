@@ -321,6 +321,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_tetra_average_degenerate_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long]),
         "spexhip_tetra_reduce_parents_dev": (I, [_vp, I, I, I, I, _vp, _c_long, _vp, I, _vp,
                                                  _c_long]),
+        "spexhip_tetra_average_equivalent_dev": (I, [_vp, I, I, I, I, _vp, I, _vp, I, _vp, I,
+                                                     _vp, _c_long]),
         "spexhip_tetrahedron_weights": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp,
                                             _vp, I, _vp, _vp, _vp]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
@@ -587,6 +589,17 @@ class Context:
         kp = np.ascontiguousarray(kpt1p, np.int32)
         self._check(lib().spexhip_tetra_reduce_parents_dev(
             self.h, nk, nocc, nunocc, nfrqs, w_full.ptr, ldw, _ptr(kp), nkpt1, w_red.ptr, ldr))
+
+    def tetra_average_equivalent_dev(self, nocc: int, nunocc: int, nfrqs: int, kpt1, kpt1p,
+                                     kptsym, ikpt: int, W: DevBuf, ldw: int) -> None:
+        """In place: the mean over each group of equivalent k points of kpt1
+        under the symmetries that keep q (kptsym (nk, nsym), 0-based)."""
+        k1 = np.ascontiguousarray(kpt1, np.int32)
+        kp = np.ascontiguousarray(kpt1p, np.int32)
+        ks = np.asfortranarray(kptsym, np.int32)
+        self._check(lib().spexhip_tetra_average_equivalent_dev(
+            self.h, len(k1), nocc, nunocc, nfrqs, _ptr(k1), ks.shape[0], _ptr(kp), ks.shape[1],
+            _ptr(ks), ikpt, W.ptr, ldw))
 
     def tetrahedron_weights(self, nkpt3, rlat, e_occ: np.ndarray, e_unocc: np.ndarray,
                             efermi: float, frqs: np.ndarray, kpt1p=None, nkpt1: int = 0,

@@ -231,3 +231,55 @@ def test_degenerate_averaging_rejects_inconsistent_deg(ctx):
         ctx.tetra_average_degenerate_dev(nk, nocc, nunocc, dDo, dDu, nf, dW, n)
     for b in (dW, dDo, dDu):
         b.free()
+
+
+def _average_equivalent_dev(ctx, W, nocc, nunocc, kpt1, kpt1p, kptsym, ikpt):
+    n, nf = W.shape
+    dW = ctx.to_dev(np.ascontiguousarray(W.T))
+    try:
+        ctx.tetra_average_equivalent_dev(nocc, nunocc, nf, kpt1, kpt1p, kptsym, ikpt, dW, n)
+        return dW.download(np.float64, (nf, n)).T
+    finally:
+        dW.free()
+
+
+def test_equivalent_k_averaging_vs_oracle(ctx):
+    """A 4^3 cubic mesh with bands of cubic symmetry, q = (1,0,0)/4, every k
+    in kpt1: device tetrahedron weights, then the averaging over k points
+    equivalent under the little group of q -- bit-exact with the oracle and
+    equal on each orbit."""
+    from tests.test_oracle_tetra import _symmetric_mesh
+    n, q = 4, (1, 0, 0)
+    pts, ops, kptsym, iq, eo, eu = _symmetric_mesh(n, q)
+    nk = n ** 3
+    frq = 0.05 * (1.25 ** np.arange(40) - 1) + 0.01
+    w = _run(ctx, (n, n, n), np.eye(3), eo, eu, frq)
+    kpt1 = kpt1p = np.arange(nk)
+    got = _average_equivalent_dev(ctx, w, 1, 2, kpt1, kpt1p, kptsym, iq)
+    want = tetra.average_equivalent(w, 1, 2, kpt1, kpt1p, kptsym, iq)
+    assert np.abs(want - w).max() > 1e-8  # the averaging changes something
+    np.testing.assert_array_equal(got, want)
+
+
+def test_equivalent_k_averaging_overlapping_groups_vs_oracle(ctx):
+    """Irregular tables whose groups reuse members already averaged (the
+    reference then averages them again, in its loop order) and count some
+    images more than once; one k set with no equivalents is left alone."""
+    g = np.random.default_rng(0)
+    nk, nsym, nkpt1, nocc, nunocc, nf = 10, 4, 7, 3, 5, 6
+    ks = g.integers(0, nk, (nk, nsym))
+    ks[:, 0] = np.arange(nk)
+    ks[3, :] = 3
+    kpt1 = g.choice(nk, nkpt1, replace=False)
+    kp = g.integers(-1, nkpt1, nk)
+    groups = tetra.equivalent_groups(kpt1, kp, ks, 3)
+    mem = np.concatenate([m for m, _ in groups])
+    assert len(mem) > len(set(mem.tolist())) and max(c.max() for _, c in groups) > 1
+    W = g.uniform(0, 1, (nkpt1 * nocc * nunocc, nf))
+    got = _average_equivalent_dev(ctx, W, nocc, nunocc, kpt1, kp, ks, 3)
+    np.testing.assert_array_equal(got, tetra.average_equivalent(W, nocc, nunocc, kpt1, kp, ks, 3))
+    ident = np.repeat(np.arange(nk)[:, None], nsym, 1)
+    got = _average_equivalent_dev(ctx, W, nocc, nunocc, kpt1, kp, ident, 3)
+    np.testing.assert_array_equal(got, W)
+    with pytest.raises(spexhip.SpexError, match="kptsym out of range"):
+        _average_equivalent_dev(ctx, W, nocc, nunocc, kpt1, kp, ks + nk, 3)

@@ -3,6 +3,7 @@ against exact integration over the clipped tetrahedron (a convex polytope
 triangulated by scipy), the volume rule, the decomposition choice, and for
 the whole mesh: flat bands (all weight in one bin), the Brillouin-zone sum
 rule, and an independent assembly from the clipped-polytope integrals."""
+import itertools
 import math
 
 import numpy as np
@@ -330,3 +331,99 @@ def test_parents_reduction_is_the_reference_scatter():
     want[..., 1:] = np.diff(cum, axis=-1) / (2 * dw[1:] * 24 * nk)
     want[..., 0] = cum[..., 0] / (2 * dw[0] * 24 * nk)
     np.testing.assert_allclose(got, want.reshape(got.shape), rtol=1e-12, atol=1e-14)
+
+
+def _cubic_ops():
+    """The 48 signed permutation matrices (the cubic point group)."""
+    ops = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            R = np.zeros((3, 3), int)
+            for i in range(3):
+                R[i, perm[i]] = signs[i]
+            ops.append(R)
+    return ops
+
+
+def _symmetric_mesh(n, q):
+    """A cubic n^3 mesh with bands invariant under the cubic group, the
+    image table kptsym(k, isym) and the index of q."""
+    pk = tetra.mesh_index((n, n, n))
+    pts = np.array([(i1, i2, i3) for i1 in range(n) for i2 in range(n) for i3 in range(n)])
+    ops = _cubic_ops()
+    kptsym = np.array([[pk(*(R @ p)) for R in ops] for p in pts])
+
+    def band(p, shift):
+        x = 2 * np.pi * (p + shift) / n
+        return np.cos(x).sum(-1)
+    eo = np.stack([-3.2 + 0.3 * band(pts, 0)], 1)
+    eu = np.stack([1.5 - 0.4 * band(pts, np.array(q)), 2.0 - 0.2 * band(pts, np.array(q))], 1)
+    return pts, ops, kptsym, pk(*q), eo, eu
+
+
+def test_equivalent_average_is_the_orbit_mean_under_the_little_group():
+    """All k of a 4^3 mesh in kpt1 (no reduction): each k's weights become
+    the mean over its orbit under the symmetries that keep q = (1,0,0)/4
+    (an orbit built as a set here), equal on every member, and the
+    decomposition's symmetry breaking is really there to remove."""
+    n, q = 4, (1, 0, 0)
+    pts, ops, kptsym, iq, eo, eu = _symmetric_mesh(n, q)
+    nk = n ** 3
+    frq = 0.05 * (1.25 ** np.arange(40) - 1) + 0.01
+    w = tetra.tetrahedron5_weights((n, n, n), np.eye(3), eo, eu, 0.0, frq)
+    kpt1, kpt1p = np.arange(nk), np.arange(nk)
+    got = tetra.average_equivalent(w, 1, 2, kpt1, kpt1p, kptsym, iq).reshape(nk, 2, -1)
+    W = w.reshape(nk, 2, -1)
+    little = [R for R in ops if tuple(R @ np.array(q) % n) == q]
+    assert len(little) == 8
+    pk = tetra.mesh_index((n, n, n))
+    broken = 0.0
+    for k in range(nk):
+        orbit = sorted({pk(*(R @ pts[k])) for R in little})
+        np.testing.assert_allclose(got[k], W[orbit].mean(0), rtol=1e-13, atol=1e-15)
+        for m in orbit:
+            np.testing.assert_array_equal(got[m], got[k])
+        broken = max(broken, np.abs(W[orbit] - W[orbit].mean(0)).max())
+    assert broken > 1e-6 * np.abs(W).max()
+    # the whole group (not the little group) would mix other orbits in
+    full = [sorted({pk(*(R @ p)) for R in ops}) for p in pts]
+    assert any(len(o) > len({pk(*(R @ pts[k])) for R in little}) for k, o in enumerate(full))
+
+
+def test_equivalent_average_weights_by_counts_and_skips_reduced_sets():
+    """Hand-worked cases: two symmetries (identity and one that sends point
+    0 to point 2, whose parent is 1), kpt1 = (0, 1): point 0's images have
+    parents 0 and 1, each counted once -> the plain mean; three symmetries
+    whose images of point 0 have parents (0, 0, 2) -> counts (2, 1) and the
+    mean (2 w0 + w2)/3, point 1 untouched; a symmetry that moves q is not
+    used; a k set that is irreducible under the little group is left as it
+    is."""
+    w = np.array([[1.0, 10.0], [4.0, 40.0]])  # (nkpt1 * 1 * 1, nfrqs)
+    kptsym = np.array([[0, 2], [1, 1], [2, 0]])  # (nk, nsym): symmetry 1 swaps 0 and 2
+    got = tetra.average_equivalent(w, 1, 1, [0, 1], [0, 1, 1], kptsym, 1)
+    np.testing.assert_array_equal(got, [[2.5, 25.0], [2.5, 25.0]])
+    # three points, counts (2, 1): identity twice on 0's image table
+    kptsym3 = np.array([[0, 0, 3], [1, 1, 1], [2, 2, 2], [3, 3, 0]])
+    w3 = np.array([[3.0], [5.0], [7.0]])
+    got3 = tetra.average_equivalent(w3, 1, 1, [0, 1, 2], [0, 1, 2, 2], kptsym3, 1)
+    np.testing.assert_array_equal(got3[[0, 2]], [[(2 * 3.0 + 7.0) / 3]] * 2)
+    assert got3[1, 0] == 5.0
+    # q outside the symmetry's little group: symmetry 1 does not keep q = 0
+    kq = np.array([[0, 2], [1, 0], [2, 1]])
+    np.testing.assert_array_equal(tetra.average_equivalent(w, 1, 1, [0, 1], [0, 1, 1], kq, 0), w)
+    # irreducible set: every image has the point's own parent
+    n, q = 4, (1, 0, 0)
+    pts, ops, kptsym, iq, _, _ = _symmetric_mesh(n, q)
+    pk = tetra.mesh_index((n, n, n))
+    little = [s for s, R in enumerate(ops) if kptsym[iq, s] == iq]
+    rep = {}
+    kpt1, kpt1p = [], np.full(n ** 3, -1)
+    for k in range(n ** 3):
+        key = min(kptsym[k, s] for s in little)
+        if key not in rep:
+            rep[key] = len(kpt1)
+            kpt1.append(k)
+        kpt1p[k] = rep[key]
+    wr = np.random.default_rng(5).uniform(0, 1, (len(kpt1) * 2, 3))
+    assert tetra.equivalent_groups(kpt1, kpt1p, kptsym, iq) == []
+    np.testing.assert_array_equal(tetra.average_equivalent(wr, 1, 2, kpt1, kpt1p, kptsym, iq), wr)
