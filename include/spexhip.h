@@ -192,6 +192,13 @@ int spexhip_tetra_average_equivalent_dev(spexhip_ctx *ctx, int nkpt1, int nocc, 
                                          int nsym, const int *kptsym, int ikpt, void *wtetra_dev,
                                          long ldwt);
 
+/* Host drop-in of the same step: wtetra (nkpt1*nocc*nunocc x nfrqs,
+ * column-major host array, as spexhip_tetrahedron_weights returns it) is
+ * updated in place; the other arguments as above.                         */
+int spexhip_tetra_average_equivalent(spexhip_ctx *ctx, int nkpt1, int nocc, int nunocc, int nfrqs,
+                                     const int *kpt1, int nk, const int *kpt1p, int nsym,
+                                     const int *kptsym, int ikpt, double *wtetra);
+
 /* Host drop-in for tetrahedron5_init on host arrays: e_occ [nk][nocc],
  * e_unocc [nk][nunocc] (full mesh), frqs [nfrqs]; kpt1p [nk] (or NULL: the
  * full mesh, nkpt1 ignored) and deg_occ [nkpt1][nocc], deg_unocc

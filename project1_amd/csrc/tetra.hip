@@ -792,4 +792,36 @@ int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double
   return rc;
 }
 
+int spexhip_tetra_average_equivalent(spexhip_ctx *ctx, int nkpt1, int nocc, int nunocc, int nfrqs,
+                                     const int *kpt1, int nk, const int *kpt1p, int nsym,
+                                     const int *kptsym, int ikpt, double *wtetra) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, wtetra && nkpt1 > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0,
+            "tetra_average_equivalent: bad arguments");
+  const size_t bytes = 8 * (size_t)nkpt1 * nocc * nunocc * nfrqs;
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *dW = nullptr;
+  int rc = spx_alloc(ctx, bytes, &dW);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(dW, wtetra, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "tetra_average_equivalent: upload failed: %s",
+                      hipGetErrorString(e));
+  }
+  if (!rc)
+    rc = spexhip_tetra_average_equivalent_dev(ctx, nkpt1, nocc, nunocc, nfrqs, kpt1, nk, kpt1p,
+                                              nsym, kptsym, ikpt, dW,
+                                              (long)nkpt1 * nocc * nunocc);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(wtetra, dW, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "tetra_average_equivalent: download failed: %s",
+                      hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  if (dW) spx_free(ctx, dW);
+  return rc;
+}
+
 }  // extern "C"

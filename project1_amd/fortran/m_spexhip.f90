@@ -42,7 +42,7 @@ module m_spexhip
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
   public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
-  public :: spexhip_dielecc_inverse
+  public :: spexhip_dielecc_inverse, spexhip_tetrahedron5_equiv
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -98,6 +98,12 @@ module m_spexhip
       type(c_ptr), value :: ctx, nkpt3, rlat, eocc, eunocc, frqs, kpt1p, degocc, degunocc, wtetra
       integer(c_int), value :: nocc, nunocc, nfrqs, nkpt1
       real(c_double), value :: efermi
+    end function
+    integer(c_int) function c_tetra_average_equivalent(ctx, nkpt1, nocc, nunocc, nfrqs, kpt1, &
+        nk, kpt1p, nsym, kptsym, ikpt, wtetra) bind(C, name='spexhip_tetra_average_equivalent')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, kpt1, kpt1p, kptsym, wtetra
+      integer(c_int), value :: nkpt1, nocc, nunocc, nfrqs, nk, nsym, ikpt
     end function
     integer(c_int) function c_dielectric_inverse_gamma(ctx, nb, nw, chi, sqcoul, head, wing, &
         einv) bind(C, name='spexhip_dielectric_inverse_gamma')
@@ -301,6 +307,24 @@ contains
     call check(ctx, c_tetrahedron_weights(ctx, c_loc(k3), c_loc(rlat), int(nocc, c_int), &
                int(nunocc, c_int), c_loc(ene_occ), c_loc(ene_unocc), efermi, int(nfrqs, c_int), &
                c_loc(frqs), pk, nk1, pdo, pdu, c_loc(wtetra)), 'tetrahedron5')
+  end subroutine
+
+  ! The last step of tetrahedron5_init (tetrahedron.f:1282-1318) on the
+  ! wtetra of spexhip_tetrahedron5: the mean over k points of kpt1 that are
+  ! equivalent under the symmetries keeping q.  1-based kpt1(nkpt1),
+  ! kpt1p(nk) (0: no parent), kptsym(nk, nsym) and ikpt, as in the reference.
+  subroutine spexhip_tetrahedron5_equiv(ctx, nocc, nunocc, nfrqs, kpt1, kpt1p, kptsym, ikpt, &
+                                        wtetra)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nocc, nunocc, nfrqs, kpt1(:), kpt1p(:), kptsym(:,:), ikpt
+    real(c_double), intent(inout), target :: wtetra(*)
+    integer(c_int), allocatable, target :: k1(:), kp(:), ks(:,:)
+    allocate(k1(size(kpt1)), kp(size(kpt1p)), ks(size(kptsym,1), size(kptsym,2)))
+    k1 = kpt1 - 1 ; kp = kpt1p - 1 ; ks = kptsym - 1
+    call check(ctx, c_tetra_average_equivalent(ctx, int(size(kpt1), c_int), int(nocc, c_int), &
+               int(nunocc, c_int), int(nfrqs, c_int), c_loc(k1), int(size(kptsym,1), c_int), &
+               c_loc(kp), int(size(kptsym,2), c_int), c_loc(ks), int(ikpt-1, c_int), &
+               c_loc(wtetra)), 'tetrahedron5 equivalent k')
   end subroutine
 
   ! wavefproducts2_mt(cprod,k1,nk,ik0,ispin1,ispin2,b1low,b1up,b2low,b2up)

@@ -323,6 +323,7 @@ def _declare(L: C.CDLL) -> None:
                                                  _c_long]),
         "spexhip_tetra_average_equivalent_dev": (I, [_vp, I, I, I, I, _vp, I, _vp, I, _vp, I,
                                                      _vp, _c_long]),
+        "spexhip_tetra_average_equivalent": (I, [_vp, I, I, I, I, _vp, I, _vp, I, _vp, I, _vp]),
         "spexhip_tetrahedron_weights": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp,
                                             _vp, I, _vp, _vp, _vp]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
@@ -600,6 +601,20 @@ class Context:
         self._check(lib().spexhip_tetra_average_equivalent_dev(
             self.h, len(k1), nocc, nunocc, nfrqs, _ptr(k1), ks.shape[0], _ptr(kp), ks.shape[1],
             _ptr(ks), ikpt, W.ptr, ldw))
+
+    def tetra_average_equivalent(self, wtetra: np.ndarray, nocc: int, nunocc: int, kpt1, kpt1p,
+                                 kptsym, ikpt: int) -> np.ndarray:
+        """Host drop-in: wtetra (nkpt1*nocc*nunocc, nfrqs) averaged over
+        equivalent k points; returns a new Fortran-ordered array."""
+        W = np.array(wtetra, np.float64, order="F")
+        k1 = np.ascontiguousarray(kpt1, np.int32)
+        kp = np.ascontiguousarray(kpt1p, np.int32)
+        ks = np.asfortranarray(kptsym, np.int32)
+        assert W.shape[0] == len(k1) * nocc * nunocc
+        self._check(lib().spexhip_tetra_average_equivalent(
+            self.h, len(k1), nocc, nunocc, W.shape[1], _ptr(k1), ks.shape[0], _ptr(kp),
+            ks.shape[1], _ptr(ks), ikpt, _ptr(W)))
+        return W
 
     def tetrahedron_weights(self, nkpt3, rlat, e_occ: np.ndarray, e_unocc: np.ndarray,
                             efermi: float, frqs: np.ndarray, kpt1p=None, nkpt1: int = 0,

@@ -15,7 +15,7 @@ def test_module_declares_every_shim():
     for name in ["spexhip_pairsum", "spexhip_wfprod2_mt", "spexhip_wfprod2_pw",
                  "spexhip_dielec_inverse", "spexhip_minverse", "spexhip_screen",
                  "spexhip_rpa_int", "spexhip_pairsum_hilbert", "spexhip_headwings",
-                 "spexhip_dielec_inverse_gamma", "spexhip_tetrahedron5",
+                 "spexhip_dielec_inverse_gamma", "spexhip_tetrahedron5", "spexhip_tetrahedron5_equiv",
                  "spexhip_dielecc_inverse"]:
         assert f"subroutine {name}" in src
     # every bind(C) name exists in the C header

@@ -283,3 +283,18 @@ def test_equivalent_k_averaging_overlapping_groups_vs_oracle(ctx):
     np.testing.assert_array_equal(got, W)
     with pytest.raises(spexhip.SpexError, match="kptsym out of range"):
         _average_equivalent_dev(ctx, W, nocc, nunocc, kpt1, kp, ks + nk, 3)
+
+
+def test_equivalent_k_host_dropin_vs_oracle(ctx):
+    """spexhip_tetra_average_equivalent on host arrays (the Fortran shim's
+    path): the irregular tables of the test above, bit-exact."""
+    g = np.random.default_rng(0)
+    nk, nsym, nkpt1, nocc, nunocc, nf = 10, 4, 7, 3, 5, 6
+    ks = g.integers(0, nk, (nk, nsym))
+    ks[:, 0] = np.arange(nk)
+    ks[3, :] = 3
+    kpt1 = g.choice(nk, nkpt1, replace=False)
+    kp = g.integers(-1, nkpt1, nk)
+    W = g.uniform(0, 1, (nkpt1 * nocc * nunocc, nf))
+    got = ctx.tetra_average_equivalent(W, nocc, nunocc, kpt1, kp, ks, 3)
+    np.testing.assert_array_equal(got, tetra.average_equivalent(W, nocc, nunocc, kpt1, kp, ks, 3))
