@@ -7,10 +7,16 @@
 // is C_w = L_w^T conj(M) with L_w(t,I) = wght(t,w) M(t,I), computed per
 // 64x64 output tile of the upper triangle with MFMA:
 //   * FP64 form: v_mfma_f64_16x16x4f64, 4 real products per complex step;
-//   * split-FP32 form: L and M are split into hi = fp32(x), lo = fp32(x-hi);
-//     hi*hi, hi*lo and lo*hi are issued as v_mfma_f32_16x16x4f32 into two
-//     FP32 accumulators (hi*hi and the cross terms) that are flushed into
-//     FP64 accumulators after every K-chunk of 16 transitions.
+//   * split-FP32 form: both sides are sqrt|D| M on a grid relative to their
+//     column maximum, split into an integer part h (<= 10 bits, so h*h and
+//     its K-chunk sums are exact in FP32) and an FP32 rest l; h*h goes into
+//     one FP32 accumulator, h*l + l*fp32(h+l) into another, both flushed
+//     into FP64 accumulators after every K-chunk of 16 transitions
+//     (chi0_tile.hpp).  A scale pass (colscale.hpp) reads M once first.
+#include <algorithm>
+#include <type_traits>
+
+#include "colscale.hpp"
 #include "common.hpp"
 
 #include "chi0_tile.hpp"
@@ -100,10 +106,130 @@ static void launch_cfg(hipStream_t st, const Problem &P, int colmajor, int split
   }
 }
 
+// ------------------------------------------------------- column scales --
+namespace colscale {
+
+// scale[w][I] via atomicMax on the bit pattern (non-negative doubles order
+// like unsigned integers); the caller zeroes scale first.  One thread per
+// column I and tchunk transitions.  The weights of a stage of 64
+// transitions are the same for every thread: the workgroup stages them in
+// LDS (NWP = nw rounded up to 4, rows w >= nw read as 0) and each thread
+// reads them back as broadcasts, eight transitions at a time.
+template <bool COLMAJOR, int NWP, bool FROM_D>
+__global__ __launch_bounds__(256) void col_scale_kernel(long n, int nb,
+                                                        const double2 *__restrict__ M, long ldm,
+                                                        int nw, const double *__restrict__ W,
+                                                        long tchunk,
+                                                        unsigned long long *__restrict__ scale) {
+  constexpr int TS = 64;
+  __shared__ double ss[NWP][TS];
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = I < nb;
+  const long t0 = (long)blockIdx.y * tchunk;
+  const long t1 = t0 + tchunk < n ? t0 + tchunk : n;
+  double m[NWP];
+#pragma unroll
+  for (int w = 0; w < NWP; w++) m[w] = 0.0;
+  for (long tb = t0; tb < t1; tb += TS) {
+    const int len = (int)(t1 - tb < TS ? t1 - tb : TS);
+    __syncthreads();  // the previous stage has been read
+    for (int e = threadIdx.x; e < NWP * TS; e += blockDim.x) {
+      const int w = e / TS, j = e % TS;
+      double v = 0.0;
+      if (w < nw && j < len) {
+        v = W[(long)w * n + tb + j];
+        if (FROM_D) v = sqrt(fabs(v));
+      }
+      ss[w][j] = v;
+    }
+    __syncthreads();
+    if (live) {
+      int j0 = 0;
+      for (; j0 + 8 <= len; j0 += 8) {  // no branches: eight loads in flight
+        double2 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const long t = tb + j0 + j;
+          v[j] = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep the eight loads ahead of their uses
+        double a[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) a[j] = fmax(fabs(v[j].x), fabs(v[j].y));
+#pragma unroll
+        for (int w = 0; w < NWP; w++) {
+          double x = a[0] * ss[w][j0];
+#pragma unroll
+          for (int j = 1; j < 8; j++) x = fmax(x, a[j] * ss[w][j0 + j]);
+          m[w] = fmax(m[w], x);
+        }
+      }
+      for (; j0 < len; j0++) {
+        const long t = tb + j0;
+        const double2 v = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
+        const double a = fmax(fabs(v.x), fabs(v.y));
+#pragma unroll
+        for (int w = 0; w < NWP; w++) m[w] = fmax(m[w], a * ss[w][j0]);
+      }
+    }
+  }
+  if (!live) return;
+#pragma unroll
+  for (int w = 0; w < NWP; w++)
+    if (w < nw) atomicMax(scale + (long)w * nb + I, (unsigned long long)__double_as_longlong(m[w]));
+}
+
+__global__ void zero_u64_kernel(unsigned long long *__restrict__ p, long count) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < count) p[g] = 0ull;
+}
+
+}  // namespace colscale
+
+size_t spx_split_ws_bytes(int nb, int nw) { return 8 * (size_t)nb * nw; }
+
+int spx_col_scales(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,
+                   int colmajor, int nw, const double *W, int from_d, double *scale) {
+  const long cnt = (long)nb * nw;
+  hipLaunchKernelGGL(colscale::zero_u64_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0,
+                     st, (unsigned long long *)scale, cnt);
+  SPX_LAUNCHED(ctx);
+  if (n == 0) return SPEXHIP_OK;
+  const long tchunk = 1024;  // enough workgroups to keep HBM busy; few atomics per column
+  dim3 grid((nb + 255) / 256, (unsigned)((n + tchunk - 1) / tchunk));
+  for (int w0 = 0; w0 < nw; w0 += 16) {
+    const int g = std::min(16, nw - w0);
+    const double *Wg = W + (long)w0 * n;  // rows of n transitions
+    unsigned long long *Sg = (unsigned long long *)scale + (long)w0 * nb;
+    auto go = [&](auto cm, auto np, auto fd) {
+      hipLaunchKernelGGL((colscale::col_scale_kernel<decltype(cm)::value, decltype(np)::value,
+                                                     decltype(fd)::value>),
+                         grid, dim3(256), 0, st, n, nb, (const double2 *)M, ldm, g, Wg, tchunk,
+                         Sg);
+    };
+    auto go_nw = [&](auto cm, auto fd) {
+      switch ((g + 3) / 4 * 4) {
+        case 4: go(cm, std::integral_constant<int, 4>{}, fd); break;
+        case 8: go(cm, std::integral_constant<int, 8>{}, fd); break;
+        case 12: go(cm, std::integral_constant<int, 12>{}, fd); break;
+        default: go(cm, std::integral_constant<int, 16>{}, fd); break;
+      }
+    };
+    auto go_fd = [&](auto cm) {
+      if (from_d) go_nw(cm, std::true_type{});
+      else go_nw(cm, std::false_type{});
+    };
+    if (colmajor) go_fd(std::true_type{});
+    else go_fd(std::false_type{});
+    SPX_LAUNCHED(ctx);
+  }
+  return SPEXHIP_OK;
+}
+
 int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
                     const void *M, long ldm, int colmajor, int nw,
                     const void *wght, int precision, double alpha, double beta,
-                    void *chi, int ldchi) {
+                    void *chi, int ldchi, void *scale_ws) {
   SPX_CHECK(ctx, n >= 0 && nb > 0 && nw > 0, "chi0: bad sizes n=%ld nb=%d nw=%d", n, nb, nw);
   SPX_CHECK(ctx, ldchi >= nb, "chi0: ldchi=%d < nb=%d", ldchi, nb);
   SPX_CHECK(ctx, colmajor ? ldm >= n : ldm >= nb, "chi0: ldm=%ld too small", ldm);
@@ -113,9 +239,23 @@ int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
   if (precision == SPEXHIP_PREC_OZAKI_INT8)
     return spx_chi0_ozaki(ctx, st, n, nb, M, ldm, colmajor, nw, wght, alpha, beta, chi, ldchi,
                           ctx->ozaki_ws_limit);
+  const int split = precision == SPEXHIP_PREC_SPLIT_FP32 && n > 0;
+  double *scale = nullptr;
+  if (split) {
+    if (scale_ws) {
+      scale = (double *)scale_ws;
+    } else {
+      SPX_CHECK(ctx, st == ctx->stream, "chi0: split form off the context stream needs a workspace");
+      void *p = nullptr;
+      int rc = spx_colscale_buf(ctx, spx_split_ws_bytes(nb, nw), &p);
+      if (rc) return rc;
+      scale = (double *)p;
+    }
+    int rc = spx_col_scales(ctx, st, n, nb, M, ldm, colmajor, nw, (const double *)wght, 1, scale);
+    if (rc) return rc;
+  }
   Problem P{n, nb, nw, (const double2 *)M, ldm, (const double *)wght, (double2 *)chi,
-            (long)ldchi, n == 0 ? 0.0 : alpha, beta};
-  const int split = precision == SPEXHIP_PREC_SPLIT_FP32;
+            (long)ldchi, n == 0 ? 0.0 : alpha, beta, scale};
   launch_cfg<64, 16>(st, P, colmajor, split);
   SPX_LAUNCHED(ctx);
   return SPEXHIP_OK;

@@ -9,12 +9,14 @@
 // chi0_tile.hpp (FP64 or split-FP32 MFMA), i.e. susceptibility.f:1183-1240
 // per q.  Tile shapes (TM x TM outputs, KC-deep K slices) and the number of
 // resident workgroups per CU are chosen by an autotuner that times each
-// configuration on the actual batch.
+// configuration on the actual batch.  In the split-FP32 form each problem's
+// column scales (colscale.hpp) are computed right before the grouped launch.
 #include <algorithm>
 #include <numeric>
 #include <vector>
 
 #include "chi0_tile.hpp"
+#include "colscale.hpp"
 #include "internal.hpp"
 
 namespace {
@@ -85,10 +87,12 @@ int ncu_of(spexhip_ctx *ctx) {
 struct spexhip_group {
   int nprob = 0, cfg = 0, split = 0;
   long total = 0;
-  void *dev = nullptr;  // [Problem x nprob][long x (nprob+1)][uint counter]
+  // [Problem x nprob][long x (nprob+1)][uint counter][split: column scales]
+  void *dev = nullptr;
   Problem *probs = nullptr;
   long *start = nullptr;
   unsigned int *counter = nullptr;
+  std::vector<Problem> host;  // the problems as uploaded (scale passes)
 };
 
 static int group_build(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *pp, int precision,
@@ -116,7 +120,7 @@ static int group_build(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *
               order[i]);
     probs[i] = Problem{q.n, q.nb, q.nw, (const double2 *)q.M_dev, q.ldm,
                        (const double *)q.wght_dev, (double2 *)q.chi_dev, (long)q.ldchi,
-                       q.n == 0 ? 0.0 : q.alpha, q.beta};
+                       q.n == 0 ? 0.0 : q.alpha, q.beta, nullptr};
     const int nbt = (q.nb + tm - 1) / tm;
     start[i + 1] = start[i] + (long)nbt * (nbt + 1) / 2 * q.nw;
   }
@@ -125,11 +129,22 @@ static int group_build(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *
   g->split = precision == SPEXHIP_PREC_SPLIT_FP32;
   g->total = start[nprob];
   const size_t pb = sizeof(Problem) * nprob, sb = sizeof(long) * (nprob + 1);
-  int rc = spx_alloc(ctx, pb + sb + 64, &g->dev);
+  size_t scb = 0;
+  if (g->split)
+    for (const auto &p : probs) scb += (spx_split_ws_bytes(p.nb, p.nw) + 255) & ~(size_t)255;
+  int rc = spx_alloc(ctx, pb + sb + 64 + 256 + scb, &g->dev);
   if (rc) return rc;
   g->probs = (Problem *)g->dev;
   g->start = (long *)((char *)g->dev + pb);
   g->counter = (unsigned int *)((char *)g->dev + pb + sb);
+  if (g->split) {
+    char *sc = (char *)g->dev + ((pb + sb + 64 + 255) & ~(size_t)255);
+    for (auto &p : probs) {
+      p.scale = (const double *)sc;
+      sc += (spx_split_ws_bytes(p.nb, p.nw) + 255) & ~(size_t)255;
+    }
+  }
+  g->host = probs;
   SPX_HIP(ctx, hipMemcpy(g->probs, probs.data(), pb, hipMemcpyHostToDevice));
   SPX_HIP(ctx, hipMemcpy(g->start, start.data(), sb, hipMemcpyHostToDevice));
   return SPEXHIP_OK;
@@ -137,6 +152,12 @@ static int group_build(spexhip_ctx *ctx, int nprob, const spexhip_chi0_problem *
 
 static int group_launch(spexhip_ctx *ctx, hipStream_t st, const spexhip_group *g) {
   const Cfg c = kCfgs[g->cfg];
+  if (g->split)
+    for (const Problem &p : g->host) {
+      const int rc = spx_col_scales(ctx, st, p.n, p.nb, p.M, p.ldm, 0, p.nw, p.wght, 1,
+                                    (double *)p.scale);
+      if (rc) return rc;
+    }
   hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, st, g->counter);
   SPX_LAUNCHED(ctx);
   const long grid = std::min<long>(g->total, (long)ncu_of(ctx) * c.per_cu);

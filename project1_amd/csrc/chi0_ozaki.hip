@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <type_traits>
 
+#include "colscale.hpp"
 #include "common.hpp"
 
 typedef int i4_t __attribute__((ext_vector_type(4)));
@@ -82,73 +83,6 @@ __global__ __launch_bounds__(256) void ozk_sqd_kernel(long n, int nw, const doub
   if ((threadIdx.x & 63) == 0 && (bp || bn)) atomicOr(&fl, (bp ? 1 : 0) | (bn ? 2 : 0));
   __syncthreads();
   if (threadIdx.x == 0 && fl) atomicOr(flags + w, fl);
-}
-
-// scale[w][I] = max_t sqd[w][t] * max(|Re M|, |Im M|), via atomicMax on the
-// bit pattern (non-negative doubles order like unsigned integers).  One
-// thread per column I and tchunk transitions.  The sqd values of a stage of
-// 64 transitions are the same for every thread: the workgroup stages them
-// in LDS (NWP = nw rounded up to 4; sqd has zero rows up to NWP) and each
-// thread reads them back as broadcasts, eight transitions at a time.
-template <bool COLMAJOR, int NWP>
-__global__ __launch_bounds__(256) void ozk_scale_kernel(long n, int nb,
-                                                        const double2 *__restrict__ M, long ldm,
-                                                        int nw, const double *__restrict__ sqd,
-                                                        long tchunk,
-                                                        unsigned long long *__restrict__ scale) {
-  constexpr int TS = 64;
-  __shared__ double ss[NWP][TS];
-  const int I = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = I < nb;
-  const long t0 = (long)blockIdx.y * tchunk;
-  const long t1 = t0 + tchunk < n ? t0 + tchunk : n;
-  double m[NWP];
-#pragma unroll
-  for (int w = 0; w < NWP; w++) m[w] = 0.0;
-  for (long tb = t0; tb < t1; tb += TS) {
-    const int len = (int)(t1 - tb < TS ? t1 - tb : TS);
-    __syncthreads();  // the previous stage has been read
-    for (int e = threadIdx.x; e < NWP * TS; e += blockDim.x) {
-      const int w = e / TS, j = e % TS;
-      ss[w][j] = j < len ? sqd[(long)w * n + tb + j] : 0.0;
-    }
-    __syncthreads();
-    if (live) {
-      auto mag = [&](long t) {
-        const double2 v = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
-        return fmax(fabs(v.x), fabs(v.y));
-      };
-      int j0 = 0;
-      for (; j0 + 8 <= len; j0 += 8) {  // no branches: eight loads in flight
-        double2 v[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          const long t = tb + j0 + j;
-          v[j] = COLMAJOR ? M[t + ldm * (long)I] : M[t * ldm + I];
-        }
-        __builtin_amdgcn_sched_barrier(0);  // keep the eight loads ahead of their uses
-        double a[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) a[j] = fmax(fabs(v[j].x), fabs(v[j].y));
-#pragma unroll
-        for (int w = 0; w < NWP; w++) {
-          double x = a[0] * ss[w][j0];
-#pragma unroll
-          for (int j = 1; j < 8; j++) x = fmax(x, a[j] * ss[w][j0 + j]);
-          m[w] = fmax(m[w], x);
-        }
-      }
-      for (; j0 < len; j0++) {
-        const double a = mag(tb + j0);
-#pragma unroll
-        for (int w = 0; w < NWP; w++) m[w] = fmax(m[w], a * ss[w][j0]);
-      }
-    }
-  }
-  if (!live) return;
-#pragma unroll
-  for (int w = 0; w < NWP; w++)
-    if (w < nw) atomicMax(scale + (long)w * nb + I, (unsigned long long)__double_as_longlong(m[w]));
 }
 
 __device__ inline void slice4(double u, int a[NS]) {
@@ -541,7 +475,7 @@ size_t spx_ozaki_slice_bytes(long n, int nb) {
   return (size_t)nchunk * NPL * nbp * TK;
 }
 
-// sqd rows are padded to a multiple of 4 frequencies (see ozk_scale_kernel)
+// sqd rows are padded to a multiple of 4 frequencies (see col_scale_kernel)
 static int ozk_nwp(int nw) { return (nw + 3) / 4 * 4; }
 
 static size_t ozaki_fixed_bytes(long n, int nb, int nw) {
@@ -588,38 +522,17 @@ int spx_chi0_ozaki_async(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const
   int *flags = (int *)(sgn + nw);
   unsigned char *S = (unsigned char *)ws + fixed;
   const bool timed = st == ctx->stream;
-  {  // zero the scale maxima and the sign flags (fill kernels: capture-safe)
-    const long cnt = (long)nb * nw;
-    hipLaunchKernelGGL(ozk_zero_kernel, dim3((cnt + 255) / 256), dim3(256), 0, st,
-                       (unsigned long long *)scale, cnt);
-    SPX_LAUNCHED(ctx);
-    hipLaunchKernelGGL(ozk_zero_kernel, dim3(1), dim3(64), 0, st, (unsigned long long *)flags, 8L);
-    SPX_LAUNCHED(ctx);
-  }
+  // zero the sign flags (a fill kernel: capture-safe)
+  hipLaunchKernelGGL(ozk_zero_kernel, dim3(1), dim3(64), 0, st, (unsigned long long *)flags, 8L);
+  SPX_LAUNCHED(ctx);
   hipLaunchKernelGGL(ozk_sqd_kernel, dim3((unsigned)std::min<long>((n + 255) / 256, 128), nwp),
                      dim3(256), 0, st, n, nw, (const double *)wght, sqd, flags);
   SPX_LAUNCHED(ctx);
   hipLaunchKernelGGL(ozk_sign_kernel, dim3(1), dim3(64), 0, st, nw, flags, sgn, err_dev);
   SPX_LAUNCHED(ctx);
   {
-    const long tchunk = 1024;  // enough workgroups to keep HBM busy; few atomics per column
-    dim3 grid((nb + 255) / 256, (n + tchunk - 1) / tchunk);
-    auto go = [&](auto cm, auto np) {
-      hipLaunchKernelGGL((ozk_scale_kernel<decltype(cm)::value, decltype(np)::value>), grid,
-                         dim3(256), 0, st, n, nb, (const double2 *)M, ldm, nw, sqd, tchunk,
-                         (unsigned long long *)scale);
-    };
-    auto go_nw = [&](auto cm) {
-      switch (nwp) {
-        case 4: go(cm, std::integral_constant<int, 4>{}); break;
-        case 8: go(cm, std::integral_constant<int, 8>{}); break;
-        case 12: go(cm, std::integral_constant<int, 12>{}); break;
-        default: go(cm, std::integral_constant<int, 16>{}); break;
-      }
-    };
-    if (colmajor) go_nw(std::true_type{});
-    else go_nw(std::false_type{});
-    SPX_LAUNCHED(ctx);
+    const int rc = spx_col_scales(ctx, st, n, nb, M, ldm, colmajor, nw, sqd, 0, scale);
+    if (rc) return rc;
   }
   const int nbt = nbp / TT;
   const int nib = nbp / 16;

@@ -7,8 +7,22 @@
 //
 // Tile shape: TM x TM complex outputs per 256-thread workgroup, 4 waves in a
 // 2x2 layout, each wave owning (TM/32)^2 blocks of 16x16; K-slices of KC
-// transitions staged through double-buffered LDS with the weight applied to
-// the row operand while staging.
+// transitions staged through double-buffered LDS with the weight applied
+// while staging.
+//
+// Split-FP32 form.  With B(t,I) = sqrt|D(t,w)| M(t,I) and s(I,w) its column
+// maximum (colscale.hpp), every real component is staged as
+//     u = B * R / s(I,w),  |u| <= R = 512,   h = rint(u),  l = fp32(u - h),
+// the row side carrying the sign of D, so that chi = s_I s_J / R^2 *
+// sum_t sgn(D) u_I conj(u_J).  h is an integer of at most 10 bits on a grid
+// common to all transitions of a column, so h_I*h_J and their sums over a
+// K-slice (2 KC products of magnitude <= R^2 per real accumulator, <= 2^24)
+// are exact in FP32.  The remaining part h_I*l_J + l_I*x_J, x = fp32(h + l)
+// ~ fp32(u), is 2^-9 smaller and goes into a second FP32 accumulator; both
+// are flushed into FP64 after every K-slice.  Small entries keep full FP32
+// relative precision in l, and no term of the product is dropped beyond
+// the FP32 rounding of the cross term: 12 real MFMAs per complex step, as
+// for the plain hi/lo split, but without its O(2^-24) error on hi*hi.
 #pragma once
 #include "common.hpp"
 
@@ -24,9 +38,15 @@ __device__ inline void tri_index(int x, int &bi, int &bj) {
   bi = x - j * (j + 1) / 2;
 }
 
-__device__ inline void split2(double x, float &hi, float &lo) {
-  hi = (float)x;
-  lo = (float)(x - (double)hi);
+constexpr double SPLIT_R = 512.0;  // grid levels of h per column maximum
+
+// u = x * inv (|u| <= SPLIT_R) -> h = rint(u), l = fp32(u - h) (u - h is
+// exact in FP64).
+__device__ inline void split_fx(double x, double inv, float &h, float &l) {
+  const double u = x * inv;
+  const double r = rint(u);
+  h = (float)r;
+  l = (float)(u - r);
 }
 
 template <int TM, int KC, bool SPLIT>
@@ -50,6 +70,7 @@ struct Problem {
   double2 *chi;           // chi[w*ldc*nb + I + ldc*J]
   long ldc;
   double alpha, beta;
+  const double *scale;    // split-FP32: s(I,w) at scale[w*nb + I] (colscale.hpp)
 };
 
 // Computes tile (bi, bj) of frequency w.  Must be called by all 256 threads
@@ -59,6 +80,7 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
   constexpr int WB = TM / 32;             // 16x16 blocks per wave per dimension
   constexpr int PER = KC * TM / NT;       // elements per thread per stage
   static_assert(PER >= 1 && KC % 4 == 0, "tile shape");
+  static_assert(2.0 * KC * SPLIT_R * SPLIT_R <= 16777216.0, "h*h sums of a K-slice exact in FP32");
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int wr = wv >> 1, wc = wv & 1, lr = lane & 15, lk = lane >> 4;
   const int I0 = bi * TM, J0 = bj * TM;
@@ -87,6 +109,20 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
       i = e % TM;
     }
   };
+  // split form: R / s of the columns this thread stages (0 for padding and
+  // all-zero columns)
+  double inva[SPLIT ? PER : 1], invb[SPLIT ? PER : 1];
+  if constexpr (SPLIT) {
+    const double *S = P.scale + (long)w * nb;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      int k, i;
+      coords(tid + NT * r, k, i);
+      const double sa = I0 + i < nb ? S[I0 + i] : 0.0, sb = J0 + i < nb ? S[J0 + i] : 0.0;
+      inva[r] = sa > 0.0 ? SPLIT_R / sa : 0.0;
+      invb[r] = sb > 0.0 ? SPLIT_R / sb : 0.0;
+    }
+  }
   auto fetch = [&](long st) {
     const long t0 = st * KC;
 #pragma unroll
@@ -111,17 +147,20 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
       int k, i;
       coords(tid + NT * r, k, i);
       if constexpr (SPLIT) {
+        // the same sqrt|D| as the scale pass, so |sq*M| <= s bit for bit
+        const double sq = sqrt(fabs(dv[r]));
+        const double sa = dv[r] < 0.0 ? -sq : sq;
         float h, l;
-        split2(dv[r] * va[r].x, h, l);
+        split_fx(sa * va[r].x, inva[r], h, l);
         s.Lrh[buf][k][i] = h;
         s.Lrl[buf][k][i] = l;
-        split2(dv[r] * va[r].y, h, l);
+        split_fx(sa * va[r].y, inva[r], h, l);
         s.Lih[buf][k][i] = h;
         s.Lil[buf][k][i] = l;
-        split2(vb[r].x, h, l);
+        split_fx(sq * vb[r].x, invb[r], h, l);
         s.Rrh[buf][k][i] = h;
         s.Rrl[buf][k][i] = l;
-        split2(vb[r].y, h, l);
+        split_fx(sq * vb[r].y, invb[r], h, l);
         s.Rih[buf][k][i] = h;
         s.Ril[buf][k][i] = l;
       } else {
@@ -155,6 +194,7 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
       for (int kk = 0; kk < KC / 4; kk++) {
         const int k = kk * 4 + lk;
         float arh[WB], arl[WB], aih[WB], ail[WB], brh[WB], brl[WB], bih[WB], bil[WB];
+        float brx[WB], bix[WB];
 #pragma unroll
         for (int a = 0; a < WB; a++) {
           const int ia = wr * (TM / 2) + a * 16 + lr, jb = wc * (TM / 2) + a * 16 + lr;
@@ -166,7 +206,11 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
           brl[a] = s.Rrl[buf][k][jb];
           bih[a] = s.Rih[buf][k][jb];
           bil[a] = s.Ril[buf][k][jb];
+          brx[a] = brh[a] + brl[a];
+          bix[a] = bih[a] + bil[a];
         }
+        // Re = ar br + ai bi, Im = ai br - ar bi with a = h + l on the row
+        // side: h*h exactly into (hr, hi_), h*l + l*x into (xr, xi)
 #pragma unroll
         for (int a = 0; a < WB; a++)
 #pragma unroll
@@ -176,13 +220,13 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
             hi_[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], brh[b], hi_[a][b], 0, 0, 0);
             hi_[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arh[a], bih[b], hi_[a][b], 0, 0, 0);
             xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(arh[a], brl[b], xr[a][b], 0, 0, 0);
-            xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(arl[a], brh[b], xr[a][b], 0, 0, 0);
+            xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(arl[a], brx[b], xr[a][b], 0, 0, 0);
             xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], bil[b], xr[a][b], 0, 0, 0);
-            xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ail[a], bih[b], xr[a][b], 0, 0, 0);
+            xr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ail[a], bix[b], xr[a][b], 0, 0, 0);
             xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(aih[a], brl[b], xi[a][b], 0, 0, 0);
-            xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ail[a], brh[b], xi[a][b], 0, 0, 0);
+            xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ail[a], brx[b], xi[a][b], 0, 0, 0);
             xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arh[a], bil[b], xi[a][b], 0, 0, 0);
-            xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arl[a], bih[b], xi[a][b], 0, 0, 0);
+            xi[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(-arl[a], bix[b], xi[a][b], 0, 0, 0);
           }
       }
       // FP64 correction pass of this K-slice
@@ -238,7 +282,11 @@ __device__ void tile(const Problem &P, int bi, int bj, int w, Lds<TM, KC, SPLIT>
         const int I = I0 + wr * (TM / 2) + a * 16 + (SPLIT ? 4 * lk + r : lk + 4 * r);
         if (I >= nb) continue;
         double2 *p = C + I + P.ldc * (long)J;
-        double2 o = make_double2(P.alpha * accr[a][b][r], P.alpha * acci[a][b][r]);
+        double f = P.alpha;
+        if constexpr (SPLIT)
+          f *= (P.scale[(long)w * nb + I] * (1.0 / SPLIT_R)) *
+               (P.scale[(long)w * nb + J] * (1.0 / SPLIT_R));
+        double2 o = make_double2(f * accr[a][b][r], f * acci[a][b][r]);
         if (P.beta != 0.0) {
           const double2 old = *p;
           o.x += P.beta * old.x;

@@ -31,6 +31,10 @@ struct spexhip_ctx {
   // scratch for host drop-in calls (grown on demand, owned by the context)
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // column scales of the split-FP32 chi0 form for calls on the context
+  // stream (grown on demand; pipelines and batches carry their own)
+  void *colscale = nullptr;
+  size_t colscale_bytes = 0;
   // largest slice workspace of the Ozaki form (frequencies are grouped to fit)
   size_t ozaki_ws_limit = (size_t)24 << 30;
   // events bracketing the phases of the last Ozaki chi0 call (slice of the
@@ -73,6 +77,8 @@ int spx_aux_stream(spexhip_ctx *ctx);
 int spx_free(spexhip_ctx *ctx, void *p);
 // Context-owned scratch of at least `bytes` (contents undefined).
 int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p);
+// Context-owned buffer for split-FP32 column scales (context stream only).
+int spx_colscale_buf(spexhip_ctx *ctx, size_t bytes, void **p);
 // Is `p` (or p+bytes-1) inside an allocation of this context?
 bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes);
 
@@ -80,7 +86,11 @@ bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes);
 int spx_chi0_ozaki(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,
                    int colmajor, int nw, const void *wght, double alpha, double beta, void *chi,
                    int ldchi, size_t ws_limit);
+// scale_ws: nb*nw doubles for the split-FP32 column scales; nullptr takes
+// the context's own buffer (then st must be the context stream).
 int spx_chi0_launch(spexhip_ctx *ctx, hipStream_t st, long n, int nb,
                     const void *M, long ldm, int colmajor, int nw,
                     const void *wght, int precision, double alpha, double beta,
-                    void *chi, int ldchi);
+                    void *chi, int ldchi, void *scale_ws = nullptr);
+// Bytes of scale_ws for the split-FP32 form.
+size_t spx_split_ws_bytes(int nb, int nw);

@@ -131,6 +131,25 @@ int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p) {
   return SPEXHIP_OK;
 }
 
+int spx_colscale_buf(spexhip_ctx *ctx, size_t bytes, void **p) {
+  if (ctx->colscale_bytes >= bytes) {
+    *p = ctx->colscale;
+    return SPEXHIP_OK;
+  }
+  if (ctx->colscale) {
+    SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = spx_free(ctx, ctx->colscale);
+    ctx->colscale = nullptr;
+    ctx->colscale_bytes = 0;
+    if (rc) return rc;
+  }
+  int rc = spx_alloc(ctx, bytes, &ctx->colscale);
+  if (rc) return rc;
+  ctx->colscale_bytes = bytes;
+  *p = ctx->colscale;
+  return SPEXHIP_OK;
+}
+
 extern "C" {
 
 int spexhip_create(int device, size_t budget, spexhip_ctx **out) {

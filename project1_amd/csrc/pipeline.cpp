@@ -54,6 +54,7 @@ Sizes sizes_of(const spexhip_qdesc &d) {
   size_t ws = 16 * (size_t)d.nb * d.nb * d.nw;          // dense v^1/2 temporary
   ws = std::max(ws, spx_inverse_ws(d.nb, d.nw));
   ws = std::max(ws, 16 * (size_t)d.nb * (d.nb + 1) / 2 * d.nw);  // packing
+  ws = std::max(ws, spx_split_ws_bytes(d.nb, d.nw));  // split-FP32 column scales
   if (d.precision == SPEXHIP_PREC_OZAKI_INT8)
     ws = std::max(ws, spx_ozaki_ws_bytes(n, d.nb, d.nw, std::min(d.nw, d.ozaki_group > 0 ? d.ozaki_group : d.nw)));
   if (d.mt_plan) ws = std::max(ws, spx_wfprod_mt_ws(d.mt_plan, kp, d.nocc, d.nunocc));
@@ -110,7 +111,7 @@ int enqueue(spexhip_ctx *ctx, spexhip_qpipe *q) {
                                 d.nb, q->ws, q->wsb, q->flags + 1);
     else
       rc = spx_chi0_launch(ctx, st, np, d.nb, Mp, q->ldm, 0, d.nw, q->D, d.precision, d.alpha,
-                           beta, q->X, d.nb);
+                           beta, q->X, d.nb, q->ws);
     if (rc) return rc;
   }
   rc = spx_dielectric(ctx, st, d.nb, d.nw, q->X, d.nb, d.vkind, d.vhalf_dev, q->E, d.nb, q->ws);

@@ -39,7 +39,7 @@ def per_q(lat, reqs):
 
 
 @pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-12),
-                                           (spexhip.PREC_SPLIT_FP32, 3e-7)])
+                                           (spexhip.PREC_SPLIT_FP32, 1e-8)])
 def test_batch_matches_per_q_pipelines(lattice, precision, tol):
     reqs = uneven(lattice, precision)
     ref = per_q(lattice, [serving.Request(lattice, r.spec, spexhip.PREC_FP64) for r in reqs])
