@@ -10,11 +10,18 @@
 // frequency stride ld*nb).  v^1/2 is either diagonal (sqrt(coul), the
 // Coulomb eigenbasis of the reference) or a dense Hermitian matrix; the
 // dense products use an FP64 MFMA complex GEMM.  The inverse is an in-place
-// blocked Gauss-Jordan elimination without pivoting (eps is Hermitian
-// positive definite on the imaginary axis, eigenvalues >= 1, so every
-// leading block is HPD and no pivoting is needed), with 64-wide panels:
-// one workgroup inverts the diagonal block in LDS, an MFMA GEMM forms the
-// row panel, and a rank-64 MFMA update sweeps the whole matrix.
+// blocked Gauss-Jordan elimination with 64-wide panels: one workgroup
+// inverts the diagonal block in registers, an MFMA GEMM forms the row panel,
+// and a rank-64 MFMA update sweeps the whole matrix.  eps is Hermitian
+// positive definite on the imaginary axis (eigenvalues >= 1), so every
+// leading block is HPD and no pivoting is needed there.  The real-frequency
+// eps (dielecc) is a general complex matrix, which the reference inverts by
+// LU with partial pivoting (zgetrf/zgetri, wrapper.f:2357-2358); here each
+// panel first picks its pivot rows by an LU with partial pivoting of the
+// tall panel (a copy), the rows are interchanged in the whole working array,
+// the panel is eliminated as above, and the columns of the result are
+// interchanged back in reverse order at the end (Gauss-Jordan with row
+// interchanges: inv(A) = inv(PA) P).
 #include "gemm_f64.hpp"
 #include "internal.hpp"
 
@@ -91,8 +98,9 @@ constexpr int GB = 64;  // panel width
 // logdet (optional): logdet[blockIdx.y] = sum of log(pivot) over this block
 // -- the pivots of Gauss-Jordan on an HPD matrix are the Schur-complement
 // diagonals, so summed over all blocks they give log det.
-// GEN: a general complex matrix (real-frequency eps): complex pivots, no
-// pivoting; a zero or non-finite pivot sets the flag.
+// GEN: a general complex matrix (real-frequency eps) whose rows were
+// interchanged by gj_pivot_kernel: complex pivots; a zero or non-finite
+// pivot sets the flag.
 template <bool GEN>
 __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
                                                       long ldx, long sX, int k0,
@@ -167,6 +175,131 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__r
       double ld = 0.0;
       for (int p = 0; p < GB; p++) ld += piv[p];
       logdet[blockIdx.y] = ld;
+    }
+  }
+}
+
+// Pivot rows of panel k0 (general matrices): LU with partial pivoting of
+// the tall panel X(k0:nb, k0:k0+w) on a copy in C (ld ldc), one workgroup
+// of 16 waves per matrix (blockIdx.y).  Column p's pivot is the first row
+// of largest |Re| + |Im| at or below p (izamax, as zgetrf chooses it);
+// piv[k0+p] = its absolute row.  Only the pivot choice is kept: the
+// multipliers are not stored, since the Gauss-Jordan step recomputes the
+// elimination on the interchanged matrix.  A zero or non-finite pivot sets
+// the flag.
+__global__ __launch_bounds__(1024) void gj_pivot_kernel(int nb, const double2 *__restrict__ X,
+                                                        long ldx, long sX, int k0,
+                                                        double2 *__restrict__ Cw, long ldc,
+                                                        int *__restrict__ piv, long nbp,
+                                                        int *__restrict__ flag) {
+  constexpr int NTP = 1024, NWV = NTP / 64;
+  __shared__ double bv[NWV];
+  __shared__ int bi[NWV];
+  __shared__ int prow;
+  const double2 *Xb = X + sX * blockIdx.y;
+  double2 *C = Cw + ldc * GB * blockIdx.y;
+  int *pv = piv + nbp * blockIdx.y;
+  const int m = nb - k0, w = m < GB ? m : GB;
+  for (long e = threadIdx.x; e < (long)m * w; e += NTP) {
+    const int c = (int)(e / m), r = (int)(e % m);
+    C[r + ldc * c] = Xb[(k0 + r) + ldx * (long)(k0 + c)];
+  }
+  __syncthreads();
+  bool bad = false;
+  for (int p = 0; p < w; p++) {
+    double best = -1.0;
+    int bidx = p;
+    for (int r = p + (int)threadIdx.x; r < m; r += NTP) {
+      const double2 z = C[r + ldc * p];
+      const double v = fabs(z.x) + fabs(z.y);
+      if (v > best) {
+        best = v;
+        bidx = r;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bidx, off);
+      if (ov > best || (ov == best && oi < bidx)) {
+        best = ov;
+        bidx = oi;
+      }
+    }
+    if ((threadIdx.x & 63) == 0) {
+      bv[threadIdx.x >> 6] = best;
+      bi[threadIdx.x >> 6] = bidx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double bb = bv[0];
+      int ii = bi[0];
+      for (int k = 1; k < NWV; k++)
+        if (bv[k] > bb || (bv[k] == bb && bi[k] < ii)) {
+          bb = bv[k];
+          ii = bi[k];
+        }
+      prow = ii;
+      pv[k0 + p] = k0 + ii;
+    }
+    __syncthreads();
+    const int q = prow;
+    if (q != p && (int)threadIdx.x < w) {
+      const double2 t = C[p + ldc * threadIdx.x];
+      C[p + ldc * threadIdx.x] = C[q + ldc * threadIdx.x];
+      C[q + ldc * threadIdx.x] = t;
+    }
+    __syncthreads();
+    const double2 d = C[p + ldc * p];
+    const double m2 = d.x * d.x + d.y * d.y;
+    bad |= !(m2 > 0.0) || !isfinite(m2);
+    const double2 inv = m2 > 0.0 ? make_double2(d.x / m2, -d.y / m2) : make_double2(0.0, 0.0);
+    for (int r = p + 1 + (int)threadIdx.x; r < m; r += NTP) {
+      const double2 a = C[r + ldc * p];
+      const double2 l = make_double2(a.x * inv.x - a.y * inv.y, a.x * inv.y + a.y * inv.x);
+      for (int c = p + 1; c < w; c++) {
+        const double2 u = C[p + ldc * c];
+        double2 v = C[r + ldc * c];
+        v.x -= l.x * u.x - l.y * u.y;
+        v.y -= l.x * u.y + l.y * u.x;
+        C[r + ldc * c] = v;
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && bad) flag[0] = 1;
+}
+
+// Row interchanges of panel k0 in every column of the working array.
+__global__ void gj_swap_rows_kernel(int nb, double2 *__restrict__ X, long ldx, long sX, int k0,
+                                    const int *__restrict__ piv, long nbp) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nb) return;
+  double2 *col = X + sX * blockIdx.y + ldx * (long)j;
+  const int *pv = piv + nbp * blockIdx.y;
+  const int w = nb - k0 < GB ? nb - k0 : GB;
+  for (int p = k0; p < k0 + w; p++) {
+    const int q = pv[p];
+    if (q != p) {
+      const double2 t = col[p];
+      col[p] = col[q];
+      col[q] = t;
+    }
+  }
+}
+
+// inv(A) = inv(PA) P: column interchanges in reverse order, one thread per row.
+__global__ void gj_unswap_cols_kernel(int nb, double2 *__restrict__ X, long ldx, long sX,
+                                      const int *__restrict__ piv, long nbp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  double2 *Xb = X + sX * blockIdx.y;
+  const int *pv = piv + nbp * blockIdx.y;
+  for (int p = nb - 1; p >= 0; p--) {
+    const int q = pv[p];
+    if (q != p) {
+      const double2 t = Xb[i + ldx * (long)p];
+      Xb[i + ldx * (long)p] = Xb[i + ldx * (long)q];
+      Xb[i + ldx * (long)q] = t;
     }
   }
 }
@@ -293,28 +426,40 @@ int spx_hermitize(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, lon
   return SPEXHIP_OK;
 }
 
-// Workspace (bytes) of spx_inverse for nb, nw.
+// Workspace (bytes) of spx_inverse for nb, nw; the last 256 bytes are free
+// for the caller's flag.
 size_t spx_inverse_ws(int nb, int nw) {
   const long nbp = ((nb + GB - 1) / GB) * GB;
-  return 16 * (size_t)nw * ((size_t)GB * GB + (size_t)GB * nbp + (size_t)nbp * GB) + 256;
+  const size_t piv = ((4 * (size_t)nw * nbp) + 255) & ~(size_t)255;
+  return 16 * (size_t)nw * ((size_t)GB * GB + (size_t)GB * nbp + (size_t)nbp * GB) + piv + 256;
 }
 
-// In-place inverse of nw HPD matrices (full storage).  ws: spx_inverse_ws
-// bytes; flag: device int, set non-zero if a pivot is not positive.
+// In-place inverse of nw matrices (full storage): HPD (general = 0) or
+// general with partial pivoting (general != 0).  ws: spx_inverse_ws bytes;
+// flag: device int, set non-zero if a pivot is not positive (HPD) or is
+// zero (general).
 int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx, void *ws,
                 int *flag, double *logdet_parts, int general) {
   const long nbp = ((nb + GB - 1) / GB) * GB;
   double2 *P = (double2 *)ws;
   double2 *R = P + (long)nw * GB * GB;        // GB x nbp per matrix (ld GB)
   double2 *Cp = R + (long)nw * GB * nbp;      // nbp x GB per matrix (ld nbp)
+  int *piv = (int *)(Cp + (long)nw * nbp * GB);  // nbp per matrix (general)
   const long sX = ldx * (long)nb;
   const int nt = (int)(nbp / GB);
   for (int kb = 0; kb < nt; kb++) {
     const int k0 = kb * GB;
-    if (general)
+    if (general) {
+      // Cp is free until gj_copy_col_kernel below: the pivot search's copy
+      hipLaunchKernelGGL(gj_pivot_kernel, dim3(1, nw), dim3(1024), 0, st, nb, (const double2 *)X,
+                         ldx, sX, k0, Cp, nbp, piv, nbp, flag);
+      SPX_LAUNCHED(ctx);
+      hipLaunchKernelGGL(gj_swap_rows_kernel, dim3((nb + 255) / 256, nw), dim3(256), 0, st, nb,
+                         (double2 *)X, ldx, sX, k0, (const int *)piv, nbp);
+      SPX_LAUNCHED(ctx);
       hipLaunchKernelGGL(gj_diag_kernel<true>, dim3(1, nw), dim3(NT), 0, st, nb,
                          (const double2 *)X, ldx, sX, k0, P, flag, nullptr);
-    else
+    } else
       hipLaunchKernelGGL(gj_diag_kernel<false>, dim3(1, nw), dim3(NT), 0, st, nb,
                          (const double2 *)X, ldx, sX, k0, P, flag,
                          logdet_parts ? logdet_parts + (long)kb * nw : nullptr);
@@ -330,6 +475,11 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
     hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nw), dim3(NT), 0, st, nb, (double2 *)X, ldx,
                        sX, k0, (const double2 *)P, (const double2 *)R, (long)GB, (long)GB * nbp,
                        (const double2 *)Cp, nbp);
+    SPX_LAUNCHED(ctx);
+  }
+  if (general) {
+    hipLaunchKernelGGL(gj_unswap_cols_kernel, dim3((nb + 255) / 256, nw), dim3(256), 0, st, nb,
+                       (double2 *)X, ldx, sX, (const int *)piv, nbp);
     SPX_LAUNCHED(ctx);
   }
   return SPEXHIP_OK;

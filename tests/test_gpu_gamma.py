@@ -325,13 +325,48 @@ def test_general_dielectric_and_inverse_vs_numpy(ctx, vkind):
         b.free()
 
 
-def test_general_inverse_rejects_a_zero_pivot(ctx):
+def test_general_inverse_pivots_like_zgetrf(ctx):
+    """Partial pivoting (wrapper.f:2357-2358 calls zgetrf/zgetri): an
+    invertible matrix with a zero leading pivot, and well-conditioned
+    matrices whose rows are scrambled across the 64-row panels so that the
+    natural pivots are tiny (1e-12) or zero, invert to numpy's inverse."""
     nb = 70
     X = np.eye(nb, dtype=complex)
     X[0, 0] = 0.0
     X[0, 1] = X[1, 0] = 1.0  # invertible, but the first pivot is zero
     dX = _dev_f(ctx, X, np.complex128)
-    with pytest.raises(spexhip.SpexError):
+    ctx.inverse_general_dev(dX, nb, 1, nb)
+    got = dX.download(np.complex128, (nb, nb)).T
+    np.testing.assert_allclose(got, np.linalg.inv(X), rtol=0, atol=1e-15)
+    dX.free()
+    nb, nw = 200, 3
+    g = np.random.default_rng(31)
+    mats = []
+    for f in range(nw):
+        B = np.eye(nb) + 0.3 * (g.standard_normal((nb, nb)) + 1j * g.standard_normal((nb, nb))) / np.sqrt(nb)
+        A = B[g.permutation(nb)]  # same condition number, pivots scattered over the panels
+        A[0, 0] = 1e-12 if f == 0 else 0.0
+        A[64, 64] = 1e-13  # the first pivot of the second panel, before elimination
+        mats.append(A)
+    A = np.stack(mats, 2)
+    dX = _dev_f(ctx, A, np.complex128)
+    ctx.inverse_general_dev(dX, nb, nw, nb)
+    inv = dX.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)
+    for f in range(nw):
+        want = np.linalg.inv(A[:, :, f])
+        mx, fro = normwise(inv[:, :, f], want)
+        print(f"pivoted inverse f={f}: cond {np.linalg.cond(A[:, :, f]):.1f} max-rel {mx:.2e}")
+        assert mx < 1e-12, (f, mx)
+    dX.free()
+
+
+def test_general_inverse_rejects_a_singular_matrix(ctx):
+    nb = 90
+    g = np.random.default_rng(32)
+    X = g.standard_normal((nb, nb)) + 1j * g.standard_normal((nb, nb))
+    X[:, 70] = 0.0  # no pivot exists in column 70
+    dX = _dev_f(ctx, X, np.complex128)
+    with pytest.raises(spexhip.SpexError, match="pivot"):
         ctx.inverse_general_dev(dX, nb, 1, nb)
     dX.free()
 
