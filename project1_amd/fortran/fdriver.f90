@@ -1,8 +1,8 @@
 ! Fortran host exercising the ISO_C_BINDING boundary: one packet of the
 ! HILBERT NONE pair sum and the dielectric inverse, compared with a plain
 ! Fortran pair loop written the way susceptibility.f:1183-1240 does it; the
-! screened interaction, the RPA integrand, the HILBERT pair sum and the
-! Gamma-point head and wings likewise.
+! screened interaction, the RPA integrand, the HILBERT pair sum, the
+! Gamma-point head and wings and the equivalent-k averaging likewise.
 ! Exit status 0 on agreement (normwise relative error < 1e-12).
 program fdriver
   use, intrinsic :: iso_c_binding
@@ -231,6 +231,51 @@ program fdriver
     err = maxval(abs(e2 - e1)) / maxval(abs(e1))
     write(*, '(A,ES10.3)') 'fortran dielec_inverse_gamma body max-rel error: ', err
     if (err > 1d-12 .or. any(wh(:, 1, :) /= 0)) stop 9
+  end block
+  ! Averaging over equivalent k points (tetrahedron.f:1282-1318) through the
+  ! shim with a kptsym table of nkpt2 = 2*nkpt rows, as SPEX allocates it when
+  ! a kptadd vector is present; the second half maps into the shifted set
+  ! (indices > nkpt).  The shim must hand over only the mesh rows, so its
+  ! result equals the call with the first nkpt rows alone.  With
+  ! SPEXHIP_FDRIVER_OUT set, tables and weights are written there for a
+  ! comparison with the oracle.
+  block
+    integer, parameter :: nk = 10, nsym = 4, nkpt1 = 7, no = 3, nu = 5, nf = 6, ik = 4
+    integer :: ks2(2*nk, nsym), kp1(nkpt1), kpp(nk), s, u
+    real(c_double), target :: w0(nkpt1*no*nu, nf), w1(nkpt1*no*nu, nf), w2(nkpt1*no*nu, nf)
+    character(len=512) :: outdir
+    integer :: ln
+    do s = 1, nsym
+      do k = 1, nk
+        ks2(k, s) = mod(7*k + 3*s + k*s, nk) + 1
+        ks2(nk + k, s) = nk + mod(5*k + s, nk) + 1
+      end do
+    end do
+    ks2(1:nk, 1) = [(k, k = 1, nk)]
+    ks2(ik, :) = ik
+    kp1 = [3, 1, 8, 5, 10, 6, 2]
+    do k = 1, nk
+      kpp(k) = mod(3*k, nkpt1 + 1)
+    end do
+    do f = 1, nf
+      do i = 1, nkpt1*no*nu
+        w0(i, f) = 0.5d0 + 0.3d0*sin(0.1d0*i + 0.7d0*f)
+      end do
+    end do
+    w1 = w0
+    call spexhip_tetrahedron5_equiv(ctx, no, nu, nf, kp1, kpp, ks2, ik, w1)
+    w2 = w0
+    call spexhip_tetrahedron5_equiv(ctx, no, nu, nf, kp1, kpp, ks2(1:nk, :), ik, w2)
+    write(*, '(A,ES10.3)') 'fortran tetrahedron5_equiv (2*nkpt kptsym) change: ', &
+        maxval(abs(w1 - w0))
+    if (any(w1 /= w2) .or. maxval(abs(w1 - w0)) == 0) stop 10
+    call get_environment_variable('SPEXHIP_FDRIVER_OUT', outdir, ln)
+    if (ln > 0) then
+      open(newunit=u, file=outdir(1:ln)//'/equiv.bin', access='stream', form='unformatted', &
+           status='replace')
+      write(u) nk, nsym, nkpt1, no, nu, nf, ik, kp1, kpp, ks2, w0, w1
+      close(u)
+    end if
   end block
   rc = spexhip_destroy(ctx)
   write(*, '(A)') 'fortran driver OK'

@@ -188,6 +188,13 @@ contains
     error stop 1
   end subroutine
 
+  ! An argument shape the C side cannot check (the sizes of host arrays).
+  subroutine size_error(where, msg)
+    character(*), intent(in) :: where, msg
+    write(0, '(A)') 'SPEX-ERROR (spexhip, '//where//'): '//msg
+    error stop 1
+  end subroutine
+
   ! Pair loop of one memory packet, HILBERT NONE (susceptibility.f:1159-1251):
   ! susceph(ij,ifrq) += sum_k wghtSr(k,ifrq) * cprod(k,i) * conjg(cprod(k,j)).
   subroutine spexhip_pairsum(ctx, n, dim, cprod, nfrqh, wghtSr, susceph, precision)
@@ -298,6 +305,9 @@ contains
     k3 = nkpt3
     pk = c_null_ptr ; pdo = c_null_ptr ; pdu = c_null_ptr ; nk1 = 0
     if (present(kpt1p)) then
+      ! the C side reads one parent per mesh point
+      if (size(kpt1p) /= product(nkpt3)) call size_error('tetrahedron5', &
+          'kpt1p must have one entry per k point of the mesh (product(nkpt3))')
       allocate(kp(size(kpt1p))) ; kp = kpt1p - 1 ; pk = c_loc(kp) ; nk1 = nkpt1
     end if
     if (present(deg_occ)) then
@@ -312,17 +322,25 @@ contains
   ! The last step of tetrahedron5_init (tetrahedron.f:1282-1318) on the
   ! wtetra of spexhip_tetrahedron5: the mean over k points of kpt1 that are
   ! equivalent under the symmetries keeping q.  1-based kpt1(nkpt1),
-  ! kpt1p(nk) (0: no parent), kptsym(nk, nsym) and ikpt, as in the reference.
+  ! kpt1p(nkpt) (0: no parent), kptsym(nkpt2, nsym) and ikpt, as in the
+  ! reference.  SPEX allocates kptsym with nkpt2 = 2*nkpt rows when a kptadd
+  ! vector is present (the shifted set follows the mesh); the averaging reads
+  ! only rows of mesh points (kptsym(kpt1(k),:), kptsym(ikpt,:)), so the
+  ! first nkpt rows are passed.
   subroutine spexhip_tetrahedron5_equiv(ctx, nocc, nunocc, nfrqs, kpt1, kpt1p, kptsym, ikpt, &
                                         wtetra)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: nocc, nunocc, nfrqs, kpt1(:), kpt1p(:), kptsym(:,:), ikpt
     real(c_double), intent(inout), target :: wtetra(*)
     integer(c_int), allocatable, target :: k1(:), kp(:), ks(:,:)
-    allocate(k1(size(kpt1)), kp(size(kpt1p)), ks(size(kptsym,1), size(kptsym,2)))
-    k1 = kpt1 - 1 ; kp = kpt1p - 1 ; ks = kptsym - 1
+    integer :: nk
+    nk = size(kpt1p)
+    if (size(kptsym,1) < nk) call size_error('tetrahedron5 equivalent k', &
+        'kptsym needs a row for every k point of kpt1p')
+    allocate(k1(size(kpt1)), kp(nk), ks(nk, size(kptsym,2)))
+    k1 = kpt1 - 1 ; kp = kpt1p - 1 ; ks = kptsym(1:nk, :) - 1
     call check(ctx, c_tetra_average_equivalent(ctx, int(size(kpt1), c_int), int(nocc, c_int), &
-               int(nunocc, c_int), int(nfrqs, c_int), c_loc(k1), int(size(kptsym,1), c_int), &
+               int(nunocc, c_int), int(nfrqs, c_int), c_loc(k1), int(nk, c_int), &
                c_loc(kp), int(size(kptsym,2), c_int), c_loc(ks), int(ikpt-1, c_int), &
                c_loc(wtetra)), 'tetrahedron5 equivalent k')
   end subroutine
