@@ -58,15 +58,30 @@ def gl_frequencies(nw: int):
     return omega, wq
 
 
-def mesh_frequencies(nfreq: int, freqmax: float):
-    """SPEX MESH grid, correlation.f:2486-2496 (no additional points)."""
-    f = np.empty(nfreq)
+def mesh_frequencies(nfreq: int, freqmax: float, freqadd: int = 0):
+    """SPEX MESH grid on the imaginary axis (keyword ``MESH n[+a] wmax``,
+    correlation.f:2478-2507): x_i = 0.9(i-1)/(n-1) (wmax > 0) or (i-1)/n
+    (wmax < 0), w = x/(1-x) rescaled so that the last point is |wmax|; with
+    ``+a`` the m-th of a rounds puts m equally spaced points into the
+    interval (w_{a-m+1}, w_{a-m+2}) (1-based), and the mesh is sorted."""
+    if nfreq <= 0:
+        raise ValueError("MESH: the number of frequencies must be positive")
+    if nfreq == 1 and freqmax > 0:
+        # the reference's 0.9*(i-1)/(nfreq-1) is 0/0 here
+        raise ValueError("MESH: one frequency needs wmax < 0")
+    f = np.empty(nfreq + freqadd * (freqadd + 1) // 2)
     for i in range(1, nfreq + 1):
         r = 0.9 * (i - 1) / (nfreq - 1) if freqmax > 0 else 1.0 * (i - 1) / nfreq
         f[i - 1] = r / (1 - r)
     if nfreq > 1:
-        f *= abs(freqmax) / f[-1]
-    return f
+        f[:nfreq] *= abs(freqmax) / f[nfreq - 1]
+    j = nfreq
+    for m in range(1, freqadd + 1):
+        lo, hi = f[freqadd - m], f[freqadd - m + 1]  # freq(a-m+1), freq(a-m+2), 1-based
+        for i in range(1, m + 1):
+            f[j] = lo + (hi - lo) / (m + 1) * i
+            j += 1
+    return np.sort(f)
 
 
 def energies(g: np.random.Generator, nk: int, nocc: int, nunocc: int):
