@@ -223,6 +223,23 @@ int spexhip_chi0_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev,
                      int precision, double alpha, double beta, void *chi_dev,
                      int ldchi);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): spexhip_chi0_dev with the same arguments and results,
+ * bit for bit, but no hipMalloc, hipFree or other allocation.  The
+ * split-FP32 column scales (nb*nw doubles) live in a fixed device array of
+ * SPEXHIP_NOALLOC_MAX_SCALES doubles (1 MiB per GPU, held until the process
+ * ends) instead of a context buffer; nb*nw above that limit, and the Ozaki
+ * form (its slice workspace is gigabytes), return SPEXHIP_E_INVALID.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * (other contexts or threads) race.  The array is rewritten by every call
+ * before it is read.                                                       */
+#define SPEXHIP_NOALLOC_MAX_SCALES 131072
+int spexhip_chi0_noalloc_dev(spexhip_ctx *ctx, long n, int nb, const void *M_dev,
+                             long ldm, int colmajor, int nw, const void *wght_dev,
+                             int precision, double alpha, double beta, void *chi_dev,
+                             int ldchi);
+
 /* Real-frequency form (suscepc, susceptibility.f:1231-1236): complex weights
  * Dr + i*Di (from wghtSr and wghtSi of the real frequencies) and the whole
  * matrix, chi(I,J,w) = alpha * sum_t (Dr + i*Di)(t,w) M(t,I) conj(M(t,J))

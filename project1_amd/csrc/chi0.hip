@@ -188,6 +188,22 @@ __global__ void zero_u64_kernel(unsigned long long *__restrict__ p, long count) 
 
 size_t spx_split_ws_bytes(int nb, int nw) { return 8 * (size_t)nb * nw; }
 
+// Column scales of spexhip_chi0_noalloc_dev: a fixed array in device memory
+// (one copy per GPU, held until the process ends) instead of a context
+// allocation.  Host code never passes `scratch` itself to a kernel (on the
+// host its name is a stand-in, not a device address); spx_noalloc_setup
+// looks up the device address once per context, with the context's device
+// current, and the entry point passes that address.
+static __device__ __align__(256) double scratch[SPEXHIP_NOALLOC_MAX_SCALES];
+
+int spx_noalloc_setup(spexhip_ctx *ctx) {
+  void *ptr = nullptr;
+  SPX_HIP(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  SPX_CHECK(ctx, ptr != nullptr, "no-alloc scratch: null device address");
+  ctx->noalloc_scales = ptr;
+  return SPEXHIP_OK;
+}
+
 int spx_col_scales(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,
                    int colmajor, int nw, const double *W, int from_d, double *scale) {
   const long cnt = (long)nb * nw;
@@ -326,6 +342,31 @@ int spexhip_chi0_dev(spexhip_ctx *ctx, long n, int nb, const void *M, long ldm,
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   return spx_chi0_launch(ctx, ctx->stream, n, nb, M, ldm, colmajor, nw, wght, precision,
                          alpha, beta, chi, ldchi);
+}
+
+int spexhip_chi0_noalloc_dev(spexhip_ctx *ctx, long n, int nb, const void *M, long ldm,
+                             int colmajor, int nw, const void *wght, int precision,
+                             double alpha, double beta, void *chi, int ldchi) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, precision == SPEXHIP_PREC_FP64 || precision == SPEXHIP_PREC_SPLIT_FP32,
+            "chi0_noalloc: precision %d not available without allocation (the Ozaki form "
+            "needs a slice workspace; use spexhip_chi0_dev)", precision);
+  SPX_CHECK(ctx, nb > 0 && nw > 0 && n >= 0 && ldchi >= nb, "chi0_noalloc: bad sizes");
+  SPX_CHECK(ctx, (long)nb * nw <= SPEXHIP_NOALLOC_MAX_SCALES,
+            "chi0_noalloc: nb*nw = %ld exceeds the fixed scratch of %d column scales",
+            (long)nb * nw, SPEXHIP_NOALLOC_MAX_SCALES);
+  SPX_CHECK(ctx, ctx->noalloc_scales != nullptr, "chi0_noalloc: scratch address not set up");
+  if (n > 0) {
+    const size_t mbytes = colmajor ? 16 * ((size_t)ldm * (nb - 1) + n)
+                                   : 16 * ((size_t)ldm * (n - 1) + nb);
+    SPX_CHECK(ctx, spx_owns(ctx, M, mbytes), "chi0_noalloc: M is not a context allocation of the stated shape");
+    SPX_CHECK(ctx, spx_owns(ctx, wght, sizeof(double) * n * nw), "chi0_noalloc: wght is not a context allocation of n*nw doubles");
+  }
+  SPX_CHECK(ctx, spx_owns(ctx, chi, 16 * ((size_t)ldchi * nb * (nw - 1) + (size_t)ldchi * (nb - 1) + nb)),
+            "chi0_noalloc: chi is not a context allocation of the stated shape");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  return spx_chi0_launch(ctx, ctx->stream, n, nb, M, ldm, colmajor, nw, wght, precision,
+                         alpha, beta, chi, ldchi, ctx->noalloc_scales);
 }
 
 int spexhip_pack_upper_d2h(spexhip_ctx *ctx, int nb, int nw, const void *X, int ldx,

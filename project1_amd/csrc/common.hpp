@@ -35,6 +35,9 @@ struct spexhip_ctx {
   // stream (grown on demand; pipelines and batches carry their own)
   void *colscale = nullptr;
   size_t colscale_bytes = 0;
+  // device address of the fixed column-scale array of
+  // spexhip_chi0_noalloc_dev on this context's device (set at creation)
+  void *noalloc_scales = nullptr;
   // largest slice workspace of the Ozaki form (frequencies are grouped to fit)
   size_t ozaki_ws_limit = (size_t)24 << 30;
   // events bracketing the phases of the last Ozaki chi0 call (slice of the
@@ -81,6 +84,9 @@ int spx_scratch(spexhip_ctx *ctx, size_t bytes, void **p);
 int spx_colscale_buf(spexhip_ctx *ctx, size_t bytes, void **p);
 // Is `p` (or p+bytes-1) inside an allocation of this context?
 bool spx_owns(spexhip_ctx *ctx, const void *p, size_t bytes);
+// Looks up the device address of the no-alloc column-scale array (chi0.hip)
+// on the current device and stores it in ctx->noalloc_scales.
+int spx_noalloc_setup(spexhip_ctx *ctx);
 
 // Internal launchers shared between translation units.
 int spx_chi0_ozaki(spexhip_ctx *ctx, hipStream_t st, long n, int nb, const void *M, long ldm,

@@ -168,6 +168,13 @@ int spexhip_create(int device, size_t budget, spexhip_ctx **out) {
   }
   for (auto &e : c->phase_ev)
     if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+  if (spx_noalloc_setup(c) != SPEXHIP_OK) {  // `device` is current
+    for (auto &e : c->phase_ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return SPEXHIP_E_DEVICE;
+  }
   *out = c;
   return SPEXHIP_OK;
 }

@@ -27,6 +27,7 @@ _LIB_PATH = Path(__file__).resolve().parent / "libspexhip.so"
 
 OK, E_INVALID, E_NOMEM, E_DEVICE, E_NOTPD, E_AGAIN = range(6)
 PREC_FP64, PREC_SPLIT_FP32, PREC_OZAKI_INT8 = 0, 1, 2
+NOALLOC_MAX_SCALES = 131072  # SPEXHIP_NOALLOC_MAX_SCALES: nb*nw limit of chi0_noalloc_dev
 
 _c_long = C.c_long
 _vp = C.c_void_p
@@ -307,6 +308,8 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_weights_none_dev": (I, [_vp, I, I, I, _vp, _vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_chi0_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
                                  C.c_double, C.c_double, _vp, I]),
+        "spexhip_chi0_noalloc_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
+                                         C.c_double, C.c_double, _vp, I]),
         "spexhip_weights_hilbert_dev": (I, [_vp, _c_long, I, _vp, _c_long, C.c_double, I, _vp,
                                             I, _vp, _c_long]),
         "spexhip_chi0_cplx_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, _vp, I,
@@ -652,6 +655,15 @@ class Context:
                  nw: int, wght: DevBuf, precision: int, chi: DevBuf, ldchi: int,
                  alpha: float = 1.0, beta: float = 0.0) -> None:
         self._check(lib().spexhip_chi0_dev(
+            self.h, n, nb, M.ptr, ldm, int(colmajor), nw, wght.ptr, precision,
+            alpha, beta, chi.ptr, ldchi))
+
+    def chi0_noalloc_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool,
+                         nw: int, wght: DevBuf, precision: int, chi: DevBuf, ldchi: int,
+                         alpha: float = 1.0, beta: float = 0.0) -> None:
+        """chi0_dev without any device allocation (FP64 and split-FP32; nb*nw at most
+        NOALLOC_MAX_SCALES)."""
+        self._check(lib().spexhip_chi0_noalloc_dev(
             self.h, n, nb, M.ptr, ldm, int(colmajor), nw, wght.ptr, precision,
             alpha, beta, chi.ptr, ldchi))
 
