@@ -59,14 +59,15 @@ def parse():
                    help="CPU baseline sample: all pairs among this many columns")
     p.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                    help="only run 5 plain device copies of MIB MiB (byte-counter calibration)")
-    p.add_argument("--only", choices=[f for f, _ in FORMS], default=None,
-                   help="profiling aid: run one chi0 form and print its timings only")
+    p.add_argument("--only", choices=[f for f, _ in FORMS] + ["theta"], default=None,
+                   help="profiling aid: run one chi0 form (or the theta products leg) and "
+                        "print its timings only")
     p.add_argument("--no-batch", action="store_true", help="skip the 32-q batched leg")
     p.add_argument("--no-serve", action="store_true", help="skip the serving leg")
-    p.add_argument("--serve-seconds", type=float, default=20.0,
+    p.add_argument("--serve-seconds", type=float, default=60.0,
                    help="open-loop window of the serving leg per precision")
-    p.add_argument("--serve-cap", type=int, default=8,
-                   help="requests of the closed-loop capacity run per precision")
+    p.add_argument("--serve-cap", type=int, default=24,
+                   help="requests of the sustained-capacity run per precision")
     return p.parse_args()
 
 
@@ -313,6 +314,51 @@ def hilbert_leg(pipe, a, nfrqs=64, reps=3):
                     "reads a host frequency table); the events bracket the calls"}
 
 
+def theta_leg(pipe, a, sb=3, reps=3):
+    """The interstitial products of the step's workload with the step
+    function theta(r) on the caller's 36^3 grid (SPEX's default FULLPW,
+    wavefproducts.f:630-632): with theta the product is no longer a pure
+    convolution, so the plan keeps the caller's grid instead of the 24^3
+    alias-free one the timed step uses.  theta is synthetic, band-limited
+    to |G_i| <= sb.  HIP events on the context stream around `reps` calls;
+    not part of the timed step."""
+    ctx, sy = pipe.ctx, pipe.sys
+    g = np.random.default_rng(a.seed + 9)
+    w = 2 * sb + 1
+    cs = (g.standard_normal((w, w, w)) + 1j * g.standard_normal((w, w, w))) * 0.02
+    cs = 0.5 * (cs + np.conj(cs[::-1, ::-1, ::-1]))
+    cs[sb, sb, sb] = 0.75
+    N = sy.nfft
+    box = np.zeros(N[::-1], complex)
+    for z in range(-sb, sb + 1):
+        for y in range(-sb, sb + 1):
+            for x in range(-sb, sb + 1):
+                box[z % N[2], y % N[1], x % N[0]] = cs[z + sb, y + sb, x + sb]
+    th = np.ascontiguousarray((np.fft.ifftn(box) * np.prod(N)).real).reshape(-1)
+    plan = ctx.pw_plan(N, sy.gpt, sy.ngpt, sy.gptm, sy.vol, step_r=th)
+    run = lambda: ctx.wfprod_pw_dev(plan, pipe.dcpw, sy.nband, sy.k1, sy.k2, sy.g2, 0, a.nocc,
+                                    a.nocc, a.nunocc, pipe.dM, a.nb, sy.nbasp)
+    run()
+    e0, e1 = ctx.event(), ctx.event()
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    ctx.sync()
+    ms = e1.ms_since(e0) / reps
+    plan.destroy()
+    for e in (e0, e1):
+        e.destroy()
+    pb = 16 * pipe.n * sy.ngptm + sy.cpw.nbytes  # IPW columns of M written, cpw read once
+    gbs = pb / (ms * 1e6)
+    return {"kernel": "pw_inverse_kernel + pw_pair_fused_kernel (caller grid)",
+            "grid": list(N), "ms": round(ms, 3), "achieved": round(gbs, 1), "peak": 8000.0,
+            "unit": "GB/s", "frac": round(gbs / 8000.0, 4), "bytes_per_call": int(pb),
+            "note": "IPW products with theta(r) on the 36^3 caller grid (SPEX's default FULLPW), "
+                    "all %d transitions x %d IPWs; outside the timed step (which runs the "
+                    "product without theta on the 24^3 alias-free grid)" % (pipe.n, sy.ngptm)}
+
+
 def gamma_leg(pipe, a, reps=5):
     """The q = 0 head and wings (headwing_kernel) on this step's M and
     weights, with a synthetic momentum matrix, and the head/wing update of
@@ -435,6 +481,11 @@ def main():
         return
     ctx.set_workspace_limit(64 << 30)  # one frequency group for the Ozaki form at this size
     pipe = Pipeline(ctx, a)
+    if a.only == "theta":
+        print(json.dumps({"only": "theta", "theta_products": theta_leg(pipe, a)}), flush=True)
+        pipe.free()
+        ctx.close()
+        return
     res, out = {}, {}
     flops = chi0_flops(a.nw, pipe.n, a.nb)
     for name, pname in FORMS:
@@ -535,6 +586,9 @@ def main():
         "err_vs_fp64": errs_,
         "roofline": roof,
     }
+    if pipe.products:
+        line["theta_products"] = theta_leg(pipe, a)
+        log(f"theta {json.dumps(line['theta_products'])}")
     line["gamma_headwings"] = gamma_leg(pipe, a)
     line["hilbert_weights"] = hilbert_leg(pipe, a)
     log(f"hilbert {json.dumps(line['hilbert_weights'])}")

@@ -54,6 +54,8 @@ enum {
 };
 
 typedef struct spexhip_ctx spexhip_ctx;
+/* setup of the product and momentum kernels (see the plan_create calls) */
+typedef struct spexhip_plan spexhip_plan;
 
 /* ---- context ---------------------------------------------------------- */
 /* No reference counterpart (the reference has no device); the HBM budget
@@ -322,6 +324,79 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
                                    const double *wghtSi, int precision, double *susceph,
                                    double *suscepc);
 
+/* ---- products in the Coulomb eigenbasis ---------------------------------- */
+/* susceptibility.f:806-832: cprod = conjg(conjg(cprod) . T), i.e.
+ *   out(t, j) = sum_I M(t, I) conj(T(I, j)),  j < dim <= nbasm,
+ * for the transition-major M_dev (t*ldm + I) and T_dev (nbasm x dim
+ * complex, column-major, ld ldt): ctrafo for products <M phi|phi>
+ * (fullpw), else ctrafo1 from spexhip_ctrafo_olap_dev.  out_dev (t*ldo + j)
+ * must not overlap M_dev; out_dev NULL or equal to M_dev transforms in place
+ * (columns 0..dim-1 of each row), block by block through the context's
+ * scratch.  With the transformed products chi0 comes out in the Coulomb
+ * eigenbasis, where v^1/2 is diagonal (sqcoul), as in the reference.       */
+int spexhip_coulomb_trafo_dev(spexhip_ctx *ctx, long n, int nbasm, void *M_dev, long ldm,
+                              const void *T_dev, long ldt, int dim, void *out_dev, long ldo);
+/* ctrafo1 of susceptibility.f:815-819: rows < nbasp of T_dev copied, and
+ * every interstitial column through olapvec (overlap.f:298-315) over the
+ * mixed-basis plane waves gptm (host, [ngptm][3]) of the q point,
+ *   T1(nbasp + i, c) = sum_j conj(cstep(G_j - G_i)) T(nbasp + j, c),
+ * with cstep (host, complex, x fastest, extents 2*cext+1) covering every
+ * G_j - G_i.  T_dev, T1_dev: (nbasp + ngptm) x dim, column-major.          */
+int spexhip_ctrafo_olap_dev(spexhip_ctx *ctx, int nbasp, int ngptm, int dim, const void *T_dev,
+                            long ldt, const int *gptm, const double *cstep, const int *cext,
+                            void *T1_dev, long ldt1);
+
+/* ---- Gamma point (q = 0): momentum matrix -------------------------------- */
+/* momentum_matrix (dwavefproducts.f:89-284), one spin, k points stored in
+ * full, no SOC:  momentum(b2, b1, k, j) = <phi_b1,k | -i d/dx_j | phi_b2,k>,
+ * the muffin-tin part from the gradient formula for spherical harmonics with
+ * the caller's radial integrals qmat1 (l -> l-1) and qmat2 (l -> l+1)
+ * (:145-173 form them from the radial basis), the plane-wave part from the
+ * interstitial overlap olapvec (overlap.f:298-315) with the step function's
+ * Fourier coefficients cstep.  The setup is a plan (kind "momentum").      */
+typedef struct {
+  int lcut;             /* lcut(itype)                                      */
+  const int *nindx;     /* nindx(0:lcut, itype)                             */
+  int neq;              /* neq(itype); the centres of a type are consecutive */
+  int maxindx;          /* leading dimensions of qmat1 and qmat2            */
+  const double *qmat1;  /* qmat1(maxindx, maxindx, 0:lcut) of the type:
+                           qmat1(n1, n2, l), n1 a function of l-1, n2 of l  */
+  const double *qmat2;  /* qmat2(n1, n2, l), n1 a function of l+1, n2 of l  */
+} spexhip_mom_type;
+
+typedef struct {
+  int ntype;
+  const spexhip_mom_type *types;
+  int maxlmindx;        /* first dimension of cmt                           */
+  const double *rlat;   /* rlat[i + 3*j]: reciprocal basis vector j          */
+  int nkpt, maxgpt;
+  const int *gpt;       /* [nkpt][maxgpt][3] G vectors of each k's basis    */
+  const int *ngpt;      /* [nkpt]                                           */
+  const double *kpt;    /* [nkpt][3] k points in the reciprocal basis       */
+  const double *cstep;  /* cstep(-c1:c1,-c2:c2,-c3:c3) complex, x fastest;
+                           must cover every G_j - G_i of one k's basis      */
+  int cext[3];          /* c1, c2, c3                                       */
+} spexhip_mom_desc;
+
+int spexhip_momentum_plan_create(spexhip_ctx *ctx, const spexhip_mom_desc *desc,
+                                 spexhip_plan **plan);
+/* Device form: cmt_dev = cmt(maxlmindx, ncent, nband, nkpt), cpw_dev =
+ * cpw(maxgpt, nband, nkpt) (the arrays of the product plans); k1 (host, nk
+ * 0-based k indices); bands b1lo.. (nocc of them) and b2lo.. (nunocc).
+ * P_dev (n x 3 complex, t = (k*nocc + b1)*nunocc + b2 fastest, ld ldp) is
+ * overwritten, or added to with accumulate != 0 (the reference's lacc).
+ * spexhip_momentum_scale_dev then forms dcprod.                            */
+int spexhip_momentum_dev(spexhip_ctx *ctx, const spexhip_plan *plan, const void *cmt_dev,
+                         int ncent, const void *cpw_dev, int nband, int nk, const int *k1,
+                         int b1lo, int nocc, int b2lo, int nunocc, void *P_dev, long ldp,
+                         int accumulate);
+/* Host drop-in with the reference layout momentum(b2, b1, k, 3) (column-
+ * major, interleaved complex): cmt, cpw host arrays as above.              */
+int spexhip_momentum_matrix(spexhip_ctx *ctx, const spexhip_plan *plan, const double *cmt,
+                            int ncent, const double *cpw, int nband, int nk, const int *k1,
+                            int b1lo, int nocc, int b2lo, int nunocc, double *momentum,
+                            int accumulate);
+
 /* ---- Gamma point (q = 0): head and wings -------------------------------- */
 /* Replaces the momentum-matrix scaling of susceptibility.f:858-873
  * (multdiff_ = 0): dcprod = dcprod / sqrt(vol) / (e_unocc - e_occ), zero
@@ -412,11 +487,40 @@ int spexhip_dielectric_inverse_gamma(spexhip_ctx *ctx, int nb, int nw, const dou
                                      const double *sqcoul, double *head, double *wing,
                                      double *einv_packed);
 
+/* ---- FFT grid setup (host, once per run) -------------------------------- */
+/* rlat[i + 3*j]: component i of reciprocal basis vector j (the reference's
+ * rlat(i,j)); imat, imati: row-major 3x3 integer matrices, imat[3*i + j] =
+ * imat(i,j) (the layout of spexhip_pw_desc.imati, so a Fortran caller
+ * passes transpose(imati)).                                                 */
+/* determine_imat (fft.f:163-238): the unimodular basis change with entries
+ * in [-2,2] that minimises the non-orthogonality of rlat*imat (sum of the
+ * three squared cosines; a candidate replaces the best one when it is lower
+ * by more than 1e-12, candidates in the reference's loop order), and its
+ * inverse imati.                                                            */
+int spexhip_determine_imat(const double *rlat, int *imat, int *imati);
+/* fft_init's grid (fft.f:46-51) via gcount_sphere (fft.f:101-158): the G
+ * points of rlat*imat inside the sphere of radius gcutf, walked in shells
+ * |ix|+|iy|+|iz| = n until a shell is empty; nfft(i) = the smallest
+ * 2^a 3^b 5^c 7^d >= 2*max(i)+1.  npoints (optional) = points in the
+ * sphere.  SPEXHIP_E_INVALID for more than 2^16 points along an axis.     */
+int spexhip_fft_grid(const double *rlat, const int *imat, double gcutf, int *nfft,
+                     long *npoints);
+/* The step function on the FFT grid as FULLPW builds it
+ * (wavefproducts.f:576-590): grid frequency g folded into (-N/2, N/2],
+ * g1 = imat*g, cstep(g1) where |rlat*g1| <= gmax (= 2*gcut + gcutm), 0
+ * elsewhere, then the unnormalised backward DFT.  cstep: the reference
+ * cstep(-c1:c1,-c2:c2,-c3:c3) (complex, x fastest) with cext = {c1,c2,c3};
+ * theta: nfft[0]*nfft[1]*nfft[2] complex values, x fastest (complex in
+ * general: an even grid keeps the +N/2 frequency but not -N/2).  Pass it as
+ * spexhip_pw_desc.step_c.  SPEXHIP_E_INVALID if a kept g1 lies outside
+ * cstep.                                                                   */
+int spexhip_step_grid(const int *nfft, const int *imat, const double *rlat, double gmax,
+                      const double *cstep, const int *cext, double *theta);
+
 /* ---- wave-function products -------------------------------------------- */
 /* The reference reads the basis and wave functions from module global
  * (cmt, cpw, gpt, gptm, nfft, imati, integral, gaunt -- wavefproducts.f:
  * 101-222, 476-560; fft.f:17-28).  Here that setup is a plan built once. */
-typedef struct spexhip_plan spexhip_plan;
 
 /* One muffin-tin atom type (wavefproducts.f:177-222).  integral(d1,d2,d3)
  * is the reference's radial-integral array integral(ln,ln1,lln) for the
@@ -448,6 +552,9 @@ typedef struct {
   const int *gptm;    /* [ngptm][3]                                        */
   const double *step_r; /* optional theta(r) on the grid (x fastest), applied
                            to phi1 like fullpw (:631-633); NULL for none    */
+  const double *step_c; /* optional complex theta(r) (interleaved, x fastest),
+                           e.g. from spexhip_step_grid; used instead of
+                           step_r when set                                  */
 } spexhip_pw_desc;
 
 int spexhip_mt_plan_create(spexhip_ctx *ctx, int ntype, const spexhip_mt_type *types,

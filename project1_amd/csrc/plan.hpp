@@ -28,7 +28,23 @@ struct spexhip_plan {
   std::vector<int> ngpt;          // per k
   std::vector<int> gptm;          // ngptm x 3 (mixed-basis IPWs of q)
   int *gidx = nullptr;            // device [nkpt][maxgpt] box index of each G
-  double *step = nullptr;         // device theta(r) or null
+  double2 *step = nullptr;        // device theta(r) (complex) or null
+  // momentum matrix (kind 3): per atom type the three Cartesian operators
+  // B_j (nn x nn complex, column-major, device) that map the muffin-tin
+  // coefficients of band 1 to the left vectors; the plane-wave basis of
+  // every k (device G vectors) and the step function's coefficients
+  struct MomType {
+    int neq = 0, nn = 0;
+    double2 *B = nullptr;  // 3 consecutive nn x nn matrices
+  };
+  std::vector<MomType> mom;
+  int mom_ncent = 0;
+  int *mom_gpt = nullptr;         // device [nkpt][maxgpt][3]
+  int *mom_ngpt = nullptr;        // device [nkpt]
+  double *mom_kpt = nullptr;      // device [nkpt][3], in the reciprocal basis
+  double2 *mom_cstep = nullptr;   // device cstep, x fastest
+  int cext[3] = {0, 0, 0};
+  double rlat[9] = {0};           // rlat[i + 3*j]
 };
 
 // Box of Fourier components handled by the plane-wave kernels: each axis

@@ -63,7 +63,7 @@ template <int NB>
 __global__ __launch_bounds__(NT) void pw_inverse_kernel(
     const double2 *__restrict__ cpw, int maxgpt, int nband, const int *__restrict__ gidx,
     const int *__restrict__ ngpt, int kA, int bandA, int nA, int kB, int bandB, int N1, int N2,
-    int N3, double scaleA, double scaleB, const double *__restrict__ step,
+    int N3, double scaleA, double scaleB, const double2 *__restrict__ step,
     double2 *__restrict__ grids) {
   __shared__ double Cr[PW_BOX * PW_BOX * PW_BOX], Ci[PW_BOX * PW_BOX * PW_BOX];
   __shared__ double2 tz[64][PW_BOX];  // N3 <= 64
@@ -142,11 +142,7 @@ __global__ __launch_bounds__(NT) void pw_inverse_kernel(
             double2 v = make_double2(orr[r], oi[r]);
             if (occ) {
               v = make_double2(v.x * scale, -v.y * scale);
-              if (step) {
-                const double th = step[g];
-                v.x *= th;
-                v.y *= th;
-              }
+              if (step) v = cmul(v, step[g]);  // fullpw: conj(psi1)/sqrt(vol) * theta
             } else {
               v.x *= scale;
               v.y *= scale;
@@ -309,20 +305,32 @@ __global__ __launch_bounds__(NT) void pw_pair_fused_kernel(
 
 constexpr int gcd16(int n) { return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : (n % 2 == 0) ? 2 : 1; }
 
+// Planes packed per x pass of pw_pair_joint_kernel: all 16/gcd(N2,16)
+// planes that fill whole row blocks when that is at most five blocks, else
+// two planes with the last block partly padded (28: 56 of 64 rows, 36: 72
+// of 80).
+constexpr int joint_planes(int n2) {
+  return (16 / gcd16(n2)) * n2 / 16 <= 5 ? 16 / gcd16(n2) : 2;
+}
+// Waves per SIMD the register budget allows: three up to three row blocks
+// (168 registers), two above (the x-pass accumulators and loads of four or
+// five blocks would spill at three).
+constexpr int joint_waves(int n2) { return (joint_planes(n2) * n2 + 15) / 16 <= 3 ? 3 : 2; }
+
 // pw_pair_fused_kernel for compile-time in-plane sizes N1 x N2: the x pass
-// covers G = 16/gcd(N2,16) consecutive z planes at once, with the G*N2
-// rows (y, z) packed into RB = G*N2/16 MFMA row blocks and no padding rows
-// (a 24-point grid: 2 planes = 3 blocks instead of 2 x 2 half-empty ones).
+// covers G consecutive z planes at once, with the G*N2 rows (y, z) packed
+// into RB = ceil(G*N2/16) MFMA row blocks (a 24-point grid: 2 planes = 3
+// blocks instead of 2 x 2 half-empty ones; rows past G*N2 read as zero).
 // Row R of the packed x-pass result is (y, zz) = (R % N2, R / N2); the y
 // pass of plane zz takes K-step kq (y = 4kq..4kq+3) from row block
 // (N2*zz + 4kq)/16, register ((N2*zz + 4kq) % 16)/4 -- the accumulator
 // layout (rows lane/16 + 4r) is again the operand layout.
 template <int N1, int N2>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 3))) void pw_pair_joint_kernel(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(N2), joint_waves(N2)))) void pw_pair_joint_kernel(
     const double2 *__restrict__ grids, int nocc, int nunocc, int N3, const int *__restrict__ oidx,
     int ngptm, double2 *__restrict__ M, long ldm, long t0, int col0) {
   static_assert(N1 % 4 == 0 && N2 % 4 == 0 && N1 >= PW_BOX && N2 >= PW_BOX, "grid");
-  constexpr int G = 16 / gcd16(N2), RB = G * N2 / 16, KX = N1 / 4, KY = N2 / 4;
+  constexpr int G = joint_planes(N2), RB = (G * N2 + 15) / 16, KX = N1 / 4, KY = N2 / 4;
   // z twiddles, N3 x PW_BOX (dynamic: 6 KB for N3 = 24 instead of a 64-plane
   // table, so four workgroups fit in a CU's LDS)
   extern __shared__ double2 tz[];
@@ -377,7 +385,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(3, 3))) void
       for (int rb = 0; rb < RB; rb++) {
         const int R = 16 * rb + lr, y = R % N2, z = z0 + R / N2;
         double2 a = make_double2(0.0, 0.0);
-        if (z < N3) {
+        if (R < G * N2 && z < N3) {
           const long g = x + (long)N1 * (y + (long)N2 * z);
           a = cmul(p1[g], p2[g]);
         }
@@ -478,7 +486,9 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
                        nocc, k2[ik], b2lo, N1, N2, N3, 1.0 / std::sqrt(p->vol), 1.0 / (double)NN,
                        p->step, grids);
     SPX_LAUNCHED(ctx);
-    if (p->ngptm <= 64 * PW_FOUT && N1 == N2 && (N1 == 16 || N1 == 20 || N1 == 24 || N1 == 32)) {
+    if (p->ngptm <= 64 * PW_FOUT && N1 == N2 &&
+        (N1 == 16 || N1 == 20 || N1 == 24 || N1 == 28 || N1 == 32 || N1 == 36 || N1 == 40 ||
+         N1 == 48)) {
       const int ngrp = (nocc + 3) / 4;
       const dim3 grid((unsigned)(((nunocc + 7) / 8) * 8 * ngrp));
       const size_t tzb = (size_t)N3 * PW_BOX * sizeof(double2);
@@ -488,7 +498,11 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
         case 16: hipLaunchKernelGGL((pw_pair_joint_kernel<16, 16>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
         case 20: hipLaunchKernelGGL((pw_pair_joint_kernel<20, 20>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
         case 24: hipLaunchKernelGGL((pw_pair_joint_kernel<24, 24>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        default: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 28: hipLaunchKernelGGL((pw_pair_joint_kernel<28, 28>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 32: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 36: hipLaunchKernelGGL((pw_pair_joint_kernel<36, 36>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 40: hipLaunchKernelGGL((pw_pair_joint_kernel<40, 40>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        default: hipLaunchKernelGGL((pw_pair_joint_kernel<48, 48>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
       }
       SPX_LAUNCHED(ctx);
       continue;
@@ -555,7 +569,8 @@ int spx_pw_plan_build(spexhip_ctx *ctx, const spexhip_pw_desc *d, spexhip_plan *
   // the 16-point box); a caller grid that aliases is kept as given.
   // Sizes are rounded up to 16, 20, 24 or 32 (the in-plane sizes with a
   // specialised pair kernel) when that is still below the caller's grid.
-  if (!d->step_r)
+  const bool theta = d->step_r || d->step_c;
+  if (!theta)
     for (int i = 0; i < 3; i++) {
       int nmin = std::max(PW_BOX, hi[i] >= lo[i] ? hi[i] - lo[i] + PW_OFF + 1 : PW_BOX);
       for (int nice : {16, 20, 24, 32})
@@ -569,12 +584,16 @@ int spx_pw_plan_build(spexhip_ctx *ctx, const spexhip_pw_desc *d, spexhip_plan *
   if (rc) return rc;
   p->owned.push_back(p->gidx);
   SPX_HIP(ctx, hipMemcpy(p->gidx, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
-  if (d->step_r) {
+  if (theta) {
     const size_t nn = (size_t)p->nfft[0] * p->nfft[1] * p->nfft[2];
-    rc = spx_alloc(ctx, 8 * nn, (void **)&p->step);
+    std::vector<double2> th(nn);
+    for (size_t i = 0; i < nn; i++)
+      th[i] = d->step_c ? make_double2(d->step_c[2 * i], d->step_c[2 * i + 1])
+                        : make_double2(d->step_r[i], 0.0);
+    rc = spx_alloc(ctx, 16 * nn, (void **)&p->step);
     if (rc) return rc;
     p->owned.push_back(p->step);
-    SPX_HIP(ctx, hipMemcpy(p->step, d->step_r, 8 * nn, hipMemcpyHostToDevice));
+    SPX_HIP(ctx, hipMemcpy(p->step, th.data(), 16 * nn, hipMemcpyHostToDevice));
   }
   return SPEXHIP_OK;
 }

@@ -277,6 +277,35 @@ program fdriver
       close(u)
     end if
   end block
+  ! FFT grid setup (getinput.f:2472-2473) for a simple cubic reciprocal lattice
+  ! of spacing 1.1 written in a skewed basis: determine_imat must make the
+  ! basis orthogonal again, and the grid of the 8.3 sphere is then that of
+  ! the cubic lattice (half-width 7 -> 15 points).  With SPEXHIP_FDRIVER_OUT
+  ! set, rlat, nfft, imat and imati go there for the oracle.
+  block
+    real(c_double) :: rlat(3, 3), r1(3, 3), c
+    integer :: nfft(3), imat(3, 3), imati(3, 3), u, ln, p, q
+    character(len=512) :: outdir
+    rlat = 1.1d0 * reshape([1d0, 0d0, 1d0, 1d0, 1d0, 1d0, 0d0, -1d0, 1d0], [3, 3])
+    call spexhip_fft_setup(rlat, 8.3d0, nfft, imat, imati)
+    r1 = matmul(rlat, dble(imat))
+    c = 0
+    do p = 1, 3
+      do q = p + 1, 3
+        c = c + dot_product(r1(:, p), r1(:, q))**2
+      end do
+    end do
+    write(*, '(A,3I4,A,ES10.3)') 'fortran fft_setup nfft:', nfft, '  non-orthogonality: ', c
+    if (any(matmul(imat, imati) /= reshape([1, 0, 0, 0, 1, 0, 0, 0, 1], [3, 3]))) stop 11
+    if (c > 1d-20 .or. any(nfft /= 15)) stop 12
+    call get_environment_variable('SPEXHIP_FDRIVER_OUT', outdir, ln)
+    if (ln > 0) then
+      open(newunit=u, file=outdir(1:ln)//'/fftgrid.bin', access='stream', form='unformatted', &
+           status='replace')
+      write(u) rlat, nfft, imat, imati
+      close(u)
+    end if
+  end block
   rc = spexhip_destroy(ctx)
   write(*, '(A)') 'fortran driver OK'
 end program fdriver

@@ -35,6 +35,7 @@ module m_spexhip
     integer(c_int) :: ngptm
     type(c_ptr)    :: gptm
     type(c_ptr)    :: step_r
+    type(c_ptr)    :: step_c
   end type
 
   public :: spexhip_create, spexhip_destroy, spexhip_mt_plan_create, spexhip_pw_plan_create
@@ -43,6 +44,7 @@ module m_spexhip
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
   public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
   public :: spexhip_dielecc_inverse, spexhip_tetrahedron5_equiv
+  public :: spexhip_fft_setup, spexhip_step_on_grid
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -163,6 +165,22 @@ module m_spexhip
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, packed
       integer(c_int), value :: nb, count
+    end function
+    integer(c_int) function c_determine_imat(rlat, imat, imati) bind(C, name='spexhip_determine_imat')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: rlat, imat, imati
+    end function
+    integer(c_int) function c_fft_grid(rlat, imat, gcutf, nfft, npoints) bind(C, name='spexhip_fft_grid')
+      import :: c_int, c_ptr, c_double, c_long
+      type(c_ptr), value :: rlat, imat, nfft
+      real(c_double), value :: gcutf
+      integer(c_long) :: npoints
+    end function
+    integer(c_int) function c_step_grid(nfft, imat, rlat, gmax, cstep, cext, theta) &
+        bind(C, name='spexhip_step_grid')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: nfft, imat, rlat, cstep, cext, theta
+      real(c_double), value :: gmax
     end function
   end interface
 
@@ -423,6 +441,49 @@ contains
     integer :: n
     n = nint(sqrt(0.25d0 + 2*size(mat)) - 0.5d0)
     call check(ctx, c_inverse_hpd_packed(ctx, int(n, c_int), 1_c_int, c_loc(mat)), 'Minverse')
+  end subroutine
+
+  ! getinput.f:2472-2473 (call determine_imat; call fft_init(gcutf)): the
+  ! basis change imat/imati of fft.f:163-238 and the grid nfft of fft.f:36-51
+  ! and 101-158 for the reciprocal basis rlat(3,3).  The C side keeps
+  ! imat/imati row-major, hence the transposes.
+  subroutine spexhip_fft_setup(rlat, gcutf, nfft, imat, imati)
+    real(c_double), intent(in), target :: rlat(3, 3)
+    real(c_double), intent(in) :: gcutf
+    integer, intent(out) :: nfft(3), imat(3, 3), imati(3, 3)
+    integer(c_int), target :: im(3, 3), imi(3, 3), nf(3)
+    integer(c_long) :: npts
+    integer(c_int) :: rc
+    rc = c_determine_imat(c_loc(rlat), c_loc(im), c_loc(imi))
+    if (rc /= 0) call size_error('fft_setup', 'determine_imat failed (degenerate lattice)')
+    imat = transpose(im)   ! im(j,i) holds imat(i,j)
+    imati = transpose(imi)
+    rc = c_fft_grid(c_loc(rlat), c_loc(im), gcutf, c_loc(nf), npts)
+    if (rc /= 0) call size_error('fft_setup', 'more than 2^16 points in one direction')
+    nfft = nf
+  end subroutine
+
+  ! The step function on the FFT grid as wavefproducts.f:576-590 builds it for
+  ! FULLPW: theta(0:nfft(1)-1,0:nfft(2)-1,0:nfft(3)-1) from cstep(-c1:c1,
+  ! -c2:c2,-c3:c3) cut at gmax = 2*gcut+gcutm.  Pass theta as the plan's
+  ! step_c.
+  subroutine spexhip_step_on_grid(nfft, imat, rlat, gmax, cstep, theta)
+    integer, intent(in) :: nfft(3), imat(3, 3)
+    real(c_double), intent(in), target :: rlat(3, 3)
+    real(c_double), intent(in) :: gmax
+    complex(c_double_complex), intent(in), target :: cstep(:, :, :)
+    complex(c_double_complex), intent(out), target :: theta(0:nfft(1)-1, 0:nfft(2)-1, 0:nfft(3)-1)
+    integer(c_int), target :: im(3, 3), nf(3), cext(3)
+    integer :: i
+    do i = 1, 3
+      if (mod(size(cstep, i), 2) /= 1) call size_error('step_on_grid', 'cstep extents must be odd')
+      cext(i) = (size(cstep, i) - 1) / 2
+    end do
+    nf = nfft
+    im = transpose(imat)
+    if (c_step_grid(c_loc(nf), c_loc(im), c_loc(rlat), gmax, c_loc(cstep), c_loc(cext), &
+                    c_loc(theta)) /= 0) &
+      call size_error('step_on_grid', 'a frequency inside gmax lies outside cstep')
   end subroutine
 
 end module m_spexhip

@@ -67,7 +67,20 @@ class PwDesc(C.Structure):
     """spexhip_pw_desc"""
     _fields_ = [("nfft", C.c_int * 3), ("imati", C.c_int * 9), ("vol", C.c_double),
                 ("nkpt", C.c_int), ("maxgpt", C.c_int), ("gpt", _vp), ("ngpt", _vp),
-                ("ngptm", C.c_int), ("gptm", _vp), ("step_r", _vp)]
+                ("ngptm", C.c_int), ("gptm", _vp), ("step_r", _vp), ("step_c", _vp)]
+
+
+class MomType(C.Structure):
+    """spexhip_mom_type"""
+    _fields_ = [("lcut", C.c_int), ("nindx", _vp), ("neq", C.c_int), ("maxindx", C.c_int),
+                ("qmat1", _vp), ("qmat2", _vp)]
+
+
+class MomDesc(C.Structure):
+    """spexhip_mom_desc"""
+    _fields_ = [("ntype", C.c_int), ("types", _vp), ("maxlmindx", C.c_int), ("rlat", _vp),
+                ("nkpt", C.c_int), ("maxgpt", C.c_int), ("gpt", _vp), ("ngpt", _vp), ("kpt", _vp),
+                ("cstep", _vp), ("cext", C.c_int * 3)]
 
 
 class Chi0Problem(C.Structure):
@@ -308,6 +321,15 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_weights_none_dev": (I, [_vp, I, I, I, _vp, _vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_chi0_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
                                  C.c_double, C.c_double, _vp, I]),
+        "spexhip_coulomb_trafo_dev": (I, [_vp, _c_long, I, _vp, _c_long, _vp, _c_long, I, _vp,
+                                          _c_long]),
+        "spexhip_ctrafo_olap_dev": (I, [_vp, I, I, I, _vp, _c_long, _vp, _vp, _vp, _vp, _c_long]),
+        "spexhip_momentum_plan_create": (I, [_vp, _vp, P(_vp)]),
+        "spexhip_momentum_dev": (I, [_vp, _vp, _vp, I, _vp, I, I, _vp, I, I, I, I, _vp, _c_long, I]),
+        "spexhip_momentum_matrix": (I, [_vp, _vp, _vp, I, _vp, I, I, _vp, I, I, I, I, _vp, I]),
+        "spexhip_determine_imat": (I, [_vp, _vp, _vp]),
+        "spexhip_fft_grid": (I, [_vp, _vp, C.c_double, _vp, P(_c_long)]),
+        "spexhip_step_grid": (I, [_vp, _vp, _vp, C.c_double, _vp, _vp, _vp]),
         "spexhip_chi0_noalloc_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
                                          C.c_double, C.c_double, _vp, I]),
         "spexhip_weights_hilbert_dev": (I, [_vp, _c_long, I, _vp, _c_long, C.c_double, I, _vp,
@@ -884,7 +906,7 @@ class Context:
                                                  _ptr(cent), maxlmindx, C.byref(h)))
         return Plan(self, h.value, keep)
 
-    def pw_plan(self, nfft, gpt, ngpt, gptm, vol, imati=None, step_r=None) -> Plan:
+    def pw_plan(self, nfft, gpt, ngpt, gptm, vol, imati=None, step_r=None, step_c=None) -> Plan:
         gpt = np.ascontiguousarray(gpt, np.int32)
         ngpt = np.ascontiguousarray(ngpt, np.int32)
         gptm = np.ascontiguousarray(gptm, np.int32)
@@ -901,9 +923,78 @@ class Context:
             st = _f64(step_r)
             keep.append(st)
             d.step_r = _ptr(st)
+        if step_c is not None:
+            sc = _c128(step_c)
+            keep.append(sc)
+            d.step_c = _ptr(sc)
         h = _vp()
         self._check(lib().spexhip_pw_plan_create(self.h, C.byref(d), C.byref(h)))
         return Plan(self, h.value, keep)
+
+    def coulomb_trafo_dev(self, n: int, nbasm: int, M: DevBuf, ldm: int, T: DevBuf, ldt: int,
+                          dim: int, out: DevBuf | None = None, ldo: int = 0) -> None:
+        """out(t, j) = sum_I M(t, I) conj(T(I, j)) (j < dim); in place without out."""
+        self._check(lib().spexhip_coulomb_trafo_dev(self.h, n, nbasm, M.ptr, ldm, T.ptr, ldt, dim,
+                                                    out.ptr if out is not None else None, ldo))
+
+    def ctrafo_olap_dev(self, nbasp: int, gptm, dim: int, T: DevBuf, ldt: int, cstep,
+                        T1: DevBuf, ldt1: int) -> None:
+        """ctrafo1: MT rows of T copied, IPW columns through olapvec with cstep ([z][y][x])."""
+        g = np.ascontiguousarray(gptm, np.int32).reshape(-1, 3)
+        cs = _c128(cstep)
+        cext = np.array([cs.shape[2] // 2, cs.shape[1] // 2, cs.shape[0] // 2], np.int32)
+        self._check(lib().spexhip_ctrafo_olap_dev(self.h, nbasp, g.shape[0], dim, T.ptr, ldt,
+                                                  _ptr(g), _ptr(cs), _ptr(cext), T1.ptr, ldt1))
+
+    def momentum_plan(self, types, maxlmindx, rlat, kpt, gpt, ngpt, cstep) -> Plan:
+        """momentum_matrix setup: types are dicts lcut, nindx, neq, qmat1, qmat2 (arrays
+        [n1, n2, l]); cstep a [z][y][x] complex array of odd extents."""
+        arr = (MomType * len(types))()
+        keep = []
+        for t, T in zip(arr, types):
+            nx = np.ascontiguousarray(T["nindx"], np.int32)
+            q1 = np.asfortranarray(T["qmat1"], np.float64)
+            q2 = np.asfortranarray(T["qmat2"], np.float64)
+            assert q1.shape == q2.shape and q1.shape[0] == q1.shape[1]
+            keep += [nx, q1, q2]
+            t.lcut, t.nindx, t.neq, t.maxindx = int(T["lcut"]), _ptr(nx), int(T["neq"]), q1.shape[0]
+            t.qmat1, t.qmat2 = q1.ctypes.data, q2.ctypes.data
+        gpt = np.ascontiguousarray(gpt, np.int32)
+        ngpt = np.ascontiguousarray(ngpt, np.int32)
+        kpt = _f64(kpt)
+        r = _rlat_f(rlat)
+        cs = _c128(cstep)
+        keep += [arr, gpt, ngpt, kpt, r, cs]
+        d = MomDesc()
+        d.ntype, d.types, d.maxlmindx, d.rlat = len(types), C.cast(arr, _vp), int(maxlmindx), _ptr(r)
+        d.nkpt, d.maxgpt, d.gpt, d.ngpt, d.kpt = gpt.shape[0], gpt.shape[1], _ptr(gpt), _ptr(ngpt), _ptr(kpt)
+        d.cstep = _ptr(cs)
+        d.cext[:] = [cs.shape[2] // 2, cs.shape[1] // 2, cs.shape[0] // 2]
+        h = _vp()
+        self._check(lib().spexhip_momentum_plan_create(self.h, C.byref(d), C.byref(h)))
+        return Plan(self, h.value, keep)
+
+    def momentum_dev(self, plan: Plan, cmt: DevBuf, ncent: int, cpw: DevBuf, nband: int, k1,
+                     b1lo: int, nocc: int, b2lo: int, nunocc: int, P: DevBuf, ldp: int,
+                     accumulate: bool = False) -> None:
+        k1 = np.ascontiguousarray(k1, np.int32)
+        self._check(lib().spexhip_momentum_dev(self.h, plan.h, cmt.ptr, ncent, cpw.ptr, nband,
+                                               k1.size, _ptr(k1), b1lo, nocc, b2lo, nunocc,
+                                               P.ptr, ldp, int(accumulate)))
+
+    def momentum_matrix(self, plan: Plan, cmt, cpw, k1, b1lo, nocc, b2lo, nunocc, out=None):
+        """Host drop-in: (n, 3) complex, the reference momentum(b2, b1, k, 3) layout."""
+        k1 = np.ascontiguousarray(k1, np.int32)
+        cm = np.asfortranarray(cmt, np.complex128)
+        cp = np.asfortranarray(cpw, np.complex128)
+        n = k1.size * nocc * nunocc
+        acc = out is not None
+        res = np.asfortranarray(out if acc else np.zeros((n, 3)), np.complex128)
+        self._check(lib().spexhip_momentum_matrix(self.h, plan.h, cm.ctypes.data, cm.shape[1],
+                                                  cp.ctypes.data, cp.shape[1], k1.size, _ptr(k1),
+                                                  b1lo, nocc, b2lo, nunocc, res.ctypes.data,
+                                                  int(acc)))
+        return res
 
     def wfprod_mt_dev(self, plan: Plan, cmt: DevBuf, ncent, nband, nkpt, q, k1, k2, b1lo,
                       nocc, b2lo, nunocc, M: DevBuf, ldm: int, col0: int = 0) -> None:
@@ -1045,6 +1136,54 @@ class Context:
         nb = int(round((np.sqrt(8 * npack + 1) - 1) / 2))
         self._check(lib().spexhip_inverse_hpd_packed(self.h, nb, count, _ptr(p)))
         return p
+
+
+def _rlat_f(rlat) -> np.ndarray:
+    """rlat (3x3, column j = reciprocal basis vector j) as rlat[i + 3*j]."""
+    r = np.asarray(rlat, np.float64)
+    assert r.shape == (3, 3)
+    return np.ascontiguousarray(r.T).reshape(-1)
+
+
+def _check_host(rc: int, what: str) -> None:
+    if rc != OK:
+        raise SpexError(rc, what)
+
+
+def determine_imat(rlat):
+    """(imat, imati) as 3x3 int arrays (fft.f:163-238, host)."""
+    imat, imati = np.zeros(9, np.int32), np.zeros(9, np.int32)
+    r = _rlat_f(rlat)
+    _check_host(lib().spexhip_determine_imat(_ptr(r), _ptr(imat), _ptr(imati)),
+                "determine_imat: bad lattice")
+    return imat.reshape(3, 3), imati.reshape(3, 3)
+
+
+def fft_grid(rlat, imat, gcutf):
+    """(nfft, points in the sphere) of fft_init / gcount_sphere (host)."""
+    r = _rlat_f(rlat)
+    im = np.ascontiguousarray(imat, np.int32).reshape(-1)
+    nfft = np.zeros(3, np.int32)
+    npts = _c_long(0)
+    _check_host(lib().spexhip_fft_grid(_ptr(r), _ptr(im), float(gcutf), _ptr(nfft),
+                                       C.byref(npts)), "fft_grid: bad arguments")
+    return tuple(int(v) for v in nfft), int(npts.value)
+
+
+def step_grid(nfft, imat, rlat, gmax, cstep):
+    """Complex theta(r) on the grid (x fastest) from cstep given as a [z][y][x]
+    array of extents (2c3+1, 2c2+1, 2c1+1) (wavefproducts.f:576-590, host)."""
+    cs = _c128(cstep)
+    assert cs.ndim == 3 and all(n % 2 == 1 for n in cs.shape)
+    cext = np.array([cs.shape[2] // 2, cs.shape[1] // 2, cs.shape[0] // 2], np.int32)
+    nf = np.asarray(nfft, np.int32)
+    out = np.zeros(int(np.prod(nf)), np.complex128)
+    r = _rlat_f(rlat)
+    im = np.ascontiguousarray(imat, np.int32).reshape(-1)
+    _check_host(lib().spexhip_step_grid(_ptr(nf), _ptr(im), _ptr(r), float(gmax), _ptr(cs),
+                                        _ptr(cext), _ptr(out)),
+                "step_grid: bad arguments or a kept frequency outside cstep")
+    return out
 
 
 def exported_symbols() -> list[str]:

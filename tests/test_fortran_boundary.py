@@ -16,7 +16,7 @@ def test_module_declares_every_shim():
                  "spexhip_dielec_inverse", "spexhip_minverse", "spexhip_screen",
                  "spexhip_rpa_int", "spexhip_pairsum_hilbert", "spexhip_headwings",
                  "spexhip_dielec_inverse_gamma", "spexhip_tetrahedron5", "spexhip_tetrahedron5_equiv",
-                 "spexhip_dielecc_inverse"]:
+                 "spexhip_dielecc_inverse", "spexhip_fft_setup", "spexhip_step_on_grid"]:
         assert f"subroutine {name}" in src
     # every bind(C) name exists in the C header
     import re
@@ -68,3 +68,14 @@ def test_fortran_driver_runs_on_gpu(tmp_path):
     want = tetra.average_equivalent(w0, no, nu, kp1 - 1, kpp - 1, ks2[:nk] - 1, ik - 1)
     assert np.abs(want - w0).max() > 0
     np.testing.assert_array_equal(w1, want)
+    # FFT grid setup through the Fortran shim (transposed row-major tables)
+    from oracle import fftgrid
+
+    raw = (tmp_path / "fftgrid.bin").read_bytes()
+    rlat = np.frombuffer(raw[:72], np.float64).reshape((3, 3), order="F")
+    ints = np.frombuffer(raw[72:], np.int32)
+    nfft, imat, imati = ints[:3], ints[3:12].reshape((3, 3), order="F"), ints[12:21].reshape((3, 3), order="F")
+    want, wanti = fftgrid.determine_imat(rlat)
+    np.testing.assert_array_equal(imat, want)
+    np.testing.assert_array_equal(imati, wanti)
+    assert tuple(nfft) == fftgrid.gcount_sphere(rlat, want, 8.3)[0] == (15, 15, 15)

@@ -51,8 +51,9 @@ def test_mt_full_lmax6_subset(ctx):
     assert got.shape[1] == 294 and mx < TOL
 
 
-def _pw_dev(ctx, s, b1lo, nocc, b2lo, nunocc, step=None, nfft=None):
-    plan = ctx.pw_plan(nfft or s.nfft, s.gpt, s.ngpt, s.gptm, s.vol, step_r=step)
+def _pw_dev(ctx, s, b1lo, nocc, b2lo, nunocc, step=None, nfft=None, imati=None, step_c=None):
+    plan = ctx.pw_plan(nfft or s.nfft, s.gpt, s.ngpt, s.gptm, s.vol, imati=imati, step_r=step,
+                       step_c=step_c)
     n = len(s.k1) * nocc * nunocc
     dC = ctx.to_dev(np.asfortranarray(s.cpw).ravel(order="F"))
     ldm = plan.ncols + 5
@@ -137,3 +138,122 @@ def test_pw_plan_rejects_box_overflow(ctx):
     with pytest.raises(spexhip.SpexError) as e:
         ctx.pw_plan(s.nfft, s.gpt, s.ngpt, s.gptm, s.vol)
     assert e.value.code == spexhip.E_INVALID
+
+
+def _theta_from_cstep(cs, sb, N):
+    """theta(r) on an N grid from its Fourier coefficients cstep(G), |G_i| <= sb
+    (FFTW BACKWARD, unnormalised: theta(r) = sum_G cstep(G) e^{+iGr})."""
+    a = np.zeros(N[::-1], complex)
+    for z in range(-sb, sb + 1):
+        for y in range(-sb, sb + 1):
+            for x in range(-sb, sb + 1):
+                a[z % N[2], y % N[1], x % N[0]] = cs[z + sb, y + sb, x + sb]
+    th = np.fft.ifftn(a) * np.prod(N)
+    assert np.abs(th.imag).max() < 1e-12
+    return np.ascontiguousarray(th.real).reshape(-1)
+
+
+def test_pw_realistic_with_step_function_36(ctx):
+    """The default FULLPW product (theta(r) on the caller's grid,
+    wavefproducts.f:630-632) at the realistic size: Ng ~ 1500 per state,
+    906 IPWs, 36^3 grid, against the oracle's explicit convolution with the
+    step function's Fourier coefficients (wavefproducts.f:870-888).  With
+    theta band-limited to |G_i| <= 3 the 36^3 FFT is alias-free for these
+    outputs, so the two reference algorithms must agree."""
+    s = synth.product_system(nmesh=4, nband=8, gcut=7.1, ngptm=906, nfft=(36, 36, 36), ntype=1,
+                             lcut=0, nrad=1, lcutm=0, nradm=1, seed=46)
+    g = np.random.default_rng(47)
+    sb = 3
+    w = 2 * sb + 1
+    cs = (g.standard_normal((w, w, w)) + 1j * g.standard_normal((w, w, w))) * 0.02
+    cs = 0.5 * (cs + np.conj(cs[::-1, ::-1, ::-1]))  # real theta(r)
+    cs[sb, sb, sb] = 0.75
+    th = _theta_from_cstep(cs, sb, s.nfft)
+    sub = [0, 37]
+    s2 = synth.ProductSystem(**{**s.__dict__, "k1": s.k1[sub], "k2": s.k2[sub], "g2": s.g2[sub]})
+    nocc, nunocc = 2, 3
+    ref = oracle.wfprod_pw(s2.k1, s2.k2, s2.g2, s.gpt, s.ngpt, s.cpw, s.gptm, s.vol, 0, nocc, 4,
+                           nunocc, cstep=cs.reshape(-1), sb=sb)
+    got = _pw_dev(ctx, s2, 0, nocc, 4, nunocc, step=th)
+    mx, fro = normwise(got, ref)
+    print(f"PW theta 36^3 realistic: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < TOL
+
+
+def _skewed(s, U):
+    """The same plane-wave system described in the reciprocal basis a*U
+    (cubic coordinates G = U G'): G vectors, IPWs and umklapps mapped by U^-1."""
+    Ui = np.rint(np.linalg.inv(U)).astype(np.int64)
+    tr = lambda v: np.ascontiguousarray(np.asarray(v, np.int64) @ Ui.T, np.int32)
+    gpt = np.stack([tr(s.gpt[k]) for k in range(s.nkpt)])
+    return synth.ProductSystem(**{**s.__dict__, "gpt": gpt, "gptm": tr(s.gptm), "g2": tr(s.g2)})
+
+
+def test_pw_skewed_lattice_with_imat(ctx):
+    """A cubic lattice described by a skewed reciprocal basis (wavefproducts.f:
+    625-627, 675-676 map G to imati.G mod nfft): imat from determine_imat and
+    the grid from fft_init's rule (FFT EXACT, gcutf = 2 gcut + gcutm) on the
+    host, products on the device against the oracle's explicit convolution
+    in the skewed coordinates, and against the same products in the cubic
+    coordinates (the result does not depend on the basis)."""
+    from oracle import fftgrid as fg
+
+    U = np.array([[1, 1, 0], [0, 1, -1], [1, 1, 1]])
+    s = synth.product_system(nmesh=2, nband=7, gcut=3.1, gcutm=2.1, seed=48)
+    sk = _skewed(s, U)
+    rlat = U.astype(float)  # a = 1: cubic reciprocal spacing 1
+    imat, imati = spexhip.determine_imat(rlat)
+    assert not (imat == np.eye(3)).all()
+    nfft, _ = spexhip.fft_grid(rlat, imat, 2 * 3.1 + 2.1)
+    assert (nfft, imat.tolist()) == (fg.gcount_sphere(rlat, imat, 8.3)[0],
+                                     fg.determine_imat(rlat)[0].tolist())
+    nocc, nunocc = 2, 4
+    plan = ctx.pw_plan(nfft, sk.gpt, sk.ngpt, sk.gptm, sk.vol, imati=imati)
+    n = len(sk.k1) * nocc * nunocc
+    dC = ctx.to_dev(np.asfortranarray(sk.cpw).ravel(order="F"))
+    dM = ctx.alloc(16 * n * sk.ngptm).zero()
+    ctx.wfprod_pw_dev(plan, dC, sk.nband, sk.k1, sk.k2, sk.g2, 0, nocc, 3, nunocc, dM, sk.ngptm, 0)
+    got = dM.download(np.complex128, (n, sk.ngptm))
+    plan.destroy()
+    dC.free()
+    dM.free()
+    ref = oracle.wfprod_pw(sk.k1, sk.k2, sk.g2, sk.gpt, sk.ngpt, sk.cpw, sk.gptm, sk.vol, 0, nocc,
+                           3, nunocc)
+    cub = oracle.wfprod_pw(s.k1, s.k2, s.g2, s.gpt, s.ngpt, s.cpw, s.gptm, s.vol, 0, nocc, 3,
+                           nunocc)
+    np.testing.assert_allclose(ref, cub, rtol=0, atol=1e-13 * np.abs(cub).max())
+    mx, fro = normwise(got, ref)
+    print(f"PW skewed basis nfft={nfft}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < TOL
+
+
+def test_pw_skewed_lattice_complex_step_grid(ctx):
+    """FULLPW on the skewed description: the step function of two spheres
+    (overlap.f:141-172) put on an even grid by step_grid (wavefproducts.f:
+    576-590; complex, the +N/2 frequencies are kept and -N/2 are not),
+    applied on the device, against the numpy restatement of the reference
+    FFT path with the same imati and theta."""
+    from oracle import fftgrid as fg
+    from oracle.pw_fft import pw_products_fft
+
+    U = np.array([[1, 1, 0], [0, 1, -1], [1, 1, 1]])
+    s = synth.product_system(nmesh=2, nband=6, gcut=3.1, gcutm=2.1, seed=49)
+    sk = _skewed(s, U)
+    rlat = U.astype(float) * 0.9
+    imat, imati = spexhip.determine_imat(rlat)
+    nfft = (16, 18, 16)  # smaller than FFT EXACT: the sphere reaches the +N/2 planes
+    gmax = (2 * 3.1 + 2.1) * 0.9
+    g = [np.where(np.arange(n) > n // 2, np.arange(n) - n, np.arange(n)) for n in nfft]
+    G1 = np.stack(np.meshgrid(*g, indexing="ij"), -1).reshape(-1, 3) @ imat.T
+    v = G1 @ rlat.T
+    ext = np.abs(G1[(v * v).sum(1) <= gmax**2]).max(0)
+    cs = fg.stepfunction_coeffs(rlat, sk.vol, [1.1, 0.8], [[0, 0, 0], [0.5, 0.3, 0.2]], ext)
+    th = spexhip.step_grid(nfft, imat, rlat, gmax, cs)
+    assert np.abs(th.imag).max() > 1e-6  # complex theta: the asymmetric Nyquist frequencies
+    nocc, nunocc = 2, 3
+    ss = synth.ProductSystem(**{**sk.__dict__, "nfft": nfft})
+    ref = pw_products_fft(ss, 0, nocc, 2, nunocc, step_r=th, imati=imati)
+    got = _pw_dev(ctx, ss, 0, nocc, 2, nunocc, step=None, nfft=nfft, imati=imati, step_c=th)
+    mx, fro = normwise(got, ref)
+    print(f"PW skewed basis, complex theta nfft={nfft}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < TOL
