@@ -153,10 +153,20 @@ class Mesh:
             damp[: len(s), k] = np.exp(-np.einsum("ij,ij->i", s, s) / 8.0)
         nb_, ma = lat.nband, lat.max_atoms
         seed = lat.seed * 100 + nmesh
-        self.dcmt = ctx.alloc(16 * lat.maxlm * ma * nb_ * nk)
-        ctx.synth_normal(self.dcmt, 2 * lat.maxlm * ma * nb_ * nk, seed, 1.0 / np.sqrt(lat.maxlm))
-        ddamp = ctx.to_dev(np.asfortranarray(damp).ravel(order="F"))
-        self.dcpw = ctx.alloc(16 * maxgpt * nb_ * nk)
+        held = []  # freed again if a later allocation fails (nothing half-built stays)
+        try:
+            self.dcmt = ctx.alloc(16 * lat.maxlm * ma * nb_ * nk)
+            held.append(self.dcmt)
+            ctx.synth_normal(self.dcmt, 2 * lat.maxlm * ma * nb_ * nk, seed,
+                             1.0 / np.sqrt(lat.maxlm))
+            ddamp = ctx.to_dev(np.asfortranarray(damp).ravel(order="F"))
+            held.append(ddamp)
+            self.dcpw = ctx.alloc(16 * maxgpt * nb_ * nk)
+        except spexhip.SpexError:
+            ctx.sync()
+            for b in held:
+                b.free()
+            raise
         ctx.synth_normal_rows(self.dcpw, maxgpt, nb_, nk, seed + 1, ddamp)
         ctx.sync()
         ddamp.free()
@@ -267,6 +277,7 @@ class Driver:
         self.records: list[dict] = []
         self.log = log
         self._pending: list[tuple] = []  # (index, request, server id, submit time)
+        self._finishing = 0  # taken off _pending, not yet recorded
         self._cv = threading.Condition()
         self._closed = False
         self._thr = threading.Thread(target=self._collect, daemon=True)
@@ -294,8 +305,11 @@ class Driver:
             with self._cv:
                 for p in done:
                     self._pending.remove(p)
+                self._finishing += len(done)
             for idx, req, rid, t_sub in done:
                 self._finish(idx, req, rid, t_sub)
+                with self._cv:
+                    self._finishing -= 1
 
     def _finish(self, idx, req, rid, t_sub):
         err = None
@@ -316,6 +330,11 @@ class Driver:
             self.records.append(rec)
         if self.log:
             self.log(rec)
+
+    def outstanding(self) -> int:
+        """Requests submitted and not yet finished."""
+        with self._cv:
+            return len(self._pending) + self._finishing
 
     def drain(self) -> None:
         with self._cv:
@@ -378,11 +397,35 @@ def run_closed(lat: Lattice, specs: list[Spec], precision: int, max_inflight: in
     return summarize(drv.records, t0, stats), drv.results
 
 
+def run_sustained(lat: Lattice, specs: list[Spec], precision: int, max_inflight: int,
+                  hbm_budget: int, log=None):
+    """Capacity as the stream sees it: every request is staged by the
+    submitting thread (as in the open loop) and submitted as soon as fewer
+    than max_inflight + 1 are outstanding, so staging, admission and the
+    device work overlap the way they do under load.  Completed requests
+    per second from the first submission to the last completion."""
+    drv = Driver(lat, precision, max_inflight, hbm_budget, log=log)
+    t0 = time.perf_counter()
+    try:
+        for i, sp in enumerate(specs):
+            req = _stage(lat, sp, precision)
+            while drv.outstanding() > max_inflight:
+                time.sleep(0.001)
+            drv.submit(i, req)
+    finally:
+        stats = drv.close()
+    return summarize(drv.records, t0, stats)
+
+
 def run_poisson(lat: Lattice, specs: list[Spec], precision: int, rate: float, seconds: float,
-                max_inflight: int, hbm_budget: int, seed: int = 5, log=None):
+                max_inflight: int, hbm_budget: int, seed: int = 5, log=None,
+                patience_s: float = 120.0):
     """Open loop: Poisson arrivals at ``rate`` requests/s for ``seconds``
     (the spec list is cycled); each request's inputs are staged before its
-    arrival time, and latency runs from submission to completion."""
+    arrival time, and latency runs from submission to completion.  If a
+    request cannot be staged within ``patience_s`` (the device stays full),
+    the requests already submitted are drained and the error is raised
+    with their summary attached as ``partial``."""
     g = synth.rng(seed)
     arrivals = []
     t = 0.0
@@ -393,25 +436,33 @@ def run_poisson(lat: Lattice, specs: list[Spec], precision: int, rate: float, se
         arrivals.append(t)
     drv = Driver(lat, precision, max_inflight, hbm_budget, log=log)
     t0 = time.perf_counter()
+    err = None
     try:
         for i, ta in enumerate(arrivals):
-            req = _stage(lat, specs[i % len(specs)], precision)
+            req = _stage(lat, specs[i % len(specs)], precision, patience_s)
             dt = t0 + ta - time.perf_counter()
             if dt > 0:
                 time.sleep(dt)
             drv.submit(i, req)
+    except spexhip.SpexError as e:
+        err = e
     finally:
         stats = drv.close()  # drains the server even when staging fails
     out = summarize(drv.records, t0, stats)
     out.update(offered_rate=round(rate, 4), window_s=seconds, submitted=len(arrivals))
+    if err is not None:
+        err.partial = out
+        raise err
     return out
 
 
-def serve_benchmark(ctx: spexhip.Context, precisions: dict, seconds: float, cap_requests: int,
+def serve_benchmark(ctx: spexhip.Context, precisions: dict, seconds, cap_requests: int,
                     max_inflight: int = 4, hbm_budget: int = 200 << 30, load: float = 0.8,
                     seed: int = 2027, log=print) -> dict:
-    """Capacity (closed loop) then an open-loop Poisson run at ``load`` x
-    capacity, for each named precision."""
+    """Sustained capacity (``cap_requests`` requests staged by the submitting
+    thread, at most max_inflight + 1 outstanding) then an open-loop Poisson
+    run at ``load`` x capacity, for each named precision.  ``seconds`` is the
+    window of every precision, or a dict of windows by name."""
     lat = Lattice(ctx)
     specs = request_mix(4096, seed)
     for m in sorted({s.nmesh for s in specs}):
@@ -423,15 +474,19 @@ def serve_benchmark(ctx: spexhip.Context, precisions: dict, seconds: float, cap_
                    "hbm_budget_gib": hbm_budget / 2**30, "load": load}}
     try:
         for name, prec in precisions.items():
-            cap, _ = run_closed(lat, specs[:cap_requests], prec, max_inflight, hbm_budget,
+            cap = run_sustained(lat, specs[:cap_requests], prec, max_inflight, hbm_budget,
                                 log=log)
             if log:
-                log(f"[serve] {name} capacity {cap['qpoints_per_s']} q/s over "
+                log(f"[serve] {name} sustained capacity {cap['qpoints_per_s']} q/s over "
                     f"{cap['completed']} requests")
             rate = load * cap["qpoints_per_s"]
-            res = run_poisson(lat, specs[cap_requests:], prec, rate, seconds, max_inflight,
+            win = seconds[name] if isinstance(seconds, dict) else seconds
+            res = run_poisson(lat, specs[cap_requests:], prec, rate, win, max_inflight,
                               hbm_budget, seed=seed + 1, log=log)
             res["capacity_qpoints_per_s"] = cap["qpoints_per_s"]
+            res["capacity_requests"] = cap["completed"]
+            # the queue kept up when the window's throughput is near the offered rate
+            res["completed_over_offered"] = round(res["qpoints_per_s"] / rate, 3)
             if log:
                 log(f"[serve] {name} " + json.dumps(res))
             out[name] = res

@@ -109,3 +109,49 @@ def test_context_close_drains_a_live_server():
     ctx2.close()
     assert srv.h is None
     srv.destroy()  # a no-op once the context has taken the server down
+
+
+def test_staging_oom_drains_then_raises_and_context_closes():
+    """A context whose HBM budget holds the lattice and two small pipelines
+    but never the wave functions of an 8x8x8 k mesh: the open-loop driver
+    stages and submits two small requests, then cannot stage the third (its
+    mesh does not fit) within its patience.  It must drain the two submitted
+    requests before raising SPEXHIP_E_NOMEM, leave nothing of theirs in use,
+    and the context must then close cleanly."""
+    kw = dict(nocc=4, nunocc=12, lcut=2, nrad=2, gcut=3.1, nfft=(16, 16, 16), max_atoms=3,
+              nb_max=100, ipw_target=40, seed=6)
+    small = [serving.Spec(nmesh=2, nb=80, nw=3, seed=200 + i) for i in range(2)]
+    big = serving.Spec(nmesh=8, nb=80, nw=3, seed=210)
+    with spexhip.Context(0) as probe:  # sizes, on an unlimited context
+        lat = serving.Lattice(probe, **kw)
+        lat.mesh(2)
+        lat.mt_plan(lat.natoms_for(80))
+        base = probe.mem_used()[0]
+        req = serving.Request(lat, small[0], spexhip.PREC_FP64)
+        per_req = probe.mem_used()[0] - base + req.footprint
+        req.free()
+        before = probe.mem_used()[0]
+        lat.mesh(8)
+        mesh8 = probe.mem_used()[0] - before
+        lat.free()
+    assert mesh8 > 2 * per_req + (1 << 20), (mesh8, per_req)
+    budget = base + 2 * per_req + (mesh8 - 2 * per_req) // 2
+    ctx2 = spexhip.Context(0, hbm_budget=budget)
+    lat = serving.Lattice(ctx2, **kw)
+    lat.mesh(2)
+    lat.mt_plan(lat.natoms_for(80))
+    used0 = ctx2.mem_used()[0]
+    with pytest.raises(spexhip.SpexError) as e:
+        serving.run_poisson(lat, small + [big], spexhip.PREC_FP64, rate=1e4, seconds=1.0,
+                            max_inflight=2, hbm_budget=budget, patience_s=1.0)
+    assert e.value.code == spexhip.E_NOMEM
+    part = e.value.partial
+    print("partial", part)
+    assert part["completed"] == 2 and part["errors"] == 0
+    # inputs and pipelines of the drained requests are released (freed blocks
+    # go to the cache, which is not counted as used); the failed mesh left
+    # nothing behind
+    assert 8 not in lat.meshes
+    assert ctx2.mem_used()[0] <= used0 + per_req
+    lat.free()
+    ctx2.close()
