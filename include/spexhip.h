@@ -178,6 +178,17 @@ int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nun
                                      const void *w_full_dev, long ldw, const int *kpt1p,
                                      int nkpt1, void *w_red_dev, long ldr);
 
+/* The same rows under the TETRAF keyword (getinput.f:1605): a cube
+ * corner contributes only when it is itself listed in kpt1 (tetrahedron.f:
+ * 1092-1096), and the row of parent p is then multiplied by the number of
+ * mesh points whose parent is p (:1212-1216).  So the row of p is
+ * count(kpt1p == p) times the sum of the full-mesh rows of the points k
+ * listed in kpt1 with kpt1p(k) = p -- normally the one row of kpt1(p).
+ * kpt1 (host, nkpt1 full-mesh indices, 0-based); the rest as above.       */
+int spexhip_tetra_reduce_tetraf_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, int nfrqs,
+                                    const void *w_full_dev, long ldw, const int *kpt1, int nkpt1,
+                                    const int *kpt1p, void *w_red_dev, long ldr);
+
 /* The last step of tetrahedron5_init (tetrahedron.f:1282-1318): where kpt1
  * holds k points that are equivalent under the symmetries keeping q
  * (kptsym(ikpt, isym) == ikpt), the weights of each group are replaced by
@@ -211,6 +222,38 @@ int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double
                                 double efermi, int nfrqs, const double *frqs, const int *kpt1p,
                                 int nkpt1, const int *deg_occ, const int *deg_unocc,
                                 double *wtetra);
+
+/* Host drop-in for tetrahedron5_init with band windows that may cut a
+ * degenerate group (tetrahedron.f:984-1012, 1080-1093, 1191-1194,
+ * 1237-1273), kflag 1.  ene1 [nk][nb1] at k and ene2 [nk][nb2] at k+q hold
+ * all bands of the caller (sorted per k), deg1/deg2 [nk][nb] SPEX's deg
+ * tables of the same bands, 0-based (at the first band of a group its last
+ * band, at the others the first band); windows b1low..b1up, b2low..b2up
+ * (0-based, inclusive).  Per k the windows are widened to the whole groups
+ * of their edge bands, the weights of the bands outside go to the nearest
+ * window row, and the rows are averaged over each group with the whole
+ * group's size: every window row gets the mean over its whole degenerate
+ * group.  kpt1p [nk] (or NULL: the full mesh) with kpt1 [nkpt1] (the groups
+ * of row p are those at kpt1[p]; the degeneracies must be the same on every
+ * member of a star); tetraf nonzero gives the TETRAF rows.  wtetra (host,
+ * nkr*nocc*nunocc x nfrqs, t = (r*nocc + b1)*nunocc + b2 fastest, nocc =
+ * b1up - b1low + 1, nkr = nkpt1 or nk) is overwritten.                    */
+int spexhip_tetrahedron_weights_bands(spexhip_ctx *ctx, const int *nkpt3, const double *rlat,
+                                      int nb1, const double *ene1, const int *deg1, int b1low,
+                                      int b1up, int nb2, const double *ene2, const int *deg2,
+                                      int b2low, int b2up, double efermi, int nfrqs,
+                                      const double *frqs, const int *kpt1, int nkpt1,
+                                      const int *kpt1p, int tetraf, double *wtetra);
+
+/* The host drop-in with TETRAF set: kpt1 [nkpt1] and kpt1p [nk] are both
+ * required, the rows are those of spexhip_tetra_reduce_tetraf_dev; the
+ * other arguments as spexhip_tetrahedron_weights.                         */
+int spexhip_tetrahedron_weights_tetraf(spexhip_ctx *ctx, const int *nkpt3, const double *rlat,
+                                       int nocc, int nunocc, const double *e_occ,
+                                       const double *e_unocc, double efermi, int nfrqs,
+                                       const double *frqs, const int *kpt1, int nkpt1,
+                                       const int *kpt1p, const int *deg_occ,
+                                       const int *deg_unocc, double *wtetra);
 
 /* ---- chi0 pair sum ----------------------------------------------------- */
 /* Device form of the pair loop susceptibility.f:1159-1251 with
@@ -396,6 +439,34 @@ int spexhip_momentum_matrix(spexhip_ctx *ctx, const spexhip_plan *plan, const do
                             int ncent, const double *cpw, int nband, int nk, const int *k1,
                             int b1lo, int nocc, int b2lo, int nunocc, double *momentum,
                             int accumulate);
+
+/* ---- Gamma point (q = 0): Drude term of metals ------------------------- */
+/* Fermi-surface weights wintgr4 of tetrahedron4_init (tetrahedron.f:708-838)
+ * on a full cubic mesh (k = (i1*N2 + i2)*N3 + i3, no symmetry reduction):
+ * ene_dev and w_dev are [nk][nband] doubles; degenerate states (within
+ * 1e-8) share their mean weight.  In the lowest cross-section regime
+ * (efermi between the two lowest corner energies) the reference's triangle
+ * area is 3 f1 f2/(e3 - e1) (:779), kept as it is.                        */
+int spexhip_fermi_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nband,
+                              const void *ene_dev, double efermi, void *w_dev);
+/* Plasma frequency (susceptibility.f:1404-1441, 1693-1700): plasma^2 =
+ * 4 pi/(3 vol) * spin_factor * sum_{k,n} w(k,n) sum_j (Re of the mean of
+ * P(i,i,k,j) over the degenerate group of n)^2, P_dev the momentum matrix
+ * of bands 0..nband-1 with themselves (spexhip_momentum_dev with b1lo =
+ * b2lo = 0, nocc = nunocc = nband; n x 3, ld ldp), deg (host [nk][nband])
+ * SPEX's degeneracy table, 0-based: at the first band of a group its last
+ * band (the other entries are not read); spin_factor 2
+ * for one spin of a spin-unpolarised system.  Below 1e-6 eV the result is
+ * 0; SPEXHIP_E_INVALID if plasma^2 < -1e-10.                              */
+int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void *w_dev,
+                                 const void *P_dev, long ldp, const int *deg, double vol,
+                                 double spin_factor, double *plasma);
+/* correlation.f:847-862: head(j,j,w) -= drude(freq(w)) = -plasma^2/freq^2
+ * (infinity at freq 0) on the 3 x 3 x nw head_dev, and headc(j,j,w) -=
+ * plasma^2/freqc^2 on the 3 x 3 x nwc headc_dev (freqc host complex).
+ * Nothing happens for plasma <= 0.                                         */
+int spexhip_gamma_drude_dev(spexhip_ctx *ctx, int nw, const double *freq, double plasma,
+                            void *head_dev, int nwc, const double *freqc, void *headc_dev);
 
 /* ---- Gamma point (q = 0): head and wings -------------------------------- */
 /* Replaces the momentum-matrix scaling of susceptibility.f:858-873

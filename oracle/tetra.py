@@ -22,7 +22,10 @@ Restates, in plain numpy/Python:
 - the weights of a symmetry-reduced k set (kpt1p): the sum over the
   full-mesh points that map to each parent;
 - the final averaging over equivalent k points of kpt1 under the symmetries
-  that keep q (:1282-1318).
+  that keep q (:1282-1318);
+- the TETRAF rows of a reduced set (:1092-1096, 1212-1216);
+- band windows that cut a degenerate group (:984-1012, 1191-1194,
+  1237-1273): widened windows, clamped rows and the whole-group average.
 
 k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
 (i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
@@ -291,6 +294,23 @@ def reduce_to_parents(w, nk, nocc, nunocc, kpt1p, nkpt1):
     return out.reshape(nkpt1 * nocc * nunocc, -1)
 
 
+def reduce_tetraf(w, nk, nocc, nunocc, kpt1, kpt1p, nkpt1):
+    """The rows of a reduced k set under TETRAF (getinput.f:1605): a cube
+    corner k contributes only if k is itself listed in kpt1 (tetrahedron.f:
+    1092-1096), to its parent kpt1p(k), and the row of parent p is then
+    multiplied by count(kpt1p == p) (:1212-1216).  w (n_full, nfrqs);
+    kpt1 (nkpt1,) full-mesh indices; kpt1p (nk,) 0-based parent or -1."""
+    W = np.asarray(w, float).reshape(nk, nocc, nunocc, -1)
+    listed = set(int(k) for k in kpt1)
+    out = np.zeros((nkpt1, nocc, nunocc, W.shape[-1]))
+    for k in range(nk):
+        if kpt1p[k] >= 0 and k in listed:
+            out[kpt1p[k]] += W[k]
+    for p in range(nkpt1):
+        out[p] *= np.count_nonzero(np.asarray(kpt1p) == p)
+    return out.reshape(nkpt1 * nocc * nunocc, -1)
+
+
 def _group(deg_row, b):
     """(first, last) of band b's degenerate group from SPEX's deg convention:
     deg(b) is the last band of the group when b is its first band, else the
@@ -321,6 +341,71 @@ def average_degenerate(w, nk, nocc, nunocc, deg_occ, deg_unocc):
             W[k, :, first:last + 1] = (W[k, :, first:last + 1].sum(1) / (last - first + 1))[:, None]
             n = last + 1
     return W.reshape(nk * nocc * nunocc, -1)
+
+
+def tetrahedron5_weights_bands(nkpt3, rlat, ene1, ene2, deg1, deg2, b1low, b1up, b2low, b2up,
+                               efermi, frq, kpt1=None, kpt1p=None, nkpt1=None, tetraf=False):
+    """tetrahedron5_init with band windows that may cut a degenerate group
+    (tetrahedron.f:984-1012, 1080-1093, 1191-1194, 1237-1273), kflag 1.
+    ene1 (nk, nb1) at k and ene2 (nk, nb2) at k+q hold all bands, deg1 and
+    deg2 the SPEX deg tables of the same bands (0-based, SPEX's convention)
+    at k and k+q; windows b1low..b1up, b2low..b2up 0-based inclusive.
+
+    Per mesh point k the windows are widened to the whole groups of their
+    edge bands (d1low(k) .. d1up(k), d2low(k) .. d2up(k)); a transition (n1,
+    n2) of the widened windows adds its weight to the row of the window band
+    nearest to it (n clamped into the window) of its corner k; the rows of a
+    reduced set are formed from those (reduce_to_parents, or reduce_tetraf
+    under TETRAF); then the reference's averaging over the groups at kpt1(p)
+    divides each group's in-window sum by the whole group's size.  Returns
+    (nkr*nocc*nunocc, nfrqs), nkr = nkpt1 or nk."""
+    ene1, ene2 = np.asarray(ene1, float), np.asarray(ene2, float)
+    nk = ene1.shape[0]
+    nocc, nunocc = b1up - b1low + 1, b2up - b2low + 1
+    lo1 = np.array([_group(deg1[k], b1low)[0] for k in range(nk)])
+    up1 = np.array([_group(deg1[k], b1up)[1] for k in range(nk)])
+    lo2 = np.array([_group(deg2[k], b2low)[0] for k in range(nk)])
+    up2 = np.array([_group(deg2[k], b2up)[1] for k in range(nk)])
+    a1, z1, a2, z2 = lo1.min(), up1.max(), lo2.min(), up2.max()
+    nx1, nx2 = z1 - a1 + 1, z2 - a2 + 1
+    Wx = tetrahedron5_weights_metal(nkpt3, rlat, ene1[:, a1:z1 + 1], ene2[:, a2:z2 + 1], efermi,
+                                    frq).reshape(nk, nx1, nx2, -1)
+    acc = np.zeros((nk, nocc, nunocc, Wx.shape[-1]))
+    for k in range(nk):
+        for n1 in range(lo1[k], up1[k] + 1):
+            r1 = min(b1up, max(b1low, n1)) - b1low
+            for n2 in range(lo2[k], up2[k] + 1):
+                r2 = min(b2up, max(b2low, n2)) - b2low
+                acc[k, r1, r2] += Wx[k, n1 - a1, n2 - a2]
+    acc = acc.reshape(nk * nocc * nunocc, -1)
+    if kpt1p is None:
+        rows, kd = acc, np.arange(nk)
+    else:
+        kd = np.asarray(kpt1)
+        if tetraf:
+            rows = reduce_tetraf(acc, nk, nocc, nunocc, kpt1, kpt1p, nkpt1)
+        else:
+            rows = reduce_to_parents(acc, nk, nocc, nunocc, kpt1p, nkpt1)
+    W = rows.reshape(len(kd), nocc, nunocc, -1).copy()
+    for r, k in enumerate(kd):  # occupied groups (:1237-1255)
+        n = _group(deg1[k], b1low)[0]
+        while n <= b1up:
+            nn = _group(deg1[k], n)[1]
+            lo, hi = max(b1low, n), min(b1up, nn)
+            if lo <= hi:
+                W[r, lo - b1low:hi - b1low + 1] = (W[r, lo - b1low:hi - b1low + 1].sum(0)
+                                                   / (nn - n + 1))
+            n = nn + 1
+    for r, k in enumerate(kd):  # unoccupied groups (:1257-1273)
+        n = _group(deg2[k], b2low)[0]
+        while n <= b2up:
+            nn = _group(deg2[k], n)[1]
+            lo, hi = max(b2low, n), min(b2up, nn)
+            if lo <= hi:
+                W[r, :, lo - b2low:hi - b2low + 1] = (W[r, :, lo - b2low:hi - b2low + 1].sum(1)
+                                                      / (nn - n + 1))[:, None]
+            n = nn + 1
+    return W.reshape(len(kd) * nocc * nunocc, -1)
 
 
 def equivalent_groups(kpt1, kpt1p, kptsym, ikpt):

@@ -22,6 +22,7 @@
 // The averaging over degenerate states is a separate pass (two
 // out-of-place kernels: occupied groups, then unoccupied groups).
 #include "common.hpp"
+#include "tetra_mesh.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
@@ -32,24 +33,6 @@ namespace {
 constexpr int TW_NT = 64;  // one wave: its 27 x 64 LDS values (13.5 KB) leave room for 11 waves per CU
 constexpr double TW_ETA = 1e-14;  // cuttetraN's tolerance
 
-// the four decompositions of get_tetra (tetrahedron.f:253-268), corners
-// 0..7, one row of four corners per tetrahedron; DEC is the index of the
-// shortest diagonal (3-6, 2-7, 1-8, 4-5)
-__device__ constexpr int kDecomp[4][6][4] = {
-    {{0, 1, 2, 5}, {4, 6, 2, 5}, {0, 4, 2, 5}, {1, 3, 2, 5}, {3, 7, 2, 5}, {6, 7, 2, 5}},
-    {{4, 5, 1, 6}, {0, 4, 1, 6}, {0, 2, 1, 6}, {7, 5, 1, 6}, {3, 2, 1, 6}, {7, 3, 1, 6}},
-    {{1, 5, 0, 7}, {1, 3, 0, 7}, {2, 3, 0, 7}, {2, 6, 0, 7}, {4, 6, 0, 7}, {4, 5, 0, 7}},
-    {{1, 5, 3, 4}, {0, 1, 3, 4}, {0, 2, 3, 4}, {2, 6, 3, 4}, {6, 7, 3, 4}, {5, 7, 3, 4}}};
-
-// position of corner p in tetrahedron it of decomposition DEC, or -1
-template <int DEC>
-__device__ constexpr int corner_pos(int it, int p) {
-  return kDecomp[DEC][it][0] == p   ? 0
-         : kDecomp[DEC][it][1] == p ? 1
-         : kDecomp[DEC][it][2] == p ? 2
-         : kDecomp[DEC][it][3] == p ? 3
-                                    : -1;
-}
 
 // Weight of corner position `q` (0..3) of one tetrahedron with corner values
 // e_in and threshold ef: cuttetraN's wtetra(q) (tetrahedron.f:347-412).
@@ -447,13 +430,13 @@ __global__ void deg_average_kernel(int nk, int nocc, int nunocc, const int *__re
   out[t + ldo * i] = s / (last - first + 1);
 }
 
-// Row of parent p = sum over its star (full-mesh points kp with
-// kpt1p(kp) = p, ascending) of the full-mesh rows; one thread per reduced
-// transition (p, b1, b2) and bin.
+// Row of parent p = sum over its members (full-mesh points, ascending) of
+// the full-mesh rows, times scale[p] when a scale table is given; one
+// thread per reduced transition (p, b1, b2) and bin.
 __global__ void star_sum_kernel(int nkpt1, int nocc, int nunocc, const int *__restrict__ off,
-                                const int *__restrict__ mem, int nfrq,
-                                const double *__restrict__ in, long ldi, double *__restrict__ out,
-                                long ldo) {
+                                const int *__restrict__ mem, const double *__restrict__ scale,
+                                int nfrq, const double *__restrict__ in, long ldi,
+                                double *__restrict__ out, long ldo) {
   const long per = (long)nocc * nunocc;
   const long nr = (long)nkpt1 * per;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -463,7 +446,34 @@ __global__ void star_sum_kernel(int nkpt1, int nocc, int nunocc, const int *__re
   const long bb = t - (long)p * per;
   double s = 0.0;
   for (int m = off[p]; m < off[p + 1]; m++) s += in[(long)mem[m] * per + bb + ldi * i];
-  out[t + ldo * i] = s;
+  out[t + ldo * i] = scale ? s * scale[p] : s;
+}
+
+// Weights of window rows whose degenerate groups may reach beyond the
+// window: the mean over the whole groups of the rows of the widened
+// windows.  g1/g2 hold (first, last) per (row, window band), relative to
+// the widened windows (nx1 x nx2 band pairs per row); one thread per
+// window transition (r, b1, b2) and bin.
+__global__ void group_mean_kernel(int nrow, int nocc, int nunocc, int nx1, int nx2,
+                                  const int2 *__restrict__ g1, const int2 *__restrict__ g2,
+                                  int nfrq, const double *__restrict__ in, long ldi,
+                                  double *__restrict__ out, long ldo) {
+  const long per = (long)nocc * nunocc;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (t >= (long)nrow * per || i >= nfrq) return;
+  const int r = (int)(t / per);
+  const int b1 = (int)((t - (long)r * per) / nunocc);
+  const int b2 = (int)(t - (long)r * per - (long)b1 * nunocc);
+  const int2 G1 = g1[(long)r * nocc + b1], G2 = g2[(long)r * nunocc + b2];
+  const double *col = in + ldi * i + (long)r * nx1 * nx2;
+  double s = 0.0;
+  for (int m1 = G1.x; m1 <= G1.y; m1++) {
+    double s2 = 0.0;
+    for (int m2 = G2.x; m2 <= G2.y; m2++) s2 += col[(long)m1 * nx2 + m2];
+    s += s2;
+  }
+  out[t + ldo * i] = s / ((double)(G1.y - G1.x + 1) * (double)(G2.y - G2.x + 1));
 }
 
 // Mean over a group of equivalent k points: one thread per (group, band
@@ -490,6 +500,63 @@ __global__ void equiv_average_kernel(long per, int ngroup, const int *__restrict
   for (int m = off[g]; m < off[g + 1]; m++) col[(long)mem[m] * per + bb] = avg;
 }
 
+// Rows of the reduced set from the full-mesh rows.  Without kpt1: the
+// star sum over every point with a parent.  With kpt1 (TETRAF): only the
+// points listed in kpt1 contribute, and the row of parent p is scaled by
+// the number of mesh points whose parent is p.
+int reduce_rows(spexhip_ctx *ctx, const char *name, int nk, int nocc, int nunocc, int nfrqs,
+                const void *w_full, long ldw, const int *kpt1, const int *kpt1p, int nkpt1,
+                void *w_red, long ldr) {
+  SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0 && nkpt1 > 0 && kpt1p,
+            "%s: bad arguments", name);
+  const long n = (long)nk * nocc * nunocc, nr = (long)nkpt1 * nocc * nunocc;
+  SPX_CHECK(ctx, ldw >= n && ldr >= nr, "%s: leading dimensions too small", name);
+  std::vector<char> listed(nk, kpt1 ? 0 : 1);
+  if (kpt1)
+    for (int p = 0; p < nkpt1; p++) {
+      SPX_CHECK(ctx, kpt1[p] >= 0 && kpt1[p] < nk, "%s: kpt1 out of range", name);
+      listed[kpt1[p]] = 1;
+    }
+  // members in CSR form, ascending (the order of the reference's scatter)
+  std::vector<int> off(nkpt1 + 1, 0), mem;
+  std::vector<double> cnt(kpt1 ? nkpt1 : 0, 0.0);
+  for (int k = 0; k < nk; k++) {
+    SPX_CHECK(ctx, kpt1p[k] >= -1 && kpt1p[k] < nkpt1, "%s: kpt1p out of range", name);
+    if (kpt1p[k] < 0) continue;
+    if (listed[k]) off[kpt1p[k] + 1]++;
+    if (kpt1) cnt[kpt1p[k]] += 1.0;
+  }
+  for (int p = 0; p < nkpt1; p++) off[p + 1] += off[p];
+  mem.resize(off[nkpt1]);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int k = 0; k < nk; k++)
+    if (kpt1p[k] >= 0 && listed[k]) mem[fill[kpt1p[k]]++] = k;
+  SPX_CHECK(ctx, spx_owns(ctx, w_full, 8 * ((size_t)ldw * (nfrqs - 1) + n)) &&
+                     spx_owns(ctx, w_red, 8 * ((size_t)ldr * (nfrqs - 1) + nr)),
+            "%s: arrays are not context allocations of the stated shapes", name);
+  SPX_CHECK(ctx, w_full != w_red, "%s: output must not alias the input", name);
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *ws;
+  const size_t ints = off.size() + mem.size();
+  int rc = spx_scratch(ctx, 8 * cnt.size() + 4 * ints + 16, &ws);
+  if (rc) return rc;
+  double *dcnt = (double *)ws;
+  int *doff = (int *)(dcnt + cnt.size()), *dmem = doff + off.size();
+  if (!cnt.empty())
+    SPX_HIP(ctx, hipMemcpyAsync(dcnt, cnt.data(), 8 * cnt.size(), hipMemcpyHostToDevice, ctx->stream));
+  SPX_HIP(ctx, hipMemcpyAsync(doff, off.data(), 4 * off.size(), hipMemcpyHostToDevice, ctx->stream));
+  if (!mem.empty())
+    SPX_HIP(ctx, hipMemcpyAsync(dmem, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(star_sum_kernel, dim3((unsigned)((nr + 255) / 256), (unsigned)nfrqs),
+                     dim3(256), 0, ctx->stream, nkpt1, nocc, nunocc, doff, dmem,
+                     kpt1 ? (const double *)dcnt : nullptr, nfrqs, (const double *)w_full, ldw,
+                     (double *)w_red, ldr);
+  SPX_LAUNCHED(ctx);
+  // the host tables are read by the copies above
+  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SPEXHIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -498,40 +565,17 @@ int spexhip_tetra_reduce_parents_dev(spexhip_ctx *ctx, int nk, int nocc, int nun
                                      const void *w_full, long ldw, const int *kpt1p, int nkpt1,
                                      void *w_red, long ldr) {
   if (!ctx) return SPEXHIP_E_INVALID;
-  SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0 && nkpt1 > 0 && kpt1p,
-            "tetra_reduce_parents: bad arguments");
-  const long n = (long)nk * nocc * nunocc, nr = (long)nkpt1 * nocc * nunocc;
-  SPX_CHECK(ctx, ldw >= n && ldr >= nr, "tetra_reduce_parents: leading dimensions too small");
-  // stars in CSR form, members ascending (the order of the reference's scatter)
-  std::vector<int> off(nkpt1 + 1, 0), mem;
-  for (int k = 0; k < nk; k++) {
-    SPX_CHECK(ctx, kpt1p[k] >= -1 && kpt1p[k] < nkpt1, "tetra_reduce_parents: kpt1p out of range");
-    if (kpt1p[k] >= 0) off[kpt1p[k] + 1]++;
-  }
-  for (int p = 0; p < nkpt1; p++) off[p + 1] += off[p];
-  mem.resize(off[nkpt1]);
-  std::vector<int> fill(off.begin(), off.end() - 1);
-  for (int k = 0; k < nk; k++)
-    if (kpt1p[k] >= 0) mem[fill[kpt1p[k]]++] = k;
-  SPX_CHECK(ctx, spx_owns(ctx, w_full, 8 * ((size_t)ldw * (nfrqs - 1) + n)) &&
-                     spx_owns(ctx, w_red, 8 * ((size_t)ldr * (nfrqs - 1) + nr)),
-            "tetra_reduce_parents: arrays are not context allocations of the stated shapes");
-  SPX_CHECK(ctx, w_full != w_red, "tetra_reduce_parents: output must not alias the input");
-  SPX_HIP(ctx, hipSetDevice(ctx->device));
-  void *ws;
-  int rc = spx_scratch(ctx, 4 * (off.size() + mem.size()) + 16, &ws);
-  if (rc) return rc;
-  int *doff = (int *)ws, *dmem = doff + off.size();
-  SPX_HIP(ctx, hipMemcpyAsync(doff, off.data(), 4 * off.size(), hipMemcpyHostToDevice, ctx->stream));
-  if (!mem.empty())
-    SPX_HIP(ctx, hipMemcpyAsync(dmem, mem.data(), 4 * mem.size(), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(star_sum_kernel, dim3((unsigned)((nr + 255) / 256), (unsigned)nfrqs),
-                     dim3(256), 0, ctx->stream, nkpt1, nocc, nunocc, doff, dmem, nfrqs,
-                     (const double *)w_full, ldw, (double *)w_red, ldr);
-  SPX_LAUNCHED(ctx);
-  // the host star tables are read by the copies above
-  SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return SPEXHIP_OK;
+  return reduce_rows(ctx, "tetra_reduce_parents", nk, nocc, nunocc, nfrqs, w_full, ldw, nullptr,
+                     kpt1p, nkpt1, w_red, ldr);
+}
+
+int spexhip_tetra_reduce_tetraf_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, int nfrqs,
+                                    const void *w_full, long ldw, const int *kpt1, int nkpt1,
+                                    const int *kpt1p, void *w_red, long ldr) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, kpt1 != nullptr, "tetra_reduce_tetraf: kpt1 is NULL");
+  return reduce_rows(ctx, "tetra_reduce_tetraf", nk, nocc, nunocc, nfrqs, w_full, ldw, kpt1,
+                     kpt1p, nkpt1, w_red, ldr);
 }
 
 int spexhip_tetra_average_equivalent_dev(spexhip_ctx *ctx, int nkpt1, int nocc, int nunocc,
@@ -678,24 +722,7 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
   f1[nfrqs - 1] = frqs[nfrqs - 1] + (1 - a) * (frqs[nfrqs - 1] - frqs[nfrqs - 2]);
   const double norm = 4.0 * 6 * (double)nk;
   for (int i = 0; i < nfrqs; i++) den[i] = (2 * (f1[i] - frqs[i])) * norm;
-  // get_tetra (mode 1): the decomposition around the shortest diagonal of
-  // the mesh cell spanned by rlat(:,i)/nkpt3(i), first on ties
-  double rl[3][3];
-  for (int j = 0; j < 3; j++)
-    for (int x = 0; x < 3; x++) rl[j][x] = rlat[x + 3 * j] / nkpt3[j];
-  double diag[4];
-  for (int dsel = 0; dsel < 4; dsel++) {
-    static const int sg[4][3] = {{1, -1, 1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1}};
-    double s = 0.0;
-    for (int x = 0; x < 3; x++) {
-      const double v = sg[dsel][0] * rl[0][x] + sg[dsel][1] * rl[1][x] + sg[dsel][2] * rl[2][x];
-      s += v * v;
-    }
-    diag[dsel] = s;
-  }
-  int idmin = 0;
-  for (int dsel = 1; dsel < 4; dsel++)
-    if (diag[dsel] < diag[idmin]) idmin = dsel;
+  const int idmin = tetra_choice(nkpt3, rlat);
   SPX_CHECK(ctx, spx_owns(ctx, e_occ, 8 * (size_t)nk * nocc) &&
                      spx_owns(ctx, e_unocc, 8 * (size_t)nk * nunocc) &&
                      spx_owns(ctx, wtetra, 8 * ((size_t)ldwt * (nfrqs - 1) + n)),
@@ -727,17 +754,17 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
 
 }  // extern "C"
 
-extern "C" {
+namespace {
 
 // Host drop-in for tetrahedron5_init (tetrahedron.f:903-1332) on host
 // arrays: the full-mesh weights, then (kpt1p given) the rows of the reduced
-// set and (deg arrays given) the averaging over degenerate states.
-int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
-                                int nunocc, const double *e_occ, const double *e_unocc,
-                                double efermi, int nfrqs, const double *frqs, const int *kpt1p,
-                                int nkpt1, const int *deg_occ, const int *deg_unocc,
-                                double *wtetra) {
-  if (!ctx) return SPEXHIP_E_INVALID;
+// set -- the TETRAF rows when kpt1 is given too -- and (deg arrays given)
+// the averaging over degenerate states.
+int tetrahedron_weights_host(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
+                             int nunocc, const double *e_occ, const double *e_unocc,
+                             double efermi, int nfrqs, const double *frqs, const int *kpt1,
+                             const int *kpt1p, int nkpt1, const int *deg_occ,
+                             const int *deg_unocc, double *wtetra) {
   SPX_CHECK(ctx, nkpt3 && e_occ && e_unocc && wtetra && nocc > 0 && nunocc > 0,
             "tetrahedron_weights: bad arguments");
   SPX_CHECK(ctx, (deg_occ == nullptr) == (deg_unocc == nullptr),
@@ -772,8 +799,8 @@ int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double
                                    dW, n);
   void *res = dW;
   if (!rc && kpt1p) {
-    rc = spexhip_tetra_reduce_parents_dev(ctx, (int)nk, nocc, nunocc, nfrqs, dW, n, kpt1p, nkr, dR,
-                                          nr);
+    rc = reduce_rows(ctx, kpt1 ? "tetra_reduce_tetraf" : "tetra_reduce_parents", (int)nk, nocc,
+                     nunocc, nfrqs, dW, n, kpt1, kpt1p, nkr, dR, nr);
     res = dR;
   }
   if (!rc && deg_occ)
@@ -790,6 +817,166 @@ int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double
   for (void *p : {dEo, dEu, dW, dR, dDo, dDu})
     if (p) spx_free(ctx, p);
   return rc;
+}
+
+// The degenerate group (first, last) of band b in one SPEX deg row of nb
+// bands (0-based): deg[b] >= b marks the first band of a group and holds its
+// last, else deg[b] is the group's first band.  False if the row is not
+// consistent there.
+bool host_group(const int *deg, int nb, int b, int &first, int &last) {
+  const int d = deg[b];
+  if (d >= b) {
+    first = b;
+    last = d;
+  } else if (d >= 0) {
+    first = d;
+    last = deg[d];
+  } else {
+    return false;
+  }
+  return first >= 0 && first <= b && last >= b && last < nb;
+}
+
+}  // namespace
+
+extern "C" {
+
+int spexhip_tetrahedron_weights_bands(spexhip_ctx *ctx, const int *nkpt3, const double *rlat,
+                                      int nb1, const double *ene1, const int *deg1, int b1low,
+                                      int b1up, int nb2, const double *ene2, const int *deg2,
+                                      int b2low, int b2up, double efermi, int nfrqs,
+                                      const double *frqs, const int *kpt1, int nkpt1,
+                                      const int *kpt1p, int tetraf, double *wtetra) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, nkpt3 && rlat && ene1 && ene2 && deg1 && deg2 && frqs && wtetra,
+            "tetrahedron_weights_bands: null argument");
+  SPX_CHECK(ctx, nkpt3[0] > 0 && nkpt3[1] > 0 && nkpt3[2] > 0 && nfrqs > 0,
+            "tetrahedron_weights_bands: bad sizes");
+  SPX_CHECK(ctx, 0 <= b1low && b1low <= b1up && b1up < nb1 && 0 <= b2low && b2low <= b2up &&
+                     b2up < nb2,
+            "tetrahedron_weights_bands: band windows outside the band arrays");
+  SPX_CHECK(ctx, !kpt1p || (kpt1 && nkpt1 > 0),
+            "tetrahedron_weights_bands: a reduced set needs kpt1 and nkpt1");
+  const long nk = (long)nkpt3[0] * nkpt3[1] * nkpt3[2];
+  const int nocc = b1up - b1low + 1, nunocc = b2up - b2low + 1;
+  // widened windows: the whole groups of the edge bands at every k
+  // (tetrahedron.f:984-1012)
+  int a1 = b1low, z1 = b1up, a2 = b2low, z2 = b2up;
+  for (long k = 0; k < nk; k++) {
+    int f, l;
+    SPX_CHECK(ctx, host_group(deg1 + k * nb1, nb1, b1low, f, l),
+              "tetrahedron_weights_bands: deg1 not consistent at k=%ld band %d", k, b1low);
+    a1 = std::min(a1, f);
+    SPX_CHECK(ctx, host_group(deg1 + k * nb1, nb1, b1up, f, l),
+              "tetrahedron_weights_bands: deg1 not consistent at k=%ld band %d", k, b1up);
+    z1 = std::max(z1, l);
+    SPX_CHECK(ctx, host_group(deg2 + k * nb2, nb2, b2low, f, l),
+              "tetrahedron_weights_bands: deg2 not consistent at k=%ld band %d", k, b2low);
+    a2 = std::min(a2, f);
+    SPX_CHECK(ctx, host_group(deg2 + k * nb2, nb2, b2up, f, l),
+              "tetrahedron_weights_bands: deg2 not consistent at k=%ld band %d", k, b2up);
+    z2 = std::max(z2, l);
+  }
+  const int nx1 = z1 - a1 + 1, nx2 = z2 - a2 + 1;
+  const int nrow = kpt1p ? nkpt1 : (int)nk;
+  // group tables of the rows (groups at kpt1(p) for a reduced set), relative
+  // to the widened windows
+  std::vector<int2> g1((size_t)nrow * nocc), g2((size_t)nrow * nunocc);
+  for (int r = 0; r < nrow; r++) {
+    const long kd = kpt1p ? kpt1[r] : r;
+    SPX_CHECK(ctx, kd >= 0 && kd < nk, "tetrahedron_weights_bands: kpt1 out of range");
+    for (int b = 0; b < nocc; b++) {
+      int f, l;
+      SPX_CHECK(ctx, host_group(deg1 + kd * nb1, nb1, b1low + b, f, l) && f >= a1 && l <= z1,
+                "tetrahedron_weights_bands: deg1 not consistent at k=%ld band %d", kd, b1low + b);
+      g1[(size_t)r * nocc + b] = make_int2(f - a1, l - a1);
+    }
+    for (int b = 0; b < nunocc; b++) {
+      int f, l;
+      SPX_CHECK(ctx, host_group(deg2 + kd * nb2, nb2, b2low + b, f, l) && f >= a2 && l <= z2,
+                "tetrahedron_weights_bands: deg2 not consistent at k=%ld band %d", kd, b2low + b);
+      g2[(size_t)r * nunocc + b] = make_int2(f - a2, l - a2);
+    }
+  }
+  // energies of the widened windows
+  std::vector<double> x1((size_t)nk * nx1), x2((size_t)nk * nx2);
+  for (long k = 0; k < nk; k++) {
+    for (int b = 0; b < nx1; b++) x1[(size_t)k * nx1 + b] = ene1[k * nb1 + a1 + b];
+    for (int b = 0; b < nx2; b++) x2[(size_t)k * nx2 + b] = ene2[k * nb2 + a2 + b];
+  }
+  const long nxf = nk * nx1 * nx2, nxr = (long)nrow * nx1 * nx2, nout = (long)nrow * nocc * nunocc;
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  void *dE1 = nullptr, *dE2 = nullptr, *dW = nullptr, *dR = nullptr, *dO = nullptr,
+       *dG = nullptr;
+  int rc = spx_alloc(ctx, 8 * x1.size(), &dE1);
+  if (!rc) rc = spx_alloc(ctx, 8 * x2.size(), &dE2);
+  if (!rc) rc = spx_alloc(ctx, 8 * (size_t)nxf * nfrqs, &dW);
+  if (!rc && kpt1p) rc = spx_alloc(ctx, 8 * (size_t)nxr * nfrqs, &dR);
+  if (!rc) rc = spx_alloc(ctx, 8 * (size_t)nout * nfrqs, &dO);
+  if (!rc) rc = spx_alloc(ctx, 8 * (g1.size() + g2.size()), &dG);
+  if (!rc) {
+    hipError_t e = hipMemcpyAsync(dE1, x1.data(), 8 * x1.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dE2, x2.data(), 8 * x2.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dG, g1.data(), 8 * g1.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync((int2 *)dG + g1.size(), g2.data(), 8 * g2.size(), hipMemcpyHostToDevice,
+                         ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "tetrahedron_weights_bands: upload failed: %s",
+                      hipGetErrorString(e));
+  }
+  if (!rc)
+    rc = spexhip_tetra_weights_dev(ctx, nkpt3, rlat, nx1, nx2, dE1, dE2, efermi, nfrqs, frqs, dW,
+                                   nxf);
+  const void *rows = dW;
+  long ldr = nxf;
+  if (!rc && kpt1p) {
+    rc = reduce_rows(ctx, tetraf ? "tetra_reduce_tetraf" : "tetra_reduce_parents", (int)nk, nx1,
+                     nx2, nfrqs, dW, nxf, tetraf ? kpt1 : nullptr, kpt1p, nkpt1, dR, nxr);
+    rows = dR;
+    ldr = nxr;
+  }
+  if (!rc) {
+    hipLaunchKernelGGL(group_mean_kernel, dim3((unsigned)((nout + 255) / 256), (unsigned)nfrqs),
+                       dim3(256), 0, ctx->stream, nrow, nocc, nunocc, nx1, nx2,
+                       (const int2 *)dG, (const int2 *)dG + g1.size(), nfrqs,
+                       (const double *)rows, ldr, (double *)dO, nout);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(wtetra, dO, 8 * (size_t)nout * nfrqs, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess)
+      rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "tetrahedron_weights_bands: %s",
+                      hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : {dE1, dE2, dW, dR, dO, dG})
+    if (p) spx_free(ctx, p);
+  return rc;
+}
+
+int spexhip_tetrahedron_weights(spexhip_ctx *ctx, const int *nkpt3, const double *rlat, int nocc,
+                                int nunocc, const double *e_occ, const double *e_unocc,
+                                double efermi, int nfrqs, const double *frqs, const int *kpt1p,
+                                int nkpt1, const int *deg_occ, const int *deg_unocc,
+                                double *wtetra) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  return tetrahedron_weights_host(ctx, nkpt3, rlat, nocc, nunocc, e_occ, e_unocc, efermi, nfrqs,
+                                  frqs, nullptr, kpt1p, nkpt1, deg_occ, deg_unocc, wtetra);
+}
+
+int spexhip_tetrahedron_weights_tetraf(spexhip_ctx *ctx, const int *nkpt3, const double *rlat,
+                                       int nocc, int nunocc, const double *e_occ,
+                                       const double *e_unocc, double efermi, int nfrqs,
+                                       const double *frqs, const int *kpt1, int nkpt1,
+                                       const int *kpt1p, const int *deg_occ,
+                                       const int *deg_unocc, double *wtetra) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, kpt1 && kpt1p, "tetrahedron_weights_tetraf: kpt1 and kpt1p are required");
+  return tetrahedron_weights_host(ctx, nkpt3, rlat, nocc, nunocc, e_occ, e_unocc, efermi, nfrqs,
+                                  frqs, kpt1, kpt1p, nkpt1, deg_occ, deg_unocc, wtetra);
 }
 
 int spexhip_tetra_average_equivalent(spexhip_ctx *ctx, int nkpt1, int nocc, int nunocc, int nfrqs,

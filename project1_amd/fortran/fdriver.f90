@@ -277,6 +277,41 @@ program fdriver
       close(u)
     end if
   end block
+  ! Tetrahedron weights with band windows that cut degenerate groups, through
+  ! the shim with SPEX's 1-based deg tables; written out for the oracle.
+  block
+    integer, parameter :: nk = 8, nb = 4, nf = 10
+    integer :: n3(3), d1(nb, nk), d2(nb, nk), b, u
+    real(c_double), target :: rl(3,3), e1(nb, nk), e2(nb, nk), fr(nf), wt(nk*2*2, nf)
+    character(len=512) :: outdir
+    integer :: ln
+    n3 = [2, 2, 2]
+    rl = 0 ; rl(1,1) = 1 ; rl(2,2) = 1.1d0 ; rl(3,3) = 0.9d0
+    do k = 1, nk
+      do b = 1, nb
+        e1(b, k) = -1.2d0 + 0.25d0*b + 0.05d0*sin(1.3d0*k + b)
+        e2(b, k) = 0.3d0 + 0.2d0*b + 0.04d0*cos(0.7d0*k*b)
+      end do
+      e1(3, k) = e1(2, k)  ! occupied pair 2-3 on every k
+      e2(2, k) = e2(1, k)  ! unoccupied pair 1-2 on every k
+      d1(:, k) = [1, 3, 2, 4]
+      d2(:, k) = [2, 1, 3, 4]
+    end do
+    do f = 1, nf
+      fr(f) = 0.2d0 * (1.3d0**(f-1) - 1)
+    end do
+    ! windows 3:4 (cuts the occupied pair) and 2:3 (cuts the unoccupied pair)
+    call spexhip_tetrahedron5_bands(ctx, n3, rl, e1, d1, 3, 4, e2, d2, 2, 3, 0d0, nf, fr, wt)
+    write(*, '(A,ES10.3)') 'fortran tetrahedron5_bands weight sum: ', sum(wt)
+    if (.not. sum(wt) > 0) stop 11
+    call get_environment_variable('SPEXHIP_FDRIVER_OUT', outdir, ln)
+    if (ln > 0) then
+      open(newunit=u, file=outdir(1:ln)//'/tetra_bands.bin', access='stream', &
+           form='unformatted', status='replace')
+      write(u) rl, e1, e2, fr, d1, d2, wt
+      close(u)
+    end if
+  end block
   ! FFT grid setup (getinput.f:2472-2473) for a simple cubic reciprocal lattice
   ! of spacing 1.1 written in a skewed basis: determine_imat must make the
   ! basis orthogonal again, and the grid of the 8.3 sphere is then that of

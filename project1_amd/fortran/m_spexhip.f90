@@ -43,7 +43,7 @@ module m_spexhip
   public :: spexhip_pairsum, spexhip_wfprod2_mt, spexhip_wfprod2_pw, spexhip_dielec_inverse
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
   public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
-  public :: spexhip_dielecc_inverse, spexhip_tetrahedron5_equiv
+  public :: spexhip_dielecc_inverse, spexhip_tetrahedron5_equiv, spexhip_tetrahedron5_bands
   public :: spexhip_fft_setup, spexhip_step_on_grid
 
   interface
@@ -99,6 +99,23 @@ module m_spexhip
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx, nkpt3, rlat, eocc, eunocc, frqs, kpt1p, degocc, degunocc, wtetra
       integer(c_int), value :: nocc, nunocc, nfrqs, nkpt1
+      real(c_double), value :: efermi
+    end function
+    integer(c_int) function c_tetrahedron_weights_tetraf(ctx, nkpt3, rlat, nocc, nunocc, eocc, &
+        eunocc, efermi, nfrqs, frqs, kpt1, nkpt1, kpt1p, degocc, degunocc, wtetra) &
+        bind(C, name='spexhip_tetrahedron_weights_tetraf')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, nkpt3, rlat, eocc, eunocc, frqs, kpt1, kpt1p, degocc, degunocc, &
+                            wtetra
+      integer(c_int), value :: nocc, nunocc, nfrqs, nkpt1
+      real(c_double), value :: efermi
+    end function
+    integer(c_int) function c_tetrahedron_weights_bands(ctx, nkpt3, rlat, nb1, ene1, deg1, b1low, &
+        b1up, nb2, ene2, deg2, b2low, b2up, efermi, nfrqs, frqs, kpt1, nkpt1, kpt1p, tetraf, &
+        wtetra) bind(C, name='spexhip_tetrahedron_weights_bands')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, nkpt3, rlat, ene1, deg1, ene2, deg2, frqs, kpt1, kpt1p, wtetra
+      integer(c_int), value :: nb1, b1low, b1up, nb2, b2low, b2up, nfrqs, nkpt1, tetraf
       real(c_double), value :: efermi
     end function
     integer(c_int) function c_tetra_average_equivalent(ctx, nkpt1, nocc, nunocc, nfrqs, kpt1, &
@@ -305,9 +322,10 @@ contains
   ! coordinates), frqs(nfrqs) the Hilbert mesh; optional kpt1p(nk) (1-based parents,
   ! 0 = none) with nkpt1, and deg_occ(nocc,nkpt1), deg_unocc(nunocc,nkpt1) (1-based band
   ! numbers within the windows, SPEX's deg convention).  wtetra: nkr*nocc*nunocc x nfrqs
-  ! (nkr = nkpt1 with kpt1p, else the mesh size), transition fastest.
+  ! (nkr = nkpt1 with kpt1p, else the mesh size), transition fastest.  kpt1(nkpt1)
+  ! (1-based mesh indices) with kpt1p gives the rows of the TETRAF keyword.
   subroutine spexhip_tetrahedron5(ctx, nkpt3, rlat, nocc, nunocc, ene_occ, ene_unocc, efermi, &
-                                  nfrqs, frqs, wtetra, kpt1p, nkpt1, deg_occ, deg_unocc)
+                                  nfrqs, frqs, wtetra, kpt1p, nkpt1, deg_occ, deg_unocc, kpt1)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: nkpt3(3), nocc, nunocc, nfrqs
     real(c_double), intent(in), target :: rlat(3,3), frqs(nfrqs), &
@@ -315,9 +333,9 @@ contains
                                           ene_unocc(nunocc, nkpt3(1)*nkpt3(2)*nkpt3(3))
     real(c_double), intent(in) :: efermi
     real(c_double), intent(out), target :: wtetra(*)
-    integer, intent(in), optional :: kpt1p(:), nkpt1, deg_occ(:,:), deg_unocc(:,:)
+    integer, intent(in), optional :: kpt1p(:), nkpt1, deg_occ(:,:), deg_unocc(:,:), kpt1(:)
     integer(c_int), target :: k3(3)
-    integer(c_int), allocatable, target :: kp(:), dgo(:,:), dgu(:,:)
+    integer(c_int), allocatable, target :: kp(:), dgo(:,:), dgu(:,:), k1(:)
     type(c_ptr) :: pk, pdo, pdu
     integer(c_int) :: nk1
     k3 = nkpt3
@@ -332,9 +350,66 @@ contains
       allocate(dgo(size(deg_occ,1),size(deg_occ,2)), dgu(size(deg_unocc,1),size(deg_unocc,2)))
       dgo = deg_occ - 1 ; dgu = deg_unocc - 1 ; pdo = c_loc(dgo) ; pdu = c_loc(dgu)
     end if
+    if (present(kpt1)) then
+      if (.not. present(kpt1p)) call size_error('tetrahedron5', 'kpt1 needs kpt1p and nkpt1')
+      if (size(kpt1) /= nkpt1) call size_error('tetrahedron5', 'kpt1 must have nkpt1 entries')
+      allocate(k1(size(kpt1))) ; k1 = kpt1 - 1
+      call check(ctx, c_tetrahedron_weights_tetraf(ctx, c_loc(k3), c_loc(rlat), int(nocc, c_int), &
+                 int(nunocc, c_int), c_loc(ene_occ), c_loc(ene_unocc), efermi, &
+                 int(nfrqs, c_int), c_loc(frqs), c_loc(k1), nk1, pk, pdo, pdu, c_loc(wtetra)), &
+                 'tetrahedron5')
+      return
+    end if
     call check(ctx, c_tetrahedron_weights(ctx, c_loc(k3), c_loc(rlat), int(nocc, c_int), &
                int(nunocc, c_int), c_loc(ene_occ), c_loc(ene_unocc), efermi, int(nfrqs, c_int), &
                c_loc(frqs), pk, nk1, pdo, pdu, c_loc(wtetra)), 'tetrahedron5')
+  end subroutine
+
+  ! tetrahedron5_init with band windows b1low:b1up, b2low:b2up that may cut a degenerate
+  ! group (tetrahedron.f:984-1012, 1191-1194, 1237-1273): ene1(nb1,nk) at k and
+  ! ene2(nb2,nk) at k+q over all bands, deg1/deg2 SPEX's deg tables of the same bands
+  ! (1-based, as global.f holds them), the windows 1-based; optional kpt1(nkpt1) and
+  ! kpt1p(nk) (1-based, 0 = no parent) for a reduced set, tetraf for the TETRAF rows.
+  ! wtetra: nkr*nocc*nunocc x nfrqs, transition fastest.
+  subroutine spexhip_tetrahedron5_bands(ctx, nkpt3, rlat, ene1, deg1, b1low, b1up, ene2, deg2, &
+                                        b2low, b2up, efermi, nfrqs, frqs, wtetra, kpt1, kpt1p, &
+                                        tetraf)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nkpt3(3), b1low, b1up, b2low, b2up, nfrqs
+    real(c_double), intent(in), target :: rlat(3,3), frqs(nfrqs), ene1(:,:), ene2(:,:)
+    integer, intent(in) :: deg1(:,:), deg2(:,:)
+    real(c_double), intent(in) :: efermi
+    real(c_double), intent(out), target :: wtetra(*)
+    integer, intent(in), optional :: kpt1(:), kpt1p(:)
+    logical, intent(in), optional :: tetraf
+    integer(c_int), target :: k3(3)
+    integer(c_int), allocatable, target :: d1(:,:), d2(:,:), k1(:), kp(:)
+    real(c_double), allocatable, target :: e1(:,:), e2(:,:)
+    type(c_ptr) :: pk1, pkp
+    integer(c_int) :: nk1, tf
+    k3 = nkpt3
+    if (size(ene1,2) /= product(nkpt3) .or. size(ene2,2) /= product(nkpt3) .or. &
+        any(shape(deg1) /= shape(ene1)) .or. any(shape(deg2) /= shape(ene2))) &
+      call size_error('tetrahedron5_bands', 'ene/deg must be (nband, product(nkpt3)) with equal shapes')
+    allocate(e1, source=ene1) ; allocate(e2, source=ene2)
+    allocate(d1(size(deg1,1),size(deg1,2)), d2(size(deg2,1),size(deg2,2)))
+    d1 = deg1 - 1 ; d2 = deg2 - 1
+    pk1 = c_null_ptr ; pkp = c_null_ptr ; nk1 = 0 ; tf = 0
+    if (present(kpt1p)) then
+      if (.not. present(kpt1)) call size_error('tetrahedron5_bands', 'kpt1p needs kpt1')
+      if (size(kpt1p) /= product(nkpt3)) call size_error('tetrahedron5_bands', &
+          'kpt1p must have one entry per k point of the mesh (product(nkpt3))')
+      allocate(k1(size(kpt1)), kp(size(kpt1p))) ; k1 = kpt1 - 1 ; kp = kpt1p - 1
+      pk1 = c_loc(k1) ; pkp = c_loc(kp) ; nk1 = size(kpt1)
+    end if
+    if (present(tetraf)) then
+      if (tetraf) tf = 1
+    end if
+    call check(ctx, c_tetrahedron_weights_bands(ctx, c_loc(k3), c_loc(rlat), &
+               int(size(ene1,1), c_int), c_loc(e1), c_loc(d1), int(b1low-1, c_int), &
+               int(b1up-1, c_int), int(size(ene2,1), c_int), c_loc(e2), c_loc(d2), &
+               int(b2low-1, c_int), int(b2up-1, c_int), efermi, int(nfrqs, c_int), c_loc(frqs), &
+               pk1, nk1, pkp, tf, c_loc(wtetra)), 'tetrahedron5_bands')
   end subroutine
 
   ! The last step of tetrahedron5_init (tetrahedron.f:1282-1318) on the

@@ -321,6 +321,10 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_weights_none_dev": (I, [_vp, I, I, I, _vp, _vp, I, I, _vp, I, _vp, _vp]),
         "spexhip_chi0_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, I, _vp, I,
                                  C.c_double, C.c_double, _vp, I]),
+        "spexhip_fermi_weights_dev": (I, [_vp, _vp, _vp, I, _vp, C.c_double, _vp]),
+        "spexhip_plasma_frequency_dev": (I, [_vp, I, I, _vp, _vp, _c_long, _vp, C.c_double,
+                                             C.c_double, P(C.c_double)]),
+        "spexhip_gamma_drude_dev": (I, [_vp, I, _vp, C.c_double, _vp, I, _vp, _vp]),
         "spexhip_coulomb_trafo_dev": (I, [_vp, _c_long, I, _vp, _c_long, _vp, _c_long, I, _vp,
                                           _c_long]),
         "spexhip_ctrafo_olap_dev": (I, [_vp, I, I, I, _vp, _c_long, _vp, _vp, _vp, _vp, _c_long]),
@@ -346,11 +350,18 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_tetra_average_degenerate_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long]),
         "spexhip_tetra_reduce_parents_dev": (I, [_vp, I, I, I, I, _vp, _c_long, _vp, I, _vp,
                                                  _c_long]),
+        "spexhip_tetra_reduce_tetraf_dev": (I, [_vp, I, I, I, I, _vp, _c_long, _vp, I, _vp, _vp,
+                                                _c_long]),
         "spexhip_tetra_average_equivalent_dev": (I, [_vp, I, I, I, I, _vp, I, _vp, I, _vp, I,
                                                      _vp, _c_long]),
         "spexhip_tetra_average_equivalent": (I, [_vp, I, I, I, I, _vp, I, _vp, I, _vp, I, _vp]),
         "spexhip_tetrahedron_weights": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp,
                                             _vp, I, _vp, _vp, _vp]),
+        "spexhip_tetrahedron_weights_tetraf": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I,
+                                                   _vp, _vp, I, _vp, _vp, _vp, _vp]),
+        "spexhip_tetrahedron_weights_bands": (I, [_vp, _vp, _vp, I, _vp, _vp, I, I, I, _vp, _vp,
+                                                  I, I, C.c_double, I, _vp, _vp, I, _vp, I,
+                                                  _vp]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
@@ -616,6 +627,17 @@ class Context:
         self._check(lib().spexhip_tetra_reduce_parents_dev(
             self.h, nk, nocc, nunocc, nfrqs, w_full.ptr, ldw, _ptr(kp), nkpt1, w_red.ptr, ldr))
 
+    def tetra_reduce_tetraf_dev(self, nk: int, nocc: int, nunocc: int, nfrqs: int,
+                                w_full: DevBuf, ldw: int, kpt1, kpt1p, w_red: DevBuf,
+                                ldr: int) -> None:
+        """Rows of a reduced k set under TETRAF: count(kpt1p == p) times the
+        rows of the points listed in kpt1 whose parent is p."""
+        k1 = np.ascontiguousarray(kpt1, np.int32)
+        kp = np.ascontiguousarray(kpt1p, np.int32)
+        self._check(lib().spexhip_tetra_reduce_tetraf_dev(
+            self.h, nk, nocc, nunocc, nfrqs, w_full.ptr, ldw, _ptr(k1), k1.size, _ptr(kp),
+            w_red.ptr, ldr))
+
     def tetra_average_equivalent_dev(self, nocc: int, nunocc: int, nfrqs: int, kpt1, kpt1p,
                                      kptsym, ikpt: int, W: DevBuf, ldw: int) -> None:
         """In place: the mean over each group of equivalent k points of kpt1
@@ -643,9 +665,9 @@ class Context:
 
     def tetrahedron_weights(self, nkpt3, rlat, e_occ: np.ndarray, e_unocc: np.ndarray,
                             efermi: float, frqs: np.ndarray, kpt1p=None, nkpt1: int = 0,
-                            deg_occ=None, deg_unocc=None) -> np.ndarray:
+                            deg_occ=None, deg_unocc=None, kpt1=None) -> np.ndarray:
         """Host drop-in of tetrahedron5_init: returns wtetra (nkr*nocc*nunocc,
-        nfrqs), Fortran-ordered."""
+        nfrqs), Fortran-ordered.  kpt1 given (with kpt1p): the TETRAF rows."""
         k3 = np.ascontiguousarray(nkpt3, np.int32)
         rl = np.asfortranarray(rlat, np.float64)
         eo = np.ascontiguousarray(e_occ, np.float64)
@@ -659,9 +681,50 @@ class Context:
         du = None if deg_unocc is None else np.ascontiguousarray(deg_unocc, np.int32)
         out = np.zeros((nkr * nocc * nunocc, fr.size), np.float64, order="F")
         p = lambda a: None if a is None else _ptr(a)
+        if kpt1 is not None:
+            if kp is None:
+                raise ValueError("tetrahedron_weights: kpt1 needs kpt1p")
+            k1 = np.ascontiguousarray(kpt1, np.int32)
+            if k1.size != nkpt1:
+                raise ValueError("tetrahedron_weights: kpt1 must hold nkpt1 entries")
+            self._check(lib().spexhip_tetrahedron_weights_tetraf(
+                self.h, _ptr(k3), rl.ctypes.data, nocc, nunocc, _ptr(eo), _ptr(eu), efermi,
+                fr.size, _ptr(fr), _ptr(k1), nkpt1, _ptr(kp), p(do), p(du), out.ctypes.data))
+            return out
         self._check(lib().spexhip_tetrahedron_weights(
             self.h, _ptr(k3), rl.ctypes.data, nocc, nunocc, _ptr(eo), _ptr(eu), efermi, fr.size,
             _ptr(fr), p(kp), nkpt1, p(do), p(du), out.ctypes.data))
+        return out
+
+    def tetrahedron_weights_bands(self, nkpt3, rlat, ene1, deg1, w1, ene2, deg2, w2,
+                                  efermi: float, frqs, kpt1=None, kpt1p=None,
+                                  tetraf: bool = False) -> np.ndarray:
+        """Host drop-in of tetrahedron5_init with band windows w1 = (b1low,
+        b1up), w2 = (b2low, b2up) that may cut degenerate groups; ene/deg
+        (nk, nb) over all bands, 0-based.  Returns (nkr*nocc*nunocc, nfrqs),
+        Fortran-ordered."""
+        k3 = np.ascontiguousarray(nkpt3, np.int32)
+        rl = np.asfortranarray(rlat, np.float64)
+        e1 = np.ascontiguousarray(ene1, np.float64)
+        e2 = np.ascontiguousarray(ene2, np.float64)
+        g1 = np.ascontiguousarray(deg1, np.int32)
+        g2 = np.ascontiguousarray(deg2, np.int32)
+        if g1.shape != e1.shape or g2.shape != e2.shape or e1.shape[0] != e2.shape[0]:
+            raise ValueError("tetrahedron_weights_bands: ene/deg shapes differ")
+        fr = np.ascontiguousarray(frqs, np.float64)
+        nk = e1.shape[0]
+        k1 = None if kpt1 is None else np.ascontiguousarray(kpt1, np.int32)
+        kp = None if kpt1p is None else np.ascontiguousarray(kpt1p, np.int32)
+        if kp is not None and (k1 is None or kp.size != nk):
+            raise ValueError("tetrahedron_weights_bands: kpt1p needs kpt1 and one entry per k")
+        nkr = k1.size if kp is not None else nk
+        nocc, nunocc = w1[1] - w1[0] + 1, w2[1] - w2[0] + 1
+        out = np.zeros((nkr * nocc * nunocc, fr.size), np.float64, order="F")
+        p = lambda a: None if a is None else _ptr(a)
+        self._check(lib().spexhip_tetrahedron_weights_bands(
+            self.h, _ptr(k3), rl.ctypes.data, e1.shape[1], _ptr(e1), _ptr(g1), w1[0], w1[1],
+            e2.shape[1], _ptr(e2), _ptr(g2), w2[0], w2[1], efermi, fr.size, _ptr(fr), p(k1),
+            0 if k1 is None else k1.size, p(kp), int(tetraf), out.ctypes.data))
         return out
 
     # -- chi0 -------------------------------------------------------------
@@ -930,6 +993,31 @@ class Context:
         h = _vp()
         self._check(lib().spexhip_pw_plan_create(self.h, C.byref(d), C.byref(h)))
         return Plan(self, h.value, keep)
+
+    def fermi_weights_dev(self, nkpt3, rlat, nband: int, ene: DevBuf, efermi: float,
+                          w: DevBuf) -> None:
+        """tetrahedron4_init's Fermi-surface weights [nk][nband] (full mesh)."""
+        nk3 = np.ascontiguousarray(nkpt3, np.int32)
+        r = _rlat_f(rlat)
+        self._check(lib().spexhip_fermi_weights_dev(self.h, _ptr(nk3), _ptr(r), nband, ene.ptr,
+                                                    float(efermi), w.ptr))
+
+    def plasma_frequency_dev(self, nk: int, nband: int, w: DevBuf, P: DevBuf, ldp: int, deg,
+                             vol: float, spin_factor: float = 2.0) -> float:
+        d = np.ascontiguousarray(deg, np.int32)
+        out = C.c_double(0.0)
+        self._check(lib().spexhip_plasma_frequency_dev(self.h, nk, nband, w.ptr, P.ptr, ldp,
+                                                       _ptr(d), float(vol), float(spin_factor),
+                                                       C.byref(out)))
+        return out.value
+
+    def gamma_drude_dev(self, freq, plasma: float, head: DevBuf, freqc=None,
+                        headc: DevBuf | None = None) -> None:
+        f = _f64(freq)
+        fc = _c128(freqc if freqc is not None else np.zeros(0))
+        self._check(lib().spexhip_gamma_drude_dev(self.h, f.size, _ptr(f), float(plasma), head.ptr,
+                                                  fc.size, _ptr(fc) if fc.size else None,
+                                                  headc.ptr if headc is not None else None))
 
     def coulomb_trafo_dev(self, n: int, nbasm: int, M: DevBuf, ldm: int, T: DevBuf, ldt: int,
                           dim: int, out: DevBuf | None = None, ldo: int = 0) -> None:

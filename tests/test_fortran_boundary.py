@@ -16,7 +16,8 @@ def test_module_declares_every_shim():
                  "spexhip_dielec_inverse", "spexhip_minverse", "spexhip_screen",
                  "spexhip_rpa_int", "spexhip_pairsum_hilbert", "spexhip_headwings",
                  "spexhip_dielec_inverse_gamma", "spexhip_tetrahedron5", "spexhip_tetrahedron5_equiv",
-                 "spexhip_dielecc_inverse", "spexhip_fft_setup", "spexhip_step_on_grid"]:
+                 "spexhip_dielecc_inverse", "spexhip_fft_setup", "spexhip_step_on_grid",
+                 "spexhip_tetrahedron5_bands"]:
         assert f"subroutine {name}" in src
     # every bind(C) name exists in the C header
     import re
@@ -68,6 +69,21 @@ def test_fortran_driver_runs_on_gpu(tmp_path):
     want = tetra.average_equivalent(w0, no, nu, kp1 - 1, kpp - 1, ks2[:nk] - 1, ik - 1)
     assert np.abs(want - w0).max() > 0
     np.testing.assert_array_equal(w1, want)
+    # tetrahedron weights with cut windows through the shim (1-based tables)
+    raw = (tmp_path / "tetra_bands.bin").read_bytes()
+    nk, nb, nf = 8, 4, 10
+    f64 = np.frombuffer(raw[:8 * (9 + 2 * nb * nk + nf)], np.float64)
+    rl = f64[:9].reshape((3, 3), order="F")
+    e1 = f64[9:9 + nb * nk].reshape((nb, nk), order="F").T
+    e2 = f64[9 + nb * nk:9 + 2 * nb * nk].reshape((nb, nk), order="F").T
+    fr = f64[9 + 2 * nb * nk:]
+    off = f64.nbytes
+    i32 = np.frombuffer(raw[off:off + 8 * nb * nk], np.int32)
+    d1 = i32[:nb * nk].reshape((nb, nk), order="F").T - 1
+    d2 = i32[nb * nk:].reshape((nb, nk), order="F").T - 1
+    wt = np.frombuffer(raw[off + 8 * nb * nk:], np.float64).reshape((nk * 4, nf), order="F")
+    want = tetra.tetrahedron5_weights_bands((2, 2, 2), rl, e1, e2, d1, d2, 2, 3, 1, 2, 0.0, fr)
+    np.testing.assert_allclose(wt, want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
     # FFT grid setup through the Fortran shim (transposed row-major tables)
     from oracle import fftgrid
 

@@ -193,6 +193,33 @@ def test_reduce_to_parents_vs_oracle(ctx):
         b.free()
 
 
+def test_reduce_tetraf_vs_oracle(ctx):
+    """TETRAF rows: the listed point's row times the star size; a listed
+    point without a parent and a parent listed by a later member."""
+    nkpt3 = (3, 2, 2)
+    nk, nocc, nunocc = 12, 2, 3
+    g = np.random.default_rng(89)
+    eo, eu = _energies(g, nk, nocc, nunocc)
+    frq = 0.03 * (1.2 ** np.arange(16) - 1)
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3, 4, 4, 4, 1])
+    kpt1 = np.array([0, 11, 6, 7, 9])
+    nkpt1 = 5
+    n, nr = nk * nocc * nunocc, nkpt1 * nocc * nunocc
+    full = _run(ctx, nkpt3, np.eye(3), eo, eu, frq)
+    want = tetra.reduce_tetraf(full, nk, nocc, nunocc, kpt1, kpt1p, nkpt1)
+    dF = ctx.to_dev(np.ascontiguousarray(full.T))
+    dR = ctx.alloc(8 * nr * len(frq))
+    ctx.tetra_reduce_tetraf_dev(nk, nocc, nunocc, len(frq), dF, n, kpt1, kpt1p, dR, nr)
+    got = dR.download(np.float64, (len(frq), nr)).T
+    mx, _ = normwise(got, want)
+    assert mx < 1e-15
+    with pytest.raises(spexhip.SpexError, match="kpt1 out of range"):
+        ctx.tetra_reduce_tetraf_dev(nk, nocc, nunocc, len(frq), dF, n, [0, 12, 6, 7, 9], kpt1p,
+                                    dR, nr)
+    for b in (dF, dR):
+        b.free()
+
+
 def test_host_dropin_chain_vs_oracle(ctx):
     """Host drop-in: metal weights on the full mesh, star sums over kpt1p,
     then degenerate averaging, against the oracle chain."""
@@ -215,6 +242,14 @@ def test_host_dropin_chain_vs_oracle(ctx):
     got = ctx.tetrahedron_weights(nkpt3, RL_SKEW, eo, eu, ef, frq, kpt1p, nkpt1, do, du)
     full = tetra.tetrahedron5_weights_metal(nkpt3, RL_SKEW, eo, eu, ef, frq)
     red = tetra.reduce_to_parents(full, nk, nocc, nunocc, kpt1p, nkpt1)
+    want = tetra.average_degenerate(red, nkpt1, nocc, nunocc, do, du)
+    mx, _ = normwise(got, want)
+    assert mx < 1e-12
+    # TETRAF: the listed points are the first members, whose energies the
+    # deg arrays were built from
+    kpt1 = np.array([np.where(kpt1p == p)[0][0] for p in range(nkpt1)])
+    got = ctx.tetrahedron_weights(nkpt3, RL_SKEW, eo, eu, ef, frq, kpt1p, nkpt1, do, du, kpt1=kpt1)
+    red = tetra.reduce_tetraf(full, nk, nocc, nunocc, kpt1, kpt1p, nkpt1)
     want = tetra.average_degenerate(red, nkpt1, nocc, nunocc, do, du)
     mx, _ = normwise(got, want)
     assert mx < 1e-12
@@ -298,3 +333,65 @@ def test_equivalent_k_host_dropin_vs_oracle(ctx):
     W = g.uniform(0, 1, (nkpt1 * nocc * nunocc, nf))
     got = ctx.tetra_average_equivalent(W, nocc, nunocc, kpt1, kp, ks, 3)
     np.testing.assert_array_equal(got, tetra.average_equivalent(W, nocc, nunocc, kpt1, kp, ks, 3))
+
+
+def _deg_bands(g, nk, nb, sign, copies, local=()):
+    e = np.sort(sign * g.uniform(0.05, 1.2, (nk, nb)), axis=1)
+    for b in copies:
+        e[:, b] = e[:, b - 1]
+    for k in local:
+        e[k, 4] = e[k, 3]
+    return e
+
+
+@pytest.mark.parametrize("w1,w2,ef", [((2, 3), (1, 3), 0.0), ((3, 3), (0, 2), 0.0),
+                                      ((1, 4), (2, 4), 0.15)])
+def test_cut_windows_vs_oracle(ctx, w1, w2, ef):
+    """Band windows that cut degenerate groups (on every k and on a few k),
+    full mesh, an insulator and a metal (ef = 0.15 cuts the lowest
+    unoccupied bands)."""
+    g = np.random.default_rng(71)
+    nkpt3 = (3, 2, 2)
+    nk = 12
+    e1 = _deg_bands(g, nk, 6, -1.0, copies=(2,), local=(0, 5, 7))
+    e2 = _deg_bands(g, nk, 6, 1.0, copies=(3,), local=(1, 5))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.03 * (1.2 ** np.arange(24) - 1)
+    got = ctx.tetrahedron_weights_bands(nkpt3, RL_SKEW, e1, d1, w1, e2, d2, w2, ef, frq)
+    want = tetra.tetrahedron5_weights_bands(nkpt3, RL_SKEW, e1, e2, d1, d2, *w1, *w2, ef, frq)
+    assert np.abs(want).max() > 0
+    mx, _ = normwise(got, want)
+    print(f"cut windows {w1} {w2} ef={ef}: max-rel {mx:.2e}")
+    assert mx < 1e-12
+
+
+@pytest.mark.parametrize("tetraf", [False, True])
+def test_cut_windows_reduced_set_vs_oracle(ctx, tetraf):
+    g = np.random.default_rng(72)
+    nkpt3 = (3, 2, 2)
+    nk = 12
+    e1 = _deg_bands(g, nk, 5, -1.0, copies=(2,))
+    e2 = _deg_bands(g, nk, 5, 1.0, copies=(1, 4))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.03 * (1.2 ** np.arange(16) - 1)
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3, 4, 4, 4, 1])
+    kpt1 = np.array([0, 1, 3, 7, 8])
+    got = ctx.tetrahedron_weights_bands(nkpt3, np.eye(3), e1, d1, (2, 3), e2, d2, (1, 3), 0.0,
+                                        frq, kpt1, kpt1p, tetraf)
+    want = tetra.tetrahedron5_weights_bands(nkpt3, np.eye(3), e1, e2, d1, d2, 2, 3, 1, 3, 0.0,
+                                            frq, kpt1, kpt1p, 5, tetraf=tetraf)
+    mx, _ = normwise(got, want)
+    assert mx < 1e-12
+
+
+def test_cut_windows_reject_inconsistent_deg(ctx):
+    g = np.random.default_rng(73)
+    e1 = _deg_bands(g, 8, 4, -1.0, copies=(2,))
+    e2 = _deg_bands(g, 8, 4, 1.0, copies=())
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    bad = d1.copy()
+    bad[3, 2] = 7  # beyond the band array
+    frq = 0.03 * (1.2 ** np.arange(8) - 1)
+    with pytest.raises(spexhip.SpexError, match="deg1 not consistent"):
+        ctx.tetrahedron_weights_bands((2, 2, 2), np.eye(3), e1, bad, (2, 3), e2, d2, (0, 1), 0.0,
+                                      frq)

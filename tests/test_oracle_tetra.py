@@ -297,6 +297,41 @@ def test_degenerate_averaging_rules():
     np.testing.assert_array_equal(B[:, 0, 2:], P[:, 0, 2:])  # singleton groups untouched
 
 
+def _literal_scatter(nkpt3, eo, eu, frq, kpt1p, nkpt1, kpt1=None):
+    """tetrahedron5_init's cube loop for one occupied band, written out: the
+    corner weights of each cube go to kcubep = kpt1p(kcube), corners without
+    a parent skipped and, under TETRAF (kpt1 given), corners not listed in
+    kpt1 skipped too and each parent's sum scaled by count(kpt1p == p);
+    then the bin differences."""
+    nk = int(np.prod(nkpt3))
+    nunocc = eu.shape[1]
+    tt = tetra.get_tetra(np.eye(3), nkpt3)
+    pk = tetra.mesh_index(nkpt3)
+    frq1, dw = tetra.hilbert_bins(frq)
+    cum = np.zeros((nkpt1, 1, nunocc, len(frq)))
+    listed = set(range(nk)) if kpt1 is None else set(kpt1)
+    for i1 in range(nkpt3[0]):
+        for i2 in range(nkpt3[1]):
+            for i3 in range(nkpt3[2]):
+                kc = [pk(i1 + a, i2 + b, i3 + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+                for b2 in range(nunocc):
+                    dc = eu[kc, b2] - eo[kc, 0]
+                    for j, f in enumerate(frq1):
+                        wadd = np.zeros(8)
+                        for row in tt:
+                            wadd[row] += tetra.corner_weights_at(dc[row], None, f)
+                        for c in range(8):
+                            if kpt1p[kc[c]] >= 0 and kc[c] in listed:
+                                cum[kpt1p[kc[c]], 0, b2, j] += wadd[c]
+    if kpt1 is not None:
+        for p in range(nkpt1):
+            cum[p] *= np.count_nonzero(np.asarray(kpt1p) == p)
+    want = np.empty_like(cum)
+    want[..., 1:] = np.diff(cum, axis=-1) / (2 * dw[1:] * 24 * nk)
+    want[..., 0] = cum[..., 0] / (2 * dw[0] * 24 * nk)
+    return want.reshape(nkpt1 * nunocc, len(frq))
+
+
 def test_parents_reduction_is_the_reference_scatter():
     """A literal scatter with kcubep = kpt1p(kcube) (corners without a parent
     skipped) equals the star sums of the full-mesh weights."""
@@ -309,28 +344,68 @@ def test_parents_reduction_is_the_reference_scatter():
     nkpt1 = 4
     full = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
     got = tetra.reduce_to_parents(full, nk, 1, 2, kpt1p, nkpt1)
-    # literal scatter (cube loop, corner -> parent), then the same bin differences
-    tt = tetra.get_tetra(np.eye(3), nkpt3)
-    pk = tetra.mesh_index(nkpt3)
-    frq1, dw = tetra.hilbert_bins(frq)
-    cum = np.zeros((nkpt1, 1, 2, len(frq)))
-    for i1 in range(2):
-        for i2 in range(2):
-            for i3 in range(2):
-                kc = [pk(i1 + a, i2 + b, i3 + c) for c in (0, 1) for b in (0, 1) for a in (0, 1)]
-                for b2 in range(2):
-                    dc = eu[kc, b2] - eo[kc, 0]
-                    for j, f in enumerate(frq1):
-                        wadd = np.zeros(8)
-                        for row in tt:
-                            wadd[row] += tetra.corner_weights_at(dc[row], None, f)
-                        for c in range(8):
-                            if kpt1p[kc[c]] >= 0:
-                                cum[kpt1p[kc[c]], 0, b2, j] += wadd[c]
-    want = np.empty_like(cum)
-    want[..., 1:] = np.diff(cum, axis=-1) / (2 * dw[1:] * 24 * nk)
-    want[..., 0] = cum[..., 0] / (2 * dw[0] * 24 * nk)
-    np.testing.assert_allclose(got, want.reshape(got.shape), rtol=1e-12, atol=1e-14)
+    want = _literal_scatter(nkpt3, eo, eu, frq, kpt1p, nkpt1)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+
+
+def test_tetraf_reduction_is_the_reference_scatter():
+    """The same literal scatter under TETRAF (only corners listed in kpt1,
+    rows scaled by the star sizes) equals reduce_tetraf, including a parent
+    whose listed point is not the first of its star and a listed point that
+    has no parent."""
+    g = np.random.default_rng(43)
+    nkpt3 = (2, 2, 2)
+    nk = 8
+    eo, eu = _mesh_case(g, nkpt3, 1, 2)
+    frq = 0.05 * (1.3 ** np.arange(12) - 1) + 0.02
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3])
+    kpt1 = [0, 2, 6, 7]  # parent 1 listed by its second member, parent 2 by 6
+    full = tetra.tetrahedron5_weights(nkpt3, np.eye(3), eo, eu, 0.0, frq)
+    got = tetra.reduce_tetraf(full, nk, 1, 2, kpt1, kpt1p, 4)
+    want = _literal_scatter(nkpt3, eo, eu, frq, kpt1p, 4, kpt1)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+    # the rows are count * the one listed row, not the star sums
+    F = full.reshape(nk, 2, -1)
+    np.testing.assert_allclose(got.reshape(4, 2, -1)[1], 2 * F[2], rtol=1e-14)
+    assert not np.allclose(got, tetra.reduce_to_parents(full, nk, 1, 2, kpt1p, 4))
+
+
+def test_tetraf_equals_star_sums_under_exact_symmetry():
+    """Bands invariant under the 12 operations (axis permutations, with and
+    without inversion) that also keep the Kuhn decomposition along the
+    (1,1,1) diagonal: equivalent k points then carry equal full-mesh weights,
+    so count(star) * w(representative) is the star sum."""
+    n = 4
+    rl = np.array([[1.0, -0.4, -0.4], [-0.4, 1.0, -0.4], [-0.4, -0.4, 1.0]])
+    assert tetra.tetra_choice(rl, (n, n, n)) == 2  # the diagonal 1-8
+    pk = tetra.mesh_index((n, n, n))
+    pts = np.array([(i1, i2, i3) for i1 in range(n) for i2 in range(n) for i3 in range(n)])
+    ops = []
+    for perm in itertools.permutations(range(3)):
+        P = np.eye(3, dtype=int)[list(perm)]
+        ops += [P, -P]
+    x = 2 * np.pi * pts / n
+    sym = np.cos(x).sum(1) + 0.7 * np.cos(x.sum(1)) + 0.3 * np.cos(2 * x).sum(1) * np.cos(x).sum(1)
+    eo = np.stack([-2.0 + 0.2 * sym], 1)
+    eu = np.stack([1.7 - 0.2 * sym, 1.8 + 0.05 * sym ** 2], 1)
+    frq = 0.05 * (1.25 ** np.arange(30) - 1) + 0.02
+    full = tetra.tetrahedron5_weights((n, n, n), rl, eo, eu, 0.0, frq)
+    F = full.reshape(n ** 3, 2, -1)
+    for R in ops:
+        img = [pk(*(R @ p)) for p in pts]
+        np.testing.assert_allclose(F[img], F, rtol=1e-11, atol=1e-14 * np.abs(F).max())
+    kpt1, kpt1p, rep = [], np.full(n ** 3, -1), {}
+    for k, p in enumerate(pts):
+        key = min(pk(*(R @ p)) for R in ops)
+        if key not in rep:
+            rep[key] = len(kpt1)
+            kpt1.append(k)
+        kpt1p[k] = rep[key]
+    sizes = np.bincount(kpt1p)
+    assert sizes.max() > 1 and len(kpt1) < n ** 3
+    got = tetra.reduce_tetraf(full, n ** 3, 1, 2, kpt1, kpt1p, len(kpt1))
+    want = tetra.reduce_to_parents(full, n ** 3, 1, 2, kpt1p, len(kpt1))
+    np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-14 * np.abs(want).max())
 
 
 def _cubic_ops():
@@ -427,3 +502,77 @@ def test_equivalent_average_weights_by_counts_and_skips_reduced_sets():
     wr = np.random.default_rng(5).uniform(0, 1, (len(kpt1) * 2, 3))
     assert tetra.equivalent_groups(kpt1, kpt1p, kptsym, iq) == []
     np.testing.assert_array_equal(tetra.average_equivalent(wr, 1, 2, kpt1, kpt1p, kptsym, iq), wr)
+
+
+def _degenerate_bands(g, nk, nb, sign, copies, local=()):
+    """Sorted bands (nk, nb) of one sign, bands copies[i] exactly equal to
+    the band below them on every k, and at the k points in `local` one more
+    degenerate pair (bands 3 and 4)."""
+    e = np.sort(sign * g.uniform(0.1, 1.2, (nk, nb)), axis=1)
+    for b in copies:
+        e[:, b] = e[:, b - 1]
+    for k in local:
+        e[k, 4] = e[k, 3]
+    return e
+
+
+@pytest.mark.parametrize("w1,w2", [((2, 3), (1, 3)), ((1, 4), (2, 2)), ((3, 3), (0, 2))])
+def test_cut_windows_give_the_rows_of_whole_groups(w1, w2):
+    """Windows that cut degenerate groups give, in every window row, what a
+    window holding the whole groups gives after the ordinary averaging: the
+    purpose of the reference's widening.  Full mesh, degenerate pairs on
+    every k (bands 1-2 occupied, 2-3 unoccupied) and on a few k only."""
+    g = np.random.default_rng(61)
+    nkpt3 = (2, 3, 2)
+    nk = 12
+    e1 = _degenerate_bands(g, nk, 6, -1.0, copies=(2,), local=(0, 5, 7))
+    e2 = _degenerate_bands(g, nk, 6, 1.0, copies=(3,), local=(1, 5))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.04 * (1.25 ** np.arange(20) - 1)
+    got = tetra.tetrahedron5_weights_bands(nkpt3, RL_T, e1, e2, d1, d2, *w1, *w2, 0.0, frq)
+    full = tetra.tetrahedron5_weights_metal(nkpt3, RL_T, e1, e2, 0.0, frq)
+    wide = tetra.average_degenerate(full, nk, 6, 6, d1, d2).reshape(nk, 6, 6, -1)
+    want = wide[:, w1[0]:w1[1] + 1, w2[0]:w2[1] + 1].reshape(got.shape)
+    np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
+    # the windows really cut groups (at some k) for these cases
+    cut = any(tetra._group(d1[k], w1[0])[0] < w1[0] or tetra._group(d1[k], w1[1])[1] > w1[1]
+              or tetra._group(d2[k], w2[0])[0] < w2[0] or tetra._group(d2[k], w2[1])[1] > w2[1]
+              for k in range(nk))
+    assert cut
+
+
+def test_cut_windows_on_reduced_sets():
+    """The same on the rows of a reduced k set, star sums and TETRAF, with
+    the degeneracies shared by every member of a star."""
+    g = np.random.default_rng(62)
+    nkpt3 = (2, 2, 2)
+    nk = 8
+    e1 = _degenerate_bands(g, nk, 5, -1.0, copies=(2,))
+    e2 = _degenerate_bands(g, nk, 5, 1.0, copies=(1, 4))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.04 * (1.25 ** np.arange(16) - 1)
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3])
+    kpt1 = np.array([0, 1, 3, 7])
+    full = tetra.tetrahedron5_weights_metal(nkpt3, np.eye(3), e1, e2, 0.0, frq)
+    for tf in (False, True):
+        got = tetra.tetrahedron5_weights_bands(nkpt3, np.eye(3), e1, e2, d1, d2, 2, 3, 1, 3, 0.0,
+                                               frq, kpt1, kpt1p, 4, tetraf=tf)
+        red = (tetra.reduce_tetraf(full, nk, 5, 5, kpt1, kpt1p, 4) if tf
+               else tetra.reduce_to_parents(full, nk, 5, 5, kpt1p, 4))
+        wide = tetra.average_degenerate(red, 4, 5, 5, d1[kpt1], d2[kpt1]).reshape(4, 5, 5, -1)
+        want = wide[:, 2:4, 1:4].reshape(got.shape)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-15 * np.abs(want).max())
+
+
+def test_uncut_windows_equal_the_plain_chain():
+    g = np.random.default_rng(63)
+    nkpt3 = (2, 2, 3)
+    nk = 12
+    e1 = _degenerate_bands(g, nk, 5, -1.0, copies=(2,))
+    e2 = _degenerate_bands(g, nk, 5, 1.0, copies=(2,))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.04 * (1.25 ** np.arange(16) - 1)
+    got = tetra.tetrahedron5_weights_bands(nkpt3, np.eye(3), e1, e2, d1, d2, 1, 2, 1, 3, 0.0, frq)
+    w = tetra.tetrahedron5_weights_metal(nkpt3, np.eye(3), e1[:, 1:3], e2[:, 1:4], 0.0, frq)
+    want = tetra.average_degenerate(w, nk, 2, 3, d1[:, 1:3] - 1, d2[:, 1:4] - 1)
+    np.testing.assert_array_equal(got, want)
