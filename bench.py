@@ -351,7 +351,7 @@ def theta_leg(pipe, a, sb=3, reps=3):
         e.destroy()
     pb = 16 * pipe.n * sy.ngptm + sy.cpw.nbytes  # IPW columns of M written, cpw read once
     gbs = pb / (ms * 1e6)
-    return {"kernel": "pw_inverse_kernel + pw_pair_fused_kernel (caller grid)",
+    return {"kernel": "pw_inverse_kernel + pw_pair_joint_kernel<36,36> (caller grid)",
             "grid": list(N), "ms": round(ms, 3), "achieved": round(gbs, 1), "peak": 8000.0,
             "unit": "GB/s", "frac": round(gbs / 8000.0, 4), "bytes_per_call": int(pb),
             "note": "IPW products with theta(r) on the 36^3 caller grid (SPEX's default FULLPW), "
