@@ -333,8 +333,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(
   static_assert(N1 % 4 == 0 && N2 % 4 == 0 && N1 >= PW_BOX && N2 >= PW_BOX, "grid");
   constexpr int G = joint_planes(N2), RB = (G * N2 + 15) / 16, KX = N1 / 4, KY = N2 / 4;
   constexpr int GI = G % 2 == 0 ? 2 : 1;  // planes per y-pass batch
-  // z twiddles, N3 x PW_BOX (dynamic: 6 KB for N3 = 24 instead of a 64-plane
-  // table, so four workgroups fit in a CU's LDS)
+  // z twiddles, (N3 + G) x PW_BOX, zero past N3 (dynamic: sized to the grid
+  // instead of a 64-plane table)
   extern __shared__ double2 tz[];
   __shared__ double2 Qs[NT / 64][GI * PW_BOX * PW_BOX];
   __shared__ double2 F1s[KX][64], F2s[KY][64];
@@ -349,11 +349,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(
   const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
   const int b2 = (r / ngrp) * 8 + xcd, grp = r % ngrp;
   if (b2 >= nunocc) return;  // whole workgroup, before any barrier
-  for (int e = tid; e < N3 * PW_BOX; e += NT) {
+  for (int e = tid; e < (N3 + G) * PW_BOX; e += NT) {
     const int z = e / PW_BOX, g = e % PW_BOX;
-    tz[e] = twid((long)(g - PW_OFF) * z, N3, -1.0);
+    tz[e] = z < N3 ? twid((long)(g - PW_OFF) * z, N3, -1.0) : make_double2(0.0, 0.0);
   }
-  for (int e = tid; e < 64 * PW_FOUT; e += NT) obs[e] = e < ngptm ? oidx[e] : -1;
+  for (int e = tid; e < 64 * PW_FOUT; e += NT) obs[e] = e < ngptm ? oidx[e] : 0;
   for (int e = tid; e < KX * 64; e += NT) {
     const int kk = e >> 6, ln = e & 63;
     F1s[kk][ln] = twid((long)((ln & 15) - PW_OFF) * (4 * kk + (ln >> 4)), N1, -1.0);
@@ -449,21 +449,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(
           Q[zi * PW_BOX * PW_BOX + (lk + 4 * r) * PW_BOX + lr] = make_double2(qr[zi][r], qi[zi][r]);
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      const int nz = min(GI, N3 - (z0 + zb));  // planes past N3 carry no data
+      // Branch-free: slots past ngptm hold index 0 and accumulate values
+      // that are never stored, so the LDS reads of all slots can be in
+      // flight together.
+      // Planes past N3 hold Q = 0 (their x-pass rows are masked) and the
+      // twiddle table is zero-padded past N3, so they add zero.
+      // Slots are taken four at a time, their LDS reads issued together.
+      const double2 *tzp = tz + (z0 + zb) * PW_BOX;
 #pragma unroll
-      for (int j = 0; j < PW_FOUT; j++) {
-        const int o = obs[l + 64 * j];
-        if (o >= 0) {
+      for (int j0 = 0; j0 < PW_FOUT; j0 += 4) {
+        double2 T[4][GI], V[4][GI];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int o = obs[l + 64 * (j0 + j)];
+          const int gz = o >> 8, c = o & 255;
 #pragma unroll
           for (int zi = 0; zi < GI; zi++) {
-            if (zi < nz) {
-              const double2 t = tz[(z0 + zb + zi) * PW_BOX + (o >> 8)];
-              const double2 v = Q[zi * PW_BOX * PW_BOX + (o & 255)];
-              acc[j].x += t.x * v.x - t.y * v.y;
-              acc[j].y += t.x * v.y + t.y * v.x;
-            }
+            T[j][zi] = tzp[zi * PW_BOX + gz];
+            V[j][zi] = Q[zi * PW_BOX * PW_BOX + c];
           }
         }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int zi = 0; zi < GI; zi++) {
+            const double2 t = T[j][zi], v = V[j][zi];
+            double2 &a = acc[j0 + j];
+            a.x = fma(t.x, v.x, a.x);
+            a.x = fma(-t.y, v.y, a.x);
+            a.y = fma(t.x, v.y, a.y);
+            a.y = fma(t.y, v.x, a.y);
+          }
       }
       __builtin_amdgcn_wave_barrier();  // Q is rewritten by the next planes
     }
@@ -538,7 +554,7 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
          N1 == 48)) {
       const int ngrp = (nocc + 3) / 4;
       const dim3 grid((unsigned)(((nunocc + 7) / 8) * 8 * ngrp));
-      const size_t tzb = (size_t)N3 * PW_BOX * sizeof(double2);
+      const size_t tzb = (size_t)(N3 + 4) * PW_BOX * sizeof(double2);  // G <= 4 padding planes
       const int *ox = oidx_dev + (long)ik * p->ngptm;
       const long tt = (long)ik * nocc * nunocc;
       switch (N1) {
