@@ -2,7 +2,9 @@
  * spexcor -- the spex.cor checkpoint file of SPEX (C ABI, host side).
  *
  * Replaces writecor / readcor (correlation.f:1636-1952) with the file
- * layout of COR version 1 and the HDF5 wrapper's conventions (Hwrapper.f):
+ * layout of COR version 1 (version 0 files are read too) and the HDF5
+ * wrapper's conventions (Hwrapper.f, including the division of datasets
+ * above its chunk limit):
  * one group per k point named by the k vector in F12.10 format, the
  * matrices of each frequency as datasets "matrix_<i>" (packed upper) and
  * "matrixc_<i>" (full), complex arrays as double datasets with a leading
@@ -50,6 +52,27 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
                  double *coul, double *ctrafo, int *cblock, int *oselfc, double contour[4],
                  double *plasma, double *freq, double *matrix, double *head, double *wing,
                  double *freqc, double *matrixc, double *headc, double *wingc);
+
+/* The chunk limit of the HDF5 wrapper (MaxChunk, cppmacro.h:143-144) in
+ * bytes, 0 for the default 2^31-1: arrays of more than limit/(element
+ * bytes) elements are stored as "<name>_0" and "<name>_1", divided again
+ * until each part fits (Hwrapper.f:413-426).  Reads follow the same rule,
+ * so a file must be read with the limit it was written with.             */
+int spexcor_set_max_chunk(size_t bytes);
+
+/* The name writecor writes to: prefix.cor, or prefix.cor.<color> for an
+ * MPI color > 0 (correlation.f:1828).                                     */
+int spexcor_file_name(const char *prefix, int color, char *fname, size_t len);
+
+/* readcor's choice of file for k point kvec (correlation.f:1655-1672): the
+ * file prefix.cor.<n> named by prefix.cor.map/<k point> when the map
+ * directory holds "present", else prefix.cor.                             */
+int spexcor_locate(const char *prefix, const double kvec[3], char *fname, size_t len);
+
+/* writecor's map bookkeeping after a write (correlation.f:1911-1946): for
+ * color > 0 the map directory, its "present" file and the entry holding
+ * color are created; for color 0 an existing entry is removed.            */
+int spexcor_map_record(const char *prefix, const double kvec[3], int color);
 
 #ifdef __cplusplus
 }

@@ -19,13 +19,29 @@
 // writes a rank-1 space of size 1).  A complex array is a double dataset
 // with a leading dimension 2 (real, imaginary: Hwrapper.f hdf_rdwr_c), and
 // Fortran dimensions appear reversed in HDF5's row-major view: a complex
-// (nbas, nbas) matrix is the double dataset {nbas, nbas, 2}.  Datasets are
-// split only above 2^31-1 elements (MaxChunk = huge(0)), larger than any
-// here.
+// (nbas, nbas) matrix is the double dataset {nbas, nbas, 2}.
+//
+// Large datasets are divided as Hwrapper.f's hdf_rdwr divides them
+// (Hwrapper.f:413-426, array_divide :573-585; cppmacro.h:143-144): an array
+// of more than (2^31-1)/(element bytes) elements is written as "<name>_0"
+// and "<name>_1", split at (ub-lb)/2+lb of its last Fortran dimension with
+// more than one element, each half divided again by the same rule; reads
+// follow the same rule.  The halves of a column-major array split along its
+// slowest non-trivial dimension are contiguous.
+//
+// COR version 0 files (no version attribute, or 0) are read as readcor reads
+// them: one "matrix" dataset (all nfreq packed matrices) and one "matrixc"
+// dataset (correlation.f:1799-1807).  The spex.cor.map directory of MPI
+// runs (correlation.f:1655-1672, 1911-1946) maps a k point to the file
+// spex.cor.<n> that holds it.
 #include "../../include/spexcor.h"
 
 #include <hdf5.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +51,8 @@
 namespace {
 
 thread_local std::string g_err;
+// MaxChunk of the HDF5 wrapper in bytes (huge(0)); settable for tests
+size_t g_max_chunk = 2147483647;
 
 int fail(const std::string &m) {
   g_err = m;
@@ -101,24 +119,84 @@ bool read_attr(hid_t loc, const char *name, hid_t type, void *buf) {
   return att >= 0 && H5Aread(att, type, buf) >= 0;
 }
 
+bool exists(hid_t loc, const char *name) { return H5Lexists(loc, name, H5P_DEFAULT) > 0; }
+
 // datasets (dims in HDF5 order: the Fortran dimensions reversed) ---------
+// The division of an array that exceeds the wrapper's chunk limit: the
+// HDF5 dimension j (the last Fortran dimension with more than one element)
+// and the extent h of the first half, or j = -1 if the array is written
+// whole.
+int divide(const std::vector<hsize_t> &dims, size_t esize, hsize_t &h) {
+  hsize_t n = 1;
+  for (hsize_t d : dims) n *= d;
+  if (n <= (hsize_t)(g_max_chunk / esize)) return -1;
+  for (size_t j = 0; j < dims.size(); j++)
+    if (dims[j] > 1) {
+      h = (dims[j] - 1) / 2 + 1;  // i - lb + 1 with i = (ub - lb)/2 + lb
+      return (int)j;
+    }
+  return -1;
+}
+
+// skip_existing: hdf_rdwr's mode 2 (an existing dataset is left alone),
+// applied to each part as the reference applies it after dividing.
 bool write_set(hid_t loc, const std::string &name, hid_t type, const void *buf,
-               std::vector<hsize_t> dims) {
+               std::vector<hsize_t> dims, bool skip_existing = false) {
+  const size_t es = H5Tget_size(type);
+  hsize_t h = 0;
+  const int j = divide(dims, es, h);
+  if (j >= 0) {
+    hsize_t inner = 1;
+    for (size_t k = j + 1; k < dims.size(); k++) inner *= dims[k];
+    std::vector<hsize_t> d0 = dims, d1 = dims;
+    d0[j] = h;
+    d1[j] = dims[j] - h;
+    return write_set(loc, name + "_0", type, buf, d0, skip_existing) &&
+           write_set(loc, name + "_1", type, (const char *)buf + es * h * inner, d1,
+                     skip_existing);
+  }
+  if (skip_existing && exists(loc, name.c_str())) return true;
   H spc(H5Screate_simple((int)dims.size(), dims.data(), nullptr), H5Sclose);
   H set(H5Dcreate2(loc, name.c_str(), type, spc, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT),
         H5Dclose);
   return spc >= 0 && set >= 0 && H5Dwrite(set, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) >= 0;
 }
 
-bool read_set(hid_t loc, const std::string &name, hid_t type, void *buf, hsize_t expect) {
+bool read_set(hid_t loc, const std::string &name, hid_t type, void *buf,
+              std::vector<hsize_t> dims) {
+  const size_t es = H5Tget_size(type);
+  hsize_t h = 0;
+  const int j = divide(dims, es, h);
+  if (j >= 0) {
+    hsize_t inner = 1;
+    for (size_t k = j + 1; k < dims.size(); k++) inner *= dims[k];
+    std::vector<hsize_t> d0 = dims, d1 = dims;
+    d0[j] = h;
+    d1[j] = dims[j] - h;
+    return read_set(loc, name + "_0", type, buf, d0) &&
+           read_set(loc, name + "_1", type, (char *)buf + es * h * inner, d1);
+  }
   H set(H5Dopen2(loc, name.c_str(), H5P_DEFAULT), H5Dclose);
   if (set < 0) return false;
   H spc(H5Dget_space(set), H5Sclose);
+  hsize_t expect = 1;
+  for (hsize_t d : dims) expect *= d;
   if (H5Sget_simple_extent_npoints(spc) != (hssize_t)expect) return false;
   return H5Dread(set, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) >= 0;
 }
 
-bool exists(hid_t loc, const char *name) { return H5Lexists(loc, name, H5P_DEFAULT) > 0; }
+int file_version(hid_t file) {
+  int version = 0;  // default (correlation.f:1679)
+  if (H5Aexists(file, "version") > 0) read_attr(file, "version", H5T_NATIVE_INT, &version);
+  return version;
+}
+
+std::string map_dir(const char *prefix) { return std::string(prefix) + ".cor.map"; }
+
+bool file_exists(const std::string &f) {
+  struct stat st;
+  return stat(f.c_str(), &st) == 0;
+}
 
 }  // namespace
 
@@ -180,8 +258,7 @@ int spexcor_write(const char *fname, const double kvec[3], int typ, const char *
   } else if (!write_attr(grp, "nbasm", H5T_NATIVE_INT, &nbasm, 1)) {
     return fail("spexcor_write: cannot write nbasm");
   }
-  if (!exists(grp, "coulomb") &&
-      !write_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, {npm, 2}))
+  if (!write_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, {npm, 2}, true))
     return fail("spexcor_write: cannot write coulomb");
   // file attributes (mode 2)
   const int nn[2] = {nfreq, nfreqc};
@@ -239,9 +316,9 @@ int spexcor_query(const char *fname, const double kvec[3], int *typ, int *nbasm,
   H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
   H file(H5Fopen(fname, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
   if (file < 0) return fail(std::string("File ") + fname + " not found.");
-  int version = 0;
-  if (H5Aexists(file, "version") > 0) read_attr(file, "version", H5T_NATIVE_INT, &version);
-  if (version != 1) return fail("COR version number unknown or not supported: " + std::to_string(version));
+  const int version = file_version(file);
+  if (version != 0 && version != 1)
+    return fail("COR version number unknown: " + std::to_string(version));
   const std::string gname = group_name(kvec);
   *typ = -1;  // the reference's "nothing for this k point"
   if (!exists(file, gname.c_str())) return 0;
@@ -282,10 +359,11 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
     if (H5Aread(att, typ_s, buf.data()) < 0) return fail("spexcor_read: label");
     std::memcpy(label, buf.data(), len < 6 ? len : 6);
   }
-  if ((coulomb && !read_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, 2 * npm)) ||
-      (coul && !read_set(grp, "coul", H5T_NATIVE_DOUBLE, coul, nb)) ||
-      (ctrafo && !read_set(grp, "ctrafo", H5T_NATIVE_DOUBLE, ctrafo, 2 * nb * nbasm)) ||
-      (cblock && ncblock && !read_set(grp, "cblock", H5T_NATIVE_INT, cblock, ncblock)) ||
+  const hsize_t nbm = nbasm;
+  if ((coulomb && !read_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, {npm, 2})) ||
+      (coul && !read_set(grp, "coul", H5T_NATIVE_DOUBLE, coul, {nb})) ||
+      (ctrafo && !read_set(grp, "ctrafo", H5T_NATIVE_DOUBLE, ctrafo, {nb, nbm, 2})) ||
+      (cblock && ncblock && !read_set(grp, "cblock", H5T_NATIVE_INT, cblock, {(hsize_t)ncblock})) ||
       (oselfc && !read_attr(grp, "oselfc", H5T_NATIVE_INT, oselfc)) ||
       (contour && !read_attr(grp, "contour", H5T_NATIVE_DOUBLE, contour)) ||
       (plasma && gamma && !read_attr(grp, "plasma", H5T_NATIVE_DOUBLE, plasma)))
@@ -294,21 +372,88 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
     return fail("spexcor_read: freq");
   if (nfreqc && freqc && !read_attr(file, "freqc", H5T_NATIVE_DOUBLE, freqc))
     return fail("spexcor_read: freqc");
-  for (int i = 0; matrix && i < nfreq; i++)
-    if (!read_set(grp, "matrix_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE, matrix + 2 * np * i,
-                  2 * np))
-      return fail("spexcor_read: matrix_" + std::to_string(i + 1));
-  for (int i = 0; matrixc && i < nfreqc; i++)
-    if (!read_set(grp, "matrixc_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
-                  matrixc + 2 * nb * nb * i, 2 * nb * nb))
-      return fail("spexcor_read: matrixc_" + std::to_string(i + 1));
+  const hsize_t nf = nfreq, nfc = nfreqc;
+  if (file_version(file) == 0) {
+    // one dataset per matrix kind (correlation.f:1799-1807)
+    if (matrix && nfreq && !read_set(grp, "matrix", H5T_NATIVE_DOUBLE, matrix, {nf, np, 2}))
+      return fail("spexcor_read: matrix");
+    if (matrixc && nfreqc &&
+        !read_set(grp, "matrixc", H5T_NATIVE_DOUBLE, matrixc, {nfc, nb, nb, 2}))
+      return fail("spexcor_read: matrixc");
+  } else {
+    for (int i = 0; matrix && i < nfreq; i++)
+      if (!read_set(grp, "matrix_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
+                    matrix + 2 * np * i, {np, 2}))
+        return fail("spexcor_read: matrix_" + std::to_string(i + 1));
+    for (int i = 0; matrixc && i < nfreqc; i++)
+      if (!read_set(grp, "matrixc_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
+                    matrixc + 2 * nb * nb * i, {nb, nb, 2}))
+        return fail("spexcor_read: matrixc_" + std::to_string(i + 1));
+  }
   if (gamma) {
-    if ((nfreq && head && !read_set(grp, "head", H5T_NATIVE_DOUBLE, head, 12 * (hsize_t)nfreq)) ||
-        (nfreq && wing && !read_set(grp, "wing", H5T_NATIVE_DOUBLE, wing, 6 * nb * nfreq)) ||
-        (nfreqc && headc && !read_set(grp, "headc", H5T_NATIVE_DOUBLE, headc, 18 * (hsize_t)nfreqc)) ||
-        (nfreqc && wingc && !read_set(grp, "wingc", H5T_NATIVE_DOUBLE, wingc, 12 * nb * nfreqc)))
+    if ((nfreq && head && !read_set(grp, "head", H5T_NATIVE_DOUBLE, head, {nf, 6, 2})) ||
+        (nfreq && wing && !read_set(grp, "wing", H5T_NATIVE_DOUBLE, wing, {nf, nb, 3, 2})) ||
+        (nfreqc && headc && !read_set(grp, "headc", H5T_NATIVE_DOUBLE, headc, {nfc, 3, 3, 2})) ||
+        (nfreqc && wingc && !read_set(grp, "wingc", H5T_NATIVE_DOUBLE, wingc, {nfc, 2, nb, 3, 2})))
       return fail("spexcor_read: head or wings");
   }
+  return 0;
+}
+
+int spexcor_set_max_chunk(size_t bytes) {
+  g_max_chunk = bytes ? bytes : 2147483647;
+  return 0;
+}
+
+int spexcor_file_name(const char *prefix, int color, char *fname, size_t len) {
+  if (!prefix || !fname || !len || color < 0) return fail("spexcor_file_name: bad arguments");
+  std::string f = std::string(prefix) + ".cor";
+  if (color > 0) f += "." + std::to_string(color);
+  if (f.size() + 1 > len) return fail("spexcor_file_name: buffer too small");
+  std::memcpy(fname, f.c_str(), f.size() + 1);
+  return 0;
+}
+
+int spexcor_locate(const char *prefix, const double kvec[3], char *fname, size_t len) {
+  if (!prefix || !kvec || !fname || !len) return fail("spexcor_locate: bad arguments");
+  int color = 0;
+  const std::string dir = map_dir(prefix);
+  if (file_exists(dir + "/present")) {
+    const std::string entry = dir + "/" + group_name(kvec);
+    if (file_exists(entry)) {
+      FILE *f = std::fopen(entry.c_str(), "r");
+      if (!f || std::fscanf(f, "%d", &color) != 1) {
+        if (f) std::fclose(f);
+        return fail("Could not read corfile handle.");
+      }
+      std::fclose(f);
+      if (color <= 0) return fail("corfile handle non-positive.");
+    }
+  }
+  return spexcor_file_name(prefix, color, fname, len);
+}
+
+int spexcor_map_record(const char *prefix, const double kvec[3], int color) {
+  if (!prefix || !kvec || color < 0) return fail("spexcor_map_record: bad arguments");
+  const std::string dir = map_dir(prefix), present = dir + "/present";
+  const std::string entry = dir + "/" + group_name(kvec);
+  if (color > 0) {
+    if (mkdir(dir.c_str(), 0777) != 0 && errno != EEXIST)
+      return fail("Could not create directory " + dir);
+    if (!file_exists(present)) {
+      FILE *f = std::fopen(present.c_str(), "a");
+      if (!f) return fail("Could not create empty file in directory " + dir);
+      std::fclose(f);
+    }
+    FILE *f = std::fopen(entry.c_str(), "w");
+    if (!f) return fail("Could not write " + entry);
+    std::fprintf(f, "%12d\n", color);  // list-directed integer, as write(iunit,*)
+    std::fclose(f);
+    return 0;
+  }
+  // a k point written to spex.cor itself: a stale entry is removed
+  if (file_exists(present) && file_exists(entry) && unlink(entry.c_str()) != 0)
+    return fail("Could not delete " + entry);
   return 0;
 }
 

@@ -31,7 +31,12 @@ def lib() -> C.CDLL:
                                     _vp]
         L.spexcor_query.argtypes = [C.c_char_p, _vp] + [C.POINTER(I)] * 6
         L.spexcor_read.argtypes = [C.c_char_p, _vp, C.c_char_p] + [_vp] * 15
-        for f in (L.spexcor_write, L.spexcor_query, L.spexcor_read):
+        L.spexcor_set_max_chunk.argtypes = [C.c_size_t]
+        L.spexcor_file_name.argtypes = [C.c_char_p, I, C.c_char_p, C.c_size_t]
+        L.spexcor_locate.argtypes = [C.c_char_p, _vp, C.c_char_p, C.c_size_t]
+        L.spexcor_map_record.argtypes = [C.c_char_p, _vp, I]
+        for f in (L.spexcor_write, L.spexcor_query, L.spexcor_read, L.spexcor_set_max_chunk,
+                  L.spexcor_file_name, L.spexcor_locate, L.spexcor_map_record):
             f.restype = I
         _LIB = L
     return _LIB
@@ -110,3 +115,30 @@ def read(fname, kvec, gamma=False) -> dict:
                               g("wing"), g("freqc"), g("matrixc"), g("headc"), g("wingc")))
     out.update(q, label=lab.raw[:6].decode(), oselfc=osf.value, plasma=pl.value)
     return out
+
+
+def set_max_chunk(nbytes: int = 0) -> None:
+    """The HDF5 wrapper's chunk limit in bytes (0: the default 2^31-1);
+    larger arrays are stored divided into "<name>_0" / "<name>_1"."""
+    _check(lib().spexcor_set_max_chunk(int(nbytes)))
+
+
+def file_name(prefix, color: int = 0) -> str:
+    """The file writecor writes to: prefix.cor or prefix.cor.<color>."""
+    buf = C.create_string_buffer(4096)
+    _check(lib().spexcor_file_name(str(prefix).encode(), color, buf, 4096))
+    return buf.value.decode()
+
+
+def locate(prefix, kvec) -> str:
+    """readcor's file for k point kvec (through prefix.cor.map when present)."""
+    buf = C.create_string_buffer(4096)
+    _check(lib().spexcor_locate(str(prefix).encode(), _p(np.ascontiguousarray(kvec, np.float64)),
+                                buf, 4096))
+    return buf.value.decode()
+
+
+def map_record(prefix, kvec, color: int) -> None:
+    """writecor's map bookkeeping after writing k point kvec with this color."""
+    _check(lib().spexcor_map_record(str(prefix).encode(),
+                                    _p(np.ascontiguousarray(kvec, np.float64)), color))

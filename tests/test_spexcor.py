@@ -117,3 +117,159 @@ def test_group_name_is_fortran_f12_10(tmp_path):
     # F12.10: 10 decimals, width 12 (the leading zero of a negative value dropped)
     assert "/0.1250000000,0.3333333333,-.2500000000" in out, out
     assert spexcor.query(f, [0.125, 1.0 / 3.0, -0.25])["type"] == 1
+
+
+def _divided_parts(name, fshape, limit_elems):
+    """Hwrapper.f:413-426 with array_divide (:573-585), restated on Fortran
+    shapes: an array of more than limit_elems elements is written as
+    name_0 (lb .. i of its last dimension with more than one element,
+    i = (ub - lb)/2 + lb) and name_1 (i+1 .. ub), each divided again.
+    Returns [(dataset name, Fortran shape)] in memory order."""
+    if int(np.prod(fshape)) <= limit_elems:
+        return [(name, tuple(fshape))]
+    d = max(k for k in range(len(fshape)) if fshape[k] > 1)
+    n = fshape[d]
+    h = (n - 1) // 2 + 1
+    s0, s1 = list(fshape), list(fshape)
+    s0[d], s1[d] = h, n - h
+    return (_divided_parts(name + "_0", s0, limit_elems)
+            + _divided_parts(name + "_1", s1, limit_elems))
+
+
+@needs_lib
+@pytest.mark.skipif(not Path(H5DUMP).exists(), reason="h5dump not available")
+def test_large_datasets_are_divided_as_hwrapper(tmp_path):
+    """With the chunk limit lowered to 256 bytes (32 doubles), every array
+    above it is stored in the parts the reference's rule gives (names and
+    HDF5 shapes from h5dump), the parts hold consecutive pieces of the
+    array, and the round trip is exact."""
+    g = np.random.default_rng(7)
+    f = tmp_path / "spex.cor"
+    nbasm, nbas, nf, nfc = 9, 7, 3, 2
+    d = _case(g, nbasm, nbas, nf, nfc)
+    spexcor.set_max_chunk(256)
+    try:
+        spexcor.write(f, [0, 0, 0], 3, "GW", gamma=True, plasma=0.5, **d)
+        r = spexcor.read(f, [0, 0, 0], gamma=True)
+    finally:
+        spexcor.set_max_chunk(0)
+    for key in d:
+        np.testing.assert_array_equal(r[key].reshape(np.shape(d[key])), d[key], err_msg=key)
+    out = re.sub(r"\s+", " ", subprocess.run([H5DUMP, "-H", str(f)], capture_output=True,
+                                             text=True).stdout)
+    np_ = nbas * (nbas + 1) // 2
+    shapes = {"coulomb": (2, nbasm * (nbasm + 1) // 2), "ctrafo": (2, nbasm, nbas),
+              "matrix_1": (2, np_), "matrixc_1": (2, nbas, nbas), "wing": (2, 3, nbas, nf),
+              "wingc": (2, 3, nbas, 2, nfc), "head": (2, 6, nf)}
+    for name, fs in shapes.items():
+        parts = _divided_parts(name, fs, 32)
+        assert len(parts) > 1, name
+        for pname, ps in parts:
+            dims = ", ".join(str(x) for x in reversed(ps))
+            assert re.search(rf'DATASET "{pname}" {{ DATATYPE H5T_IEEE_F64LE DATASPACE SIMPLE '
+                             rf'{{ \( {re.escape(dims)} \)', out), (pname, dims)
+        assert f'DATASET "{name}" ' not in out or len(parts) == 1
+    # the parts of matrixc_1 are consecutive column blocks of the matrix
+    offset = 0
+    flat = d["matrixc"][:, :, 0].ravel(order="F")
+    for pname, ps in _divided_parts("matrixc_1", (2, nbas, nbas), 32):
+        txt = subprocess.run([H5DUMP, "-d", f"/0.0000000000,0.0000000000,0.0000000000/{pname}",
+                              "-y", "-w", "0", "-m", "%.17g", str(f)], capture_output=True,
+                             text=True).stdout
+        body = txt.split("DATA {", 1)[1].split("}", 1)[0]
+        vals = np.array([float(x) for x in re.findall(r"[-+0-9.eE]+", body)])
+        n = int(np.prod(ps)) // 2
+        np.testing.assert_array_equal(vals[0::2] + 1j * vals[1::2], flat[offset:offset + n])
+        offset += n
+    assert offset == flat.size
+
+
+def _hdf5():
+    """libhdf5 through ctypes (test-side file surgery)."""
+    import ctypes as C
+    L = C.CDLL("/opt/conda/lib/libhdf5.so")
+    L.H5open()
+    hid = C.c_int64
+    for fn, args in (("H5Fopen", [C.c_char_p, C.c_uint, hid]), ("H5Gopen2", [hid, C.c_char_p, hid]),
+                     ("H5Adelete", [hid, C.c_char_p]), ("H5Ldelete", [hid, C.c_char_p, hid]),
+                     ("H5Screate_simple", [C.c_int, C.c_void_p, C.c_void_p]),
+                     ("H5Dcreate2", [hid, C.c_char_p, hid, hid, hid, hid, hid]),
+                     ("H5Dwrite", [hid, hid, hid, hid, hid, C.c_void_p]),
+                     ("H5Dclose", [hid]), ("H5Sclose", [hid]), ("H5Gclose", [hid]),
+                     ("H5Fclose", [hid])):
+        getattr(L, fn).argtypes = args
+    for fn in ("H5Fopen", "H5Gopen2", "H5Screate_simple", "H5Dcreate2"):
+        getattr(L, fn).restype = hid
+    return L, hid.in_dll(L, "H5T_NATIVE_DOUBLE_g").value
+
+
+@needs_lib
+@pytest.mark.skipif(not Path("/opt/conda/lib/libhdf5.so").exists(), reason="libhdf5 not found")
+def test_reads_cor_version_0(tmp_path):
+    """A version-0 file (no version attribute; all frequencies in one
+    "matrix" and one "matrixc" dataset, correlation.f:1799-1807) made from a
+    version-1 file by moving the data: read back unchanged."""
+    g = np.random.default_rng(8)
+    f = tmp_path / "spex.cor"
+    nbasm, nbas, nf, nfc = 9, 7, 3, 2
+    d = _case(g, nbasm, nbas, nf, nfc)
+    spexcor.write(f, [0, 0, 0], 3, "GW", gamma=True, plasma=0.5, **d)
+    L, dbl = _hdf5()
+    import ctypes as C
+    fid = L.H5Fopen(str(f).encode(), 1, 0)
+    gid = L.H5Gopen2(fid, b"0.0000000000,0.0000000000,0.0000000000", 0)
+    assert fid >= 0 and gid >= 0
+    assert L.H5Adelete(fid, b"version") >= 0
+    for i in range(nf):
+        assert L.H5Ldelete(gid, f"matrix_{i + 1}".encode(), 0) >= 0
+    for i in range(nfc):
+        assert L.H5Ldelete(gid, f"matrixc_{i + 1}".encode(), 0) >= 0
+    np_ = nbas * (nbas + 1) // 2
+    for name, arr, dims in (("matrix", d["matrix"], (nf, np_, 2)),
+                            ("matrixc", d["matrixc"], (nfc, nbas, nbas, 2))):
+        buf = np.ascontiguousarray(np.asfortranarray(arr).ravel(order="F").view(np.float64))
+        dd = (C.c_uint64 * len(dims))(*dims)
+        sid = L.H5Screate_simple(len(dims), dd, None)
+        did = L.H5Dcreate2(gid, name.encode(), dbl, sid, 0, 0, 0)
+        assert did >= 0 and L.H5Dwrite(did, dbl, 0, 0, 0, buf.ctypes.data) >= 0
+        L.H5Dclose(did)
+        L.H5Sclose(sid)
+    L.H5Gclose(gid)
+    L.H5Fclose(fid)
+    r = spexcor.read(f, [0, 0, 0], gamma=True)
+    for key in d:
+        np.testing.assert_array_equal(r[key].reshape(np.shape(d[key])), d[key], err_msg=key)
+
+
+@needs_lib
+def test_cor_map_directory(tmp_path):
+    """readcor's file choice and writecor's bookkeeping of spex.cor.map
+    (correlation.f:1655-1672, 1911-1946): an entry names spex.cor.<n>; a k
+    point written by the root (color 0) drops its stale entry."""
+    pre = str(tmp_path / "spex")
+    k = [0.25, 0.5, 0.0]
+    assert spexcor.file_name(pre) == pre + ".cor"
+    assert spexcor.file_name(pre, 3) == pre + ".cor.3"
+    assert spexcor.locate(pre, k) == pre + ".cor"
+    spexcor.map_record(pre, k, 3)
+    entry = tmp_path / "spex.cor.map" / "0.2500000000,0.5000000000,0.0000000000"
+    assert (tmp_path / "spex.cor.map" / "present").exists() and entry.exists()
+    assert int(entry.read_text()) == 3
+    assert spexcor.locate(pre, k) == pre + ".cor.3"
+    assert spexcor.locate(pre, [0.5, 0.5, 0.0]) == pre + ".cor"  # no entry
+    spexcor.map_record(pre, k, 0)
+    assert not entry.exists() and spexcor.locate(pre, k) == pre + ".cor"
+    entry.write_text("x\n")
+    with pytest.raises(spexcor.CorError, match="Could not read corfile handle"):
+        spexcor.locate(pre, k)
+    entry.write_text("  0\n")
+    with pytest.raises(spexcor.CorError, match="non-positive"):
+        spexcor.locate(pre, k)
+    # a data round trip through the located file
+    g = np.random.default_rng(9)
+    d = {kk: v for kk, v in _case(g).items() if kk not in ("head", "wing", "headc", "wingc")}
+    spexcor.map_record(pre, k, 2)
+    fname = spexcor.file_name(pre, 2)
+    spexcor.write(fname, k, 1, "SUS", **d)
+    r = spexcor.read(spexcor.locate(pre, k), k)
+    np.testing.assert_array_equal(r["matrix"], d["matrix"])
