@@ -173,3 +173,49 @@ def test_ozaki_positive_weights_and_large_k(ctx):
     mx, fro = normwise(got, ref)
     print(f"ozaki +w large K: max-rel {mx:.2e} fro-rel {fro:.2e}")
     assert mx < TOL_SPLIT
+
+
+@pytest.mark.parametrize("precision", [spexhip.PREC_SPLIT_FP32, spexhip.PREC_OZAKI_INT8])
+def test_split_forms_wide_dynamic_range(ctx, precision):
+    """Columns of M whose scales span 10^-6 .. 10^6, columns dominated by a
+    few transitions (x 10^4 spikes, so the per-column scale is far above
+    the rest of the column), and strongly correlated column pairs.
+
+    Normwise (the error metric of the results) both forms stay within 1e-8.
+    Per element, -chi is positive semidefinite (D < 0), so |chi_IJ| <=
+    sqrt(chi_II chi_JJ) is the natural scale of element (I, J): split-FP32
+    stays below 1e-8 of it.  The Ozaki form drops the slice products of
+    level >= 4 (except the real part of the third slices' square); for a
+    column dominated by a few transitions the rest of the column carries
+    full-size low digits, whose dropped products sum as a random walk over
+    the n transitions while sqrt(chi_II chi_JJ) does not grow with n: the
+    per-element error is statistically bounded by about
+    4 * sqrt(2) * 73^2 / (127^2 254^4) * sqrt(n) = 4.5e-10 sqrt(n) of that
+    scale (73: rms of a digit in [-127, 127]), here 2.8e-8."""
+    g = np.random.default_rng(31)
+    c = synth.chi0_case(nk=2, nocc=4, nunocc=500, nb=96, nw=3, seed=31)
+    M = np.array(c.M)
+    n, nb = M.shape
+    M *= 10.0 ** g.uniform(-6, 6, nb)[None, :]
+    for I in (3, 17, 40, 77, 90):
+        rows = g.choice(n, 3, replace=False)
+        M[rows, I] *= 1e4
+    M[:, 51] = 1e3 * M[:, 50] + 1e-6 * M[:, 51]   # nearly parallel columns
+    M[:, 61] = -M[:, 60] * (1 + 1e-9)
+    w = np.asfortranarray(oracle.weights_none(c.e_occ, c.e_unocc, c.omega))
+    ref = oracle.chi0_herk(M, w)
+    got = run_chi0(ctx, M, w, precision)
+    mx, fro = normwise(got, ref)
+    iu, ju = np.triu_indices(nb)  # packed upper order: i <= j, j-major
+    order = np.lexsort((iu, ju))
+    iu, ju = iu[order], ju[order]
+    worst = 0.0
+    for f in range(w.shape[1]):
+        diag = np.abs(ref[iu == ju, f])  # chi_II in column order
+        scale = np.sqrt(diag[iu] * diag[ju])
+        worst = max(worst, float((np.abs(got[:, f] - ref[:, f]) / scale).max()))
+    print(f"precision {precision}: normwise max-rel {mx:.2e} fro-rel {fro:.2e}; "
+          f"max |err_IJ| / sqrt(chi_II chi_JJ) = {worst:.2e}")
+    assert mx < TOL_SPLIT and fro < TOL_SPLIT
+    bound = 1e-8 if precision == spexhip.PREC_SPLIT_FP32 else 4.5e-10 * np.sqrt(n)
+    assert worst < bound
