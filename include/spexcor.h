@@ -53,6 +53,18 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
                  double *plasma, double *freq, double *matrix, double *head, double *wing,
                  double *freqc, double *matrixc, double *headc, double *wingc);
 
+/* Layout of the arrays SPEX declares MCOMPLEX_dp (coulomb, ctrafo, matrix,
+ * head, wing): complex (inv = 0, the default) or, as an inversion-symmetric
+ * build (-DINV, cppmacro.h:50-60) stores them, real_dp (inv != 0); then
+ * those arguments of spexcor_write/spexcor_read are real arrays of the
+ * same element counts.  matrixc, headc, wingc and freqc stay complex.
+ * spexcor_read refuses a file of the other layout.                        */
+int spexcor_set_inv(int inv);
+
+/* The layout of the k point's group in fname: *inv = 1 for the real
+ * (INV build) layout, 0 for the complex one.                              */
+int spexcor_file_inv(const char *fname, const double kvec[3], int *inv);
+
 /* The chunk limit of the HDF5 wrapper (MaxChunk, cppmacro.h:143-144) in
  * bytes, 0 for the default 2^31-1: arrays of more than limit/(element
  * bytes) elements are stored as "<name>_0" and "<name>_1", divided again

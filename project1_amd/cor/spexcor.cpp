@@ -29,6 +29,11 @@
 // follow the same rule.  The halves of a column-major array split along its
 // slowest non-trivial dimension are contiguous.
 //
+// Built with -DINV, SPEX stores its MCOMPLEX_dp arrays (coulomb, ctrafo,
+// matrix, head, wing) as real_dp; spexcor_set_inv selects that layout for
+// writes and reads, and spexcor_file_inv detects it from the stored
+// Coulomb matrix.
+//
 // COR version 0 files (no version attribute, or 0) are read as readcor reads
 // them: one "matrix" dataset (all nfreq packed matrices) and one "matrixc"
 // dataset (correlation.f:1799-1807).  The spex.cor.map directory of MPI
@@ -53,6 +58,17 @@ namespace {
 thread_local std::string g_err;
 // MaxChunk of the HDF5 wrapper in bytes (huge(0)); settable for tests
 size_t g_max_chunk = 2147483647;
+// Layout of SPEX's MCOMPLEX_dp arrays (coulomb, ctrafo, matrix, head, wing):
+// complex by default, real_dp in an inversion-symmetric (-DINV) build
+// (cppmacro.h:50-60; correlation.f:89-96; global.f:153).
+bool g_inv = false;
+
+// dims of an MCOMPLEX_dp array: a trailing real/imaginary dimension unless
+// the layout is the INV build's real one
+std::vector<hsize_t> mc(std::vector<hsize_t> d) {
+  if (!g_inv) d.push_back(2);
+  return d;
+}
 
 int fail(const std::string &m) {
   g_err = m;
@@ -258,7 +274,7 @@ int spexcor_write(const char *fname, const double kvec[3], int typ, const char *
   } else if (!write_attr(grp, "nbasm", H5T_NATIVE_INT, &nbasm, 1)) {
     return fail("spexcor_write: cannot write nbasm");
   }
-  if (!write_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, {npm, 2}, true))
+  if (!write_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, mc({npm}), true))
     return fail("spexcor_write: cannot write coulomb");
   // file attributes (mode 2)
   const int nn[2] = {nfreq, nfreqc};
@@ -287,12 +303,12 @@ int spexcor_write(const char *fname, const double kvec[3], int typ, const char *
     return fail("spexcor_write: cannot write the k-point attributes");
   const hsize_t nb = nbas, nbm = nbasm, np = (hsize_t)nbas * (nbas + 1) / 2;
   if (!write_set(grp, "coul", H5T_NATIVE_DOUBLE, coul, {nb}) ||
-      !write_set(grp, "ctrafo", H5T_NATIVE_DOUBLE, ctrafo, {nb, nbm, 2}) ||
+      !write_set(grp, "ctrafo", H5T_NATIVE_DOUBLE, ctrafo, mc({nb, nbm})) ||
       (ncblock && !write_set(grp, "cblock", H5T_NATIVE_INT, cblock, {(hsize_t)ncblock})))
     return fail("spexcor_write: cannot write coul, ctrafo or cblock");
   for (int i = 0; i < nfreq; i++)
     if (!write_set(grp, "matrix_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
-                   matrix + 2 * np * i, {np, 2}))
+                   matrix + (g_inv ? 1 : 2) * np * i, mc({np})))
       return fail("spexcor_write: cannot write matrix_" + std::to_string(i + 1));
   for (int i = 0; i < nfreqc; i++)
     if (!write_set(grp, "matrixc_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
@@ -300,8 +316,8 @@ int spexcor_write(const char *fname, const double kvec[3], int typ, const char *
       return fail("spexcor_write: cannot write matrixc_" + std::to_string(i + 1));
   if (gamma) {
     const hsize_t nf = nfreq, nfc = nfreqc;
-    if ((nfreq && (!write_set(grp, "head", H5T_NATIVE_DOUBLE, head, {nf, 6, 2}) ||
-                   !write_set(grp, "wing", H5T_NATIVE_DOUBLE, wing, {nf, nb, 3, 2}))) ||
+    if ((nfreq && (!write_set(grp, "head", H5T_NATIVE_DOUBLE, head, mc({nf, 6})) ||
+                   !write_set(grp, "wing", H5T_NATIVE_DOUBLE, wing, mc({nf, nb, 3})))) ||
         (nfreqc && (!write_set(grp, "headc", H5T_NATIVE_DOUBLE, headc, {nfc, 3, 3, 2}) ||
                     !write_set(grp, "wingc", H5T_NATIVE_DOUBLE, wingc, {nfc, 2, nb, 3, 2}))))
       return fail("spexcor_write: cannot write head or wings");
@@ -351,6 +367,14 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
   const hsize_t nb = nbas, np = (hsize_t)nbas * (nbas + 1) / 2,
                 npm = (hsize_t)nbasm * (nbasm + 1) / 2;
   const bool gamma = H5Aexists(grp, "plasma") > 0;
+  {
+    int inv = -1;
+    if (spexcor_file_inv(fname, kvec, &inv)) return 1;
+    if (inv != (g_inv ? 1 : 0))
+      return fail(std::string("spexcor_read: ") + fname + " holds the " +
+                  (inv ? "real (INV build)" : "complex") +
+                  " layout; select it with spexcor_set_inv");
+  }
   if (label) {
     H att(H5Aopen(grp, "label", H5P_DEFAULT), H5Aclose);
     H typ_s(H5Aget_type(att), H5Tclose);
@@ -360,9 +384,9 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
     std::memcpy(label, buf.data(), len < 6 ? len : 6);
   }
   const hsize_t nbm = nbasm;
-  if ((coulomb && !read_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, {npm, 2})) ||
+  if ((coulomb && !read_set(grp, "coulomb", H5T_NATIVE_DOUBLE, coulomb, mc({npm}))) ||
       (coul && !read_set(grp, "coul", H5T_NATIVE_DOUBLE, coul, {nb})) ||
-      (ctrafo && !read_set(grp, "ctrafo", H5T_NATIVE_DOUBLE, ctrafo, {nb, nbm, 2})) ||
+      (ctrafo && !read_set(grp, "ctrafo", H5T_NATIVE_DOUBLE, ctrafo, mc({nb, nbm}))) ||
       (cblock && ncblock && !read_set(grp, "cblock", H5T_NATIVE_INT, cblock, {(hsize_t)ncblock})) ||
       (oselfc && !read_attr(grp, "oselfc", H5T_NATIVE_INT, oselfc)) ||
       (contour && !read_attr(grp, "contour", H5T_NATIVE_DOUBLE, contour)) ||
@@ -375,7 +399,7 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
   const hsize_t nf = nfreq, nfc = nfreqc;
   if (file_version(file) == 0) {
     // one dataset per matrix kind (correlation.f:1799-1807)
-    if (matrix && nfreq && !read_set(grp, "matrix", H5T_NATIVE_DOUBLE, matrix, {nf, np, 2}))
+    if (matrix && nfreq && !read_set(grp, "matrix", H5T_NATIVE_DOUBLE, matrix, mc({nf, np})))
       return fail("spexcor_read: matrix");
     if (matrixc && nfreqc &&
         !read_set(grp, "matrixc", H5T_NATIVE_DOUBLE, matrixc, {nfc, nb, nb, 2}))
@@ -383,7 +407,7 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
   } else {
     for (int i = 0; matrix && i < nfreq; i++)
       if (!read_set(grp, "matrix_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
-                    matrix + 2 * np * i, {np, 2}))
+                    matrix + (g_inv ? 1 : 2) * np * i, mc({np})))
         return fail("spexcor_read: matrix_" + std::to_string(i + 1));
     for (int i = 0; matrixc && i < nfreqc; i++)
       if (!read_set(grp, "matrixc_" + std::to_string(i + 1), H5T_NATIVE_DOUBLE,
@@ -391,12 +415,41 @@ int spexcor_read(const char *fname, const double kvec[3], char *label, double *c
         return fail("spexcor_read: matrixc_" + std::to_string(i + 1));
   }
   if (gamma) {
-    if ((nfreq && head && !read_set(grp, "head", H5T_NATIVE_DOUBLE, head, {nf, 6, 2})) ||
-        (nfreq && wing && !read_set(grp, "wing", H5T_NATIVE_DOUBLE, wing, {nf, nb, 3, 2})) ||
+    if ((nfreq && head && !read_set(grp, "head", H5T_NATIVE_DOUBLE, head, mc({nf, 6}))) ||
+        (nfreq && wing && !read_set(grp, "wing", H5T_NATIVE_DOUBLE, wing, mc({nf, nb, 3}))) ||
         (nfreqc && headc && !read_set(grp, "headc", H5T_NATIVE_DOUBLE, headc, {nfc, 3, 3, 2})) ||
         (nfreqc && wingc && !read_set(grp, "wingc", H5T_NATIVE_DOUBLE, wingc, {nfc, 2, nb, 3, 2})))
       return fail("spexcor_read: head or wings");
   }
+  return 0;
+}
+
+int spexcor_set_inv(int inv) {
+  g_inv = inv != 0;
+  return 0;
+}
+
+int spexcor_file_inv(const char *fname, const double kvec[3], int *inv) {
+  if (!fname || !kvec || !inv) return fail("spexcor_file_inv: bad arguments");
+  H5Eset_auto2(H5E_DEFAULT, nullptr, nullptr);
+  H file(H5Fopen(fname, H5F_ACC_RDONLY, H5P_DEFAULT), H5Fclose);
+  if (file < 0) return fail(std::string("File ") + fname + " not found.");
+  const std::string gname = group_name(kvec);
+  if (!exists(file, gname.c_str())) return fail("spexcor_file_inv: no group " + gname);
+  H grp(H5Gopen2(file, gname.c_str(), H5P_DEFAULT), H5Gclose);
+  // the packed Coulomb matrix (or its first part): rank 1 real, rank 2 complex
+  std::string name = "coulomb";
+  while (!exists(grp, name.c_str())) {
+    name += "_0";
+    if (name.size() > 64) return fail("spexcor_file_inv: no coulomb dataset in " + gname);
+  }
+  H set(H5Dopen2(grp, name.c_str(), H5P_DEFAULT), H5Dclose);
+  H spc(H5Dget_space(set), H5Sclose);
+  const int rank = H5Sget_simple_extent_ndims(spc);
+  if (rank < 1) return fail("spexcor_file_inv: cannot read " + name);
+  hsize_t dims[2] = {0, 0};
+  H5Sget_simple_extent_dims(spc, dims, nullptr);
+  *inv = (rank == 2 && dims[1] == 2) ? 0 : 1;
   return 0;
 }
 

@@ -32,6 +32,10 @@ def lib() -> C.CDLL:
         L.spexcor_query.argtypes = [C.c_char_p, _vp] + [C.POINTER(I)] * 6
         L.spexcor_read.argtypes = [C.c_char_p, _vp, C.c_char_p] + [_vp] * 15
         L.spexcor_set_max_chunk.argtypes = [C.c_size_t]
+        L.spexcor_set_inv.argtypes = [I]
+        L.spexcor_file_inv.argtypes = [C.c_char_p, _vp, C.POINTER(I)]
+        L.spexcor_set_inv.restype = I
+        L.spexcor_file_inv.restype = I
         L.spexcor_file_name.argtypes = [C.c_char_p, I, C.c_char_p, C.c_size_t]
         L.spexcor_locate.argtypes = [C.c_char_p, _vp, C.c_char_p, C.c_size_t]
         L.spexcor_map_record.argtypes = [C.c_char_p, _vp, I]
@@ -59,30 +63,44 @@ def _f(a, dt):
 
 def write(fname, kvec, typ, label, coulomb, coul, ctrafo, oselfc=1, contour=(0, 0, 0, 0),
           cblock=None, freq=None, matrix=None, freqc=None, matrixc=None, gamma=False,
-          plasma=0.0, head=None, wing=None, headc=None, wingc=None) -> None:
+          plasma=0.0, head=None, wing=None, headc=None, wingc=None, inv=False) -> None:
     """writecor: coulomb packed (nbasm(nbasm+1)/2,), coul (nbas,), ctrafo
     (nbasm, nbas), matrix (npack, nfreq), matrixc (nbas, nbas, nfreqc),
     head (6, nfreq), wing (3, nbas, nfreq), headc (3, 3, nfreqc), wingc
-    (3, nbas, 2, nfreqc)."""
-    ct = _f(ctrafo, np.complex128)
+    (3, nbas, 2, nfreqc).  inv: the real layout of an inversion-symmetric
+    build for coulomb, ctrafo, matrix, head and wing."""
+    mt = np.float64 if inv else np.complex128
+    ct = _f(ctrafo, mt)
     nbasm, nbas = ct.shape
     k = np.ascontiguousarray(kvec, np.float64)
     cont = np.ascontiguousarray(contour, np.float64)
     fr = _f(freq, np.float64)
     frc = _f(freqc, np.complex128)
-    arrs = dict(coulomb=_f(coulomb, np.complex128), coul=_f(coul, np.float64), cblock=_f(cblock, np.int32),
-                matrix=_f(matrix, np.complex128), matrixc=_f(matrixc, np.complex128),
-                head=_f(head, np.complex128), wing=_f(wing, np.complex128),
+    arrs = dict(coulomb=_f(coulomb, mt), coul=_f(coul, np.float64), cblock=_f(cblock, np.int32),
+                matrix=_f(matrix, mt), matrixc=_f(matrixc, np.complex128),
+                head=_f(head, mt), wing=_f(wing, mt),
                 headc=_f(headc, np.complex128), wingc=_f(wingc, np.complex128))
     nfreq = 0 if fr is None else fr.size
     nfreqc = 0 if frc is None else frc.size
     ncb = 0 if arrs["cblock"] is None else arrs["cblock"].size
-    _check(lib().spexcor_write(str(fname).encode(), _p(k), typ, label.encode(), nbasm,
-                               _p(arrs["coulomb"]), nbas, _p(arrs["coul"]), _p(ct), ncb,
-                               _p(arrs["cblock"]), oselfc, _p(cont), int(gamma), plasma, nfreq,
-                               _p(fr), _p(arrs["matrix"]), _p(arrs["head"]), _p(arrs["wing"]),
-                               nfreqc, _p(frc), _p(arrs["matrixc"]), _p(arrs["headc"]),
-                               _p(arrs["wingc"])))
+    _check(lib().spexcor_set_inv(int(inv)))
+    try:
+        _check(lib().spexcor_write(str(fname).encode(), _p(k), typ, label.encode(), nbasm,
+                                   _p(arrs["coulomb"]), nbas, _p(arrs["coul"]), _p(ct), ncb,
+                                   _p(arrs["cblock"]), oselfc, _p(cont), int(gamma), plasma,
+                                   nfreq, _p(fr), _p(arrs["matrix"]), _p(arrs["head"]),
+                                   _p(arrs["wing"]), nfreqc, _p(frc), _p(arrs["matrixc"]),
+                                   _p(arrs["headc"]), _p(arrs["wingc"])))
+    finally:
+        lib().spexcor_set_inv(0)
+
+
+def file_inv(fname, kvec) -> bool:
+    """True if the k point's group holds the real (INV build) layout."""
+    v = C.c_int()
+    _check(lib().spexcor_file_inv(str(fname).encode(), _p(np.ascontiguousarray(kvec, np.float64)),
+                                  C.byref(v)))
+    return bool(v.value)
 
 
 def query(fname, kvec) -> dict:
@@ -92,28 +110,37 @@ def query(fname, kvec) -> dict:
     return dict(zip(("type", "nbasm", "nbas", "nfreq", "nfreqc", "ncblock"), (x.value for x in v)))
 
 
-def read(fname, kvec, gamma=False) -> dict:
-    """readcor: the arrays written by `write` (head/wings when gamma)."""
+def read(fname, kvec, gamma=False, inv=None) -> dict:
+    """readcor: the arrays written by `write` (head/wings when gamma); inv
+    None detects the layout from the file."""
     q = query(fname, kvec)
     if q["type"] < 1:
         raise CorError("no matrices for this k point")
+    if inv is None:
+        inv = file_inv(fname, kvec)
     nbasm, nb, nf, nfc = q["nbasm"], q["nbas"], q["nfreq"], q["nfreqc"]
     z = lambda *s: np.zeros(s, np.complex128, order="F")
-    out = dict(coulomb=z(nbasm * (nbasm + 1) // 2), coul=np.zeros(nb), ctrafo=z(nbasm, nb),
+    m = (lambda *s: np.zeros(s, np.float64, order="F")) if inv else z
+    out = dict(coulomb=m(nbasm * (nbasm + 1) // 2), coul=np.zeros(nb), ctrafo=m(nbasm, nb),
                cblock=np.zeros(max(q["ncblock"], 1), np.int32), contour=np.zeros(4),
-               freq=np.zeros(max(nf, 1)), matrix=z(nb * (nb + 1) // 2, max(nf, 1)),
+               freq=np.zeros(max(nf, 1)), matrix=m(nb * (nb + 1) // 2, max(nf, 1)),
                freqc=z(max(nfc, 1)), matrixc=z(nb, nb, max(nfc, 1)))
     if gamma:
-        out.update(head=z(6, max(nf, 1)), wing=z(3, nb, max(nf, 1)), headc=z(3, 3, max(nfc, 1)),
+        out.update(head=m(6, max(nf, 1)), wing=m(3, nb, max(nf, 1)), headc=z(3, 3, max(nfc, 1)),
                    wingc=z(3, nb, 2, max(nfc, 1)))
     lab = C.create_string_buffer(7)
     osf, pl = C.c_int(), C.c_double()
     g = lambda key: _p(out.get(key))
-    _check(lib().spexcor_read(str(fname).encode(), _p(np.ascontiguousarray(kvec, np.float64)), lab,
-                              g("coulomb"), g("coul"), g("ctrafo"), g("cblock"), C.addressof(osf),
-                              g("contour"), C.addressof(pl), g("freq"), g("matrix"), g("head"),
-                              g("wing"), g("freqc"), g("matrixc"), g("headc"), g("wingc")))
-    out.update(q, label=lab.raw[:6].decode(), oselfc=osf.value, plasma=pl.value)
+    _check(lib().spexcor_set_inv(int(inv)))
+    try:
+        _check(lib().spexcor_read(str(fname).encode(), _p(np.ascontiguousarray(kvec, np.float64)),
+                                  lab, g("coulomb"), g("coul"), g("ctrafo"), g("cblock"),
+                                  C.addressof(osf), g("contour"), C.addressof(pl), g("freq"),
+                                  g("matrix"), g("head"), g("wing"), g("freqc"), g("matrixc"),
+                                  g("headc"), g("wingc")))
+    finally:
+        lib().spexcor_set_inv(0)
+    out.update(q, label=lab.raw[:6].decode(), oselfc=osf.value, plasma=pl.value, inv=inv)
     return out
 
 

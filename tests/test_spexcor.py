@@ -273,3 +273,38 @@ def test_cor_map_directory(tmp_path):
     spexcor.write(fname, k, 1, "SUS", **d)
     r = spexcor.read(spexcor.locate(pre, k), k)
     np.testing.assert_array_equal(r["matrix"], d["matrix"])
+
+
+@needs_lib
+@pytest.mark.skipif(not Path(H5DUMP).exists(), reason="h5dump not available")
+def test_inv_build_real_layout(tmp_path):
+    """An inversion-symmetric SPEX build (-DINV) declares coulomb, ctrafo,
+    matrix, head and wing real (MCOMPLEX_dp = real_dp, cppmacro.h:50-60):
+    written without the real/imaginary dimension, detected on reading,
+    round trip exact; the complex layout is refused for such a file."""
+    g = np.random.default_rng(10)
+    f = tmp_path / "spex.cor"
+    nbasm, nbas, nf, nfc = 9, 7, 3, 2
+    d = _case(g, nbasm, nbas, nf, nfc)
+    for key in ("coulomb", "ctrafo", "matrix", "head", "wing"):
+        d[key] = np.ascontiguousarray(d[key].real)
+    spexcor.write(f, [0, 0, 0], 3, "GW", gamma=True, plasma=0.5, inv=True, **d)
+    assert spexcor.file_inv(f, [0, 0, 0])
+    r = spexcor.read(f, [0, 0, 0], gamma=True)
+    assert r["inv"]
+    for key in d:
+        np.testing.assert_array_equal(r[key].reshape(np.shape(d[key])), d[key], err_msg=key)
+    with pytest.raises(spexcor.CorError, match="INV build"):
+        spexcor.read(f, [0, 0, 0], gamma=True, inv=False)
+    out = re.sub(r"\s+", " ", subprocess.run([H5DUMP, "-H", str(f)], capture_output=True,
+                                             text=True).stdout)
+    for name, dims in (("coulomb", f"{nbasm * (nbasm + 1) // 2}"), ("ctrafo", f"{nbas}, {nbasm}"),
+                       ("matrix_1", f"{nbas * (nbas + 1) // 2}"), ("head", f"{nf}, 6"),
+                       ("wing", f"{nf}, {nbas}, 3"), ("matrixc_1", f"{nbas}, {nbas}, 2"),
+                       ("headc", f"{nfc}, 3, 3, 2")):
+        assert re.search(rf'DATASET "{name}" {{ DATATYPE H5T_IEEE_F64LE DATASPACE SIMPLE '
+                         rf'{{ \( {re.escape(dims)} \)', out), (name, dims)
+    # a complex-layout file is detected as such
+    f2 = tmp_path / "spex2.cor"
+    spexcor.write(f2, [0, 0, 0], 3, "GW", gamma=True, plasma=0.5, **_case(g, nbasm, nbas, nf, nfc))
+    assert not spexcor.file_inv(f2, [0, 0, 0])
