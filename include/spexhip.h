@@ -455,7 +455,9 @@ int spexhip_fermi_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
  * of bands 0..nband-1 with themselves (spexhip_momentum_dev with b1lo =
  * b2lo = 0, nocc = nunocc = nband; n x 3, ld ldp), deg (host [nk][nband])
  * SPEX's degeneracy table, 0-based: at the first band of a group its last
- * band (the other entries are not read); spin_factor 2
+ * band (the other entries are not read); a group reaching past nband ends
+ * the sum for that k point, as the reference's loop exits there;
+ * SPEXHIP_E_INVALID if a first band's entry is below it.  spin_factor 2
  * for one spin of a spin-unpolarised system.  Below 1e-6 eV the result is
  * 0; SPEXHIP_E_INVALID if plasma^2 < -1e-10.                              */
 int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void *w_dev,

@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void plasma_sum_kernel(long nk, int nband,
     int i1 = 0;
     while (i1 < nband) {
       const int i2 = deg[k * nband + i1];
-      if (i2 >= nband || i2 < i1) break;
+      if (i2 >= nband) break;  // the group reaches past the bands given
       double re[3] = {0.0, 0.0, 0.0};
       for (int j = 0; j < 3; j++) {
         double a = 0.0;
@@ -213,8 +213,16 @@ int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void
   SPX_CHECK(ctx, spx_owns(ctx, w, 8 * (size_t)nk * nband) &&
                      spx_owns(ctx, P, 16 * ((size_t)ldp * 2 + n)),
             "plasma_frequency: w or P is not a context allocation of the stated shape");
-  for (long i = 0; i < (long)nk * nband; i++)
-    SPX_CHECK(ctx, deg[i] >= 0 && deg[i] < nband, "plasma_frequency: deg out of range");
+  // the groups as the sum walks them (susceptibility.f:1412-1420): at the
+  // first band of a group its last band; a group reaching past the bands
+  // given ends the walk for that k point
+  for (long k = 0; k < nk; k++)
+    for (int i1 = 0; i1 < nband;) {
+      const int i2 = deg[k * nband + i1];
+      if (i2 >= nband) break;
+      SPX_CHECK(ctx, i2 >= i1, "plasma_frequency: deg not consistent at k=%ld band %d", k, i1);
+      i1 = i2 + 1;
+    }
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   const size_t db = 4 * (size_t)nk * nband;
   void *ws;
