@@ -307,10 +307,12 @@ constexpr int gcd16(int n) { return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n %
 
 // Planes packed per x pass of pw_pair_joint_kernel: all 16/gcd(N2,16)
 // planes that fill whole row blocks when that is at most five blocks, else
-// two planes with the last block partly padded (28: 56 of 64 rows, 36: 72
-// of 80).
+// two planes with the last block partly padded (28: 56 of 64 rows, 40: 80
+// in 5).  36 takes one plane in three blocks (36 of 48 rows): then the
+// three-product complex MFMA fits the registers, 14 % fewer MFMAs per plane
+// than two planes in five blocks with four products.
 constexpr int joint_planes(int n2) {
-  return (16 / gcd16(n2)) * n2 / 16 <= 5 ? 16 / gcd16(n2) : 2;
+  return n2 == 36 ? 1 : (16 / gcd16(n2)) * n2 / 16 <= 5 ? 16 / gcd16(n2) : 2;
 }
 // Waves per SIMD: two (256 registers).  The x pass keeps the operands of
 // its next step in flight while the current step's MFMAs run, which needs
@@ -333,6 +335,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(
   static_assert(N1 % 4 == 0 && N2 % 4 == 0 && N1 >= PW_BOX && N2 >= PW_BOX, "grid");
   constexpr int G = joint_planes(N2), RB = (G * N2 + 15) / 16, KX = N1 / 4, KY = N2 / 4;
   constexpr int GI = G % 2 == 0 ? 2 : 1;  // planes per y-pass batch
+  constexpr bool M3 = RB <= 3;            // three-product complex MFMA where registers allow
   // z twiddles, (N3 + G) x PW_BOX, zero past N3 (dynamic: sized to the grid
   // instead of a 64-plane table)
   extern __shared__ double2 tz[];
@@ -400,15 +403,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(
   rows_of(0);
   load_step(0, c1, c2);
   for (int z0 = 0; z0 < N3; z0 += G) {
-    d4_t xr[RB], xi[RB];
+    // M3: three real MFMAs per complex product (Gauss): t1 = ar.fr,
+    // t2 = ai.fi, t3 = (ar + ai)(fr + fi); re = t1 - t2, im = t3 - t1 - t2
+    d4_t xr[RB], xi[RB], xs[M3 ? RB : 1];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
       xr[rb] = d4_t{0, 0, 0, 0};
       xi[rb] = d4_t{0, 0, 0, 0};
+      if (M3) xs[M3 ? rb : 0] = d4_t{0, 0, 0, 0};
     }
 #pragma unroll
     for (int kk = 0; kk < KX; kk++) {
       const double2 f = F1s[kk][l];
+      const double fsum = f.x + f.y;
       double2 a[RB];
 #pragma unroll
       for (int rb = 0; rb < RB; rb++)
@@ -420,25 +427,60 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(joint_waves(
         load_step(0, c1, c2);
       }
 #pragma unroll
-      for (int rb = 0; rb < RB; rb++) cmfma(xr[rb], xi[rb], a[rb], f);
+      for (int rb = 0; rb < RB; rb++) {
+        if (M3) {
+          xr[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].x, f.x, xr[rb], 0, 0, 0);
+          xi[rb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].y, f.y, xi[rb], 0, 0, 0);
+          xs[M3 ? rb : 0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb].x + a[rb].y, fsum,
+                                                                  xs[M3 ? rb : 0], 0, 0, 0);
+        } else {
+          cmfma(xr[rb], xi[rb], a[rb], f);
+        }
+      }
+    }
+    if (M3) {
+#pragma unroll
+      for (int rb = 0; rb < RB; rb++) {
+        const d4_t t1 = xr[rb], t2 = xi[rb];
+        xr[rb] = t1 - t2;
+        xi[rb] = xs[M3 ? rb : 0] - t1 - t2;
+      }
     }
     // y pass of GI planes at a time, their MFMA chains interleaved; then the
     // z pass of those planes in one sweep over the outputs
 #pragma unroll
     for (int zb = 0; zb < G; zb += GI) {
-      d4_t qr[GI], qi[GI];
+      d4_t qr[GI], qi[GI], qs[M3 ? GI : 1];
 #pragma unroll
       for (int zi = 0; zi < GI; zi++) {
         qr[zi] = d4_t{0, 0, 0, 0};
         qi[zi] = d4_t{0, 0, 0, 0};
+        if (M3) qs[M3 ? zi : 0] = d4_t{0, 0, 0, 0};
       }
 #pragma unroll
       for (int kq = 0; kq < KY; kq++) {
         const double2 f2 = F2s[kq][l];
+        const double f2sum = f2.x + f2.y;
 #pragma unroll
         for (int zi = 0; zi < GI; zi++) {
           const int R0 = N2 * (zb + zi) + 4 * kq, rb = R0 / 16, r = (R0 % 16) / 4;
-          cmfma(qr[zi], qi[zi], f2, make_double2(xr[rb][r], xi[rb][r]));
+          const double br = xr[rb][r], bi = xi[rb][r];
+          if (M3) {
+            qr[zi] = __builtin_amdgcn_mfma_f64_16x16x4f64(f2.x, br, qr[zi], 0, 0, 0);
+            qi[zi] = __builtin_amdgcn_mfma_f64_16x16x4f64(f2.y, bi, qi[zi], 0, 0, 0);
+            qs[M3 ? zi : 0] = __builtin_amdgcn_mfma_f64_16x16x4f64(f2sum, br + bi,
+                                                                    qs[M3 ? zi : 0], 0, 0, 0);
+          } else {
+            cmfma(qr[zi], qi[zi], f2, make_double2(br, bi));
+          }
+        }
+      }
+      if (M3) {
+#pragma unroll
+        for (int zi = 0; zi < GI; zi++) {
+          const d4_t u1 = qr[zi], u2 = qi[zi];
+          qr[zi] = u1 - u2;
+          qi[zi] = qs[M3 ? zi : 0] - u1 - u2;
         }
       }
       // Q(gy, gx) of plane zi: lane holds gx = lr, gy = lk + 4r
