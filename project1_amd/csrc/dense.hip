@@ -30,7 +30,7 @@ using namespace spxg;
 namespace {
 
 // C = alpha*A*B + beta*C + diag*I, batched over blockIdx.z (see ZgemmArgs).
-__global__ __launch_bounds__(NT) void zgemm_kernel(ZgemmArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void zgemm_kernel(ZgemmArgs a) {
   __shared__ GemmLds s;
   const int b = blockIdx.z;
   const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TM;

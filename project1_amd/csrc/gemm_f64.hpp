@@ -12,8 +12,11 @@ constexpr int TM = 64, KC = 16, NT = 256;
 // ------------------------------------------------- complex FP64 GEMM tile --
 // acc += A(i0:i0+64, 0:kdim) * B(0:kdim, j0:j0+64); A, B column-major.
 // Rows/cols outside [0,m) / [0,n) and k outside [0,kdim) read as zero.
+// Three real MFMAs per complex multiply-add (Gauss): the accumulators hold
+// t1 = sum ar.br, t2 = sum ai.bi, t3 = sum (ar + ai)(br + bi), and
+// tile_foreach hands out re = t1 - t2, im = t3 - t1 - t2.
 struct TileAcc {
-  d4_t r[2][2], i[2][2];
+  d4_t r[2][2], i[2][2], s[2][2];
 };
 
 __device__ inline void tile_zero(TileAcc &a) {
@@ -23,6 +26,7 @@ __device__ inline void tile_zero(TileAcc &a) {
     for (int y = 0; y < 2; y++) {
       a.r[x][y] = d4_t{0, 0, 0, 0};
       a.i[x][y] = d4_t{0, 0, 0, 0};
+      a.s[x][y] = d4_t{0, 0, 0, 0};
     }
 }
 
@@ -80,23 +84,23 @@ __device__ inline void gemm_tile(const double2 *__restrict__ A, long lda,
 #pragma unroll
     for (int kk = 0; kk < KC / 4; kk++) {
       const int k = kk * 4 + lk;
-      double ar[2], ai[2], br[2], bi[2];
+      double ar[2], ai[2], as[2], br[2], bi[2], bs[2];
 #pragma unroll
       for (int x = 0; x < 2; x++) {
         ar[x] = s.Ar[buf][k][wr * 32 + x * 16 + lr];
         ai[x] = s.Ai[buf][k][wr * 32 + x * 16 + lr];
         br[x] = s.Br[buf][k][wc * 32 + x * 16 + lr];
         bi[x] = s.Bi[buf][k][wc * 32 + x * 16 + lr];
+        as[x] = ar[x] + ai[x];
+        bs[x] = br[x] + bi[x];
       }
 #pragma unroll
       for (int x = 0; x < 2; x++)
 #pragma unroll
         for (int y = 0; y < 2; y++) {
-          // (ar + i ai)(br + i bi) = ar br - ai bi + i(ar bi + ai br)
           acc.r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc.r[x][y], 0, 0, 0);
-          acc.r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], acc.r[x][y], 0, 0, 0);
-          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acc.i[x][y], 0, 0, 0);
-          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acc.i[x][y], 0, 0, 0);
+          acc.i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], bi[y], acc.i[x][y], 0, 0, 0);
+          acc.s[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[x], bs[y], acc.s[x][y], 0, 0, 0);
         }
     }
     if (st + 1 < nst) store(buf ^ 1);
@@ -115,9 +119,11 @@ __device__ inline void tile_foreach(const TileAcc &acc, int i0, int j0, F f) {
 #pragma unroll
     for (int y = 0; y < 2; y++)
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        f(i0 + wr * 32 + x * 16 + lk + 4 * r, j0 + wc * 32 + y * 16 + lr, acc.r[x][y][r],
-          acc.i[x][y][r]);
+      for (int r = 0; r < 4; r++) {
+        const double t1 = acc.r[x][y][r], t2 = acc.i[x][y][r];
+        f(i0 + wr * 32 + x * 16 + lk + 4 * r, j0 + wc * 32 + y * 16 + lr, t1 - t2,
+          acc.s[x][y][r] - t1 - t2);
+      }
 }
 
 
