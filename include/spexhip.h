@@ -366,6 +366,20 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
                                    double wghtthr, int nfrqh, int nfrqc, const double *wghtSr,
                                    const double *wghtSi, int precision, double *susceph,
                                    double *suscepc);
+/* The same pair sum with MULTDIFF ON (multdiff_ = 1: susceptibility.f:
+ * 588-592, 912-925, 1193, 1817-1824): cprod is the packet before the
+ * scaling of its first column, frqs [nfrqs] the Hilbert mesh, e_occ
+ * [nk][nocc] at k and e_unocc [nk][nunocc] at k+q of the n = nk*nocc*nunocc
+ * transitions.  The body is the plain pair sum; row and column 0 come from
+ * spexhip_multdiff_dev.                                                   */
+int spexhip_susceptibility_hilbert_multdiff(spexhip_ctx *ctx, long n, int nb, const double *cprod,
+                                            long ldc, int nfrqs, const double *wtetra, long ldwt,
+                                            double wghtthr, int nfrqh, int nfrqc,
+                                            const double *wghtSr, const double *wghtSi,
+                                            int precision, const double *frqs, int nk, int nocc,
+                                            int nunocc, const double *e_occ,
+                                            const double *e_unocc, double *susceph,
+                                            double *suscepc);
 
 /* ---- products in the Coulomb eigenbasis ---------------------------------- */
 /* susceptibility.f:806-832: cprod = conjg(conjg(cprod) . T), i.e.
@@ -508,6 +522,35 @@ int spexhip_headwings_cplx_dev(spexhip_ctx *ctx, long n, int nb, const void *M_d
                                const void *Dr_dev, const void *Di_dev, const void *Dhr_dev,
                                const void *Dhi_dev, long ldd, double alpha, double beta,
                                void *head_dev, void *wing_dev);
+/* MULTDIFF ON at a q point other than Gamma (susceptibility.f:588-592,
+ * 912-925, 1193): the first basis function's products are multiplied by
+ * the band-energy difference e_unocc - e_occ of their transition, and the
+ * pairs that involve it take the Hilbert weights divided by frqs (row and
+ * column 0) or frqs^2 (element (0,0)).  Call after spexhip_chi0_dev (or
+ * spexhip_chi0_cplx_dev) of the same packet with the plain weights D0 and
+ * the same alpha: row and column 0 of chi_dev then receive alpha times the
+ * MULTDIFF sums in place of the plain ones,
+ *   chi(0,J) += alpha sum_t [D1 (M(t,0) de_t) - D0 M(t,0)] conj(M(t,J)),
+ *   chi(0,0) += alpha sum_t [D2 |M(t,0) de_t|^2 - D0 |M(t,0)|^2],
+ * and the lower column as the conjugate (imaginary axis) or from its own
+ * sums (real frequencies).  D0/D1/D2 (n x nw, ld ldd): the Hilbert folds of
+ * wtetra with wghtS(s,w) / frqs(s)^m (frqs(s) > 0; spexhip_weights_hilbert_dev
+ * on the divided table); e_occ_dev [nk][nocc] at k and e_unocc_dev
+ * [nk][nunocc] at k+q; M_dev as for spexhip_chi0_dev.  Returns after chi
+ * is updated.                                                             */
+int spexhip_multdiff_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const void *e_occ_dev,
+                         const void *e_unocc_dev, int nb, const void *M_dev, long ldm, int colmajor,
+                         int nw, const void *D0_dev, const void *D1_dev, const void *D2_dev,
+                         long ldd, double alpha, void *chi_dev, int ldchi);
+/* The real-frequency form with complex weights Dm = Dmr + i Dmi, on the
+ * whole matrix of spexhip_chi0_cplx_dev.                                  */
+int spexhip_multdiff_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                              const void *e_occ_dev, const void *e_unocc_dev, int nb,
+                              const void *M_dev, long ldm, int colmajor, int nw,
+                              const void *D0r_dev, const void *D0i_dev, const void *D1r_dev,
+                              const void *D1i_dev, const void *D2r_dev, const void *D2i_dev,
+                              long ldd, double alpha, void *chi_dev, int ldchi);
+
 /* Host drop-in of the two above for one memory packet: cprod (n x nb, ld
  * ldc), dcprod (n x 3, ld lddc), weights (n x nfrqh: wh of the wings, whh
  * of the head or NULL; n x nfrqc: wcr, wci, and wchr, wchi or NULL).  headh

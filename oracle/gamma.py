@@ -14,6 +14,9 @@ Restates, in plain numpy/Python:
   complex weights (sus1 + i sus2 + i susc1 - susc2 and its partner);
 - the MULTDIFF form of spectral_function (:1817-1824): the spectral function
   divided by frqs(s)**mm where frqs(s) > 0;
+- MULTDIFF ON at any q (multdiff_ = 1, :588-592): cprod(:,1) times the
+  band-energy difference (:912-925) and the pair loop with mm = the number
+  of pair indices that are the first basis function (:1193);
 - the Gamma-point dielectric step, correlation.f:570-646 (body row and
   column of the first basis function set to zero, wing(:,1,:) = 0,
   head = 1 - 4 pi head, wing = -sqrt(4 pi) sqcoul wing), the head's scalar
@@ -24,8 +27,12 @@ What pins each function is in tests/test_oracle_gamma.py: the chi0 pair sum
 of the augmented matrix [cprod | dcprod] (whose head and wing entries these
 sums are), closed forms of the scaling and of the dielectric step, the
 inverse of the bordered matrix [[head, conj(wing)], [wing^T, body]] (whose
-Schur complement and off-diagonal blocks the reference's update forms), and
-the per-transition fold of the MULTDIFF spectral function.
+Schur complement and off-diagonal blocks the reference's update forms), the
+per-transition fold of the MULTDIFF spectral function, and MULTDIFF's own
+purpose: when every transition's spectral weight sits at the mesh point
+frqs(s) equal to its energy difference, the scaled column and the 1/frqs**l
+weights cancel and the MULTDIFF matrix is the plain one (any wrong power,
+sign or direction of the scaling breaks that).
 """
 from __future__ import annotations
 
@@ -148,6 +155,52 @@ def spectral_function_mm(c, wtetra, frqs, wghtS, mm, wghtthr=0.0):
             reS[s] /= r
             imS[s] /= r
     return reS @ wghtS + 1j * (imS @ wghtS)
+
+
+def multdiff_scale(cprod, e_occ, e_unocc):
+    """susceptibility.f:912-925: cprod(:,1) times the band-energy difference
+    ene(j, k+q) - ene(i, k) of its transition (e_occ (nk, nocc) at k, e_unocc
+    (nk, nunocc) at k+q); the other columns unchanged.  Returns a copy."""
+    e_occ = np.asarray(e_occ, float)
+    e_unocc = np.asarray(e_unocc, float)
+    nk, nocc = e_occ.shape
+    nunocc = e_unocc.shape[1]
+    k, b1, b2 = transitions(nk, nocc, nunocc)
+    cp = np.array(cprod, complex)
+    for t in range(cp.shape[0]):
+        cp[t, 0] = cp[t, 0] * (e_unocc[k[t], b2[t]] - e_occ[k[t], b1[t]])
+    return cp
+
+
+def chi0_multdiff(cprod, e_occ, e_unocc, wtetra, frqs, wghtSr, wghtSi=None, wghtthr=0.0):
+    """The pair loop with MULTDIFF ON (multdiff_ = 1, susceptibility.f:588-592,
+    912-925, 1159-1236): column 0 of cprod scaled by multdiff_scale, each pair
+    i <= j with l = count([i, j] == first) passed to spectral_function_mm
+    with mm = l.  wghtSi None: the imaginary-axis matrix (susceph, :1228-1230)
+    as the full Hermitian (nb, nb, nw) array; else the real-frequency suscepc
+    (:1231-1236) from sus1 + i sus2 (wghtSr) and susc1 + i susc2 (wghtSi).
+    Literal loops: small sizes only."""
+    cp = multdiff_scale(cprod, e_occ, e_unocc)
+    nb = cp.shape[1]
+    nw = np.asarray(wghtSr).shape[1]
+    out = np.zeros((nb, nb, nw), complex)
+    for j in range(nb):
+        for i in range(j + 1):
+            c = cp[:, i] * np.conj(cp[:, j])
+            mm = int(i == 0) + int(j == 0)
+            s = spectral_function_mm(c, wtetra, frqs, wghtSr, mm, wghtthr)
+            if wghtSi is None:
+                out[i, j] = s
+                out[j, i] = np.conj(s)
+                continue
+            sc = spectral_function_mm(c, wtetra, frqs, wghtSi, mm, wghtthr)
+            sus1, sus2, susc1, susc2 = s.real, s.imag, sc.real, sc.imag
+            cdum = sus1 + 1j * susc1
+            cdum1 = -susc2 + 1j * sus2
+            out[i, j] += cdum + cdum1
+            if i != j:
+                out[j, i] += cdum - cdum1
+    return out
 
 
 def gamma_prepare(chi_packed, head, wing, sqcoul):

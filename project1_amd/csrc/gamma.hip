@@ -730,6 +730,115 @@ __global__ void gamma_head_scalar_kernel(int nw, double2 *__restrict__ E, long l
   E[lde * nb * (long)w] = val;
 }
 
+// MULTDIFF ON at any q (susceptibility.f:912-925, :1193): the first column
+// of M scaled by the band-energy difference, next to the unscaled column,
+// as the two momentum-like columns of the head/wing sums.
+// P(t,0) = M(t,0) (e_unocc(k,b2) - e_occ(k,b1)), P(t,1) = M(t,0), P(t,2) = 0.
+__global__ void multdiff_cols_kernel(int nocc, int nunocc, long n, const double2 *__restrict__ M,
+                                     long ldm, int colmajor, const double *__restrict__ eo,
+                                     const double *__restrict__ eu, double2 *__restrict__ P,
+                                     long ldp) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long per = (long)nocc * nunocc;
+  const long k = t / per;
+  const int b1 = (int)((t - k * per) / nunocc), b2 = (int)(t % nunocc);
+  const double2 m = colmajor ? M[t] : M[t * ldm];
+  const double d = eu[k * nunocc + b2] - eo[k * nocc + b1];
+  P[t] = make_double2(m.x * d, m.y * d);
+  P[t + ldp] = m;
+  P[t + 2 * ldp] = make_double2(0.0, 0.0);
+}
+
+// Row and column 0 of chi += alpha * (MULTDIFF sums - plain sums).  Wings
+// of the two passes: wA(0,J) = sum D1 M(t,J) conj(P0), wB(1,J) = sum D0
+// M(t,J) conj(M0); heads hA(0,0) = sum D2 |P0|^2, hB(1,1) = sum D0 |M0|^2.
+// cplx: wing(k,i,s,w) with s = 0 for (i,0), s = 1 for (0,i); else the
+// Hermitian (0,J) = conj of (J,0).
+__global__ void multdiff_fix_kernel(int nb, int nw, int cplx, const double2 *__restrict__ hA,
+                                    const double2 *__restrict__ wA,
+                                    const double2 *__restrict__ hB,
+                                    const double2 *__restrict__ wB, double alpha,
+                                    double2 *__restrict__ chi, long ldchi) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x, w = blockIdx.y;
+  if (J >= nb || w >= nw) return;
+  double2 *X = chi + ldchi * (long)nb * w;
+  const int ns = cplx ? 2 : 1;
+  if (J == 0) {
+    const double2 a = hA[9 * w], b = hB[4 + 9 * w];  // (0,0) and (1,1)
+    X[0].x += alpha * (a.x - b.x);
+    X[0].y += alpha * (a.y - b.y);
+    return;
+  }
+  const long wa = 3 * ((long)J + (long)nb * ns * w), wb = 1 + wa;  // k = 0 and k = 1
+  const double2 lo = make_double2(wA[wa].x - wB[wb].x, wA[wa].y - wB[wb].y);  // (J,0)
+  double2 up;
+  if (cplx) {
+    const long s1 = 3L * nb;
+    up = make_double2(wA[wa + s1].x - wB[wb + s1].x, wA[wa + s1].y - wB[wb + s1].y);
+  } else {
+    up = make_double2(lo.x, -lo.y);
+  }
+  double2 &xl = X[J], &xu = X[ldchi * (long)J];
+  xl.x += alpha * lo.x;
+  xl.y += alpha * lo.y;
+  xu.x += alpha * up.x;
+  xu.y += alpha * up.y;
+}
+
+int multdiff(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const void *eo, const void *eu, int nb,
+             const void *M, long ldm, int colmajor, int nw, const void *D0r, const void *D0i,
+             const void *D1r, const void *D1i, const void *D2r, const void *D2i, long ldd,
+             double alpha, void *chi, int ldchi) {
+  const bool cplx = D0i != nullptr;
+  SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && nb > 0 && nw > 0 && ldchi >= nb,
+            "multdiff: bad sizes");
+  SPX_CHECK(ctx, D0r && D1r && D2r && (cplx == (D1i != nullptr)) && (cplx == (D2i != nullptr)),
+            "multdiff: missing weights");
+  const long n = (long)nk * nocc * nunocc;
+  SPX_CHECK(ctx, ldd >= n, "multdiff: ldd < n");
+  SPX_CHECK(ctx, spx_owns(ctx, eo, 8 * (size_t)nk * nocc) &&
+                     spx_owns(ctx, eu, 8 * (size_t)nk * nunocc) &&
+                     spx_owns(ctx, chi, 16 * (size_t)ldchi * nb * nw),
+            "multdiff: energies or chi are not context allocations of the stated shapes");
+  SPX_HIP(ctx, hipSetDevice(ctx->device));
+  const int ns = cplx ? 2 : 1;
+  void *P = nullptr, *hA = nullptr, *hB = nullptr, *wA = nullptr, *wB = nullptr;
+  int rc = spx_alloc(ctx, 16 * 3 * (size_t)n, &P);
+  if (!rc) rc = spx_alloc(ctx, 16 * 9 * (size_t)nw, &hA);
+  if (!rc) rc = spx_alloc(ctx, 16 * 9 * (size_t)nw, &hB);
+  if (!rc) rc = spx_alloc(ctx, 16 * 3 * (size_t)nb * ns * nw, &wA);
+  if (!rc) rc = spx_alloc(ctx, 16 * 3 * (size_t)nb * ns * nw, &wB);
+  if (!rc) {
+    hipLaunchKernelGGL(multdiff_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       ctx->stream, nocc, nunocc, n, (const double2 *)M, ldm, colmajor,
+                       (const double *)eo, (const double *)eu, (double2 *)P, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "multdiff: %s", hipGetErrorString(e));
+  }
+  // MULTDIFF sums: wings with D1 (1/frqs), head with D2 (1/frqs^2); plain
+  // sums of the same row with D0
+  if (!rc)
+    rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D1r, D1i, D2r, D2i, ldd, 1.0, 0.0, hA,
+                   wA);
+  if (!rc)
+    rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D0r, D0i, D0r, D0i, ldd, 1.0, 0.0, hB,
+                   wB);
+  if (!rc) {
+    hipLaunchKernelGGL(multdiff_fix_kernel, dim3((unsigned)((nb + 63) / 64), (unsigned)nw),
+                       dim3(64), 0, ctx->stream, nb, nw, (int)cplx, (const double2 *)hA,
+                       (const double2 *)wA, (const double2 *)hB, (const double2 *)wB, alpha,
+                       (double2 *)chi, (long)ldchi);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "multdiff: %s", hipGetErrorString(e));
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *p : {P, hA, hB, wA, wB})
+    if (p) spx_free(ctx, p);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -786,6 +895,28 @@ int spexhip_headwings_cplx_dev(spexhip_ctx *ctx, long n, int nb, const void *M, 
             "headwings_cplx: give both head weights or neither");
   return headwings(ctx, n, nb, M, ldm, colmajor, P, ldp, nw, Dr, Di, Dhr, Dhi, ldd, alpha, beta,
                    head, wing);
+}
+
+int spexhip_multdiff_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const void *e_occ_dev,
+                         const void *e_unocc_dev, int nb, const void *M_dev, long ldm, int colmajor,
+                         int nw, const void *D0_dev, const void *D1_dev, const void *D2_dev,
+                         long ldd, double alpha, void *chi_dev, int ldchi) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  return multdiff(ctx, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor, nw,
+                  D0_dev, nullptr, D1_dev, nullptr, D2_dev, nullptr, ldd, alpha, chi_dev, ldchi);
+}
+
+int spexhip_multdiff_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                              const void *e_occ_dev, const void *e_unocc_dev, int nb,
+                              const void *M_dev, long ldm, int colmajor, int nw,
+                              const void *D0r_dev, const void *D0i_dev, const void *D1r_dev,
+                              const void *D1i_dev, const void *D2r_dev, const void *D2i_dev,
+                              long ldd, double alpha, void *chi_dev, int ldchi) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, D0i_dev && D1i_dev && D2i_dev, "multdiff_cplx: null imaginary weights");
+  return multdiff(ctx, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor, nw,
+                  D0r_dev, D0i_dev, D1r_dev, D1i_dev, D2r_dev, D2i_dev, ldd, alpha, chi_dev,
+                  ldchi);
 }
 
 int spexhip_gamma_prepare_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx, void *head,

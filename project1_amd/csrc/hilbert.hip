@@ -153,19 +153,22 @@ int spexhip_chi0_cplx_dev(spexhip_ctx *ctx, long n, int nb, const void *M, long 
 
 }  // extern "C"
 
-extern "C" {
+namespace {
 
-// Host drop-in for the HILBERT pair sum of one memory packet: the index
-// and threshold (susceptibility.f:1066-1077), the pair loop with
-// spectral_function (:1159-1251, :1793-1849) and its accumulation into
-// susceph (imaginary frequencies, packed upper) and suscepc (real
-// frequencies, whole matrix, :1231-1236).  All arrays are host arrays in the
-// reference layouts; susceph and suscepc are accumulated.
-int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const double *cprod,
-                                   long ldc, int nfrqs, const double *wtetra, long ldwt,
-                                   double wghtthr, int nfrqh, int nfrqc, const double *wghtSr,
-                                   const double *wghtSi, int precision, double *susceph,
-                                   double *suscepc) {
+// MULTDIFF ON (multdiff_ = 1) inputs of the host drop-in: the Hilbert mesh
+// and the band energies of the packet's transitions.
+struct MultDiff {
+  const double *frqs;
+  int nk, nocc, nunocc;
+  const double *e_occ, *e_unocc;
+};
+
+// The HILBERT pair sum of one memory packet on host arrays; md non-null adds
+// MULTDIFF's first row and column (susceptibility.f:912-925, 1193).
+int susc_hilbert(spexhip_ctx *ctx, long n, int nb, const double *cprod, long ldc, int nfrqs,
+                 const double *wtetra, long ldwt, double wghtthr, int nfrqh, int nfrqc,
+                 const double *wghtSr, const double *wghtSi, int precision, double *susceph,
+                 double *suscepc, const MultDiff *md) {
   if (!ctx) return SPEXHIP_E_INVALID;
   SPX_CHECK(ctx, n >= 0 && nb > 0 && nfrqs >= 1 && nfrqh >= 0 && nfrqc >= 0 && ldc >= n &&
                      ldwt >= n,
@@ -188,6 +191,44 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
   if (!rc) rc = spx_alloc(ctx, db, &dDr);
   if (!rc && nfrqc) rc = spx_alloc(ctx, db, &dDi);
   if (!rc) rc = spx_alloc(ctx, cb, &dC);
+  // MULTDIFF: wghtSr and wghtSi divided by frqs^l (l = 1, 2) where frqs > 0,
+  // their folds, and the energies (the plain weights D0 are dDr / dDi)
+  void *dS1 = nullptr, *dS2 = nullptr, *dS1i = nullptr, *dS2i = nullptr, *dD1 = nullptr,
+       *dD2 = nullptr, *dD1i = nullptr, *dD2i = nullptr, *dEo = nullptr, *dEu = nullptr;
+  if (md) {
+    const size_t eb = 8 * (size_t)md->nk * md->nocc, ub = 8 * (size_t)md->nk * md->nunocc;
+    if (!rc) rc = spx_alloc(ctx, sb, &dS1);
+    if (!rc) rc = spx_alloc(ctx, sb, &dS2);
+    if (!rc && nfrqc) rc = spx_alloc(ctx, sb, &dS1i);
+    if (!rc && nfrqc) rc = spx_alloc(ctx, sb, &dS2i);
+    if (!rc) rc = spx_alloc(ctx, db, &dD1);
+    if (!rc) rc = spx_alloc(ctx, db, &dD2);
+    if (!rc && nfrqc) rc = spx_alloc(ctx, db, &dD1i);
+    if (!rc && nfrqc) rc = spx_alloc(ctx, db, &dD2i);
+    if (!rc) rc = spx_alloc(ctx, eb, &dEo);
+    if (!rc) rc = spx_alloc(ctx, ub, &dEu);
+    std::vector<double> tab((size_t)nfrqs * nf);
+    auto put = [&](const double *S, int l, void *dst) {
+      for (int f = 0; f < nf; f++)
+        for (int i = 0; i < nfrqs; i++) {
+          const size_t ix = i + (size_t)nfrqs * f;
+          const double r = md->frqs[i] > 0 ? md->frqs[i] : 1.0;
+          tab[ix] = l == 1 ? S[ix] / r : S[ix] / (r * r);
+        }
+      return hipMemcpy(dst, tab.data(), sb, hipMemcpyHostToDevice);
+    };
+    if (!rc) {
+      hipError_t e = put(wghtSr, 1, dS1);
+      if (e == hipSuccess) e = put(wghtSr, 2, dS2);
+      if (e == hipSuccess && nfrqc) e = put(wghtSi, 1, dS1i);
+      if (e == hipSuccess && nfrqc) e = put(wghtSi, 2, dS2i);
+      if (e == hipSuccess) e = hipMemcpy(dEo, md->e_occ, eb, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(dEu, md->e_unocc, ub, hipMemcpyHostToDevice);
+      if (e != hipSuccess)
+        rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "susceptibility_hilbert: upload failed: %s",
+                        hipGetErrorString(e));
+    }
+  }
   std::vector<double> full;
   if (!rc) {
     hipError_t e = hipMemcpy2DAsync(dM, 16 * n, cprod, 16 * ldc, 16 * n, nb,
@@ -208,6 +249,17 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
     rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k,
                                      (const double *)dSr + (size_t)nfrqs * f0, nfrqs, dDr, n);
     if (!rc) rc = spx_chi0_launch(ctx, ctx->stream, n, nb, dM, n, 1, k, dDr, precision, 1.0, 0.0, dC, nb);
+    if (!rc && md) {
+      const size_t off = (size_t)nfrqs * f0;
+      rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k,
+                                       (const double *)dS1 + off, nfrqs, dD1, n);
+      if (!rc)
+        rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k,
+                                         (const double *)dS2 + off, nfrqs, dD2, n);
+      if (!rc)
+        rc = spexhip_multdiff_dev(ctx, md->nk, md->nocc, md->nunocc, dEo, dEu, nb, dM, n, 1, k,
+                                  dDr, dD1, dD2, n, 1.0, dC, nb);
+    }
     if (!rc)
       rc = spexhip_pack_upper_d2h(ctx, nb, k, dC, nb, susceph + 2 * (size_t)nb * (nb + 1) / 2 * f0, 1);
   }
@@ -222,6 +274,17 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
                                        nfrqs, dDi, n);
     if (!rc)
       rc = spexhip_chi0_cplx_dev(ctx, n, nb, dM, n, 1, k, dDr, dDi, precision, 1.0, dC, nb);
+    if (!rc && md) {
+      const struct {
+        void *S, *D;
+      } fold[4] = {{dS1, dD1}, {dS1i, dD1i}, {dS2, dD2}, {dS2i, dD2i}};
+      for (int j = 0; !rc && j < 4; j++)
+        rc = spexhip_weights_hilbert_dev(ctx, n, nfrqs, dT, n, wghtthr, k,
+                                         (const double *)fold[j].S + col, nfrqs, fold[j].D, n);
+      if (!rc)
+        rc = spexhip_multdiff_cplx_dev(ctx, md->nk, md->nocc, md->nunocc, dEo, dEu, nb, dM, n, 1,
+                                       k, dDr, dDi, dD1, dD1i, dD2, dD2i, n, 1.0, dC, nb);
+    }
     if (!rc) {
       full.resize(2 * (size_t)nb * nb * k);
       hipError_t e = hipMemcpyAsync(full.data(), dC, 16 * (size_t)nb * nb * k,
@@ -235,9 +298,50 @@ int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const doubl
     }
   }
   (void)hipStreamSynchronize(ctx->stream);
-  for (void *p : {dM, dT, dSr, dSi, dDr, dDi, dC})
+  for (void *p : {dM, dT, dSr, dSi, dDr, dDi, dC, dS1, dS2, dS1i, dS2i, dD1, dD2, dD1i,
+                  dD2i, dEo, dEu})
     if (p) spx_free(ctx, p);
   return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Host drop-in for the HILBERT pair sum of one memory packet: the index
+// and threshold (susceptibility.f:1066-1077), the pair loop with
+// spectral_function (:1159-1251, :1793-1849) and its accumulation into
+// susceph (imaginary frequencies, packed upper) and suscepc (real
+// frequencies, whole matrix, :1231-1236).  All arrays are host arrays in the
+// reference layouts; susceph and suscepc are accumulated.
+int spexhip_susceptibility_hilbert(spexhip_ctx *ctx, long n, int nb, const double *cprod,
+                                   long ldc, int nfrqs, const double *wtetra, long ldwt,
+                                   double wghtthr, int nfrqh, int nfrqc, const double *wghtSr,
+                                   const double *wghtSi, int precision, double *susceph,
+                                   double *suscepc) {
+  return susc_hilbert(ctx, n, nb, cprod, ldc, nfrqs, wtetra, ldwt, wghtthr, nfrqh, nfrqc, wghtSr,
+                      wghtSi, precision, susceph, suscepc, nullptr);
+}
+
+// The same with MULTDIFF ON (multdiff_ = 1, susceptibility.f:588-592,
+// 912-925, 1193, 1817-1824): cprod is the unscaled packet, frqs the Hilbert
+// mesh, e_occ [nk][nocc] at k and e_unocc [nk][nunocc] at k+q of its
+// n = nk*nocc*nunocc transitions.
+int spexhip_susceptibility_hilbert_multdiff(spexhip_ctx *ctx, long n, int nb, const double *cprod,
+                                            long ldc, int nfrqs, const double *wtetra, long ldwt,
+                                            double wghtthr, int nfrqh, int nfrqc,
+                                            const double *wghtSr, const double *wghtSi,
+                                            int precision, const double *frqs, int nk, int nocc,
+                                            int nunocc, const double *e_occ,
+                                            const double *e_unocc, double *susceph,
+                                            double *suscepc) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, frqs && e_occ && e_unocc && nk > 0 && nocc > 0 && nunocc > 0 &&
+                     (long)nk * nocc * nunocc == n,
+            "susceptibility_hilbert_multdiff: frqs, energies or n = nk*nocc*nunocc missing");
+  const MultDiff md{frqs, nk, nocc, nunocc, e_occ, e_unocc};
+  return susc_hilbert(ctx, n, nb, cprod, ldc, nfrqs, wtetra, ldwt, wghtthr, nfrqh, nfrqc, wghtSr,
+                      wghtSi, precision, susceph, suscepc, &md);
 }
 
 }  // extern "C"

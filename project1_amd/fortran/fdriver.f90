@@ -1,8 +1,9 @@
 ! Fortran host exercising the ISO_C_BINDING boundary: one packet of the
 ! HILBERT NONE pair sum and the dielectric inverse, compared with a plain
 ! Fortran pair loop written the way susceptibility.f:1183-1240 does it; the
-! screened interaction, the RPA integrand, the HILBERT pair sum, the
-! Gamma-point head and wings and the equivalent-k averaging likewise.
+! screened interaction, the RPA integrand, the HILBERT pair sum (also with
+! MULTDIFF), the Gamma-point head and wings and the equivalent-k averaging
+! likewise.
 ! Exit status 0 on agreement (normwise relative error < 1e-12).
 program fdriver
   use, intrinsic :: iso_c_binding
@@ -120,8 +121,7 @@ program fdriver
     integer, parameter :: nfs = 10, nfh = 2, nfc = 2
     real(c_double), target :: wt(n, nfs), wsr(nfs, nfh+nfc), wsi(nfs, nfh+nfc)
     complex(c_double_complex), target :: sh(npack, nfh), sc(nb, nb, nfc)
-    complex(c_double_complex) :: rh(npack, nfh), rcx(nb, nb, nfc), cdum, cdum1
-    real(c_double) :: reS(nfs), imS(nfs), s1, s2, c1, c2
+    complex(c_double_complex) :: rh(npack, nfh), rcx(nb, nb, nfc)
     integer :: s
     do s = 1, nfs
       do k = 1, n
@@ -133,47 +133,50 @@ program fdriver
         wsi(s, f) = 0.05d0 * sin(0.7d0*s + f)
       end do
     end do
-    rh = 0
-    rcx = 0
-    do j = 1, nb
-      do i = 1, j
-        do k = 1, n
-          re(k) = real(cprod(k, i) * conjg(cprod(k, j)))
-          im(k) = aimag(cprod(k, i) * conjg(cprod(k, j)))
-        end do
-        do s = 1, nfs
-          reS(s) = 0
-          imS(s) = 0
-          do k = 1, n
-            if (wt(k, s) > 1d-6) then
-              reS(s) = reS(s) + re(k) * wt(k, s)
-              imS(s) = imS(s) + im(k) * wt(k, s)
-            end if
-          end do
-        end do
-        ij = j*(j-1)/2 + i
-        do f = 1, nfh
-          rh(ij, f) = rh(ij, f) + cmplx(dot_product(reS, wsr(:, f)), dot_product(imS, wsr(:, f)), &
-                                        kind=c_double_complex)
-        end do
-        do f = 1, nfc
-          s1 = dot_product(reS, wsr(:, nfh+f))
-          s2 = dot_product(imS, wsr(:, nfh+f))
-          c1 = dot_product(reS, wsi(:, nfh+f))
-          c2 = dot_product(imS, wsi(:, nfh+f))
-          cdum = cmplx(s1, c1, kind=c_double_complex)
-          cdum1 = cmplx(-c2, s2, kind=c_double_complex)
-          rcx(i, j, f) = rcx(i, j, f) + cdum + cdum1
-          if (i /= j) rcx(j, i, f) = rcx(j, i, f) + cdum - cdum1
-        end do
-      end do
-    end do
+    call hilbert_literal(cprod, wt, wsr, wsi, nfh, .false., wsr(:, 1), rh, rcx)
     sh = 0
     sc = 0
     call spexhip_pairsum_hilbert(ctx, n, nb, cprod, nfs, wt, 1d-6, nfh, nfc, wsr, wsi, sh, sc)
     err = max(maxval(abs(sh - rh)) / maxval(abs(rh)), maxval(abs(sc - rcx)) / maxval(abs(rcx)))
     write(*, '(A,ES10.3)') 'fortran pairsum_hilbert max-rel error: ', err
     if (err > 1d-12) stop 7
+    ! MULTDIFF ON (susceptibility.f:912-925, 1193, 1817-1824): n = 2 k points
+    ! x 4 occupied x 12 unoccupied bands; the mesh starts at 0
+    block
+      integer, parameter :: nko = 2, no = 4, nu = 12
+      real(c_double), target :: eo(no, nko), eu(nu, nko), fq(nfs)
+      complex(c_double_complex) :: cm(n, nb)
+      integer :: ik, b1, b2
+      do s = 1, nfs
+        fq(s) = 0.15d0 * (s - 1)
+      end do
+      do ik = 1, nko
+        do b1 = 1, no
+          eo(b1, ik) = -0.3d0 * b1 - 0.1d0 * ik
+        end do
+        do b2 = 1, nu
+          eu(b2, ik) = 0.05d0 * b2 + 0.2d0 * ik
+        end do
+      end do
+      cm = cprod
+      k = 0
+      do ik = 1, nko
+        do b1 = 1, no
+          do b2 = 1, nu
+            k = k + 1
+            cm(k, 1) = cm(k, 1) * (eu(b2, ik) - eo(b1, ik))
+          end do
+        end do
+      end do
+      call hilbert_literal(cm, wt, wsr, wsi, nfh, .true., fq, rh, rcx)
+      sh = 0
+      sc = 0
+      call spexhip_pairsum_hilbert(ctx, n, nb, cprod, nfs, wt, 1d-6, nfh, nfc, wsr, wsi, sh, sc, &
+                                   frqs=fq, ene_occ=eo, ene_unocc=eu)
+      err = max(maxval(abs(sh - rh)) / maxval(abs(rh)), maxval(abs(sc - rcx)) / maxval(abs(rcx)))
+      write(*, '(A,ES10.3)') 'fortran pairsum_hilbert MULTDIFF max-rel error: ', err
+      if (err > 1d-12) stop 13
+    end block
   end block
 
   ! Gamma point: head and wings of the packet against plain loops written the
@@ -343,4 +346,71 @@ program fdriver
   end block
   rc = spexhip_destroy(ctx)
   write(*, '(A)') 'fortran driver OK'
+
+contains
+
+  ! The HILBERT pair loop written the way susceptibility.f:1183-1236 and
+  ! spectral_function (:1793-1849) do it; md: MULTDIFF's division of the
+  ! spectral function of a pair with l of its indices on the first basis
+  ! function by fq(s)**l where fq(s) > 0 (:1193, :1817-1824).
+  subroutine hilbert_literal(cprod, wt, wsr, wsi, nfh, md, fq, rh, rcx)
+    complex(c_double_complex), intent(in) :: cprod(:, :)
+    real(c_double), intent(in) :: wt(:, :), wsr(:, :), wsi(:, :), fq(:)
+    integer, intent(in) :: nfh
+    logical, intent(in) :: md
+    complex(c_double_complex), intent(out) :: rh(:, :), rcx(:, :, :)
+    real(c_double) :: re(size(cprod, 1)), im(size(cprod, 1)), reS(size(wt, 2)), &
+                      imS(size(wt, 2)), s1, s2, c1, c2
+    complex(c_double_complex) :: cdum, cdum1
+    integer :: i, j, k, s, f, ij, l, n, nb, nfs, nfc
+    n = size(cprod, 1)
+    nb = size(cprod, 2)
+    nfs = size(wt, 2)
+    nfc = size(wsr, 2) - nfh
+    rh = 0
+    rcx = 0
+    do j = 1, nb
+      do i = 1, j
+        do k = 1, n
+          re(k) = real(cprod(k, i) * conjg(cprod(k, j)))
+          im(k) = aimag(cprod(k, i) * conjg(cprod(k, j)))
+        end do
+        do s = 1, nfs
+          reS(s) = 0
+          imS(s) = 0
+          do k = 1, n
+            if (wt(k, s) > 1d-6) then
+              reS(s) = reS(s) + re(k) * wt(k, s)
+              imS(s) = imS(s) + im(k) * wt(k, s)
+            end if
+          end do
+        end do
+        if (md) then
+          l = count([i, j] == 1)
+          do s = 1, nfs
+            if (l > 0 .and. fq(s) > 0) then
+              reS(s) = reS(s) / fq(s)**l
+              imS(s) = imS(s) / fq(s)**l
+            end if
+          end do
+        end if
+        ij = j*(j-1)/2 + i
+        do f = 1, nfh
+          rh(ij, f) = rh(ij, f) + cmplx(dot_product(reS, wsr(:, f)), dot_product(imS, wsr(:, f)), &
+                                        kind=c_double_complex)
+        end do
+        do f = 1, nfc
+          s1 = dot_product(reS, wsr(:, nfh+f))
+          s2 = dot_product(imS, wsr(:, nfh+f))
+          c1 = dot_product(reS, wsi(:, nfh+f))
+          c2 = dot_product(imS, wsi(:, nfh+f))
+          cdum = cmplx(s1, c1, kind=c_double_complex)
+          cdum1 = cmplx(-c2, s2, kind=c_double_complex)
+          rcx(i, j, f) = rcx(i, j, f) + cdum + cdum1
+          if (i /= j) rcx(j, i, f) = rcx(j, i, f) + cdum - cdum1
+        end do
+      end do
+    end do
+  end subroutine
+
 end program fdriver

@@ -78,6 +78,15 @@ module m_spexhip
       integer(c_int), value :: nb, nfrqs, nfrqh, nfrqc, prec
       real(c_double), value :: wghtthr
     end function
+    integer(c_int) function c_susceptibility_hilbert_multdiff(ctx, n, nb, cprod, ldc, nfrqs, &
+        wtetra, ldwt, wghtthr, nfrqh, nfrqc, wghtSr, wghtSi, prec, frqs, nk, nocc, nunocc, &
+        e_occ, e_unocc, sush, susc) bind(C, name='spexhip_susceptibility_hilbert_multdiff')
+      import :: c_int, c_long, c_ptr, c_double
+      type(c_ptr), value :: ctx, cprod, wtetra, wghtSr, wghtSi, frqs, e_occ, e_unocc, sush, susc
+      integer(c_long), value :: n, ldc, ldwt
+      integer(c_int), value :: nb, nfrqs, nfrqh, nfrqc, prec, nk, nocc, nunocc
+      real(c_double), value :: wghtthr
+    end function
     integer(c_int) function c_susceptibility_headwings(ctx, n, nb, cprod, ldc, dcprod, lddc, &
         nfrqh, wh, whh, nfrqc, wcr, wci, wchr, wchi, headh, wingh, headc, wingc) &
         bind(C, name='spexhip_susceptibility_headwings')
@@ -248,9 +257,13 @@ contains
 
   ! HILBERT branch of the pair loop (susceptibility.f:1066-1251): wtetra(n,nfrqs)
   ! as filled by tetrahedron5_init (before indexing), wghtSr/wghtSi(nfrqs,
-  ! nfrqh+nfrqc) from getwghtS; susceph and suscepc are accumulated.
+  ! nfrqh+nfrqc) from getwghtS; susceph and suscepc are accumulated.  With
+  ! frqs, ene_occ(nocc,nk) (at k) and ene_unocc(nunocc,nk) (at k+q) present,
+  ! MULTDIFF is ON (multdiff_ = 1, :912-925, 1193): cprod is then the packet
+  ! before the scaling of its first column, n = nk*nocc*nunocc.
   subroutine spexhip_pairsum_hilbert(ctx, n, dim, cprod, nfrqs, wtetra, wghtthr, nfrqh, nfrqc, &
-                                     wghtSr, wghtSi, susceph, suscepc, precision)
+                                     wghtSr, wghtSi, susceph, suscepc, precision, frqs, &
+                                     ene_occ, ene_unocc)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: n, dim, nfrqs, nfrqh, nfrqc
     complex(c_double_complex), intent(in), target :: cprod(n, dim)
@@ -260,9 +273,20 @@ contains
     complex(c_double_complex), intent(inout), target :: susceph(dim*(dim+1)/2, max(1,nfrqh)), &
                                                         suscepc(dim, dim, max(1,nfrqc))
     integer(c_int), intent(in), optional :: precision
+    real(c_double), intent(in), optional, target :: frqs(nfrqs)
+    real(c_double), intent(in), optional, target, contiguous :: ene_occ(:,:), ene_unocc(:,:)
     integer(c_int) :: prec
     prec = SPEXHIP_PREC_FP64
     if (present(precision)) prec = precision
+    if (present(frqs) .and. present(ene_occ) .and. present(ene_unocc)) then
+      call check(ctx, c_susceptibility_hilbert_multdiff(ctx, int(n, c_long), int(dim, c_int), &
+                 c_loc(cprod), int(n, c_long), int(nfrqs, c_int), c_loc(wtetra), int(n, c_long), &
+                 wghtthr, int(nfrqh, c_int), int(nfrqc, c_int), c_loc(wghtSr), c_loc(wghtSi), &
+                 prec, c_loc(frqs), int(size(ene_occ, 2), c_int), int(size(ene_occ, 1), c_int), &
+                 int(size(ene_unocc, 1), c_int), c_loc(ene_occ), c_loc(ene_unocc), &
+                 c_loc(susceph), c_loc(suscepc)), 'pairsum_hilbert_multdiff')
+      return
+    end if
     call check(ctx, c_susceptibility_hilbert(ctx, int(n, c_long), int(dim, c_int), c_loc(cprod), &
                int(n, c_long), int(nfrqs, c_int), c_loc(wtetra), int(n, c_long), wghtthr, &
                int(nfrqh, c_int), int(nfrqc, c_int), c_loc(wghtSr), c_loc(wghtSi), prec, &

@@ -345,6 +345,9 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_susceptibility_none": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, I, _vp]),
         "spexhip_susceptibility_hilbert": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long,
                                                C.c_double, I, I, _vp, _vp, I, _vp, _vp]),
+        "spexhip_susceptibility_hilbert_multdiff": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp,
+                                                        _c_long, C.c_double, I, I, _vp, _vp, I,
+                                                        _vp, I, I, I, _vp, _vp, _vp, _vp]),
         "spexhip_tetra_weights_dev": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp, _vp,
                                           _c_long]),
         "spexhip_tetra_average_degenerate_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long]),
@@ -363,6 +366,11 @@ def _declare(L: C.CDLL) -> None:
                                                   I, I, C.c_double, I, _vp, _vp, I, _vp, I,
                                                   _vp]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
+        "spexhip_multdiff_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I, _vp, _vp,
+                                     _vp, _c_long, C.c_double, _vp, I]),
+        "spexhip_multdiff_cplx_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I, _vp,
+                                          _vp, _vp, _vp, _vp, _vp, _c_long, C.c_double, _vp,
+                                          I]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
         "spexhip_headwings_cplx_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I,
@@ -828,11 +836,14 @@ class Context:
     def susceptibility_hilbert(self, cprod: np.ndarray, wtetra: np.ndarray, wghtthr: float,
                                wghtSr: np.ndarray, wghtSi: np.ndarray, nfrqh: int,
                                susceph: np.ndarray, suscepc: np.ndarray,
-                               precision: int = PREC_FP64) -> None:
+                               precision: int = PREC_FP64, multdiff=None) -> None:
         """Accumulate one packet's HILBERT pair sum: ``susceph`` (npack, nfrqh)
         and ``suscepc`` (nb, nb, nfrqc), both Fortran-ordered complex128.
         ``wtetra`` (n, nfrqs), ``wghtSr``/``wghtSi`` (nfrqs, nfrqh+nfrqc)
-        Fortran-ordered real.  Mirrors ``susceptibility.f:1066-1251``."""
+        Fortran-ordered real.  Mirrors ``susceptibility.f:1066-1251``.
+        ``multdiff = (frqs, e_occ, e_unocc)`` switches MULTDIFF ON
+        (``:912-925, 1193``): the Hilbert mesh, e_occ (nk, nocc) at k and
+        e_unocc (nk, nunocc) at k+q."""
         cp = np.asfortranarray(cprod, dtype=np.complex128)
         n, nb = cp.shape
         wt = np.asfortranarray(wtetra, dtype=np.float64)
@@ -844,9 +855,19 @@ class Context:
         assert susceph.dtype == np.complex128 and susceph.size == nfrqh * nb * (nb + 1) // 2
         assert suscepc.dtype == np.complex128 and suscepc.size == nfrqc * nb * nb
         assert suscepc.flags.f_contiguous and (susceph.flags.f_contiguous or susceph.flags.c_contiguous)
-        self._check(lib().spexhip_susceptibility_hilbert(
+        if multdiff is None:
+            self._check(lib().spexhip_susceptibility_hilbert(
+                self.h, n, nb, cp.ctypes.data, n, nfrqs, wt.ctypes.data, n, wghtthr, nfrqh, nfrqc,
+                sr.ctypes.data, si.ctypes.data, precision, susceph.ctypes.data,
+                suscepc.ctypes.data))
+            return
+        fr, eo, eu = (np.ascontiguousarray(a, dtype=np.float64) for a in multdiff)
+        assert fr.shape == (nfrqs,) and eo.shape[0] == eu.shape[0]
+        self._check(lib().spexhip_susceptibility_hilbert_multdiff(
             self.h, n, nb, cp.ctypes.data, n, nfrqs, wt.ctypes.data, n, wghtthr, nfrqh, nfrqc,
-            sr.ctypes.data, si.ctypes.data, precision, susceph.ctypes.data, suscepc.ctypes.data))
+            sr.ctypes.data, si.ctypes.data, precision, fr.ctypes.data, eo.shape[0], eo.shape[1],
+            eu.shape[1], eo.ctypes.data, eu.ctypes.data, susceph.ctypes.data,
+            suscepc.ctypes.data))
 
     # -- Gamma point: head and wings --------------------------------------
     def momentum_scale_dev(self, e_occ: DevBuf, e_unocc: DevBuf, nk: int, nocc: int,
@@ -1001,6 +1022,23 @@ class Context:
         r = _rlat_f(rlat)
         self._check(lib().spexhip_fermi_weights_dev(self.h, _ptr(nk3), _ptr(r), nband, ene.ptr,
                                                     float(efermi), w.ptr))
+
+    def multdiff_dev(self, nk: int, nocc: int, nunocc: int, e_occ: DevBuf, e_unocc: DevBuf,
+                     nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int, D0: DevBuf,
+                     D1: DevBuf, D2: DevBuf, ldd: int, alpha: float, chi: DevBuf,
+                     ldchi: int) -> None:
+        """MULTDIFF ON rows of chi (imaginary axis), after chi0_dev with D0."""
+        self._check(lib().spexhip_multdiff_dev(
+            self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, nb, M.ptr, ldm, int(colmajor), nw,
+            D0.ptr, D1.ptr, D2.ptr, ldd, alpha, chi.ptr, ldchi))
+
+    def multdiff_cplx_dev(self, nk: int, nocc: int, nunocc: int, e_occ: DevBuf, e_unocc: DevBuf,
+                          nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int, D: list,
+                          ldd: int, alpha: float, chi: DevBuf, ldchi: int) -> None:
+        """Real-frequency form: D = [D0r, D0i, D1r, D1i, D2r, D2i]."""
+        self._check(lib().spexhip_multdiff_cplx_dev(
+            self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, nb, M.ptr, ldm, int(colmajor), nw,
+            *[d.ptr for d in D], ldd, alpha, chi.ptr, ldchi))
 
     def plasma_frequency_dev(self, nk: int, nband: int, w: DevBuf, P: DevBuf, ldp: int, deg,
                              vol: float, spin_factor: float = 2.0) -> float:

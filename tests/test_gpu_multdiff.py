@@ -1,0 +1,163 @@
+"""GPU parity of MULTDIFF ON at a q point other than Gamma (the first basis
+function's products times the band-energy difference, 1/frqs**l Hilbert
+weights on its row and column): chi0 with the plain weights followed by the
+MULTDIFF update, through the C ABI, against the oracle's literal pair loop
+(imaginary and real frequencies, both layouts of M)."""
+import numpy as np
+import pytest
+
+from oracle import gamma, hilbert
+from project1_amd import spexhip
+from tests.conftest import normwise
+
+pytestmark = pytest.mark.gpu
+
+
+def _f64_dev(ctx, a):
+    """(rows, cols) float64 in column-major (rows fastest) on the device."""
+    return ctx.to_dev(np.ascontiguousarray(np.asarray(a, np.float64).T))
+
+
+def _case(seed, nk, nocc, nunocc, nb, zero_mesh_point):
+    g = np.random.default_rng(seed)
+    if zero_mesh_point:  # frqs(1) = 0: its weights stay undivided
+        f = np.concatenate([[0.0], np.cumsum(g.uniform(0.02, 0.3, 23))])
+    else:
+        f = hilbert.hilbert_mesh((0.005, 1.02), 0.05, 3.0)
+    eo = g.uniform(-1.5, -0.1, (nk, nocc))
+    eu = g.uniform(0.1, 2.0, (nk, nunocc))
+    n = nk * nocc * nunocc
+    wt = np.zeros((n, len(f)))
+    s0 = g.integers(0, len(f) - 4, n)
+    for k in range(4):
+        wt[np.arange(n), s0 + k] = g.uniform(0, 1, n)
+    M = (g.standard_normal((n, nb)) + 1j * g.standard_normal((n, nb))) / np.sqrt(nb)
+    return g, f, eo, eu, wt, M
+
+
+def _divided_weights(ctx, f, wt, wS, n, nw):
+    """D_l for l = 0, 1, 2 folded on the GPU from wghtS / frqs**l."""
+    scale = np.where(f > 0, f, 1.0)
+    dWt = _f64_dev(ctx, wt)
+    out = []
+    for ll in range(3):
+        dS = _f64_dev(ctx, wS / scale[:, None] ** ll)
+        dD = ctx.alloc(8 * n * nw)
+        ctx.weights_hilbert_dev(n, len(f), dWt, n, 0.0, nw, dS, len(f), dD, n)
+        dS.free()
+        out.append(dD)
+    dWt.free()
+    return out
+
+
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-12),
+                                           (spexhip.PREC_SPLIT_FP32, 1e-8)])
+@pytest.mark.parametrize("colmajor,zero_mesh_point", [(False, False), (True, True)])
+def test_multdiff_imaginary_axis_vs_oracle(ctx, precision, tol, colmajor, zero_mesh_point):
+    nk, nocc, nunocc, nb = 3, 4, 7, 37
+    g, f, eo, eu, wt, M = _case(31, nk, nocc, nunocc, nb, zero_mesh_point)
+    n = nk * nocc * nunocc
+    freqs = 1j * np.array([0.05, 0.4, 1.3, 3.7])
+    nw = len(freqs)
+    wS = g.standard_normal((len(f), nw)) if zero_mesh_point else \
+        hilbert.hilbert_weights(freqs, f, 0.0, 0, 0)[0]
+    D0, D1, D2 = _divided_weights(ctx, f, wt, wS, n, nw)
+    dM = ctx.to_dev(np.asfortranarray(M).T if colmajor else np.ascontiguousarray(M))
+    ldm = n if colmajor else nb
+    deo, deu = ctx.to_dev(eo), ctx.to_dev(eu)
+    dC = ctx.alloc(16 * nb * nb * nw).zero()
+    ctx.chi0_dev(n, nb, dM, ldm, colmajor, nw, D0, precision, dC, nb, alpha=2.0)
+    print(f"multdiff imag prec={precision} colmajor={colmajor} zero={zero_mesh_point}", flush=True)
+    ctx.multdiff_dev(nk, nocc, nunocc, deo, deu, nb, dM, ldm, colmajor, nw, D0, D1, D2, n, 2.0,
+                     dC, nb)
+    got = ctx.pack_upper(dC, nb, nw, nb).T
+    ref = 2.0 * gamma.chi0_multdiff(M, eo, eu, wt, f, wS)
+    plain = 2.0 * hilbert.chi0_hilbert_pairsum(M, wt, wS)
+    for w in range(nw):
+        want = np.array([ref[i, j, w] for j in range(nb) for i in range(j + 1)])
+        mx, fro = normwise(got[:, w], want)
+        print(f"  w={w}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+        assert mx < tol
+        # row 0 did change: the test is not passing on the plain sums
+        assert abs(ref[0, 5, w] - plain[0, 5, w]) > 1e-3 * abs(plain[0, 5, w])
+    for b in (D0, D1, D2, dM, deo, deu, dC):
+        b.free()
+
+
+@pytest.mark.parametrize("colmajor", [False, True])
+def test_multdiff_real_frequencies_vs_oracle(ctx, colmajor):
+    nk, nocc, nunocc, nb = 2, 5, 6, 29
+    g, f, eo, eu, wt, M = _case(32, nk, nocc, nunocc, nb, colmajor)
+    n = nk * nocc * nunocc
+    if colmajor:
+        wSr = g.standard_normal((len(f), 3))
+        wSi = g.standard_normal((len(f), 3))
+    else:
+        wSr, wSi = hilbert.hilbert_weights(np.array([0.37, 1.21, 2.5]), f, 0.01, 0, 0)
+    nw = wSr.shape[1]
+    Dr = _divided_weights(ctx, f, wt, wSr, n, nw)
+    Di = _divided_weights(ctx, f, wt, wSi, n, nw)
+    dM = ctx.to_dev(np.asfortranarray(M).T if colmajor else np.ascontiguousarray(M))
+    ldm = n if colmajor else nb
+    deo, deu = ctx.to_dev(eo), ctx.to_dev(eu)
+    dC = ctx.alloc(16 * nb * nb * nw)
+    ctx.chi0_cplx_dev(n, nb, dM, ldm, colmajor, nw, Dr[0], Di[0], spexhip.PREC_FP64, dC, nb,
+                      alpha=2.0)
+    print(f"multdiff real colmajor={colmajor}", flush=True)
+    ctx.multdiff_cplx_dev(nk, nocc, nunocc, deo, deu, nb, dM, ldm, colmajor, nw,
+                          [Dr[0], Di[0], Dr[1], Di[1], Dr[2], Di[2]], n, 2.0, dC, nb)
+    got = dC.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)
+    want = 2.0 * gamma.chi0_multdiff(M, eo, eu, wt, f, wSr, wSi)
+    for w in range(nw):
+        mx, fro = normwise(got[:, :, w], want[:, :, w])
+        print(f"  w={w}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+        assert mx < 1e-12
+    for b in Dr + Di + [dM, deo, deu, dC]:
+        b.free()
+
+
+def test_multdiff_rejects_bad_arguments(ctx):
+    nk, nocc, nunocc, nb, nw = 1, 2, 2, 3, 1
+    n = nk * nocc * nunocc
+    deo, deu = ctx.to_dev(np.zeros((nk, nocc))), ctx.to_dev(np.ones((nk, nunocc)))
+    dM = ctx.to_dev(np.zeros((n, nb), np.complex128))
+    dD = ctx.to_dev(np.zeros((nw, n)))
+    dC = ctx.alloc(16 * nb * nb * nw)
+    with pytest.raises(spexhip.SpexError, match="ldd < n"):
+        ctx.multdiff_dev(nk, nocc, nunocc, deo, deu, nb, dM, nb, False, nw, dD, dD, dD, n - 1,
+                         1.0, dC, nb)
+    with pytest.raises(spexhip.SpexError, match="energies or chi"):
+        ctx.multdiff_dev(nk, nocc, nunocc + 1, deo, deu, nb, dM, nb, False, nw, dD, dD, dD,
+                         n + 2, 1.0, dC, nb)
+    for b in (deo, deu, dM, dD, dC):
+        b.free()
+
+
+def test_host_dropin_multdiff_both_axes(ctx):
+    """spexhip_susceptibility_hilbert_multdiff on host arrays: susceph and
+    suscepc accumulated, against the oracle's pair loop with the reference's
+    threshold."""
+    nk, nocc, nunocc, nb = 2, 3, 8, 21
+    g, f, eo, eu, wt, M = _case(33, nk, nocc, nunocc, nb, True)
+    wt[g.uniform(0, 1, wt.shape) < 0.05] = 1e-9
+    nfrqh, nfrqc = 3, 2
+    wSr = g.standard_normal((len(f), nfrqh + nfrqc))
+    wSi = g.standard_normal((len(f), nfrqh + nfrqc))
+    npack = nb * (nb + 1) // 2
+    base_h = g.standard_normal((npack, nfrqh)) + 1j * g.standard_normal((npack, nfrqh))
+    base_c = g.standard_normal((nb, nb, nfrqc)) + 1j * g.standard_normal((nb, nb, nfrqc))
+    sh = np.asfortranarray(base_h.copy())
+    sc = np.asfortranarray(base_c.copy())
+    print("multdiff host drop-in", flush=True)
+    ctx.susceptibility_hilbert(M, wt, 1e-6, wSr, wSi, nfrqh, sh, sc, multdiff=(f, eo, eu))
+    rh = gamma.chi0_multdiff(M, eo, eu, wt, f, wSr[:, :nfrqh], None, 1e-6)
+    rc = gamma.chi0_multdiff(M, eo, eu, wt, f, wSr[:, nfrqh:], wSi[:, nfrqh:], 1e-6)
+    for w in range(nfrqh):
+        want = np.array([rh[i, j, w] for j in range(nb) for i in range(j + 1)])
+        mx, _ = normwise(sh[:, w] - base_h[:, w], want)
+        print(f"  imag w={w}: max-rel {mx:.2e}")
+        assert mx < 1e-12
+    for w in range(nfrqc):
+        mx, _ = normwise(sc[:, :, w] - base_c[:, :, w], rc[:, :, w])
+        print(f"  real w={w}: max-rel {mx:.2e}")
+        assert mx < 1e-12

@@ -171,6 +171,64 @@ def test_multdiff_spectral_function_is_a_fold_with_scaled_hilbert_weights():
         np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
 
 
+def _multdiff_on_mesh(g, nk, nocc, nunocc, nfrqs):
+    """Energies whose every transition difference is a positive mesh point
+    frqs(s_t), and tetrahedron weights nonzero only at that s_t."""
+    frqs = np.concatenate([[0.0], np.cumsum(g.uniform(0.05, 0.4, nfrqs - 1))])
+    c = g.uniform(-1.0, 0.0, nk)
+    s_of = g.integers(1, nfrqs, (nk, nunocc))
+    eo = np.repeat(c[:, None], nocc, axis=1)
+    eu = c[:, None] + frqs[s_of]
+    n = nk * nocc * nunocc
+    k, b1, b2 = gamma.transitions(nk, nocc, nunocc)
+    wt = np.zeros((n, nfrqs))
+    wt[np.arange(n), s_of[k, b2]] = g.uniform(0.1, 1.0, n)
+    return frqs, eo, eu, wt
+
+
+def test_multdiff_cancels_when_weight_sits_at_the_transition_energy():
+    """D_l = D_0 / Delta^l for every transition, so the scaled column times the
+    divided weights gives back the plain pair sum, on both axes."""
+    g = np.random.default_rng(21)
+    nk, nocc, nunocc, nfrqs, nb, nw = 3, 2, 4, 9, 5, 4
+    frqs, eo, eu, wt = _multdiff_on_mesh(g, nk, nocc, nunocc, nfrqs)
+    n = nk * nocc * nunocc
+    cp, _ = _rand(g, n, nb)
+    wSr = g.standard_normal((nfrqs, nw))
+    wSi = g.standard_normal((nfrqs, nw))
+    plain = hilbert.chi0_hilbert_pairsum(cp, wt, wSr)
+    md = gamma.chi0_multdiff(cp, eo, eu, wt, frqs, wSr)
+    np.testing.assert_allclose(md, plain, rtol=1e-12, atol=1e-12)
+    Dr = hilbert.fold_weights(wt, wSr)
+    Di = hilbert.fold_weights(wt, wSi)
+    plainc = hilbert.chi0_complex_weights(cp, Dr, Di)
+    mdc = gamma.chi0_multdiff(cp, eo, eu, wt, frqs, wSr, wSi)
+    np.testing.assert_allclose(mdc, plainc, rtol=1e-12, atol=1e-12)
+    # the column is scaled by e_unocc(k+q) - e_occ(k) (not its inverse or
+    # negative) and only the first column changes
+    sc = gamma.multdiff_scale(cp, eo, eu)
+    kk, bb1, bb2 = gamma.transitions(nk, nocc, nunocc)
+    np.testing.assert_array_equal(sc[:, 1:], cp[:, 1:])
+    np.testing.assert_allclose(sc[:, 0], cp[:, 0] * (eu[kk, bb2] - eo[kk, bb1]), rtol=1e-15)
+
+
+def test_multdiff_with_weight_off_the_transition_energy_differs():
+    """Moving the weight one mesh point away breaks the cancellation in row
+    and column 0 only: the body (i, j >= 1) is untouched."""
+    g = np.random.default_rng(22)
+    nk, nocc, nunocc, nfrqs, nb, nw = 2, 2, 3, 8, 4, 3
+    frqs, eo, eu, wt = _multdiff_on_mesh(g, nk, nocc, nunocc, nfrqs)
+    wt = np.roll(wt, 1, axis=1)
+    wt[:, 0] = 0.0
+    n = nk * nocc * nunocc
+    cp, _ = _rand(g, n, nb)
+    wS = g.standard_normal((nfrqs, nw))
+    plain = hilbert.chi0_hilbert_pairsum(cp, wt, wS)
+    md = gamma.chi0_multdiff(cp, eo, eu, wt, frqs, wS)
+    np.testing.assert_allclose(md[1:, 1:], plain[1:, 1:], rtol=1e-12, atol=1e-12)
+    assert np.abs(md[0] - plain[0]).max() > 1e-3 * np.abs(plain[0]).max()
+
+
 def test_head_scalar_closed_forms():
     """trace/3 is the angular average of v^T head v over unit vectors (the
     mean of v v^T over the sphere is I/3), and a projection on an axis picks
