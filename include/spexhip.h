@@ -490,7 +490,9 @@ int spexhip_gamma_drude_dev(spexhip_ctx *ctx, int nw, const double *freq, double
  * where |e_unocc - e_occ| <= 1e-8, in place.  P_dev (n x 3 complex, t
  * fastest, ld ldp) holds the momentum matrix <u|p|o> of the transitions
  * t = (k*nocc + b1)*nunocc + b2; e_occ_dev [nk][nocc], e_unocc_dev
- * [nk][nunocc] (energies at k).                                           */
+ * [nk][nunocc] (energies at k).  Both energy pointers NULL: the MULTDIFF
+ * form (multdiff_ > 0, the default at Gamma, :844-857), dcprod / sqrt(vol)
+ * only, the energy difference going into the weights.                    */
 int spexhip_momentum_scale_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
                                const void *e_occ_dev, const void *e_unocc_dev, double vol,
                                void *P_dev, long ldp);
@@ -542,14 +544,37 @@ int spexhip_multdiff_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const v
                          const void *e_unocc_dev, int nb, const void *M_dev, long ldm, int colmajor,
                          int nw, const void *D0_dev, const void *D1_dev, const void *D2_dev,
                          long ldd, double alpha, void *chi_dev, int ldchi);
-/* The real-frequency form with complex weights Dm = Dmr + i Dmi, on the
- * whole matrix of spexhip_chi0_cplx_dev.                                  */
+/* MULTDIFF INTON (multdiff_ = 2 at any q, susceptibility.f:591, 1088-1090,
+ * 1168-1170): the body pass (ll = 0) skips every pair with the first basis
+ * function, and the passes ll = 1 and ll = 2 (Hilbert weights getwghtS(l)
+ * integrating 1/w and 1/w^2, hilbert.f:106-300) take only the pair (1,1)
+ * (1-based) of the scaled column.  Call after spexhip_chi0_dev with the
+ * md = 0 weights D0 and the same alpha: row and column 0 lose their plain
+ * sums and chi(0,0) becomes alpha sum_t (Dmd1 + Dmd2) |M(t,0) de_t|^2, as
+ * in the reference.  Arguments as spexhip_multdiff_dev; Dmd1, Dmd2 the
+ * folds of wtetra with the md = 1 and md = 2 Hilbert weights.  At Gamma
+ * (MULTDIFF INT) the first row and column are replaced by head and wings
+ * anyway, and this call is not needed.                                    */
+int spexhip_multdiff_inton_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                               const void *e_occ_dev, const void *e_unocc_dev, int nb,
+                               const void *M_dev, long ldm, int colmajor, int nw,
+                               const void *D0_dev, const void *Dmd1_dev, const void *Dmd2_dev,
+                               long ldd, double alpha, void *chi_dev, int ldchi);
+/* The real-frequency forms of the two above with complex weights
+ * Dm = Dmr + i Dmi, on the whole matrix of spexhip_chi0_cplx_dev.         */
 int spexhip_multdiff_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
                               const void *e_occ_dev, const void *e_unocc_dev, int nb,
                               const void *M_dev, long ldm, int colmajor, int nw,
                               const void *D0r_dev, const void *D0i_dev, const void *D1r_dev,
                               const void *D1i_dev, const void *D2r_dev, const void *D2i_dev,
                               long ldd, double alpha, void *chi_dev, int ldchi);
+int spexhip_multdiff_inton_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                                    const void *e_occ_dev, const void *e_unocc_dev, int nb,
+                                    const void *M_dev, long ldm, int colmajor, int nw,
+                                    const void *D0r_dev, const void *D0i_dev,
+                                    const void *Dmd1r_dev, const void *Dmd1i_dev,
+                                    const void *Dmd2r_dev, const void *Dmd2i_dev, long ldd,
+                                    double alpha, void *chi_dev, int ldchi);
 
 /* Host drop-in of the two above for one memory packet: cprod (n x nb, ld
  * ldc), dcprod (n x 3, ld lddc), weights (n x nfrqh: wh of the wings, whh

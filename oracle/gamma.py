@@ -16,7 +16,9 @@ Restates, in plain numpy/Python:
   divided by frqs(s)**mm where frqs(s) > 0;
 - MULTDIFF ON at any q (multdiff_ = 1, :588-592): cprod(:,1) times the
   band-energy difference (:912-925) and the pair loop with mm = the number
-  of pair indices that are the first basis function (:1193);
+  of pair indices that are the first basis function (:1193); MULTDIFF
+  INTON (multdiff_ = 2): the three passes ll = 0, 1, 2 over the pairs they
+  visit (:1088-1090, 1168-1170), each with its own Hilbert weights;
 - the Gamma-point dielectric step, correlation.f:570-646 (body row and
   column of the first basis function set to zero, wing(:,1,:) = 0,
   head = 1 - 4 pi head, wing = -sqrt(4 pi) sqcoul wing), the head's scalar
@@ -56,7 +58,12 @@ def momentum_scale(mom, e_occ, e_unocc, vol):
     """dcprod of susceptibility.f:858-873 for multdiff_ = 0: mom (n, 3)
     complex, e_occ (nk, nocc), e_unocc (nk, nunocc) at k (q = 0).  The
     division of a complex by a real is taken componentwise (what the complex
-    division gives for a zero imaginary divisor)."""
+    division gives for a zero imaginary divisor).  e_occ None: multdiff_ > 0,
+    dcprod / sqrt(vol) only (:857)."""
+    if e_occ is None:
+        p = np.array(mom, complex)
+        sv = math.sqrt(vol)
+        return (p.real / sv) + 1j * (p.imag / sv)
     e_occ = np.asarray(e_occ, float)
     e_unocc = np.asarray(e_unocc, float)
     nk, nocc = e_occ.shape
@@ -200,6 +207,43 @@ def chi0_multdiff(cprod, e_occ, e_unocc, wtetra, frqs, wghtSr, wghtSi=None, wght
             out[i, j] += cdum + cdum1
             if i != j:
                 out[j, i] += cdum - cdum1
+    return out
+
+
+def chi0_multdiff_inton(cprod, e_occ, e_unocc, wtetra, wghtS, wghtSi=None, wghtthr=0.0):
+    """The pair loop with MULTDIFF INTON (multdiff_ = 2, with trafo;
+    susceptibility.f:591, 1088-1090, 1159-1250): column 0 scaled by
+    multdiff_scale; for ll = 0, 1, 2 the Hilbert weights of md = ll
+    (wghtS[ll], (nfrqs, nw) each; wghtSi likewise for real frequencies);
+    ll = 0 takes the pairs i <= j with neither index on the first basis
+    function, ll > 0 the pairs with j = first (so only (first, first)), all
+    with mm = 0.  Entries not visited stay 0.  Shapes and axes as
+    chi0_multdiff."""
+    cp = multdiff_scale(cprod, e_occ, e_unocc)
+    nb = cp.shape[1]
+    nfrqs = np.asarray(wtetra).shape[1]
+    nw = np.asarray(wghtS[0]).shape[1]
+    out = np.zeros((nb, nb, nw), complex)
+    for ll in range(3):
+        for j in range(nb):
+            for i in range(j + 1):
+                if ll == 0 and (i == 0 or j == 0):
+                    continue
+                if ll > 0 and j > 0:
+                    continue
+                c = cp[:, i] * np.conj(cp[:, j])
+                s = spectral_function_mm(c, wtetra, np.zeros(nfrqs), wghtS[ll], 0, wghtthr)
+                if wghtSi is None:
+                    out[i, j] += s
+                    if i != j:
+                        out[j, i] += np.conj(s)
+                    continue
+                sc = spectral_function_mm(c, wtetra, np.zeros(nfrqs), wghtSi[ll], 0, wghtthr)
+                cdum = s.real + 1j * sc.real
+                cdum1 = -sc.imag + 1j * s.imag
+                out[i, j] += cdum + cdum1
+                if i != j:
+                    out[j, i] += cdum - cdum1
     return out
 
 

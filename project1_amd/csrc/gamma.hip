@@ -51,6 +51,14 @@ __global__ void momentum_scale_kernel(int nk, int nocc, int nunocc, const double
   const long n = (long)nk * nocc * nunocc;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  if (!eo) {  // MULTDIFF (multdiff_ > 0): the numerator only
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const double2 p = P[t + ldp * c];
+      P[t + ldp * c] = make_double2(p.x / sv, p.y / sv);
+    }
+    return;
+  }
   const int b2 = (int)(t % nunocc);
   const long r = t / nunocc;
   const int b1 = (int)(r % nocc);
@@ -755,7 +763,11 @@ __global__ void multdiff_cols_kernel(int nocc, int nunocc, long n, const double2
 // M(t,J) conj(M0); heads hA(0,0) = sum D2 |P0|^2, hB(1,1) = sum D0 |M0|^2.
 // cplx: wing(k,i,s,w) with s = 0 for (i,0), s = 1 for (0,i); else the
 // Hermitian (0,J) = conj of (J,0).
-__global__ void multdiff_fix_kernel(int nb, int nw, int cplx, const double2 *__restrict__ hA,
+// inton: MULTDIFF INTON (multdiff_ = 2) instead, from one pass whose head
+// weights are Dmd1 + Dmd2 and wing weights D0 (hB, wB unused): row and column
+// 0 lose their plain sums, (0,0) becomes the sum with Dmd1 + Dmd2.
+__global__ void multdiff_fix_kernel(int nb, int nw, int cplx, int inton,
+                                    const double2 *__restrict__ hA,
                                     const double2 *__restrict__ wA,
                                     const double2 *__restrict__ hB,
                                     const double2 *__restrict__ wB, double alpha,
@@ -764,18 +776,21 @@ __global__ void multdiff_fix_kernel(int nb, int nw, int cplx, const double2 *__r
   if (J >= nb || w >= nw) return;
   double2 *X = chi + ldchi * (long)nb * w;
   const int ns = cplx ? 2 : 1;
+  const long wa = 3 * ((long)J + (long)nb * ns * w), wb = 1 + wa;  // k = 0 and k = 1
   if (J == 0) {
-    const double2 a = hA[9 * w], b = hB[4 + 9 * w];  // (0,0) and (1,1)
+    // (0,0) and the plain (1,1) sum (inton: the k = 1 wing of column 0)
+    const double2 a = hA[9 * w], b = inton ? wA[wb] : hB[4 + 9 * w];
     X[0].x += alpha * (a.x - b.x);
     X[0].y += alpha * (a.y - b.y);
     return;
   }
-  const long wa = 3 * ((long)J + (long)nb * ns * w), wb = 1 + wa;  // k = 0 and k = 1
-  const double2 lo = make_double2(wA[wa].x - wB[wb].x, wA[wa].y - wB[wb].y);  // (J,0)
+  const double2 lo = inton ? make_double2(-wA[wb].x, -wA[wb].y)
+                           : make_double2(wA[wa].x - wB[wb].x, wA[wa].y - wB[wb].y);  // (J,0)
   double2 up;
   if (cplx) {
     const long s1 = 3L * nb;
-    up = make_double2(wA[wa + s1].x - wB[wb + s1].x, wA[wa + s1].y - wB[wb + s1].y);
+    up = inton ? make_double2(-wA[wb + s1].x, -wA[wb + s1].y)
+               : make_double2(wA[wa + s1].x - wB[wb + s1].x, wA[wa + s1].y - wB[wb + s1].y);
   } else {
     up = make_double2(lo.x, -lo.y);
   }
@@ -786,10 +801,22 @@ __global__ void multdiff_fix_kernel(int nb, int nw, int cplx, const double2 *__r
   xu.y += alpha * up.y;
 }
 
-int multdiff(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const void *eo, const void *eu, int nb,
-             const void *M, long ldm, int colmajor, int nw, const void *D0r, const void *D0i,
-             const void *D1r, const void *D1i, const void *D2r, const void *D2i, long ldd,
-             double alpha, void *chi, int ldchi) {
+__global__ void weight_sum_kernel(long n, int nw, long ldd, const double *__restrict__ A,
+                                  const double *__restrict__ B, double *__restrict__ S) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long o = t + ldd * blockIdx.y;
+  S[o] = A[o] + B[o];
+}
+
+// The MULTDIFF update of chi's row and column 0 after the plain pair sum.
+// ON (inton = false): D1/D2 are the Hilbert folds with wghtS / frqs^l.
+// INTON (inton = true): D1/D2 are the folds with the md = 1 and md = 2
+// Hilbert weights (getwghtS(l), hilbert.f:106-300).
+int multdiff(spexhip_ctx *ctx, bool inton, int nk, int nocc, int nunocc, const void *eo,
+             const void *eu, int nb, const void *M, long ldm, int colmajor, int nw,
+             const void *D0r, const void *D0i, const void *D1r, const void *D1i, const void *D2r,
+             const void *D2i, long ldd, double alpha, void *chi, int ldchi) {
   const bool cplx = D0i != nullptr;
   SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && nb > 0 && nw > 0 && ldchi >= nb,
             "multdiff: bad sizes");
@@ -801,40 +828,64 @@ int multdiff(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const void *eo, con
                      spx_owns(ctx, eu, 8 * (size_t)nk * nunocc) &&
                      spx_owns(ctx, chi, 16 * (size_t)ldchi * nb * nw),
             "multdiff: energies or chi are not context allocations of the stated shapes");
+  const size_t dbytes = 8 * ((size_t)ldd * (nw - 1) + n);
+  SPX_CHECK(ctx, spx_owns(ctx, D1r, dbytes) && spx_owns(ctx, D2r, dbytes) &&
+                     (!cplx || (spx_owns(ctx, D1i, dbytes) && spx_owns(ctx, D2i, dbytes))),
+            "multdiff: weights are not context allocations of n x nw (ld ldd)");
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   const int ns = cplx ? 2 : 1;
-  void *P = nullptr, *hA = nullptr, *hB = nullptr, *wA = nullptr, *wB = nullptr;
+  void *P = nullptr, *hA = nullptr, *hB = nullptr, *wA = nullptr, *wB = nullptr, *Sr = nullptr,
+       *Si = nullptr;
   int rc = spx_alloc(ctx, 16 * 3 * (size_t)n, &P);
   if (!rc) rc = spx_alloc(ctx, 16 * 9 * (size_t)nw, &hA);
-  if (!rc) rc = spx_alloc(ctx, 16 * 9 * (size_t)nw, &hB);
   if (!rc) rc = spx_alloc(ctx, 16 * 3 * (size_t)nb * ns * nw, &wA);
-  if (!rc) rc = spx_alloc(ctx, 16 * 3 * (size_t)nb * ns * nw, &wB);
+  if (!rc && !inton) rc = spx_alloc(ctx, 16 * 9 * (size_t)nw, &hB);
+  if (!rc && !inton) rc = spx_alloc(ctx, 16 * 3 * (size_t)nb * ns * nw, &wB);
+  if (!rc && inton) rc = spx_alloc(ctx, 8 * (size_t)ldd * nw, &Sr);
+  if (!rc && inton && cplx) rc = spx_alloc(ctx, 8 * (size_t)ldd * nw, &Si);
   if (!rc) {
     hipLaunchKernelGGL(multdiff_cols_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                        ctx->stream, nocc, nunocc, n, (const double2 *)M, ldm, colmajor,
                        (const double *)eo, (const double *)eu, (double2 *)P, n);
+    if (inton) {
+      const dim3 g((unsigned)((n + 255) / 256), (unsigned)nw);
+      hipLaunchKernelGGL(weight_sum_kernel, g, dim3(256), 0, ctx->stream, n, nw, ldd,
+                         (const double *)D1r, (const double *)D2r, (double *)Sr);
+      if (cplx)
+        hipLaunchKernelGGL(weight_sum_kernel, g, dim3(256), 0, ctx->stream, n, nw, ldd,
+                           (const double *)D1i, (const double *)D2i, (double *)Si);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "multdiff: %s", hipGetErrorString(e));
   }
-  // MULTDIFF sums: wings with D1 (1/frqs), head with D2 (1/frqs^2); plain
-  // sums of the same row with D0
-  if (!rc)
-    rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D1r, D1i, D2r, D2i, ldd, 1.0, 0.0, hA,
-                   wA);
-  if (!rc)
-    rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D0r, D0i, D0r, D0i, ldd, 1.0, 0.0, hB,
-                   wB);
+  if (inton) {
+    // wings with D0 (the plain sums of row 0, to be removed), head with
+    // Dmd1 + Dmd2 (element (0,0) of passes ll = 1 and ll = 2)
+    if (!rc)
+      rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D0r, D0i, Sr, Si, ldd, 1.0, 0.0, hA,
+                     wA);
+  } else {
+    // wings with D1 (1/frqs), head with D2 (1/frqs^2); plain sums of the
+    // same row with D0
+    if (!rc)
+      rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D1r, D1i, D2r, D2i, ldd, 1.0, 0.0, hA,
+                     wA);
+    if (!rc)
+      rc = headwings(ctx, n, nb, M, ldm, colmajor, P, n, nw, D0r, D0i, D0r, D0i, ldd, 1.0, 0.0, hB,
+                     wB);
+  }
   if (!rc) {
     hipLaunchKernelGGL(multdiff_fix_kernel, dim3((unsigned)((nb + 63) / 64), (unsigned)nw),
-                       dim3(64), 0, ctx->stream, nb, nw, (int)cplx, (const double2 *)hA,
-                       (const double2 *)wA, (const double2 *)hB, (const double2 *)wB, alpha,
-                       (double2 *)chi, (long)ldchi);
+                       dim3(64), 0, ctx->stream, nb, nw, (int)cplx, (int)inton,
+                       (const double2 *)hA, (const double2 *)wA,
+                       (const double2 *)(inton ? hA : hB), (const double2 *)(inton ? wA : wB),
+                       alpha, (double2 *)chi, (long)ldchi);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) rc = spx_seterr(ctx, SPEXHIP_E_DEVICE, "multdiff: %s", hipGetErrorString(e));
   }
   (void)hipStreamSynchronize(ctx->stream);
-  for (void *p : {P, hA, hB, wA, wB})
+  for (void *p : {P, hA, hB, wA, wB, Sr, Si})
     if (p) spx_free(ctx, p);
   return rc;
 }
@@ -864,8 +915,10 @@ int spexhip_momentum_scale_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, c
   SPX_CHECK(ctx, nk > 0 && nocc > 0 && nunocc > 0 && vol > 0, "momentum_scale: bad sizes");
   const long n = (long)nk * nocc * nunocc;
   SPX_CHECK(ctx, ldp >= n, "momentum_scale: ldp < n");
-  SPX_CHECK(ctx, spx_owns(ctx, e_occ, 8 * (size_t)nk * nocc) &&
-                     spx_owns(ctx, e_unocc, 8 * (size_t)nk * nunocc) &&
+  SPX_CHECK(ctx, (e_occ == nullptr) == (e_unocc == nullptr),
+            "momentum_scale: give both energy arrays or neither");
+  SPX_CHECK(ctx, (!e_occ || (spx_owns(ctx, e_occ, 8 * (size_t)nk * nocc) &&
+                             spx_owns(ctx, e_unocc, 8 * (size_t)nk * nunocc))) &&
                      spx_owns(ctx, P, 16 * ((size_t)ldp * 2 + n)),
             "momentum_scale: arrays are not context allocations of the stated shapes");
   SPX_HIP(ctx, hipSetDevice(ctx->device));
@@ -902,8 +955,20 @@ int spexhip_multdiff_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const v
                          int nw, const void *D0_dev, const void *D1_dev, const void *D2_dev,
                          long ldd, double alpha, void *chi_dev, int ldchi) {
   if (!ctx) return SPEXHIP_E_INVALID;
-  return multdiff(ctx, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor, nw,
-                  D0_dev, nullptr, D1_dev, nullptr, D2_dev, nullptr, ldd, alpha, chi_dev, ldchi);
+  return multdiff(ctx, false, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor,
+                  nw, D0_dev, nullptr, D1_dev, nullptr, D2_dev, nullptr, ldd, alpha, chi_dev,
+                  ldchi);
+}
+
+int spexhip_multdiff_inton_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                               const void *e_occ_dev, const void *e_unocc_dev, int nb,
+                               const void *M_dev, long ldm, int colmajor, int nw,
+                               const void *D0_dev, const void *Dmd1_dev, const void *Dmd2_dev,
+                               long ldd, double alpha, void *chi_dev, int ldchi) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  return multdiff(ctx, true, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor,
+                  nw, D0_dev, nullptr, Dmd1_dev, nullptr, Dmd2_dev, nullptr, ldd, alpha, chi_dev,
+                  ldchi);
 }
 
 int spexhip_multdiff_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
@@ -914,9 +979,23 @@ int spexhip_multdiff_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
                               long ldd, double alpha, void *chi_dev, int ldchi) {
   if (!ctx) return SPEXHIP_E_INVALID;
   SPX_CHECK(ctx, D0i_dev && D1i_dev && D2i_dev, "multdiff_cplx: null imaginary weights");
-  return multdiff(ctx, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor, nw,
-                  D0r_dev, D0i_dev, D1r_dev, D1i_dev, D2r_dev, D2i_dev, ldd, alpha, chi_dev,
+  return multdiff(ctx, false, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor,
+                  nw, D0r_dev, D0i_dev, D1r_dev, D1i_dev, D2r_dev, D2i_dev, ldd, alpha, chi_dev,
                   ldchi);
+}
+
+int spexhip_multdiff_inton_cplx_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                                    const void *e_occ_dev, const void *e_unocc_dev, int nb,
+                                    const void *M_dev, long ldm, int colmajor, int nw,
+                                    const void *D0r_dev, const void *D0i_dev,
+                                    const void *Dmd1r_dev, const void *Dmd1i_dev,
+                                    const void *Dmd2r_dev, const void *Dmd2i_dev, long ldd,
+                                    double alpha, void *chi_dev, int ldchi) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, D0i_dev && Dmd1i_dev && Dmd2i_dev, "multdiff_inton_cplx: null imaginary weights");
+  return multdiff(ctx, true, nk, nocc, nunocc, e_occ_dev, e_unocc_dev, nb, M_dev, ldm, colmajor,
+                  nw, D0r_dev, D0i_dev, Dmd1r_dev, Dmd1i_dev, Dmd2r_dev, Dmd2i_dev, ldd, alpha,
+                  chi_dev, ldchi);
 }
 
 int spexhip_gamma_prepare_dev(spexhip_ctx *ctx, int nb, int nw, void *X, int ldx, void *head,

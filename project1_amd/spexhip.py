@@ -371,6 +371,11 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_multdiff_cplx_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I, _vp,
                                           _vp, _vp, _vp, _vp, _vp, _c_long, C.c_double, _vp,
                                           I]),
+        "spexhip_multdiff_inton_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I, _vp,
+                                           _vp, _vp, _c_long, C.c_double, _vp, I]),
+        "spexhip_multdiff_inton_cplx_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I,
+                                                _vp, _vp, _vp, _vp, _vp, _vp, _c_long,
+                                                C.c_double, _vp, I]),
         "spexhip_headwings_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I, _vp,
                                       _vp, _c_long, C.c_double, C.c_double, _vp, _vp]),
         "spexhip_headwings_cplx_dev": (I, [_vp, _c_long, I, _vp, _c_long, I, _vp, _c_long, I,
@@ -870,12 +875,14 @@ class Context:
             suscepc.ctypes.data))
 
     # -- Gamma point: head and wings --------------------------------------
-    def momentum_scale_dev(self, e_occ: DevBuf, e_unocc: DevBuf, nk: int, nocc: int,
-                           nunocc: int, vol: float, P: DevBuf, ldp: int) -> None:
+    def momentum_scale_dev(self, e_occ: DevBuf | None, e_unocc: DevBuf | None, nk: int,
+                           nocc: int, nunocc: int, vol: float, P: DevBuf, ldp: int) -> None:
         """dcprod = P / sqrt(vol) / (e_unocc - e_occ) in place, zero where the
-        difference is at most 1e-8 in size."""
+        difference is at most 1e-8 in size; energies None: P / sqrt(vol)
+        (MULTDIFF)."""
         self._check(lib().spexhip_momentum_scale_dev(
-            self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, vol, P.ptr, ldp))
+            self.h, nk, nocc, nunocc, e_occ.ptr if e_occ else None,
+            e_unocc.ptr if e_unocc else None, vol, P.ptr, ldp))
 
     def headwings_dev(self, n: int, nb: int, M: DevBuf, ldm: int, colmajor: bool, P: DevBuf,
                       ldp: int, nw: int, D: DevBuf, ldd: int, head: DevBuf, wing: DevBuf,
@@ -1034,11 +1041,21 @@ class Context:
 
     def multdiff_cplx_dev(self, nk: int, nocc: int, nunocc: int, e_occ: DevBuf, e_unocc: DevBuf,
                           nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int, D: list,
-                          ldd: int, alpha: float, chi: DevBuf, ldchi: int) -> None:
+                          ldd: int, alpha: float, chi: DevBuf, ldchi: int,
+                          inton: bool = False) -> None:
         """Real-frequency form: D = [D0r, D0i, D1r, D1i, D2r, D2i]."""
-        self._check(lib().spexhip_multdiff_cplx_dev(
+        f = lib().spexhip_multdiff_inton_cplx_dev if inton else lib().spexhip_multdiff_cplx_dev
+        self._check(f(self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, nb, M.ptr, ldm,
+                      int(colmajor), nw, *[d.ptr for d in D], ldd, alpha, chi.ptr, ldchi))
+
+    def multdiff_inton_dev(self, nk: int, nocc: int, nunocc: int, e_occ: DevBuf,
+                           e_unocc: DevBuf, nb: int, M: DevBuf, ldm: int, colmajor: bool,
+                           nw: int, D0: DevBuf, Dmd1: DevBuf, Dmd2: DevBuf, ldd: int,
+                           alpha: float, chi: DevBuf, ldchi: int) -> None:
+        """MULTDIFF INTON rows of chi (imaginary axis), after chi0_dev with D0."""
+        self._check(lib().spexhip_multdiff_inton_dev(
             self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, nb, M.ptr, ldm, int(colmajor), nw,
-            *[d.ptr for d in D], ldd, alpha, chi.ptr, ldchi))
+            D0.ptr, Dmd1.ptr, Dmd2.ptr, ldd, alpha, chi.ptr, ldchi))
 
     def plasma_frequency_dev(self, nk: int, nband: int, w: DevBuf, P: DevBuf, ldp: int, deg,
                              vol: float, spin_factor: float = 2.0) -> float:

@@ -161,3 +161,108 @@ def test_host_dropin_multdiff_both_axes(ctx):
         mx, _ = normwise(sc[:, :, w] - base_c[:, :, w], rc[:, :, w])
         print(f"  real w={w}: max-rel {mx:.2e}")
         assert mx < 1e-12
+
+
+@pytest.mark.parametrize("precision,tol", [(spexhip.PREC_FP64, 1e-12),
+                                           (spexhip.PREC_OZAKI_INT8, 1e-8)])
+def test_multdiff_inton_imaginary_axis_vs_oracle(ctx, precision, tol):
+    """INTON: chi0 with the md = 0 weights, then the update; the md = 1, 2
+    weights are the Hilbert weights integrating 1/w, 1/w^2."""
+    nk, nocc, nunocc, nb = 3, 4, 7, 41
+    g, f, eo, eu, wt, M = _case(34, nk, nocc, nunocc, nb, False)
+    n = nk * nocc * nunocc
+    freqs = 1j * np.array([0.05, 0.4, 1.3, 3.7])
+    nw = len(freqs)
+    wS = [hilbert.hilbert_weights(freqs, f, 0.0, md, 0)[0] for md in range(3)]
+    dWt = _f64_dev(ctx, wt)
+    D = []
+    for w in wS:
+        dS, dD = _f64_dev(ctx, w), ctx.alloc(8 * n * nw)
+        ctx.weights_hilbert_dev(n, len(f), dWt, n, 0.0, nw, dS, len(f), dD, n)
+        dS.free()
+        D.append(dD)
+    dM = ctx.to_dev(np.ascontiguousarray(M))
+    deo, deu = ctx.to_dev(eo), ctx.to_dev(eu)
+    dC = ctx.alloc(16 * nb * nb * nw).zero()
+    ctx.chi0_dev(n, nb, dM, nb, False, nw, D[0], precision, dC, nb, alpha=2.0)
+    print(f"multdiff inton imag prec={precision}", flush=True)
+    ctx.multdiff_inton_dev(nk, nocc, nunocc, deo, deu, nb, dM, nb, False, nw, D[0], D[1], D[2],
+                           n, 2.0, dC, nb)
+    got = ctx.pack_upper(dC, nb, nw, nb).T
+    ref = 2.0 * gamma.chi0_multdiff_inton(M, eo, eu, wt, wS)
+    for w in range(nw):
+        want = np.array([ref[i, j, w] for j in range(nb) for i in range(j + 1)])
+        mx, fro = normwise(got[:, w], want)
+        print(f"  w={w}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+        assert mx < tol
+    for b in D + [dWt, dM, deo, deu, dC]:
+        b.free()
+
+
+def test_multdiff_inton_real_frequencies_vs_oracle(ctx):
+    nk, nocc, nunocc, nb = 2, 5, 6, 23
+    g, f, eo, eu, wt, M = _case(35, nk, nocc, nunocc, nb, False)
+    n = nk * nocc * nunocc
+    w_real = np.array([0.37, 1.21, 2.5])
+    nw = len(w_real)
+    wS = [hilbert.hilbert_weights(w_real, f, 0.01, md, 0) for md in range(3)]
+    dWt = _f64_dev(ctx, wt)
+    D = []
+    for md in range(3):
+        for part in range(2):
+            dS, dD = _f64_dev(ctx, wS[md][part]), ctx.alloc(8 * n * nw)
+            ctx.weights_hilbert_dev(n, len(f), dWt, n, 0.0, nw, dS, len(f), dD, n)
+            dS.free()
+            D.append(dD)
+    dM = ctx.to_dev(np.asfortranarray(M).T)
+    deo, deu = ctx.to_dev(eo), ctx.to_dev(eu)
+    dC = ctx.alloc(16 * nb * nb * nw)
+    ctx.chi0_cplx_dev(n, nb, dM, n, True, nw, D[0], D[1], spexhip.PREC_FP64, dC, nb, alpha=2.0)
+    print("multdiff inton real", flush=True)
+    ctx.multdiff_cplx_dev(nk, nocc, nunocc, deo, deu, nb, dM, n, True, nw, D, n, 2.0, dC, nb,
+                          inton=True)
+    got = dC.download(np.complex128, (nw, nb, nb)).transpose(2, 1, 0)
+    want = 2.0 * gamma.chi0_multdiff_inton(M, eo, eu, wt, [w[0] for w in wS],
+                                           [w[1] for w in wS])
+    for w in range(nw):
+        mx, fro = normwise(got[:, :, w], want[:, :, w])
+        print(f"  w={w}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+        assert mx < 1e-12
+    for b in D + [dWt, dM, deo, deu, dC]:
+        b.free()
+
+
+def test_multdiff_gamma_head_and_wings(ctx):
+    """The default at Gamma: dcprod / sqrt(vol) only (bit-exact), then the
+    head with the 1/frqs^2 weights and the wings with the 1/frqs weights,
+    against the oracle's head and wing sums."""
+    nk, nocc, nunocc, nb = 2, 4, 9, 30
+    g, f, eo, eu, wt, M = _case(36, nk, nocc, nunocc, nb, True)
+    n = nk * nocc * nunocc
+    mom = g.standard_normal((n, 3)) + 1j * g.standard_normal((n, 3))
+    vol = 97.5
+    nw = 4
+    wS = g.standard_normal((len(f), nw))
+    D0, D1, D2 = _divided_weights(ctx, f, wt, wS, n, nw)
+    dP = ctx.to_dev(np.asfortranarray(mom).T)
+    ctx.momentum_scale_dev(None, None, nk, nocc, nunocc, vol, dP, n)
+    got_p = dP.download(np.complex128, (3, n)).T
+    want_p = gamma.momentum_scale(mom, None, None, vol)
+    assert np.array_equal(got_p, want_p)
+    dM = ctx.to_dev(np.ascontiguousarray(M))
+    dH, dW = ctx.alloc(16 * 9 * nw), ctx.alloc(16 * 3 * nb * nw)
+    print("multdiff gamma head/wings", flush=True)
+    ctx.headwings_dev(n, nb, dM, nb, False, dP, n, nw, D1, n, dH, dW, Dh=D2, alpha=2.0)
+    head = dH.download(np.complex128, (nw, 3, 3)).transpose(2, 1, 0)
+    wing = dW.download(np.complex128, (nw, nb, 3)).transpose(2, 1, 0)
+    scale = np.where(f > 0, f, 1.0)
+    Dh1 = hilbert.fold_weights(wt, wS / scale[:, None])
+    Dh2 = hilbert.fold_weights(wt, wS / scale[:, None] ** 2)
+    hp, wp = gamma.headwings_imag(M, want_p, Dh1, Dh2)
+    for w in range(nw):
+        mx, _ = normwise(head[:, :, w], 2.0 * gamma.unpack_head(hp[:, w]))
+        mw, _ = normwise(wing[:, :, w], 2.0 * wp[:, :, w])
+        print(f"  w={w}: head {mx:.2e} wings {mw:.2e}")
+        assert mx < 1e-12 and mw < 1e-12
+    for b in (D0, D1, D2, dP, dM, dH, dW):
+        b.free()

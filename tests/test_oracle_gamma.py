@@ -229,6 +229,57 @@ def test_multdiff_with_weight_off_the_transition_energy_differs():
     assert np.abs(md[0] - plain[0]).max() > 1e-3 * np.abs(plain[0]).max()
 
 
+def test_multdiff_inton_pairs_and_weights():
+    """INTON: the body (i, j >= 1) is the plain pair sum with the md = 0
+    weights; row and column 0 stay empty except (0,0), which is the sum of
+    the scaled column with the md = 1 plus md = 2 weights (linearity); on
+    both axes."""
+    g = np.random.default_rng(23)
+    nk, nocc, nunocc, nfrqs, nb, nw = 2, 3, 4, 10, 5, 3
+    frqs, eo, eu, wt = _multdiff_on_mesh(g, nk, nocc, nunocc, nfrqs)
+    n = nk * nocc * nunocc
+    cp, _ = _rand(g, n, nb)
+    wS = [g.standard_normal((nfrqs, nw)) for _ in range(3)]
+    wSi = [g.standard_normal((nfrqs, nw)) for _ in range(3)]
+    got = gamma.chi0_multdiff_inton(cp, eo, eu, wt, wS)
+    body = hilbert.chi0_hilbert_pairsum(cp, wt, wS[0])
+    np.testing.assert_allclose(got[1:, 1:], body[1:, 1:], rtol=1e-12, atol=1e-12)
+    assert not got[0, 1:].any() and not got[1:, 0].any()
+    sc = gamma.multdiff_scale(cp, eo, eu)
+    h = hilbert.chi0_hilbert_pairsum(sc[:, :1], wt, wS[1] + wS[2])
+    np.testing.assert_allclose(got[0, 0], h[0, 0], rtol=1e-12)
+    gotc = gamma.chi0_multdiff_inton(cp, eo, eu, wt, wS, wSi)
+    Dr = [hilbert.fold_weights(wt, w) for w in wS]
+    Di = [hilbert.fold_weights(wt, w) for w in wSi]
+    bodyc = hilbert.chi0_complex_weights(cp, Dr[0], Di[0])
+    np.testing.assert_allclose(gotc[1:, 1:], bodyc[1:, 1:], rtol=1e-12, atol=1e-12)
+    assert not gotc[0, 1:].any() and not gotc[1:, 0].any()
+    hc = hilbert.chi0_complex_weights(sc[:, :1], Dr[1] + Dr[2], Di[1] + Di[2])
+    np.testing.assert_allclose(gotc[0, 0], hc[0, 0], rtol=1e-12)
+
+
+def test_multdiff_gamma_head_and_wings_cancel_on_the_mesh():
+    """At Gamma, MULTDIFF keeps dcprod / sqrt(vol) and divides the head and
+    wing weights by frqs^2 and frqs; with every transition's weight at
+    frqs(s) = Delta_t that equals the multdiff_ = 0 form (dcprod / Delta,
+    plain weights)."""
+    g = np.random.default_rng(24)
+    nk, nocc, nunocc, nfrqs, nb, nw = 3, 2, 3, 8, 4, 3
+    frqs, eo, eu, wt = _multdiff_on_mesh(g, nk, nocc, nunocc, nfrqs)
+    n = nk * nocc * nunocc
+    cp, dp = _rand(g, n, nb)
+    vol = 123.4
+    wS = g.standard_normal((nfrqs, nw))
+    sc = np.where(frqs > 0, frqs, 1.0)
+    D0, D1, D2 = (hilbert.fold_weights(wt, wS / sc[:, None] ** ll) for ll in range(3))
+    h0, w0 = gamma.headwings_imag(cp, gamma.momentum_scale(dp, eo, eu, vol), D0)
+    p_md = gamma.momentum_scale(dp, None, None, vol)
+    np.testing.assert_allclose(p_md, dp / math.sqrt(vol), rtol=1e-15)
+    h1, w1 = gamma.headwings_imag(cp, p_md, D1, D2)
+    np.testing.assert_allclose(h1, h0, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(w1, w0, rtol=1e-12, atol=1e-14)
+
+
 def test_head_scalar_closed_forms():
     """trace/3 is the angular average of v^T head v over unit vectors (the
     mean of v v^T over the sphere is I/3), and a projection on an axis picks
