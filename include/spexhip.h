@@ -245,6 +245,15 @@ int spexhip_tetrahedron_weights_bands(spexhip_ctx *ctx, const int *nkpt3, const 
                                       const double *frqs, const int *kpt1, int nkpt1,
                                       const int *kpt1p, int tetraf, double *wtetra);
 
+/* The ALIGNBD input key (getinput.f:1486, a global of the reference): with
+ * on != 0, the host drop-ins spexhip_tetrahedron_weights, _tetraf and
+ * _bands stop where tetrahedron5_init returns (tetrahedron.f:1231), before
+ * the averaging over degenerate states -- window rows keep what the
+ * clamped band loop gave them (tetrahedron.f:1086, 1191-1194) -- and
+ * spexhip_tetra_average_equivalent leaves wtetra unchanged.  The device
+ * building blocks are not affected.  Off by default.                     */
+int spexhip_set_alignbd(spexhip_ctx *ctx, int on);
+
 /* The host drop-in with TETRAF set: kpt1 [nkpt1] and kpt1p [nk] are both
  * required, the rows are those of spexhip_tetra_reduce_tetraf_dev; the
  * other arguments as spexhip_tetrahedron_weights.                         */

@@ -344,7 +344,8 @@ def average_degenerate(w, nk, nocc, nunocc, deg_occ, deg_unocc):
 
 
 def tetrahedron5_weights_bands(nkpt3, rlat, ene1, ene2, deg1, deg2, b1low, b1up, b2low, b2up,
-                               efermi, frq, kpt1=None, kpt1p=None, nkpt1=None, tetraf=False):
+                               efermi, frq, kpt1=None, kpt1p=None, nkpt1=None, tetraf=False,
+                               alignbd=False):
     """tetrahedron5_init with band windows that may cut a degenerate group
     (tetrahedron.f:984-1012, 1080-1093, 1191-1194, 1237-1273), kflag 1.
     ene1 (nk, nb1) at k and ene2 (nk, nb2) at k+q hold all bands, deg1 and
@@ -357,7 +358,8 @@ def tetrahedron5_weights_bands(nkpt3, rlat, ene1, ene2, deg1, deg2, b1low, b1up,
     nearest to it (n clamped into the window) of its corner k; the rows of a
     reduced set are formed from those (reduce_to_parents, or reduce_tetraf
     under TETRAF); then the reference's averaging over the groups at kpt1(p)
-    divides each group's in-window sum by the whole group's size.  Returns
+    divides each group's in-window sum by the whole group's size (skipped
+    under alignbd: tetrahedron5_init returns first, :1231).  Returns
     (nkr*nocc*nunocc, nfrqs), nkr = nkpt1 or nk."""
     ene1, ene2 = np.asarray(ene1, float), np.asarray(ene2, float)
     nk = ene1.shape[0]
@@ -386,6 +388,8 @@ def tetrahedron5_weights_bands(nkpt3, rlat, ene1, ene2, deg1, deg2, b1low, b1up,
             rows = reduce_tetraf(acc, nk, nocc, nunocc, kpt1, kpt1p, nkpt1)
         else:
             rows = reduce_to_parents(acc, nk, nocc, nunocc, kpt1p, nkpt1)
+    if alignbd:
+        return rows
     W = rows.reshape(len(kd), nocc, nunocc, -1).copy()
     for r, k in enumerate(kd):  # occupied groups (:1237-1255)
         n = _group(deg1[k], b1low)[0]

@@ -40,6 +40,9 @@ struct spexhip_ctx {
   void *noalloc_scales = nullptr;
   // largest slice workspace of the Ozaki form (frequencies are grouped to fit)
   size_t ozaki_ws_limit = (size_t)24 << 30;
+  // the ALIGNBD input key: the tetrahedron-weight drop-ins stop before the
+  // averaging over degenerate states and equivalent k points
+  int alignbd = 0;
   // events bracketing the phases of the last Ozaki chi0 call (slice of the
   // first frequency group | rank-k updates and later groups)
   hipEvent_t phase_ev[3] = {nullptr, nullptr, nullptr};

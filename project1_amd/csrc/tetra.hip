@@ -452,11 +452,16 @@ __global__ void star_sum_kernel(int nkpt1, int nocc, int nunocc, const int *__re
 // Weights of window rows whose degenerate groups may reach beyond the
 // window: the mean over the whole groups of the rows of the widened
 // windows.  g1/g2 hold (first, last) per (row, window band), relative to
-// the widened windows (nx1 x nx2 band pairs per row); one thread per
-// window transition (r, b1, b2) and bin.
-__global__ void group_mean_kernel(int nrow, int nocc, int nunocc, int nx1, int nx2,
-                                  const int2 *__restrict__ g1, const int2 *__restrict__ g2,
-                                  int nfrq, const double *__restrict__ in, long ldi,
+// the widened windows (nx1 x nx2 band pairs per row; window band b sits at
+// b + off1 / b + off2); one thread per window transition (r, b1, b2) and
+// bin.  clamp (ALIGNBD, where tetrahedron5_init returns before averaging,
+// tetrahedron.f:1231): each window row keeps what the clamped band loop
+// gave it instead (tetrahedron.f:1086, 1191-1194) -- its own weight, and at
+// a window edge also the members of the edge group beyond the window.
+__global__ void group_mean_kernel(int nrow, int nocc, int nunocc, int nx1, int nx2, int off1,
+                                  int off2, int clamp, const int2 *__restrict__ g1,
+                                  const int2 *__restrict__ g2, int nfrq,
+                                  const double *__restrict__ in, long ldi,
                                   double *__restrict__ out, long ldo) {
   const long per = (long)nocc * nunocc;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -465,7 +470,11 @@ __global__ void group_mean_kernel(int nrow, int nocc, int nunocc, int nx1, int n
   const int r = (int)(t / per);
   const int b1 = (int)((t - (long)r * per) / nunocc);
   const int b2 = (int)(t - (long)r * per - (long)b1 * nunocc);
-  const int2 G1 = g1[(long)r * nocc + b1], G2 = g2[(long)r * nunocc + b2];
+  int2 G1 = g1[(long)r * nocc + b1], G2 = g2[(long)r * nunocc + b2];
+  if (clamp) {
+    G1 = make_int2(b1 == 0 ? G1.x : b1 + off1, b1 == nocc - 1 ? G1.y : b1 + off1);
+    G2 = make_int2(b2 == 0 ? G2.x : b2 + off2, b2 == nunocc - 1 ? G2.y : b2 + off2);
+  }
   const double *col = in + ldi * i + (long)r * nx1 * nx2;
   double s = 0.0;
   for (int m1 = G1.x; m1 <= G1.y; m1++) {
@@ -473,7 +482,7 @@ __global__ void group_mean_kernel(int nrow, int nocc, int nunocc, int nx1, int n
     for (int m2 = G2.x; m2 <= G2.y; m2++) s2 += col[(long)m1 * nx2 + m2];
     s += s2;
   }
-  out[t + ldo * i] = s / ((double)(G1.y - G1.x + 1) * (double)(G2.y - G2.x + 1));
+  out[t + ldo * i] = clamp ? s : s / ((double)(G1.y - G1.x + 1) * (double)(G2.y - G2.x + 1));
 }
 
 // Mean over a group of equivalent k points: one thread per (group, band
@@ -803,7 +812,7 @@ int tetrahedron_weights_host(spexhip_ctx *ctx, const int *nkpt3, const double *r
                      nunocc, nfrqs, dW, n, kpt1, kpt1p, nkr, dR, nr);
     res = dR;
   }
-  if (!rc && deg_occ)
+  if (!rc && deg_occ && !ctx->alignbd)
     rc = spexhip_tetra_average_degenerate_dev(ctx, nkr, nocc, nunocc, dDo, dDu, nfrqs, res, nr);
   if (!rc) {
     hipError_t e = hipMemcpyAsync(wtetra, res, 8 * (size_t)nr * nfrqs, hipMemcpyDeviceToHost,
@@ -840,6 +849,12 @@ bool host_group(const int *deg, int nb, int b, int &first, int &last) {
 }  // namespace
 
 extern "C" {
+
+int spexhip_set_alignbd(spexhip_ctx *ctx, int on) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  ctx->alignbd = on != 0;
+  return SPEXHIP_OK;
+}
 
 int spexhip_tetrahedron_weights_bands(spexhip_ctx *ctx, const int *nkpt3, const double *rlat,
                                       int nb1, const double *ene1, const int *deg1, int b1low,
@@ -940,8 +955,9 @@ int spexhip_tetrahedron_weights_bands(spexhip_ctx *ctx, const int *nkpt3, const 
   }
   if (!rc) {
     hipLaunchKernelGGL(group_mean_kernel, dim3((unsigned)((nout + 255) / 256), (unsigned)nfrqs),
-                       dim3(256), 0, ctx->stream, nrow, nocc, nunocc, nx1, nx2,
-                       (const int2 *)dG, (const int2 *)dG + g1.size(), nfrqs,
+                       dim3(256), 0, ctx->stream, nrow, nocc, nunocc, nx1, nx2, b1low - a1,
+                       b2low - a2, ctx->alignbd, (const int2 *)dG, (const int2 *)dG + g1.size(),
+                       nfrqs,
                        (const double *)rows, ldr, (double *)dO, nout);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
@@ -985,6 +1001,7 @@ int spexhip_tetra_average_equivalent(spexhip_ctx *ctx, int nkpt1, int nocc, int 
   if (!ctx) return SPEXHIP_E_INVALID;
   SPX_CHECK(ctx, wtetra && nkpt1 > 0 && nocc > 0 && nunocc > 0 && nfrqs > 0,
             "tetra_average_equivalent: bad arguments");
+  if (ctx->alignbd) return SPEXHIP_OK;  // tetrahedron5_init returns first (tetrahedron.f:1231)
   const size_t bytes = 8 * (size_t)nkpt1 * nocc * nunocc * nfrqs;
   SPX_HIP(ctx, hipSetDevice(ctx->device));
   void *dW = nullptr;

@@ -44,7 +44,7 @@ module m_spexhip
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
   public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
   public :: spexhip_dielecc_inverse, spexhip_tetrahedron5_equiv, spexhip_tetrahedron5_bands
-  public :: spexhip_fft_setup, spexhip_step_on_grid
+  public :: spexhip_fft_setup, spexhip_step_on_grid, spexhip_alignbd
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -118,6 +118,11 @@ module m_spexhip
                             wtetra
       integer(c_int), value :: nocc, nunocc, nfrqs, nkpt1
       real(c_double), value :: efermi
+    end function
+    integer(c_int) function c_set_alignbd(ctx, on) bind(C, name='spexhip_set_alignbd')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on
     end function
     integer(c_int) function c_tetrahedron_weights_bands(ctx, nkpt3, rlat, nb1, ene1, deg1, b1low, &
         b1up, nb2, ene2, deg2, b2low, b2up, efermi, nfrqs, frqs, kpt1, nkpt1, kpt1p, tetraf, &
@@ -387,6 +392,17 @@ contains
     call check(ctx, c_tetrahedron_weights(ctx, c_loc(k3), c_loc(rlat), int(nocc, c_int), &
                int(nunocc, c_int), c_loc(ene_occ), c_loc(ene_unocc), efermi, int(nfrqs, c_int), &
                c_loc(frqs), pk, nk1, pdo, pdu, c_loc(wtetra)), 'tetrahedron5')
+  end subroutine
+
+  ! The ALIGNBD key (getinput.f:1486): the tetrahedron shims stop before the
+  ! averaging over degenerate states and equivalent k points (tetrahedron.f:1231).
+  subroutine spexhip_alignbd(ctx, alignbd)
+    type(c_ptr), intent(in) :: ctx
+    logical, intent(in) :: alignbd
+    integer(c_int) :: on
+    on = 0
+    if (alignbd) on = 1
+    call check(ctx, c_set_alignbd(ctx, on), 'alignbd')
   end subroutine
 
   ! tetrahedron5_init with band windows b1low:b1up, b2low:b2up that may cut a degenerate

@@ -366,6 +366,7 @@ def _declare(L: C.CDLL) -> None:
                                                   I, I, C.c_double, I, _vp, _vp, I, _vp, I,
                                                   _vp]),
         "spexhip_momentum_scale_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, _vp, _c_long]),
+        "spexhip_set_alignbd": (I, [_vp, I]),
         "spexhip_multdiff_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I, _vp, _vp,
                                      _vp, _c_long, C.c_double, _vp, I]),
         "spexhip_multdiff_cplx_dev": (I, [_vp, I, I, I, _vp, _vp, I, _vp, _c_long, I, I, _vp,
@@ -1029,6 +1030,10 @@ class Context:
         r = _rlat_f(rlat)
         self._check(lib().spexhip_fermi_weights_dev(self.h, _ptr(nk3), _ptr(r), nband, ene.ptr,
                                                     float(efermi), w.ptr))
+
+    def set_alignbd(self, on: bool) -> None:
+        """The ALIGNBD key: tetrahedron-weight drop-ins skip the averaging."""
+        self._check(lib().spexhip_set_alignbd(self.h, int(on)))
 
     def multdiff_dev(self, nk: int, nocc: int, nunocc: int, e_occ: DevBuf, e_unocc: DevBuf,
                      nb: int, M: DevBuf, ldm: int, colmajor: bool, nw: int, D0: DevBuf,

@@ -395,3 +395,53 @@ def test_cut_windows_reject_inconsistent_deg(ctx):
     with pytest.raises(spexhip.SpexError, match="deg1 not consistent"):
         ctx.tetrahedron_weights_bands((2, 2, 2), np.eye(3), e1, bad, (2, 3), e2, d2, (0, 1), 0.0,
                                       frq)
+
+
+def test_alignbd_stops_before_the_averaging(ctx):
+    """ALIGNBD on the context: the cut-window drop-in returns the clamped
+    rows, the plain drop-in the rows before degenerate averaging, and the
+    equivalent-k averaging leaves the weights alone; off again afterwards."""
+    g = np.random.default_rng(74)
+    nkpt3 = (3, 2, 2)
+    nk = 12
+    e1 = _deg_bands(g, nk, 6, -1.0, copies=(2,), local=(0, 5, 7))
+    e2 = _deg_bands(g, nk, 6, 1.0, copies=(3,), local=(1, 5))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.03 * (1.2 ** np.arange(24) - 1)
+    kpt1p = np.array([0, 1, 1, 2, 0, 3, 2, 3, 4, 4, 4, 1])
+    kpt1 = np.array([np.where(kpt1p == p)[0][0] for p in range(5)])
+    ctx.set_alignbd(True)
+    try:
+        for w1, w2 in (((2, 3), (1, 3)), ((3, 3), (0, 2))):
+            got = ctx.tetrahedron_weights_bands(nkpt3, RL_SKEW, e1, d1, w1, e2, d2, w2, 0.0, frq)
+            want = tetra.tetrahedron5_weights_bands(nkpt3, RL_SKEW, e1, e2, d1, d2, *w1, *w2,
+                                                    0.0, frq, alignbd=True)
+            mx, _ = normwise(got, want)
+            print(f"alignbd cut windows {w1} {w2}: max-rel {mx:.2e}")
+            assert mx < 1e-12
+        # plain drop-in with deg arrays: star sums only
+        eo, eu = e1[:, 1:3], e2[:, 1:4]
+        do = tetra.deg_from_energies(eo[kpt1], 1e-12).astype(np.int32)
+        du = tetra.deg_from_energies(eu[kpt1], 1e-12).astype(np.int32)
+        got = ctx.tetrahedron_weights(nkpt3, RL_SKEW, eo, eu, 0.0, frq, kpt1p, 5, do, du)
+        full = tetra.tetrahedron5_weights_metal(nkpt3, RL_SKEW, eo, eu, 0.0, frq)
+        want = tetra.reduce_to_parents(full, nk, 2, 3, kpt1p, 5)
+        mx, _ = normwise(got, want)
+        assert mx < 1e-12
+        # the irregular tables of test_equivalent_k_host_dropin_vs_oracle
+        ge = np.random.default_rng(0)
+        ks = ge.integers(0, 10, (10, 4))
+        ks[:, 0] = np.arange(10)
+        ks[3, :] = 3
+        k1e = ge.choice(10, 7, replace=False)
+        kpe = ge.integers(-1, 7, 10)
+        W = ge.uniform(0, 1, (7 * 3 * 5, 6))
+        assert np.abs(tetra.average_equivalent(W, 3, 5, k1e, kpe, ks, 3) - W).max() > 0
+        np.testing.assert_array_equal(ctx.tetra_average_equivalent(W, 3, 5, k1e, kpe, ks, 3), W)
+    finally:
+        ctx.set_alignbd(False)
+    got = ctx.tetrahedron_weights(nkpt3, RL_SKEW, eo, eu, 0.0, frq, kpt1p, 5, do, du)
+    want = tetra.average_degenerate(tetra.reduce_to_parents(full, nk, 2, 3, kpt1p, 5), 5, 2, 3,
+                                    do, du)
+    mx, _ = normwise(got, want)
+    assert mx < 1e-12

@@ -576,3 +576,48 @@ def test_uncut_windows_equal_the_plain_chain():
     w = tetra.tetrahedron5_weights_metal(nkpt3, np.eye(3), e1[:, 1:3], e2[:, 1:4], 0.0, frq)
     want = tetra.average_degenerate(w, nk, 2, 3, d1[:, 1:3] - 1, d2[:, 1:4] - 1)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("w1,w2", [((2, 3), (1, 3)), ((3, 3), (0, 2))])
+def test_alignbd_rows_are_clamped_sums_that_the_averaging_divides(w1, w2):
+    """ALIGNBD stops before the averaging: every window row holds the
+    all-band weights of the bands clamped into it at its k (its own band,
+    and at a window edge the members of the edge group outside the window),
+    and averaging those rows over the window part of each group and dividing
+    by the whole group's size gives the rows without ALIGNBD."""
+    g = np.random.default_rng(64)
+    nkpt3 = (2, 3, 2)
+    nk = 12
+    e1 = _degenerate_bands(g, nk, 6, -1.0, copies=(2,), local=(0, 5, 7))
+    e2 = _degenerate_bands(g, nk, 6, 1.0, copies=(3,), local=(1, 5))
+    d1, d2 = tetra.deg_from_energies(e1, 1e-12), tetra.deg_from_energies(e2, 1e-12)
+    frq = 0.04 * (1.25 ** np.arange(20) - 1)
+    raw = tetra.tetrahedron5_weights_bands(nkpt3, RL_T, e1, e2, d1, d2, *w1, *w2, 0.0, frq,
+                                           alignbd=True)
+    avg = tetra.tetrahedron5_weights_bands(nkpt3, RL_T, e1, e2, d1, d2, *w1, *w2, 0.0, frq)
+    full = tetra.tetrahedron5_weights_metal(nkpt3, RL_T, e1, e2, 0.0, frq).reshape(nk, 6, 6, -1)
+    no, nu = w1[1] - w1[0] + 1, w2[1] - w2[0] + 1
+    R = raw.reshape(nk, no, nu, -1)
+    A = avg.reshape(nk, no, nu, -1)
+
+    def members(deg, k, lo, hi, b):
+        a = tetra._group(deg[k], lo)[0] if b == lo else b
+        z = tetra._group(deg[k], hi)[1] if b == hi else b
+        return range(a, z + 1)
+
+    for k in range(nk):
+        for r1 in range(no):
+            for r2 in range(nu):
+                m1 = members(d1, k, *w1, w1[0] + r1)
+                m2 = members(d2, k, *w2, w2[0] + r2)
+                want = full[k][np.ix_(list(m1), list(m2))].sum(axis=(0, 1))
+                np.testing.assert_allclose(R[k, r1, r2], want, rtol=1e-13,
+                                           atol=1e-15 * np.abs(full).max())
+                f1, l1 = tetra._group(d1[k], w1[0] + r1)
+                f2, l2 = tetra._group(d2[k], w2[0] + r2)
+                p1 = slice(max(f1, w1[0]) - w1[0], min(l1, w1[1]) - w1[0] + 1)
+                p2 = slice(max(f2, w2[0]) - w2[0], min(l2, w2[1]) - w2[0] + 1)
+                mean = R[k, p1, p2].sum(axis=(0, 1)) / ((l1 - f1 + 1) * (l2 - f2 + 1))
+                np.testing.assert_allclose(A[k, r1, r2], mean, rtol=1e-13,
+                                           atol=1e-15 * np.abs(full).max())
+    assert np.abs(raw - avg).max() > 1e-6 * np.abs(avg).max()
