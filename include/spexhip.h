@@ -153,6 +153,28 @@ int spexhip_tetra_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
                               double efermi, int nfrqs, const double *frqs, void *wtetra_dev,
                               long ldwt);
 
+/* Replaces gauss4_init (gauss.f:120-186), the Gaussian alternative to the
+ * tetrahedron weights when the GAUSS key is set (susceptibility.f:941-942,
+ * kflag 1): occupations smeared with width sigma1 (GAUSS's first value),
+ * the transition energy with width sigma2 (its second), each bin of the
+ * ascending exponential Hilbert mesh frqs (host, >= 3 points) taking that
+ * Gaussian's integral over the bin divided by the bin width; bins at
+ * frqs = 0 stay 0.  e_occ_dev [nk][nocc] at k, e_unocc_dev [nk][nunocc] at
+ * k+q (full mesh); kpt1p (host, nk entries, 0-based row or -1) sums each
+ * row over the mesh points with that parent in ascending order, or NULL for
+ * one row per mesh point; divided by nk.  wtetra_dev (nrow*nocc*nunocc x
+ * nfrqs, ld ldwt, t = (r*nocc + b1)*nunocc + b2 fastest) is overwritten.
+ * Like the reference, no averaging over degenerate states follows.       */
+int spexhip_gauss_weights_dev(spexhip_ctx *ctx, int nk, int nocc, int nunocc,
+                              const void *e_occ_dev, const void *e_unocc_dev, double efermi,
+                              double sigma1, double sigma2, int nfrqs, const double *frqs,
+                              const int *kpt1p, int nkpt1, void *wtetra_dev, long ldwt);
+/* Host drop-in of the same (host arrays, wtetra nrow*nocc*nunocc x nfrqs). */
+int spexhip_gauss_weights(spexhip_ctx *ctx, int nk, int nocc, int nunocc, const double *e_occ,
+                          const double *e_unocc, double efermi, double sigma1, double sigma2,
+                          int nfrqs, const double *frqs, const int *kpt1p, int nkpt1,
+                          double *wtetra);
+
 /* The averaging of those weights over degenerate states (tetrahedron.f:
  * 1233-1273), in place: over each group of occupied states at k, then over
  * each group of unoccupied states at k+q.  deg_occ_dev [nk][nocc] and

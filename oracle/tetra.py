@@ -25,7 +25,8 @@ Restates, in plain numpy/Python:
   that keep q (:1282-1318);
 - the TETRAF rows of a reduced set (:1092-1096, 1212-1216);
 - band windows that cut a degenerate group (:984-1012, 1191-1194,
-  1237-1273): widened windows, clamped rows and the whole-group average.
+  1237-1273): widened windows, clamped rows and the whole-group average;
+- gauss4_init (gauss.f:120-186), the Gaussian weights of the GAUSS key.
 
 k points are numbered k = (i1*N2 + i2)*N3 + i3 for the mesh point
 (i1, i2, i3)/(N1, N2, N3); cube corners 1..8 are (0,0,0), (1,0,0),
@@ -35,9 +36,14 @@ tetrahedron.f:1014-1021.
 What pins each function is in tests/test_oracle_tetra.py: exact
 integration of the barycentric coordinates over the cut tetrahedron
 (independent subdivision quadrature), the volume and sum rules, flat bands
-(all weight in one bin), and the total weight of the Brillouin zone.
+(all weight in one bin), and the total weight of the Brillouin zone; the
+Gaussian weights by the quadrature of the Gaussian over each bin, the
+telescoping sum over the tiled bins of the exponential mesh (the Gaussian
+mass inside the mesh times the occupation factors) and star sums.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
@@ -410,6 +416,61 @@ def tetrahedron5_weights_bands(nkpt3, rlat, ene1, ene2, deg1, deg2, b1low, b1up,
                                                       / (nn - n + 1))[:, None]
             n = nn + 1
     return W.reshape(len(kd) * nocc * nunocc, -1)
+
+
+def gauss4_weights(e_occ, e_unocc, efermi, sigma, frq, kpt1p=None, nkpt1=None):
+    """gauss4_init (gauss.f:120-186), kflag 1, in the reference's loop order:
+    for each mesh point k with a parent row p (kpt1p[k] >= 0; all k with
+    kpt1p None), each transition (b1, b2) and each bin i with frq(i) /= 0,
+      w(p,b1,b2,i) += r1 r2 g2fac factor (erf(gfac (f2 - d)) - erf(gfac (f1 -
+      d))) / (2 df),
+    r1 = (1 + erf((efermi - e1) / (sqrt2 sigma1))) / 2, r2 = (1 + erf((e2 -
+    efermi) / (sqrt2 sigma1))) / 2, d = e2 - e1, gfac = 1/(sqrt2 sigma2),
+    g2fac = 1/(sqrt(2 pi) sigma2), factor = sqrt(pi/2) sigma2, the bin [f1,
+    f2] = frq(i) -/+ df with df = a (frq(i+1) - frq(i)) (last bin: (1 - a)
+    (frq(i) - frq(i-1))) and a the mesh factor of :137-147; then / nk.
+    e_occ (nk, nocc) at k, e_unocc (nk, nunocc) at k+q.  Returns
+    (nrow*nocc*nunocc, nfrqs)."""
+    eo, eu = np.asarray(e_occ, float), np.asarray(e_unocc, float)
+    frq = np.asarray(frq, float)
+    nk, nocc = eo.shape
+    nunocc = eu.shape[1]
+    nf = len(frq)
+    rdum, a = 0.0, 0.0
+    for i in range(1, nf):
+        if frq[i] <= frq[i - 1]:
+            raise ValueError("frequency mesh not ascending")
+        if i != nf - 1:
+            rdum += (frq[i] - frq[i - 1]) / (frq[i + 1] - frq[i - 1])
+            r = rdum / i
+            if i > 1 and abs(r - a) > 1e-10:
+                raise ValueError("mesh not exponential")
+            a = r
+    s1, s2 = sigma
+    sqrt2 = math.sqrt(2.0)
+    gfac = 1.0 / (sqrt2 * s2)
+    g2fac = 1.0 / (math.sqrt(2 * math.pi) * s2)
+    factor = math.sqrt(0.5 * math.pi) * s2
+    nrow = nk if kpt1p is None else nkpt1
+    w = np.zeros((nrow, nocc, nunocc, nf))
+    for k in range(nk):
+        p = k if kpt1p is None else int(kpt1p[k])
+        if p < 0:
+            continue
+        for b1 in range(nocc):
+            r1 = (1 + math.erf((efermi - eo[k, b1]) / (sqrt2 * s1))) / 2
+            for b2 in range(nunocc):
+                r2 = (1 + math.erf((eu[k, b2] - efermi) / (sqrt2 * s1))) / 2
+                d = eu[k, b2] - eo[k, b1]
+                for i in range(nf):
+                    if frq[i] == 0:
+                        continue
+                    df = a * (frq[i + 1] - frq[i]) if i != nf - 1 else (1 - a) * (frq[i] - frq[i - 1])
+                    f1, f2 = frq[i] - df, frq[i] + df
+                    w[p, b1, b2, i] += (r1 * r2 * g2fac * factor
+                                        * (math.erf(gfac * (f2 - d)) - math.erf(gfac * (f1 - d)))
+                                        / (2 * df))
+    return (w / nk).reshape(nrow * nocc * nunocc, nf)
 
 
 def equivalent_groups(kpt1, kpt1p, kptsym, ikpt):

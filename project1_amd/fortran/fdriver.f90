@@ -2,8 +2,8 @@
 ! HILBERT NONE pair sum and the dielectric inverse, compared with a plain
 ! Fortran pair loop written the way susceptibility.f:1183-1240 does it; the
 ! screened interaction, the RPA integrand, the HILBERT pair sum (also with
-! MULTDIFF), the Gamma-point head and wings and the equivalent-k averaging
-! likewise.
+! MULTDIFF), the Gamma-point head and wings, the equivalent-k averaging and
+! the GAUSS weights likewise.
 ! Exit status 0 on agreement (normwise relative error < 1e-12).
 program fdriver
   use, intrinsic :: iso_c_binding
@@ -314,6 +314,61 @@ program fdriver
       write(u) rl, e1, e2, fr, d1, d2, wt
       close(u)
     end if
+  end block
+  ! GAUSS weights (gauss4_init) through the shim against the loop of
+  ! gauss.f:149-185 written out here, with the compiler's erf; a reduced set
+  ! of 3 rows, one mesh point without a parent.
+  block
+    integer, parameter :: nk = 6, no = 2, nu = 3, nf = 9
+    real(c_double), target :: eo(no, nk), eu(nu, nk), fr(nf), wt(no*nu*3, nf), rw(nu, no, 3, nf)
+    real(c_double) :: gs(2), a, r, r1, r2, d, df, gfac, g2fac, fac, pi
+    integer :: kp(nk), b1, b2, p
+    pi = 4 * atan(1d0)
+    gs = [0.03d0, 0.05d0]
+    do k = 1, nk
+      do b1 = 1, no
+        eo(b1, k) = -0.6d0 + 0.2d0*b1 + 0.03d0*sin(1.1d0*k*b1)
+      end do
+      do b2 = 1, nu
+        eu(b2, k) = -0.02d0 + 0.3d0*b2 + 0.04d0*cos(0.9d0*k + b2)
+      end do
+    end do
+    do f = 1, nf
+      fr(f) = 0.05d0 * (1.4d0**(f-1) - 1)
+    end do
+    kp = [1, 2, 2, 0, 3, 1]
+    call spexhip_gauss4(ctx, nk, no, nu, eo, eu, 0d0, gs, nf, fr, wt, kp, 3)
+    r = 0
+    do f = 2, nf - 1
+      r = r + (fr(f) - fr(f-1)) / (fr(f+1) - fr(f-1))
+      a = r / (f - 1)
+    end do
+    gfac = 1 / (sqrt(2d0) * gs(2))
+    g2fac = 1 / (sqrt(2*pi) * gs(2))
+    fac = sqrt(0.5d0*pi) * gs(2)
+    rw = 0
+    do k = 1, nk
+      p = kp(k) ; if (p == 0) cycle
+      do b1 = 1, no
+        r1 = (1 + erf((0d0 - eo(b1, k)) / (sqrt(2d0) * gs(1)))) / 2
+        do b2 = 1, nu
+          r2 = (1 + erf((eu(b2, k) - 0d0) / (sqrt(2d0) * gs(1)))) / 2
+          d = eu(b2, k) - eo(b1, k)
+          do f = 1, nf
+            if (fr(f) == 0) cycle
+            if (f /= nf) then ; df = a * (fr(f+1) - fr(f))
+            else              ; df = (1 - a) * (fr(f) - fr(f-1))
+            end if
+            rw(b2, b1, p, f) = rw(b2, b1, p, f) + r1 * r2 * g2fac * fac * &
+                (erf(gfac * (fr(f) + df - d)) - erf(gfac * (fr(f) - df - d))) / (2*df)
+          end do
+        end do
+      end do
+    end do
+    rw = rw / nk
+    err = maxval(abs(wt - reshape(rw, [no*nu*3, nf]))) / maxval(abs(rw))
+    write(*, '(A,ES10.3)') 'fortran gauss4 max-rel error: ', err
+    if (err > 1d-12 .or. any(wt(:, 1) /= 0)) stop 14
   end block
   ! FFT grid setup (getinput.f:2472-2473) for a simple cubic reciprocal lattice
   ! of spacing 1.1 written in a skewed basis: determine_imat must make the

@@ -44,7 +44,7 @@ module m_spexhip
   public :: spexhip_minverse, spexhip_screen, spexhip_rpa_int, spexhip_pairsum_hilbert
   public :: spexhip_headwings, spexhip_dielec_inverse_gamma, spexhip_tetrahedron5
   public :: spexhip_dielecc_inverse, spexhip_tetrahedron5_equiv, spexhip_tetrahedron5_bands
-  public :: spexhip_fft_setup, spexhip_step_on_grid, spexhip_alignbd
+  public :: spexhip_fft_setup, spexhip_step_on_grid, spexhip_alignbd, spexhip_gauss4
 
   interface
     integer(c_int) function spexhip_create(device, budget, ctx) bind(C, name='spexhip_create')
@@ -118,6 +118,13 @@ module m_spexhip
                             wtetra
       integer(c_int), value :: nocc, nunocc, nfrqs, nkpt1
       real(c_double), value :: efermi
+    end function
+    integer(c_int) function c_gauss_weights(ctx, nk, nocc, nunocc, e_occ, e_unocc, efermi, &
+        sigma1, sigma2, nfrqs, frqs, kpt1p, nkpt1, wtetra) bind(C, name='spexhip_gauss_weights')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, e_occ, e_unocc, frqs, kpt1p, wtetra
+      integer(c_int), value :: nk, nocc, nunocc, nfrqs, nkpt1
+      real(c_double), value :: efermi, sigma1, sigma2
     end function
     integer(c_int) function c_set_alignbd(ctx, on) bind(C, name='spexhip_set_alignbd')
       import :: c_int, c_ptr
@@ -392,6 +399,32 @@ contains
     call check(ctx, c_tetrahedron_weights(ctx, c_loc(k3), c_loc(rlat), int(nocc, c_int), &
                int(nunocc, c_int), c_loc(ene_occ), c_loc(ene_unocc), efermi, int(nfrqs, c_int), &
                c_loc(frqs), pk, nk1, pdo, pdu, c_loc(wtetra)), 'tetrahedron5')
+  end subroutine
+
+  ! gauss4_init (gauss.f:120-186, kflag 1) for the GAUSS key (susceptibility.f:941-942):
+  ! ene_occ(nocc,nk) at k and ene_unocc(nunocc,nk) at k+q over the full mesh, gauss the
+  ! key's two widths, wtetra(nocc*nunocc*nrow, nfrqs); optional kpt1p(nk) (1-based,
+  ! 0 = no parent) and nkpt1 for the rows of a reduced set.
+  subroutine spexhip_gauss4(ctx, nk, nocc, nunocc, ene_occ, ene_unocc, efermi, gauss, nfrqs, &
+                            frqs, wtetra, kpt1p, nkpt1)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: nk, nocc, nunocc, nfrqs
+    real(c_double), intent(in), target :: ene_occ(nocc, nk), ene_unocc(nunocc, nk), frqs(nfrqs)
+    real(c_double), intent(in) :: efermi, gauss(2)
+    real(c_double), intent(out), target :: wtetra(*)
+    integer, intent(in), optional :: kpt1p(:), nkpt1
+    integer(c_int), allocatable, target :: kp(:)
+    type(c_ptr) :: pk
+    integer(c_int) :: nk1
+    pk = c_null_ptr ; nk1 = 0
+    if (present(kpt1p)) then
+      if (size(kpt1p) /= nk) call size_error('gauss4', 'kpt1p must have one entry per k point')
+      if (.not. present(nkpt1)) call size_error('gauss4', 'kpt1p needs nkpt1')
+      allocate(kp(nk)) ; kp = kpt1p - 1 ; pk = c_loc(kp) ; nk1 = nkpt1
+    end if
+    call check(ctx, c_gauss_weights(ctx, int(nk, c_int), int(nocc, c_int), int(nunocc, c_int), &
+               c_loc(ene_occ), c_loc(ene_unocc), efermi, gauss(1), gauss(2), int(nfrqs, c_int), &
+               c_loc(frqs), pk, nk1, c_loc(wtetra)), 'gauss4')
   end subroutine
 
   ! The ALIGNBD key (getinput.f:1486): the tetrahedron shims stop before the

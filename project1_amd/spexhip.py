@@ -360,6 +360,10 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_tetra_average_equivalent": (I, [_vp, I, I, I, I, _vp, I, _vp, I, _vp, I, _vp]),
         "spexhip_tetrahedron_weights": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I, _vp,
                                             _vp, I, _vp, _vp, _vp]),
+        "spexhip_gauss_weights_dev": (I, [_vp, I, I, I, _vp, _vp, C.c_double, C.c_double,
+                                          C.c_double, I, _vp, _vp, I, _vp, _c_long]),
+        "spexhip_gauss_weights": (I, [_vp, I, I, I, _vp, _vp, C.c_double, C.c_double, C.c_double,
+                                      I, _vp, _vp, I, _vp]),
         "spexhip_tetrahedron_weights_tetraf": (I, [_vp, _vp, _vp, I, I, _vp, _vp, C.c_double, I,
                                                    _vp, _vp, I, _vp, _vp, _vp, _vp]),
         "spexhip_tetrahedron_weights_bands": (I, [_vp, _vp, _vp, I, _vp, _vp, I, I, I, _vp, _vp,
@@ -676,6 +680,33 @@ class Context:
             self.h, len(k1), nocc, nunocc, W.shape[1], _ptr(k1), ks.shape[0], _ptr(kp),
             ks.shape[1], _ptr(ks), ikpt, _ptr(W)))
         return W
+
+    def gauss_weights_dev(self, nk: int, nocc: int, nunocc: int, e_occ: DevBuf, e_unocc: DevBuf,
+                          efermi: float, sigma: tuple, frqs, kpt1p, nkpt1: int, out: DevBuf,
+                          ldw: int) -> None:
+        """GAUSS weights of the Hilbert mesh (gauss4_init) into out."""
+        fr = np.ascontiguousarray(frqs, np.float64)
+        kp = None if kpt1p is None else np.ascontiguousarray(kpt1p, np.int32)
+        self._check(lib().spexhip_gauss_weights_dev(
+            self.h, nk, nocc, nunocc, e_occ.ptr, e_unocc.ptr, efermi, sigma[0], sigma[1],
+            fr.size, _ptr(fr), None if kp is None else _ptr(kp), nkpt1, out.ptr, ldw))
+
+    def gauss_weights(self, e_occ: np.ndarray, e_unocc: np.ndarray, efermi: float, sigma: tuple,
+                      frqs, kpt1p=None, nkpt1: int = 0) -> np.ndarray:
+        """Host drop-in of gauss4_init: wtetra (nrow*nocc*nunocc, nfrqs),
+        Fortran-ordered; e_occ (nk, nocc) at k, e_unocc (nk, nunocc) at k+q."""
+        eo = np.ascontiguousarray(e_occ, np.float64)
+        eu = np.ascontiguousarray(e_unocc, np.float64)
+        fr = np.ascontiguousarray(frqs, np.float64)
+        nk, nocc = eo.shape
+        nunocc = eu.shape[1]
+        kp = None if kpt1p is None else np.ascontiguousarray(kpt1p, np.int32)
+        nrow = nkpt1 if kp is not None else nk
+        out = np.zeros((nrow * nocc * nunocc, fr.size), np.float64, order="F")
+        self._check(lib().spexhip_gauss_weights(
+            self.h, nk, nocc, nunocc, _ptr(eo), _ptr(eu), efermi, sigma[0], sigma[1], fr.size,
+            _ptr(fr), None if kp is None else _ptr(kp), nkpt1, out.ctypes.data))
+        return out
 
     def tetrahedron_weights(self, nkpt3, rlat, e_occ: np.ndarray, e_unocc: np.ndarray,
                             efermi: float, frqs: np.ndarray, kpt1p=None, nkpt1: int = 0,

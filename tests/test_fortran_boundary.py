@@ -17,7 +17,7 @@ def test_module_declares_every_shim():
                  "spexhip_rpa_int", "spexhip_pairsum_hilbert", "spexhip_headwings",
                  "spexhip_dielec_inverse_gamma", "spexhip_tetrahedron5", "spexhip_tetrahedron5_equiv",
                  "spexhip_dielecc_inverse", "spexhip_fft_setup", "spexhip_step_on_grid",
-                 "spexhip_tetrahedron5_bands"]:
+                 "spexhip_tetrahedron5_bands", "spexhip_gauss4", "spexhip_alignbd"]:
         assert f"subroutine {name}" in src
     # every bind(C) name exists in the C header
     import re

@@ -445,3 +445,41 @@ def test_alignbd_stops_before_the_averaging(ctx):
                                     do, du)
     mx, _ = normwise(got, want)
     assert mx < 1e-12
+
+
+@pytest.mark.parametrize("reduced", [False, True])
+def test_gauss_weights_vs_oracle(ctx, reduced):
+    """GAUSS weights (gauss4_init) on the device against the oracle's loop:
+    a metal (efermi inside the bands), a mesh starting at 0, and the rows of
+    a reduced set with a point that has no parent."""
+    g = np.random.default_rng(82)
+    nk, nocc, nunocc = 12, 3, 5
+    eo = g.uniform(-0.8, 0.05, (nk, nocc))
+    eu = g.uniform(-0.05, 0.9, (nk, nunocc))
+    frq = np.concatenate([[0.0], np.cumsum(0.01 * 1.2 ** np.arange(25))])
+    sig = (0.02, 0.04)
+    kpt1p = np.array([0, 1, 1, 2, 0, -1, 2, 3, 4, 4, 4, 1]) if reduced else None
+    nrow = 5 if reduced else nk
+    dEo, dEu = ctx.to_dev(eo), ctx.to_dev(eu)
+    nr = nrow * nocc * nunocc
+    dW = ctx.alloc(8 * nr * len(frq))
+    ctx.gauss_weights_dev(nk, nocc, nunocc, dEo, dEu, 0.0, sig, frq, kpt1p, nrow, dW, nr)
+    got = dW.download(np.float64, (len(frq), nr)).T
+    want = tetra.gauss4_weights(eo, eu, 0.0, sig, frq, kpt1p, nrow)
+    mx, fro = normwise(got, want)
+    print(f"gauss weights reduced={reduced}: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < 1e-12 and (got[:, 0] == 0).all()
+    host = ctx.gauss_weights(eo, eu, 0.0, sig, frq, kpt1p, nrow)
+    np.testing.assert_array_equal(host, got)
+    for b in (dEo, dEu, dW):
+        b.free()
+
+
+def test_gauss_weights_reject_bad_meshes(ctx):
+    eo, eu = np.full((2, 1), -0.5), np.full((2, 1), 0.5)
+    with pytest.raises(spexhip.SpexError, match="not exponential"):
+        ctx.gauss_weights(eo, eu, 0.0, (0.01, 0.01), np.array([0.1, 0.2, 0.3, 0.5, 0.55]))
+    with pytest.raises(spexhip.SpexError, match="at least 3"):
+        ctx.gauss_weights(eo, eu, 0.0, (0.01, 0.01), np.array([0.1, 0.2]))
+    with pytest.raises(spexhip.SpexError, match="widths"):
+        ctx.gauss_weights(eo, eu, 0.0, (0.01, 0.0), np.array([0.1, 0.2, 0.31]))

@@ -621,3 +621,69 @@ def test_alignbd_rows_are_clamped_sums_that_the_averaging_divides(w1, w2):
                 np.testing.assert_allclose(A[k, r1, r2], mean, rtol=1e-13,
                                            atol=1e-15 * np.abs(full).max())
     assert np.abs(raw - avg).max() > 1e-6 * np.abs(avg).max()
+
+
+def _exp_mesh(n, w0, r, zero=False):
+    """An exponential Hilbert mesh: steps growing by r, optionally from 0."""
+    steps = w0 * r ** np.arange(n - 1)
+    start = 0.0 if zero else w0
+    return start + np.concatenate([[0.0], np.cumsum(steps)])
+
+
+def test_gauss_weights_are_bin_integrals_of_the_gaussian():
+    """Each weight times the bin width is the quadrature of the occupation
+    factors times the normal density of width sigma2 around d over the bin
+    (the bin edges from the mesh factor a = 1/(1+r) of an exponential mesh),
+    and bins at w = 0 are empty."""
+    from scipy import integrate
+    frq = _exp_mesh(12, 0.05, 1.3, zero=True)
+    eo = np.array([[-0.4, -0.05]])
+    eu = np.array([[0.02, 0.6, 1.1]])
+    ef, s1, s2 = 0.0, 0.03, 0.08
+    w = tetra.gauss4_weights(eo, eu, ef, (s1, s2), frq).reshape(2, 3, -1)
+    assert (w[..., 0] == 0).all()
+    a = 1 / (1 + 1.3)
+    for b1 in range(2):
+        for b2 in range(3):
+            d = eu[0, b2] - eo[0, b1]
+            occ = (1 + math.erf((ef - eo[0, b1]) / (math.sqrt(2) * s1))) / 2
+            unocc = (1 + math.erf((eu[0, b2] - ef) / (math.sqrt(2) * s1))) / 2
+            for i in range(1, len(frq)):
+                df = a * (frq[i + 1] - frq[i]) if i < len(frq) - 1 else (1 - a) * (frq[i] - frq[i - 1])
+                mass, _ = integrate.quad(
+                    lambda x: math.exp(-(x - d) ** 2 / (2 * s2 ** 2)) / (math.sqrt(2 * math.pi) * s2),
+                    frq[i] - df, frq[i] + df, epsabs=1e-15, epsrel=1e-13)
+                np.testing.assert_allclose(w[b1, b2, i] * 2 * df, occ * unocc * mass, rtol=1e-10,
+                                           atol=1e-16)
+
+
+def test_gauss_weights_tile_the_mesh_and_sum_over_stars():
+    """The bins of an exponential mesh tile [w1 - df1, w_n + df_n], so
+    sum_i w(i) 2 df(i) is the Gaussian mass there times the occupations
+    (divided by nk); the rows of a reduced set are the sums over their
+    members, and mesh points without a parent drop out."""
+    g = np.random.default_rng(81)
+    nk, nocc, nunocc = 6, 2, 3
+    eo = g.uniform(-1.0, -0.1, (nk, nocc))
+    eu = g.uniform(0.1, 1.0, (nk, nunocc))
+    frq = _exp_mesh(30, 0.01, 1.15)
+    s = (0.02, 0.05)
+    w = tetra.gauss4_weights(eo, eu, 0.0, s, frq).reshape(nk, nocc, nunocc, -1)
+    a = 1 / (1 + 1.15)  # steps grow by r = 1.15
+    df = np.append(a * np.diff(frq), (1 - a) * (frq[-1] - frq[-2]))
+    lo, hi = frq[0] - df[0], frq[-1] + df[-1]
+    np.testing.assert_allclose(frq[:-1] + df[:-1], frq[1:] - df[1:], rtol=1e-13)
+    for k in range(nk):
+        for b1 in range(nocc):
+            for b2 in range(nunocc):
+                d = eu[k, b2] - eo[k, b1]
+                occ = (1 + math.erf(-eo[k, b1] / (math.sqrt(2) * s[0]))) / 2
+                unocc = (1 + math.erf(eu[k, b2] / (math.sqrt(2) * s[0]))) / 2
+                mass = (math.erf((hi - d) / (math.sqrt(2) * s[1]))
+                        - math.erf((lo - d) / (math.sqrt(2) * s[1]))) / 2
+                np.testing.assert_allclose((w[k, b1, b2] * 2 * df).sum(), occ * unocc * mass / nk,
+                                           rtol=1e-12)
+    kpt1p = np.array([0, 1, 1, -1, 0, 2])
+    red = tetra.gauss4_weights(eo, eu, 0.0, s, frq, kpt1p, 3).reshape(3, nocc, nunocc, -1)
+    for p in range(3):
+        np.testing.assert_allclose(red[p], w[kpt1p == p].sum(0), rtol=1e-13, atol=1e-300)
