@@ -508,6 +508,16 @@ int spexhip_fermi_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
 int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void *w_dev,
                                  const void *P_dev, long ldp, const int *deg, double vol,
                                  double spin_factor, double *plasma);
+/* The two halves of the above, for spin-polarised metals, whose plasma^2
+ * sums both spins before the root (susceptibility.f:1395-1441):
+ * spexhip_plasma_sum_dev adds 4 pi/(3 vol) * sum_{k,n} w |Re <p>|^2 of one
+ * spin to *plasma2 (arguments as above); spexhip_plasma_finish multiplies
+ * by spin_factor (2 for a spin-unpolarised system without SOC, else 1),
+ * rejects plasma^2 < -1e-10 and takes the root, 0 below 1e-6 eV.         */
+int spexhip_plasma_sum_dev(spexhip_ctx *ctx, int nk, int nband, const void *w_dev,
+                           const void *P_dev, long ldp, const int *deg, double vol,
+                           double *plasma2);
+int spexhip_plasma_finish(spexhip_ctx *ctx, double plasma2, double spin_factor, double *plasma);
 /* correlation.f:847-862: head(j,j,w) -= drude(freq(w)) = -plasma^2/freq^2
  * (infinity at freq 0) on the 3 x 3 x nw head_dev, and headc(j,j,w) -=
  * plasma^2/freqc^2 on the 3 x 3 x nwc headc_dev (freqc host complex).

@@ -13,7 +13,8 @@ Restates, in plain numpy/Python loops:
   spin: integrand(n, k) = sum_j (real part of the mean of momentum(i, i, k,
   j) over the degenerate group of n)^2, plasma^2 = 4 pi/(3 vol) sum
   wintgr4 * integrand, times 2 for a spin-unpolarised system, then the
-  square root, set to zero below 1e-6 eV;
+  square root, set to zero below 1e-6 eV; for two spins the squares are
+  summed before the root (the spin loop of :1395-1437);
 - drude / drudec (correlation.f:1144-1169) and the head update of
   correlation.f:847-862: head(j, j, w) -= drude(freq(w)).
 
@@ -127,6 +128,36 @@ def plasma_frequency(wintgr4, mom_diag, deg, vol, spin_factor=2.0):
             integrand[k, i1:i2 + 1] = np.sum(np.abs(re) ** 2)
             i1 = i2 + 1
     p2 = 4 * math.pi / 3 / vol * np.sum(W * integrand) * spin_factor
+    if p2 < -1e-10:
+        raise ValueError("plasma frequency undefined")
+    p = math.sqrt(max(0.0, p2))
+    if p * HARTREE < 1e-6:
+        p = 0.0
+    return p
+
+
+def plasma_frequency_spins(spins, vol, spin_factor=1.0):
+    """The loop over spins of susceptibility.f:1395-1441: plasma += 4 pi/(3
+    vol) sum(wdrude * integrand) for each (wintgr4, mom_diag, deg) of
+    `spins`, times spin_factor (2 for one spin of an unpolarised system
+    without SOC, :1439), then the checks and the root of :1440-1441,
+    1693-1700."""
+    p2 = 0.0
+    for W, mom_diag, deg in spins:
+        W = np.asarray(W, float)
+        nk, nband = W.shape
+        integrand = np.zeros((nk, nband))
+        for k in range(nk):
+            i1 = 0
+            while i1 < nband:
+                i2 = int(deg[k][i1])
+                if i2 >= nband:
+                    break
+                re = np.real(mom_diag[k, i1:i2 + 1, :].sum(0)) / (i2 - i1 + 1)
+                integrand[k, i1:i2 + 1] = np.sum(np.abs(re) ** 2)
+                i1 = i2 + 1
+        p2 = p2 + 4 * math.pi / 3 / vol * np.sum(W * integrand)
+    p2 = p2 * spin_factor
     if p2 < -1e-10:
         raise ValueError("plasma frequency undefined")
     p = math.sqrt(max(0.0, p2))

@@ -202,11 +202,10 @@ int spexhip_fermi_weights_dev(spexhip_ctx *ctx, const int *nkpt3, const double *
   return SPEXHIP_OK;
 }
 
-int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void *w,
-                                 const void *P, long ldp, const int *deg, double vol,
-                                 double spin_factor, double *plasma) {
+int spexhip_plasma_sum_dev(spexhip_ctx *ctx, int nk, int nband, const void *w, const void *P,
+                           long ldp, const int *deg, double vol, double *plasma2) {
   if (!ctx) return SPEXHIP_E_INVALID;
-  SPX_CHECK(ctx, nk > 0 && nband > 0 && deg && plasma && vol > 0 && spin_factor > 0,
+  SPX_CHECK(ctx, nk > 0 && nband > 0 && deg && plasma2 && vol > 0,
             "plasma_frequency: bad arguments");
   const long n = (long)nk * nband * nband;
   SPX_CHECK(ctx, ldp >= n, "plasma_frequency: ldp < nk*nband^2");
@@ -237,12 +236,30 @@ int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void
   double s = 0.0;
   SPX_HIP(ctx, hipMemcpyAsync(&s, sum, 8, hipMemcpyDeviceToHost, ctx->stream));
   SPX_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const double p2 = 4 * M_PI / 3 / vol * s * spin_factor;
+  *plasma2 = *plasma2 + 4 * M_PI / 3 / vol * s;  // susceptibility.f:1433
+  return SPEXHIP_OK;
+}
+
+int spexhip_plasma_finish(spexhip_ctx *ctx, double plasma2, double spin_factor, double *plasma) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, plasma && spin_factor > 0, "plasma_frequency: bad arguments");
+  const double p2 = plasma2 * spin_factor;  // :1439
   SPX_CHECK(ctx, p2 >= -1e-10, "plasma_frequency: Plasma frequency undefined (%g)", p2);
   double p = std::sqrt(p2 > 0.0 ? p2 : 0.0);
   if (p * 27.2113831243217 < 1e-6) p = 0.0;  // below 1e-6 eV: zero (susceptibility.f:1696)
   *plasma = p;
   return SPEXHIP_OK;
+}
+
+int spexhip_plasma_frequency_dev(spexhip_ctx *ctx, int nk, int nband, const void *w,
+                                 const void *P, long ldp, const int *deg, double vol,
+                                 double spin_factor, double *plasma) {
+  if (!ctx) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, plasma && spin_factor > 0, "plasma_frequency: bad arguments");
+  double p2 = 0.0;
+  const int rc = spexhip_plasma_sum_dev(ctx, nk, nband, w, P, ldp, deg, vol, &p2);
+  if (rc) return rc;
+  return spexhip_plasma_finish(ctx, p2, spin_factor, plasma);
 }
 
 int spexhip_gamma_drude_dev(spexhip_ctx *ctx, int nw, const double *freq, double plasma,

@@ -324,6 +324,9 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_fermi_weights_dev": (I, [_vp, _vp, _vp, I, _vp, C.c_double, _vp]),
         "spexhip_plasma_frequency_dev": (I, [_vp, I, I, _vp, _vp, _c_long, _vp, C.c_double,
                                              C.c_double, P(C.c_double)]),
+        "spexhip_plasma_sum_dev": (I, [_vp, I, I, _vp, _vp, _c_long, _vp, C.c_double,
+                                       P(C.c_double)]),
+        "spexhip_plasma_finish": (I, [_vp, C.c_double, C.c_double, P(C.c_double)]),
         "spexhip_gamma_drude_dev": (I, [_vp, I, _vp, C.c_double, _vp, I, _vp, _vp]),
         "spexhip_coulomb_trafo_dev": (I, [_vp, _c_long, I, _vp, _c_long, _vp, _c_long, I, _vp,
                                           _c_long]),
@@ -1100,6 +1103,20 @@ class Context:
         self._check(lib().spexhip_plasma_frequency_dev(self.h, nk, nband, w.ptr, P.ptr, ldp,
                                                        _ptr(d), float(vol), float(spin_factor),
                                                        C.byref(out)))
+        return out.value
+
+    def plasma_frequency_spins_dev(self, spins: list, vol: float,
+                                   spin_factor: float = 1.0) -> float:
+        """plasma^2 summed over spins = [(nk, nband, w, P, ldp, deg), ...],
+        then spin_factor and the root (susceptibility.f:1395-1441)."""
+        p2 = C.c_double(0.0)
+        for nk, nband, w, P, ldp, deg in spins:
+            d = np.ascontiguousarray(deg, np.int32)
+            self._check(lib().spexhip_plasma_sum_dev(self.h, nk, nband, w.ptr, P.ptr, ldp,
+                                                     _ptr(d), float(vol), C.byref(p2)))
+        out = C.c_double(0.0)
+        self._check(lib().spexhip_plasma_finish(self.h, p2.value, float(spin_factor),
+                                                C.byref(out)))
         return out.value
 
     def gamma_drude_dev(self, freq, plasma: float, head: DevBuf, freqc=None,

@@ -126,3 +126,32 @@ def test_plasma_degenerate_groups_and_bad_deg(ctx):
         ctx.plasma_frequency_dev(nk, nb, dW, dP, n, bad, 7.5)
     for b in (dW, dP):
         b.free()
+
+
+def test_plasma_two_spins_vs_oracle(ctx):
+    """A spin-polarised metal: the two spins' sums (different band counts
+    and degeneracies) added before the root, against the oracle's spin
+    loop; one spin with factor 2 gives the one-call result."""
+    g = np.random.default_rng(96)
+    nk = 6
+    spins, oracle_spins = [], []
+    for nb in (4, 3):
+        W = g.uniform(0, 1, (nk, nb))
+        P = g.standard_normal((nk * nb * nb, 3)) + 1j * g.standard_normal((nk * nb * nb, 3))
+        deg = np.tile(np.arange(nb, dtype=np.int32), (nk, 1))
+        deg[2, 0], deg[2, 1] = 1, 0
+        n = nk * nb * nb
+        dW = ctx.to_dev(np.ascontiguousarray(W))
+        dP = ctx.to_dev(np.asfortranarray(P).ravel(order="F"))
+        spins.append((nk, nb, dW, dP, n, deg))
+        mom_diag = P.reshape(nk, nb, nb, 3)[:, np.arange(nb), np.arange(nb), :]
+        oracle_spins.append((W, mom_diag, deg))
+    got = ctx.plasma_frequency_spins_dev(spins, 9.25)
+    want = drude.plasma_frequency_spins(oracle_spins, 9.25)
+    print(f"two-spin plasma: device {got:.15e} oracle {want:.15e}")
+    assert want > 0 and got == pytest.approx(want, rel=1e-13)
+    one = ctx.plasma_frequency_spins_dev(spins[:1], 9.25, 2.0)
+    assert one == ctx.plasma_frequency_dev(*spins[0], 9.25)
+    for s in spins:
+        s[2].free()
+        s[3].free()

@@ -112,6 +112,29 @@ def test_plasma_closed_forms():
     assert drude.plasma_frequency(W, mom2, deg2, vol) == 0.0
 
 
+def test_plasma_spin_sum_closed_forms():
+    """Two spins: plasma^2 is the sum of the spins' squares (constant momenta
+    p_s on one band give 4 pi/(3 vol) sum_s |p_s|^2 DOS_s); two equal spins
+    give the spin-unpolarised value (one spin times 2)."""
+    n = 3
+    ene = _mesh_bands(n, 3)
+    ef, vol = 0.05, 120.0
+    W = drude.fermi_surface_weights((n, n, n), np.eye(3), ene, ef)
+    nk, nb = W.shape
+    deg = np.tile(np.arange(nb), (nk, 1))
+    p_up, p_dn = np.array([0.3, -0.2, 0.1]), np.array([0.05, 0.4, -0.25])
+    mu = np.zeros((nk, nb, 3), complex)
+    md = np.zeros((nk, nb, 3), complex)
+    mu[:, 0, :], md[:, 0, :] = p_up, p_dn
+    got = drude.plasma_frequency_spins([(W, mu, deg), (W, md, deg)], vol)
+    want = math.sqrt(4 * math.pi / 3 / vol * (np.sum(p_up**2) + np.sum(p_dn**2)) * W[:, 0].sum())
+    assert got == pytest.approx(want, rel=1e-13)
+    same = drude.plasma_frequency_spins([(W, mu, deg), (W, mu, deg)], vol)
+    assert same == pytest.approx(drude.plasma_frequency(W, mu, deg, vol), rel=1e-14)
+    assert drude.plasma_frequency_spins([(W, mu, deg)], vol, 2.0) == \
+        drude.plasma_frequency(W, mu, deg, vol)
+
+
 def test_drude_functions_and_head():
     assert drude.drude(0.5, 0.0) == 0.0
     assert drude.drude(0.5, 2.0) == pytest.approx(-16.0)
