@@ -26,6 +26,9 @@ _lib = None
 
 def build() -> Path:
     subprocess.run(["make", "-s", "-C", str(HERE), "liboracle.so"], check=True)
+    # the reference's own cuttetraN into oracle/_ref (skipped without the
+    # reference sources)
+    subprocess.run([str(HERE / "build_ref.sh")], check=True)
     return LIB
 
 

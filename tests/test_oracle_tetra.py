@@ -5,6 +5,7 @@ the whole mesh: flat bands (all weight in one bin), the Brillouin-zone sum
 rule, and an independent assembly from the clipped-polytope integrals."""
 import itertools
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -687,3 +688,48 @@ def test_gauss_weights_tile_the_mesh_and_sum_over_stars():
     red = tetra.gauss4_weights(eo, eu, 0.0, s, frq, kpt1p, 3).reshape(3, nocc, nunocc, -1)
     for p in range(3):
         np.testing.assert_allclose(red[p], w[kpt1p == p].sum(0), rtol=1e-13, atol=1e-300)
+
+
+_REF_CUT = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "libcuttetra.so"
+
+
+@pytest.mark.skipif(not _REF_CUT.exists(), reason="oracle/_ref not built (no reference sources)")
+def test_cuttetraN_matches_the_reference_build_bit_for_bit():
+    """The reference's own cuttetraN (tetrahedron.f:347-412, compiled by
+    oracle/build_ref.sh) against the restatement: random corners in every
+    regime, tied corner values (minloc order), and thresholds at the eta
+    edges and exactly at corner values."""
+    import ctypes as C
+    lib = C.CDLL(str(_REF_CUT))
+    f = lib.cuttetran_
+    f.restype = None
+    dp = C.POINTER(C.c_double)
+
+    def ref(e, ef):
+        w = (C.c_double * 4)()
+        v = (C.c_double * 3)(-7.0, -7.0, -7.0)
+        n = C.c_int(-1)
+        ea = (C.c_double * 4)(*e)
+        efa = C.c_double(ef)
+        f(w, v, C.byref(n), ea, C.byref(efa), None)
+        return np.array(w[:]), np.array(v[:]), n.value
+
+    g = np.random.default_rng(91)
+    cases = []
+    for _ in range(3000):
+        e = g.uniform(-1, 1, 4)
+        cases.append((e, g.uniform(-1.2, 1.2)))
+    for _ in range(500):
+        e = np.round(g.uniform(-1, 1, 4) * 4) / 4  # ties
+        for ef in (e.min(), e.max(), np.median(e), e.min() + 1e-14, e.max() - 1e-14,
+                   e.min() + 5e-15, e.max() - 5e-15, g.uniform(-1, 1)):
+            cases.append((e, float(ef)))
+    nreg = set()
+    for e, ef in cases:
+        wr, vr, nr = ref(e, ef)
+        wo, vo, no = tetra.cuttetraN(e, ef)
+        assert nr == no, (e, ef)
+        assert np.array_equal(wr, wo), (e, ef, wr, wo)
+        assert np.array_equal(vr[:nr], np.asarray(vo)[:nr]), (e, ef)
+        nreg.add((nr, int(np.sum(e < ef))))
+    assert len(nreg) >= 5  # every regime was visited
