@@ -14,7 +14,14 @@
 // inverts the diagonal block in registers, an MFMA GEMM forms the row panel,
 // and a rank-64 MFMA update sweeps the whole matrix.  eps is Hermitian
 // positive definite on the imaginary axis (eigenvalues >= 1), so every
-// leading block is HPD and no pivoting is needed there.  The real-frequency
+// leading block is HPD and no pivoting is needed there.  For such a matrix
+// the working array stays Hermitian up to sign while the sweep runs: with S
+// the columns already swept and U the rest, W(S,S) and W(U,U) are Hermitian
+// and W(U,S) = -W(S,U)^H.  So only the tiles on and above the diagonal are
+// updated (half the rank-64 work), the column and row panels of a step are
+// read from the upper triangle (W(i,K) = conj(W(K,i)) below block K,
+// W(K,j) = -conj(W(j,K)) left of it), and the lower triangle is filled from
+// the upper one at the end.  The real-frequency
 // eps (dielecc) is a general complex matrix, which the reference inverts by
 // LU with partial pivoting (zgetrf/zgetri, wrapper.f:2357-2358); here each
 // panel first picks its pivot rows by an LU with partial pivoting of the
@@ -355,16 +362,49 @@ __global__ void gj_copy_col_kernel(int nb, const double2 *__restrict__ X, long l
   Cp[(long)ldcp * GB * blockIdx.y + i + ldcp * (long)c] = v;
 }
 
+// The column panel Cp = W(:,K) (nb x GB, ld ldcp) and the row panel
+// Rs = W(K,:) (GB x nb, ld GB) of the Hermitian sweep, from the upper
+// triangle and the diagonal tiles of W.  blockIdx.z = 0: thread per (i, c),
+// i fastest, rows i above block K's end (Cp direct; Rs(c,i) = -conj(W(i,k))
+// left of K); 1: thread per (c, j), c fastest, columns j >= k0 (Rs direct;
+// Cp(j,c) = conj(W(k,j)) below K).  Columns of K past nb are zero.
+__global__ void gj_herm_panels_kernel(int nb, const double2 *__restrict__ X, long ldx, long sX,
+                                      int k0, double2 *__restrict__ Cp, long ldcp,
+                                      double2 *__restrict__ Rs, long sR) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  const double2 *Xb = X + sX * b;
+  double2 *Cb = Cp + ldcp * (long)GB * b, *Rb = Rs + sR * b;
+  const int kend = k0 + GB < nb ? k0 + GB : nb;
+  if (blockIdx.z == 0) {
+    if (g >= (long)kend * GB) return;
+    const int c = (int)(g / kend), i = (int)(g % kend), k = k0 + c;
+    const double2 v = k < nb ? Xb[i + ldx * (long)k] : make_double2(0.0, 0.0);
+    Cb[i + ldcp * (long)c] = v;
+    if (i < k0) Rb[c + (long)GB * i] = make_double2(-v.x, v.y);
+  } else {
+    const long cnt = (long)(nb - k0) * GB;
+    if (g >= cnt) return;
+    const int j = k0 + (int)(g / GB), c = (int)(g % GB), k = k0 + c;
+    const double2 v = k < nb ? Xb[k + ldx * (long)j] : make_double2(0.0, 0.0);
+    Rb[c + (long)GB * j] = v;
+    if (j >= kend) Cb[j + ldcp * (long)c] = make_double2(v.x, -v.y);
+  }
+}
+
 // Rank-GB sweep of Gauss-Jordan step k0 (see file header):
 //   A(K,K) = P;  A(K,J) = R(:,J);  A(I,K) = -Cp(I,:) P;  A(I,J) -= Cp(I,:) R(:,J)
+// herm: tiles below the diagonal are left alone (the Hermitian sweep).
 __global__ __launch_bounds__(NT) void gj_update_kernel(int nb, double2 *__restrict__ X, long ldx,
                                                        long sX, int k0,
                                                        const double2 *__restrict__ P,
                                                        const double2 *__restrict__ R, long ldr, long sR,
-                                                       const double2 *__restrict__ Cp, long ldcp) {
+                                                       const double2 *__restrict__ Cp, long ldcp,
+                                                       int herm) {
   __shared__ GemmLds s;
   const int b = blockIdx.z;
   const int i0 = blockIdx.x * TM, j0 = blockIdx.y * TM;
+  if (herm && i0 > j0) return;
   const bool rowK = (i0 == k0), colK = (j0 == k0);
   double2 *Xb = X + sX * b;
   const double2 *Pb = P + (long)GB * GB * b;
@@ -431,7 +471,7 @@ int spx_hermitize(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, lon
 size_t spx_inverse_ws(int nb, int nw) {
   const long nbp = ((nb + GB - 1) / GB) * GB;
   const size_t piv = ((4 * (size_t)nw * nbp) + 255) & ~(size_t)255;
-  return 16 * (size_t)nw * ((size_t)GB * GB + (size_t)GB * nbp + (size_t)nbp * GB) + piv + 256;
+  return 16 * (size_t)nw * ((size_t)GB * GB + 3 * (size_t)GB * nbp) + piv + 256;
 }
 
 // In-place inverse of nw matrices (full storage): HPD (general = 0) or
@@ -444,7 +484,8 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
   double2 *P = (double2 *)ws;
   double2 *R = P + (long)nw * GB * GB;        // GB x nbp per matrix (ld GB)
   double2 *Cp = R + (long)nw * GB * nbp;      // nbp x GB per matrix (ld nbp)
-  int *piv = (int *)(Cp + (long)nw * nbp * GB);  // nbp per matrix (general)
+  double2 *Rs = Cp + (long)nw * nbp * GB;     // GB x nbp per matrix (ld GB): W(K,:), HPD
+  int *piv = (int *)(Rs + (long)nw * GB * nbp);  // nbp per matrix (general)
   const long sX = ldx * (long)nb;
   const int nt = (int)(nbp / GB);
   for (int kb = 0; kb < nt; kb++) {
@@ -464,23 +505,40 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
                          (const double2 *)X, ldx, sX, k0, P, flag,
                          logdet_parts ? logdet_parts + (long)kb * nw : nullptr);
     SPX_LAUNCHED(ctx);
-    // R = P * X(K, :)   (GB x nb), batched
-    int rc = spx_zgemm(ctx, st, GB, nb, (nb - k0 < GB ? nb - k0 : GB), 1.0, P, GB, (long)GB * GB,
-                       (const double2 *)X + k0, ldx, sX, 0.0, R, GB, (long)GB * nbp, 0.0, nw);
-    if (rc) return rc;
-    const long tot = (long)nb * GB;
-    hipLaunchKernelGGL(gj_copy_col_kernel, dim3((tot + 255) / 256, nw), dim3(256), 0, st, nb,
-                       (const double2 *)X, ldx, sX, k0, Cp, nbp);
-    SPX_LAUNCHED(ctx);
+    const int kk = nb - k0 < GB ? nb - k0 : GB;
+    if (general) {
+      // R = P * X(K, :)   (GB x nb), batched; then the old column panel
+      int rc = spx_zgemm(ctx, st, GB, nb, kk, 1.0, P, GB, (long)GB * GB, (const double2 *)X + k0,
+                         ldx, sX, 0.0, R, GB, (long)GB * nbp, 0.0, nw);
+      if (rc) return rc;
+      const long tot = (long)nb * GB;
+      hipLaunchKernelGGL(gj_copy_col_kernel, dim3((tot + 255) / 256, nw), dim3(256), 0, st, nb,
+                         (const double2 *)X, ldx, sX, k0, Cp, nbp);
+      SPX_LAUNCHED(ctx);
+    } else {
+      // both panels from the upper triangle, then R = P * W(K, :)
+      const long tot = (long)nb * GB;
+      hipLaunchKernelGGL(gj_herm_panels_kernel, dim3((unsigned)((tot + 255) / 256), nw, 2),
+                         dim3(256), 0, st, nb, (const double2 *)X, ldx, sX, k0, Cp, nbp, Rs,
+                         (long)GB * nbp);
+      SPX_LAUNCHED(ctx);
+      int rc = spx_zgemm(ctx, st, GB, nb, kk, 1.0, P, GB, (long)GB * GB, Rs, GB, (long)GB * nbp,
+                         0.0, R, GB, (long)GB * nbp, 0.0, nw);
+      if (rc) return rc;
+    }
     hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt, nw), dim3(NT), 0, st, nb, (double2 *)X, ldx,
                        sX, k0, (const double2 *)P, (const double2 *)R, (long)GB, (long)GB * nbp,
-                       (const double2 *)Cp, nbp);
+                       (const double2 *)Cp, nbp, general ? 0 : 1);
     SPX_LAUNCHED(ctx);
   }
   if (general) {
     hipLaunchKernelGGL(gj_unswap_cols_kernel, dim3((nb + 255) / 256, nw), dim3(256), 0, st, nb,
                        (double2 *)X, ldx, sX, (const int *)piv, nbp);
     SPX_LAUNCHED(ctx);
+  } else {
+    // the lower triangle from the upper one
+    int rc = spx_hermitize(ctx, st, nb, nw, X, ldx, 0);
+    if (rc) return rc;
   }
   return SPEXHIP_OK;
 }
