@@ -746,6 +746,10 @@ int spexhip_pw_plan_create(spexhip_ctx *ctx, const spexhip_pw_desc *desc, spexhi
 int spexhip_plan_destroy(spexhip_ctx *ctx, spexhip_plan *plan);
 /* number of product columns the plan writes (nbasp or ngptm) */
 int spexhip_plan_ncols(spexhip_ctx *ctx, const spexhip_plan *plan, int *ncols);
+/* The grid an interstitial plan transforms on (nfft[3]): the caller's
+ * nfft, or the smallest alias-free grid the plan chose in its place when
+ * the caller's grid is alias-free and there is no step function.         */
+int spexhip_pw_plan_grid(spexhip_ctx *ctx, const spexhip_plan *plan, int *nfft);
 
 /* wavefproducts2_mt (wavefproducts.f:95-431) on the device.  cmt_dev is the
  * reference array cmt(maxlmindx,ncent,nband,nkpt) (complex, column-major);

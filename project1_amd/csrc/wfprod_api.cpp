@@ -85,6 +85,13 @@ int spexhip_plan_ncols(spexhip_ctx *ctx, const spexhip_plan *plan, int *ncols) {
   return SPEXHIP_OK;
 }
 
+int spexhip_pw_plan_grid(spexhip_ctx *ctx, const spexhip_plan *plan, int *nfft) {
+  if (!ctx || !plan || !nfft) return SPEXHIP_E_INVALID;
+  SPX_CHECK(ctx, plan->kind == 2, "pw_plan_grid: not an interstitial plan");
+  for (int d = 0; d < 3; d++) nfft[d] = plan->nfft[d];
+  return SPEXHIP_OK;
+}
+
 int spexhip_wfprod2_mt_dev(spexhip_ctx *ctx, const spexhip_plan *plan, const void *cmt_dev,
                            int ncent, int nband, int nkpt, const double *q, int nk, const int *k1,
                            const int *k2, int b1lo, int nocc, int b2lo, int nunocc, void *M_dev,

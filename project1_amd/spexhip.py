@@ -297,6 +297,13 @@ class Plan:
         self.ctx._check(lib().spexhip_plan_ncols(self.ctx.h, self.h, C.byref(v)))
         return v.value
 
+    @property
+    def grid(self) -> tuple:
+        """The (N1, N2, N3) an interstitial plan transforms on."""
+        v = np.zeros(3, np.int32)
+        self.ctx._check(lib().spexhip_pw_plan_grid(self.ctx.h, self.h, _ptr(v)))
+        return tuple(int(x) for x in v)
+
     def destroy(self) -> None:
         if self.h:
             self.ctx._check(lib().spexhip_plan_destroy(self.ctx.h, self.h))
@@ -444,6 +451,7 @@ def _declare(L: C.CDLL) -> None:
         "spexhip_pw_plan_create": (I, [_vp, P(PwDesc), P(_vp)]),
         "spexhip_plan_destroy": (I, [_vp, _vp]),
         "spexhip_plan_ncols": (I, [_vp, _vp, P(I)]),
+        "spexhip_pw_plan_grid": (I, [_vp, _vp, _vp]),
         "spexhip_wfprod2_mt_dev": (I, [_vp, _vp, _vp, I, I, I, _vp, I, _vp, _vp, I, I, I, I,
                                        _vp, _c_long, I]),
         "spexhip_wfprod2_pw_dev": (I, [_vp, _vp, _vp, I, I, _vp, _vp, _vp, I, I, I, I, _vp,

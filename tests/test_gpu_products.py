@@ -115,6 +115,15 @@ def test_pw_realistic_box_subset(ctx):
     mx, fro = normwise(got, ref)
     print(f"PW realistic: max-rel {mx:.2e} fro-rel {fro:.2e}")
     assert mx < TOL
+    # the plan transforms on the alias-free 24^3 grid in place of the caller's
+    # 36^3 (states of width 15 per axis: 15 + 9 = 24), and keeps 36^3 with theta
+    plan = ctx.pw_plan(s.nfft, s.gpt, s.ngpt, s.gptm, s.vol)
+    assert plan.grid == (24, 24, 24)
+    plan.destroy()
+    th = np.full(int(np.prod(s.nfft)), 0.9)
+    plan = ctx.pw_plan(s.nfft, s.gpt, s.ngpt, s.gptm, s.vol, step_r=th)
+    assert plan.grid == (36, 36, 36)
+    plan.destroy()
 
 
 def test_host_dropins_match_oracle(ctx):
