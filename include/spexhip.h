@@ -902,7 +902,10 @@ int spexhip_dielectric_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ld
 
 /* In-place inverse of nw Hermitian positive definite full device matrices;
  * replaces Minverse(dielec(:,i)) (correlation.f:709 -> inverse_zp1,
- * wrapper.f:2362-2374).  SPEXHIP_E_NOTPD if a pivot is not positive.      */
+ * wrapper.f:2362-2374).  Only the upper triangle (and the diagonal 64 x 64
+ * blocks) of X is read -- the matrix is taken Hermitian, as the reference's
+ * packed storage holds it; the whole inverse is written.  SPEXHIP_E_NOTPD
+ * if a pivot is not positive.                                             */
 int spexhip_inverse_hpd_dev(spexhip_ctx *ctx, int nb, int nw, void *X_dev, int ldx);
 
 /* Real frequencies (dielecc, correlation.f:632-636 and :701-747): eps = 1 -
