@@ -121,18 +121,43 @@ __global__ __launch_bounds__(NT) void pw_inverse_kernel(
       }
       U[kk] = make_double2(ur, ui);
     }
+    // y and x passes; for grids of up to 32 points per axis with three real
+    // MFMAs per complex product (Gauss): t1 = ar.br, t2 = ai.bi,
+    // t3 = (ar + ai)(br + bi); re = t1 - t2, im = t3 - t1 - t2.  Larger grids
+    // keep four (the third accumulators would cost the second wave per SIMD).
+    constexpr bool M3 = NB <= 2;
 #pragma unroll
     for (int mb = 0; mb < NB; mb++) {
       if (16 * mb >= N2) continue;
-      d4_t vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0};
+      d4_t vr = {0, 0, 0, 0}, vi = {0, 0, 0, 0}, vs = {0, 0, 0, 0};
 #pragma unroll
-      for (int kk = 0; kk < 4; kk++) cmfma(vr, vi, U[kk], E2[mb][kk]);
+      for (int kk = 0; kk < 4; kk++) {
+        const double2 u = U[kk], e = E2[mb][kk];
+        if (M3) {
+          vr = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x, e.x, vr, 0, 0, 0);
+          vi = __builtin_amdgcn_mfma_f64_16x16x4f64(u.y, e.y, vi, 0, 0, 0);
+          vs = __builtin_amdgcn_mfma_f64_16x16x4f64(u.x + u.y, e.x + e.y, vs, 0, 0, 0);
+        } else {
+          cmfma(vr, vi, u, e);
+        }
+      }
+      const d4_t yr = M3 ? vr - vi : vr, yi = M3 ? vs - vr - vi : vi;
 #pragma unroll
       for (int nb = 0; nb < NB; nb++) {
         if (16 * nb >= N1) continue;
-        d4_t orr = {0, 0, 0, 0}, oi = {0, 0, 0, 0};
+        d4_t tr = {0, 0, 0, 0}, ti = {0, 0, 0, 0}, ts = {0, 0, 0, 0};
 #pragma unroll
-        for (int kk = 0; kk < 4; kk++) cmfma(orr, oi, make_double2(vr[kk], vi[kk]), E1[nb][kk]);
+        for (int kk = 0; kk < 4; kk++) {
+          const double2 e = E1[nb][kk];
+          if (M3) {
+            tr = __builtin_amdgcn_mfma_f64_16x16x4f64(yr[kk], e.x, tr, 0, 0, 0);
+            ti = __builtin_amdgcn_mfma_f64_16x16x4f64(yi[kk], e.y, ti, 0, 0, 0);
+            ts = __builtin_amdgcn_mfma_f64_16x16x4f64(yr[kk] + yi[kk], e.x + e.y, ts, 0, 0, 0);
+          } else {
+            cmfma(tr, ti, make_double2(yr[kk], yi[kk]), e);
+          }
+        }
+        const d4_t orr = M3 ? tr - ti : tr, oi = M3 ? ts - tr - ti : ti;
         const int x = 16 * nb + lr;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
