@@ -85,14 +85,21 @@ __global__ __launch_bounds__(256) void ozk_sqd_kernel(long n, int nw, const doub
   if (threadIdx.x == 0 && fl) atomicOr(flags + w, fl);
 }
 
+// Digits of u in [-127, 127]: a[p] = rint(r_p), r_{p+1} = (r_p - a[p]) 254.
+// Adding 1.5 * 2^52 to |r| < 2^51 rounds it to an integer (to nearest, ties
+// to even, as rint) that the low mantissa bits of the sum then hold in two's
+// complement, so one FP64 add replaces the rounding and the conversion; the
+// sum minus 1.5 * 2^52 is rint(r) exactly.  Contraction stays off: every
+// step rounds as written (four FP64 adds and multiplies per digit).
 __device__ inline void slice4(double u, int a[NS]) {
-  // u in [-127, 127]
+#pragma clang fp contract(off)
+  constexpr double kRound = 6755399441055744.0;  // 1.5 * 2^52
   double r = u;
 #pragma unroll
   for (int p = 0; p < NS; p++) {
-    const double q = rint(r);
-    a[p] = (int)q;
-    r = (r - q) * 254.0;
+    const double s = r + kRound;
+    a[p] = __double2loint(s);
+    if (p + 1 < NS) r = (r - (s - kRound)) * 254.0;
   }
 }
 
