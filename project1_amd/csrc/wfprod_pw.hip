@@ -610,29 +610,51 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
   double2 *grids = (double2 *)ws;
   double2 *slab = grids + (long)(nocc + nunocc) * NN;
   const int *ngpt_dev = p->gidx + (long)p->nkpt * p->maxgpt;
+  const bool joint = p->ngptm <= 64 * PW_FOUT && N1 == N2 &&
+                     (N1 == 16 || N1 == 20 || N1 == 24 || N1 == 28 || N1 == 32 || N1 == 36 ||
+                      N1 == 40 || N1 == 48);
+  // The joint kernel's last round of workgroups leaves most CUs idle (1,600
+  // workgroups at 16 x 400 states are 3.1 rounds of 512).  When it runs on
+  // the context stream, outside graph capture, the k points alternate
+  // between that stream and the auxiliary one, each with its own grids (the
+  // second set in the slab region, which the joint kernel does not use), so
+  // the transforms and pair kernel of k+1 fill the tail of k's.
+  bool dual = joint && nk >= 2 && st == ctx->stream &&
+              (long)nocc * nunocc * N3 * PW_BOX * PW_BOX >= (long)(nocc + nunocc) * NN;
+  if (dual) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SPX_HIP(ctx, hipStreamIsCapturing(st, &cs));
+    dual = cs == hipStreamCaptureStatusNone;
+  }
+  if (dual) {
+    int rc = spx_aux_stream(ctx);
+    if (rc) return rc;
+    SPX_HIP(ctx, hipEventRecord(ctx->aux_ev[7], st));
+    SPX_HIP(ctx, hipStreamWaitEvent(ctx->aux, ctx->aux_ev[7], 0));
+  }
   for (int ik = 0; ik < nk; ik++) {
-    hipLaunchKernelGGL(pw_inverse_kernel<NB>, dim3(nocc + nunocc), dim3(NT), 0, st,
+    const hipStream_t s = dual && (ik & 1) ? ctx->aux : st;
+    double2 *gk = dual && (ik & 1) ? slab : grids;
+    hipLaunchKernelGGL(pw_inverse_kernel<NB>, dim3(nocc + nunocc), dim3(NT), 0, s,
                        (const double2 *)cpw, p->maxgpt, nband, p->gidx, ngpt_dev, k1[ik], b1lo,
                        nocc, k2[ik], b2lo, N1, N2, N3, 1.0 / std::sqrt(p->vol), 1.0 / (double)NN,
-                       p->step, grids);
+                       p->step, gk);
     SPX_LAUNCHED(ctx);
-    if (p->ngptm <= 64 * PW_FOUT && N1 == N2 &&
-        (N1 == 16 || N1 == 20 || N1 == 24 || N1 == 28 || N1 == 32 || N1 == 36 || N1 == 40 ||
-         N1 == 48)) {
+    if (joint) {
       const int ngrp = (nocc + 3) / 4;
       const dim3 grid((unsigned)(((nunocc + 7) / 8) * 8 * ngrp));
       const size_t tzb = (size_t)(N3 + 4) * PW_BOX * sizeof(double2);  // G <= 4 padding planes
       const int *ox = oidx_dev + (long)ik * p->ngptm;
       const long tt = (long)ik * nocc * nunocc;
       switch (N1) {
-        case 16: hipLaunchKernelGGL((pw_pair_joint_kernel<16, 16>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 20: hipLaunchKernelGGL((pw_pair_joint_kernel<20, 20>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 24: hipLaunchKernelGGL((pw_pair_joint_kernel<24, 24>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 28: hipLaunchKernelGGL((pw_pair_joint_kernel<28, 28>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 32: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 36: hipLaunchKernelGGL((pw_pair_joint_kernel<36, 36>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        case 40: hipLaunchKernelGGL((pw_pair_joint_kernel<40, 40>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
-        default: hipLaunchKernelGGL((pw_pair_joint_kernel<48, 48>), grid, dim3(NT), tzb, st, grids, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 16: hipLaunchKernelGGL((pw_pair_joint_kernel<16, 16>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 20: hipLaunchKernelGGL((pw_pair_joint_kernel<20, 20>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 24: hipLaunchKernelGGL((pw_pair_joint_kernel<24, 24>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 28: hipLaunchKernelGGL((pw_pair_joint_kernel<28, 28>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 32: hipLaunchKernelGGL((pw_pair_joint_kernel<32, 32>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 36: hipLaunchKernelGGL((pw_pair_joint_kernel<36, 36>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        case 40: hipLaunchKernelGGL((pw_pair_joint_kernel<40, 40>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
+        default: hipLaunchKernelGGL((pw_pair_joint_kernel<48, 48>), grid, dim3(NT), tzb, s, gk, nocc, nunocc, N3, ox, p->ngptm, (double2 *)M, ldm, tt, col0); break;
       }
       SPX_LAUNCHED(ctx);
       continue;
@@ -651,6 +673,10 @@ int launch_pw(spexhip_ctx *ctx, hipStream_t st, const spexhip_plan *p, const voi
                        oidx_dev + (long)ik * p->ngptm, p->ngptm, (double2 *)M, ldm,
                        (long)ik * nocc * nunocc, col0);
     SPX_LAUNCHED(ctx);
+  }
+  if (dual) {
+    SPX_HIP(ctx, hipEventRecord(ctx->aux_ev[8], ctx->aux));
+    SPX_HIP(ctx, hipStreamWaitEvent(st, ctx->aux_ev[8], 0));
   }
   return SPEXHIP_OK;
 }

@@ -126,6 +126,32 @@ def test_pw_realistic_box_subset(ctx):
     plan.destroy()
 
 
+def test_pw_alternating_streams_bitwise(ctx):
+    """Several k points in one call alternate between the context stream and
+    an auxiliary one, each with its own grids, when the joint kernel runs and
+    the workspace's slab region holds a second set of grids (4 x 12 states on
+    the 24^3 grid here).  The result equals, bit for bit, the same k points
+    computed one call each (one stream, one set of grids); one row block of a
+    k point on the auxiliary stream is checked against the oracle."""
+    s = synth.product_system(nmesh=4, nband=16, gcut=7.1, ngptm=906, nfft=(36, 36, 36), ntype=1,
+                             lcut=0, nrad=1, lcutm=0, nradm=1, seed=48)
+    sub = [0, 21, 63, 5]
+    sk = lambda idx: synth.ProductSystem(**{**s.__dict__, "k1": s.k1[idx], "k2": s.k2[idx],
+                                            "g2": s.g2[idx]})
+    nocc, nunocc = 4, 12
+    got = _pw_dev(ctx, sk(sub), 0, nocc, 4, nunocc)
+    one = np.concatenate([_pw_dev(ctx, sk([k]), 0, nocc, 4, nunocc) for k in sub])
+    assert got.shape == (len(sub) * nocc * nunocc, s.ngptm)
+    assert np.array_equal(got, one)
+    s1 = sk([21])
+    ref = oracle.wfprod_pw(s1.k1, s1.k2, s1.g2, s.gpt, s.ngpt, s.cpw, s.gptm, s.vol, 2, 1, 4,
+                           nunocc)
+    blk = got[nocc * nunocc + 2 * nunocc:nocc * nunocc + 3 * nunocc]
+    mx, fro = normwise(blk, ref)
+    print(f"PW two streams: max-rel {mx:.2e} fro-rel {fro:.2e}")
+    assert mx < TOL
+
+
 def test_host_dropins_match_oracle(ctx):
     s = synth.product_system(nmesh=2, nband=8, gcut=3.1, gcutm=2.1, lcut=2, nrad=2, lcutm=2,
                              nradm=2, seed=44)
