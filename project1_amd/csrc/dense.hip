@@ -96,9 +96,10 @@ __global__ void eps_diag_kernel(int nb, const double *__restrict__ sq,
 constexpr int GB = 64;  // panel width
 
 // P = inverse of the GB x GB diagonal block at (k0,k0) (rows/cols beyond nb
-// are treated as identity).  One workgroup of four waves per matrix
-// (blockIdx.y): wave v keeps columns 16v .. 16v+15 of the block in
-// registers, lane = row.  Pivot step p publishes column p (its owner wave)
+// are treated as identity).  One workgroup of GB/GJ_CW waves per matrix
+// (blockIdx.y): wave v keeps columns GJ_CW v .. GJ_CW v + GJ_CW-1 of the
+// block in registers, lane = row; several waves per SIMD hide each other's
+// barrier, LDS and division latencies.  Pivot step p publishes column p (its owner wave)
 // and row p (lane p of every wave) through LDS, double-buffered so that one
 // barrier per step suffices: a buffer is rewritten two steps later, after
 // every wave has passed the barrier that follows its reads.
@@ -108,33 +109,35 @@ constexpr int GB = 64;  // panel width
 // GEN: a general complex matrix (real-frequency eps) whose rows were
 // interchanged by gj_pivot_kernel: complex pivots; a zero or non-finite
 // pivot sets the flag.
+constexpr int GJ_CW = 4;                   // columns per wave
+constexpr int GJ_NT = 64 * (GB / GJ_CW);  // 16 waves
 template <bool GEN>
-__global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
+__global__ __launch_bounds__(GJ_NT) void gj_diag_kernel(int nb, const double2 *__restrict__ X,
                                                       long ldx, long sX, int k0,
                                                       double2 *__restrict__ P, int *__restrict__ flag,
                                                       double *__restrict__ logdet) {
-  static_assert(GB == 64, "one lane per row, four waves of 16 columns");
+  static_assert(GB == 64 && GB % GJ_CW == 0, "one lane per row, GJ_CW columns per wave");
   __shared__ double2 colb[2][GB], rowb[2][GB];
   __shared__ double piv[GB];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const double2 *Xb = X + sX * blockIdx.y;
-  double2 a[16];
+  double2 a[GJ_CW];
 #pragma unroll
-  for (int c = 0; c < 16; c++) {
-    const int j = wv * 16 + c;
+  for (int c = 0; c < GJ_CW; c++) {
+    const int j = wv * GJ_CW + c;
     double2 v = make_double2(lane == j ? 1.0 : 0.0, 0.0);
     if (k0 + lane < nb && k0 + j < nb) v = Xb[(k0 + lane) + ldx * (long)(k0 + j)];
     a[c] = v;
   }
   bool bad = false;
-  for (int pb = 0; pb < 4; pb++) {
+  for (int pb = 0; pb < GB / GJ_CW; pb++) {
 #pragma unroll
-    for (int pc = 0; pc < 16; pc++) {
-      const int p = pb * 16 + pc, buf = p & 1;
+    for (int pc = 0; pc < GJ_CW; pc++) {
+      const int p = pb * GJ_CW + pc, buf = p & 1;
       if (wv == pb) colb[buf][lane] = a[pc];
       if (lane == p) {
 #pragma unroll
-        for (int c = 0; c < 16; c++) rowb[buf][wv * 16 + c] = a[c];
+        for (int c = 0; c < GJ_CW; c++) rowb[buf][wv * GJ_CW + c] = a[c];
       }
       __syncthreads();
       const double2 f = colb[buf][lane];
@@ -152,8 +155,8 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__r
         pr = make_double2(1.0 / d, 0.0);
       }
 #pragma unroll
-      for (int c = 0; c < 16; c++) {
-        const int j = wv * 16 + c;
+      for (int c = 0; c < GJ_CW; c++) {
+        const int j = wv * GJ_CW + c;
         if (j == p) {
           a[c] = lane == p ? pr
                            : make_double2(-(f.x * pr.x - f.y * pr.y), -(f.x * pr.y + f.y * pr.x));
@@ -172,7 +175,7 @@ __global__ __launch_bounds__(256) void gj_diag_kernel(int nb, const double2 *__r
   }
   double2 *Pb = P + (long)GB * GB * blockIdx.y;
 #pragma unroll
-  for (int c = 0; c < 16; c++) Pb[lane + GB * (wv * 16 + c)] = a[c];
+  for (int c = 0; c < GJ_CW; c++) Pb[lane + GB * (wv * GJ_CW + c)] = a[c];
   if (threadIdx.x == 0 && bad) flag[0] = 1;
   if (!GEN && logdet) {
     __syncthreads();
@@ -498,10 +501,10 @@ int spx_inverse(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long 
       hipLaunchKernelGGL(gj_swap_rows_kernel, dim3((nb + 255) / 256, nw), dim3(256), 0, st, nb,
                          (double2 *)X, ldx, sX, k0, (const int *)piv, nbp);
       SPX_LAUNCHED(ctx);
-      hipLaunchKernelGGL(gj_diag_kernel<true>, dim3(1, nw), dim3(NT), 0, st, nb,
+      hipLaunchKernelGGL(gj_diag_kernel<true>, dim3(1, nw), dim3(GJ_NT), 0, st, nb,
                          (const double2 *)X, ldx, sX, k0, P, flag, nullptr);
     } else
-      hipLaunchKernelGGL(gj_diag_kernel<false>, dim3(1, nw), dim3(NT), 0, st, nb,
+      hipLaunchKernelGGL(gj_diag_kernel<false>, dim3(1, nw), dim3(GJ_NT), 0, st, nb,
                          (const double2 *)X, ldx, sX, k0, P, flag,
                          logdet_parts ? logdet_parts + (long)kb * nw : nullptr);
     SPX_LAUNCHED(ctx);
