@@ -61,20 +61,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   });
 }
 
-// Lower triangle <- conj(upper) (I > J), optional real diagonal.
-__global__ void hermitize_kernel(int nb, double2 *__restrict__ X, long ldx, long sX,
-                                 int real_diag) {
-  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nn = (long)nb * nb;
-  if (g >= nn) return;
-  const int j = (int)(g / nb), i = (int)(g % nb);
-  double2 *Xb = X + sX * blockIdx.y;
-  if (i > j) {
-    double2 v = Xb[j + ldx * (long)i];
-    v.y = -v.y;
-    Xb[i + ldx * (long)j] = v;
-  } else if (i == j && real_diag) {
-    Xb[i + ldx * (long)i].y = 0.0;
+// Lower triangle <- conj(upper) (I > J), optional real diagonal.  One
+// workgroup per 32x32 tile (bi, bj) on or below the diagonal (blockIdx.x =
+// bi, blockIdx.y = bj, blockIdx.z = matrix): it reads the mirrored upper
+// tile column by column and writes the lower one column by column, the
+// transposition going through LDS, so both sides are coalesced.  On a
+// diagonal tile the elements read (upper) and written (lower, diagonal)
+// are disjoint or read before the barrier.
+constexpr int HT = 32;
+__global__ __launch_bounds__(256) void hermitize_kernel(int nb, double2 *__restrict__ X, long ldx,
+                                                        long sX, int real_diag) {
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi < bj) return;  // whole workgroup, before the barrier
+  __shared__ double2 t[HT][HT + 1];
+  double2 *Xb = X + sX * blockIdx.z;
+  const int tx = threadIdx.x % HT, ty = threadIdx.x / HT;
+  // t[c][r] = X(bj*HT + r, bi*HT + c)
+  for (int c = ty; c < HT; c += 256 / HT) {
+    const int row = bj * HT + tx, col = bi * HT + c;
+    if (row < nb && col < nb) t[c][tx] = Xb[row + ldx * (long)col];
+  }
+  __syncthreads();
+  // X(bi*HT + a, bj*HT + b) = conj(t[a][b]) for row > col
+  for (int b = ty; b < HT; b += 256 / HT) {
+    const int row = bi * HT + tx, col = bj * HT + b;
+    if (row >= nb || col >= nb) continue;
+    if (row > col) {
+      const double2 v = t[tx][b];
+      Xb[row + ldx * (long)col] = make_double2(v.x, -v.y);
+    } else if (row == col && real_diag) {
+      Xb[row + ldx * (long)row].y = 0.0;
+    }
   }
 }
 
@@ -99,10 +116,11 @@ constexpr int GB = 64;  // panel width
 // are treated as identity).  One workgroup of GB/GJ_CW waves per matrix
 // (blockIdx.y): wave v keeps columns GJ_CW v .. GJ_CW v + GJ_CW-1 of the
 // block in registers, lane = row; several waves per SIMD hide each other's
-// barrier, LDS and division latencies.  Pivot step p publishes column p (its owner wave)
-// and row p (lane p of every wave) through LDS, double-buffered so that one
-// barrier per step suffices: a buffer is rewritten two steps later, after
-// every wave has passed the barrier that follows its reads.
+// barrier, LDS and division latencies.  Pivot step p publishes column p
+// (its owner wave) and row p (lane p of every wave) through LDS,
+// double-buffered so that one barrier per step suffices: a buffer is
+// rewritten two steps later, after every wave has passed the barrier that
+// follows its reads.
 // logdet (optional): logdet[blockIdx.y] = sum of log(pivot) over this block
 // -- the pivots of Gauss-Jordan on an HPD matrix are the Schur-complement
 // diagonals, so summed over all blocks they give log det.
@@ -462,9 +480,9 @@ int spx_zgemm(spexhip_ctx *ctx, hipStream_t st, int m, int n, int k, double alph
 
 int spx_hermitize(spexhip_ctx *ctx, hipStream_t st, int nb, int nw, void *X, long ldx,
                   int real_diag) {
-  const long nn = (long)nb * nb;
-  hipLaunchKernelGGL(hermitize_kernel, dim3((nn + 255) / 256, nw), dim3(256), 0, st, nb,
-                     (double2 *)X, ldx, ldx * (long)nb, real_diag);
+  const unsigned nt = (unsigned)((nb + HT - 1) / HT);
+  hipLaunchKernelGGL(hermitize_kernel, dim3(nt, nt, nw), dim3(256), 0, st, nb, (double2 *)X, ldx,
+                     ldx * (long)nb, real_diag);
   SPX_LAUNCHED(ctx);
   return SPEXHIP_OK;
 }
